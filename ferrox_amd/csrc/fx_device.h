@@ -1,0 +1,293 @@
+/*
+ * Device-side building blocks of the FerroX hot path for gfx950: field views
+ * with the reference's ghost semantics, the masked stencils of
+ * Source/Solver/DerivativeAlgorithm.H, the rotation matrix of
+ * TotalEnergyDensity.cpp:167-195, and the multigrid row operator.
+ *
+ * Layout in HBM: every cell field is a dense FP64 array over the valid box,
+ * i fastest (c = i + nx*(j + ny*k)), no ghost cells; periodic neighbours are
+ * found by index wrap, and a read across a non-periodic face returns the
+ * ghost value the reference would hold there (0 for P, see main.cpp:157).
+ * The material id is one byte per cell on a box grown by one cell, so the
+ * stencils can test the neighbour's material exactly as the reference tests
+ * MaterialMask's ghost layer (Initialization.cpp:177-205): bits 0-1 hold the
+ * mask code (FE 0, DE 1, SC 2, Channel 3), bit 2 the t-phase flag.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fxd {
+
+enum { BC_PER = 0, BC_DIR = 1, BC_NEU = 2 };
+
+struct Geom {
+    int n[3];
+    int per[3];
+    double dx[3];
+    double prob_lo[3], prob_hi[3];
+};
+
+/* P with ghost semantics: periodic image or 0 across a non-periodic face. */
+struct PView {
+    const double* __restrict__ p;
+    int nx, ny, nz, px, py, pz;
+    __device__ __forceinline__ double operator()(int i, int j, int k) const {
+        if (i < 0) { if (!px) return 0.0; do i += nx; while (i < 0); }
+        else if (i >= nx) { if (!px) return 0.0; do i -= nx; while (i >= nx); }
+        if (j < 0) { if (!py) return 0.0; do j += ny; while (j < 0); }
+        else if (j >= ny) { if (!py) return 0.0; do j -= ny; while (j >= ny); }
+        if (k < 0) { if (!pz) return 0.0; do k += nz; while (k < 0); }
+        else if (k >= nz) { if (!pz) return 0.0; do k -= nz; while (k >= nz); }
+        return p[i + nx * (j + ny * k)];
+    }
+};
+
+/* Material id on the grown box. */
+struct MView {
+    const uint8_t* __restrict__ m;
+    int sx, sy;
+    __device__ __forceinline__ int code(int i, int j, int k) const { return m[(i + 1) + sx * (j + 1) + sy * (k + 1)] & 3; }
+    __device__ __forceinline__ int raw(int i, int j, int k) const { return m[(i + 1) + sx * (j + 1) + sy * (k + 1)]; }
+};
+
+struct PBC {
+    int lo[3], hi[3];
+    double lambda;
+};
+
+/* DPDx/DPDy/DPDz, DerivativeAlgorithm.H:45-252.  Flags outside 0..4 are
+ * rejected before any launch. */
+template <int d>
+__device__ __forceinline__ double dpd(const PView& F, const MView& M, int i, int j, int k,
+                                      const double* dx, const PBC& bc) {
+    constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
+    const double h = dx[d];
+    const double lambda = bc.lambda;
+    const int m0 = M.code(i, j, k);
+    if (m0 != 0) return 0.0;
+    if (M.code(i - di, j - dj, k - dk) != 0) {
+        switch (bc.lo[d]) {
+        case 0: { const double F_lo = 0.0; return (-4. * F_lo + 3. * F(i, j, k) + F(i + di, j + dj, k + dk)) / (3. * h); }
+        case 1: { const double F_lo = F(i, j, k) / (1 + h / 2 / lambda);
+                  return (h * F_lo / lambda - F(i, j, k) + F(i + di, j + dj, k + dk)) / (2. * h); }
+        case 2: return (-F(i, j, k) + F(i + di, j + dj, k + dk)) / (2. * h);
+        case 3: return (-F(i - di, j - dj, k - dk) + F(i + di, j + dj, k + dk)) / (2. * h);
+        default: return (-F(i, j, k) + F(i + di, j + dj, k + dk)) / (h);
+        }
+    } else if (M.code(i + di, j + dj, k + dk) != 0) {
+        switch (bc.hi[d]) {
+        case 0: { const double F_hi = 0.0; return (4. * F_hi - 3. * F(i, j, k) - F(i - di, j - dj, k - dk)) / (3. * h); }
+        case 1: { const double F_hi = F(i, j, k) / (1 - h / 2 / lambda);
+                  return (h * F_hi / lambda + F(i, j, k) - F(i - di, j - dj, k - dk)) / (2. * h); }
+        case 2: return (F(i, j, k) - F(i - di, j - dj, k - dk)) / (2. * h);
+        case 3: return (F(i + di, j + dj, k + dk) - F(i - di, j - dj, k - dk)) / (2. * h);
+        default: return (F(i, j, k) - F(i - di, j - dj, k - dk)) / (h);
+        }
+    }
+    return (F(i + di, j + dj, k + dk) - F(i - di, j - dj, k - dk)) / (2. * h);
+}
+
+/* DoubleDPDx/y/z, DerivativeAlgorithm.H:257-467. */
+template <int d>
+__device__ __forceinline__ double ddpd(const PView& F, const MView& M, int i, int j, int k,
+                                       const double* dx, const PBC& bc) {
+    constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
+    const double h = dx[d];
+    const double lambda = bc.lambda;
+    const int m0 = M.code(i, j, k);
+    if (m0 != 0) return 0.0;
+    if (M.code(i - di, j - dj, k - dk) != 0) {
+        switch (bc.lo[d]) {
+        case 0: { const double F_lo = 0.0; return 4. * (2. * F_lo - 3. * F(i, j, k) + F(i + di, j + dj, k + dk)) / 3. / h / h; }
+        case 1: { const double F_lo = F(i, j, k) / (1 + h / 2 / lambda);
+                  return (-h * F_lo / lambda - F(i, j, k) + F(i + di, j + dj, k + dk)) / h / h; }
+        case 2: return (-F(i, j, k) + F(i + di, j + dj, k + dk)) / h / h;
+        case 3: return (F(i + di, j + dj, k + dk) - 2. * F(i, j, k) + F(i - di, j - dj, k - dk)) / h / h;
+        default: return (F(i + 2 * di, j + 2 * dj, k + 2 * dk) - 2. * F(i + di, j + dj, k + dk) + F(i, j, k)) / h / h;
+        }
+    } else if (M.code(i + di, j + dj, k + dk) != 0) {
+        switch (bc.hi[d]) {
+        case 0: { const double F_hi = 0.0; return 4. * (2. * F_hi - 3. * F(i, j, k) + F(i - di, j - dj, k - dk)) / 3. / h / h; }
+        case 1: { const double F_hi = F(i, j, k) / (1 - h / 2 / lambda);
+                  return (h * F_hi / lambda - F(i, j, k) + F(i - di, j - dj, k - dk)) / h / h; }
+        case 2: return (-F(i, j, k) + F(i - di, j - dj, k - dk)) / h / h;
+        case 3: return (F(i + di, j + dj, k + dk) - 2. * F(i, j, k) + F(i - di, j - dj, k - dk)) / h / h;
+        default: return (F(i, j, k) - 2. * F(i - di, j - dj, k - dk) + F(i - 2 * di, j - 2 * dj, k - 2 * dk)) / h / h;
+        }
+    }
+    return (F(i + di, j + dj, k + dk) - 2. * F(i, j, k) + F(i - di, j - dj, k - dk)) / (h * h);
+}
+
+/* DoubleDPDxDy / DxDz / DyDz, DerivativeAlgorithm.H:471-497. */
+template <int a, int b>
+__device__ __forceinline__ double ddpd_mixed(const PView& F, const MView& M, int i, int j, int k,
+                                             const double* dx, const PBC& bc) {
+    constexpr int ai = (a == 0), aj = (a == 1), ak = (a == 2);
+    return (dpd<b>(F, M, i + ai, j + aj, k + ak, dx, bc) - dpd<b>(F, M, i - ai, j - aj, k - ak, dx, bc)) / 2. / dx[a];
+}
+
+/* Rotation from Euler angles in degrees (TotalEnergyDensity.cpp:167-195). */
+__host__ __device__ inline void rotation(double a_deg, double b_deg, double t_deg, int use_Euler_angles, double R[9]) {
+    const double Pi = 3.14159265358979323846;
+    const double a = Pi / 180. * a_deg, b = Pi / 180. * b_deg, t = Pi / 180. * t_deg;
+    if (use_Euler_angles) {
+        R[0] = cos(a) * cos(t) - cos(b) * sin(a) * sin(t);
+        R[1] = sin(a) * cos(t) + cos(b) * cos(a) * sin(t);
+        R[2] = sin(b) * sin(t);
+        R[3] = -cos(b) * cos(t) * sin(a) - cos(a) * sin(t);
+        R[4] = cos(b) * cos(a) * cos(t) - sin(a) * sin(t);
+        R[5] = sin(b) * cos(t);
+        R[6] = sin(a) * sin(b);
+        R[7] = -cos(a) * sin(b);
+        R[8] = cos(b);
+    } else {
+        R[0] = cos(b) * cos(t);
+        R[1] = sin(a) * sin(b) * cos(t) - cos(a) * sin(t);
+        R[2] = cos(a) * sin(b) * cos(t) + sin(a) * sin(t);
+        R[3] = cos(b) * sin(t);
+        R[4] = sin(b) * sin(a) * sin(t) + cos(a) * cos(t);
+        R[5] = cos(a) * sin(b) * sin(t) - sin(a) * cos(t);
+        R[6] = -sin(b);
+        R[7] = sin(a) * cos(b);
+        R[8] = cos(a) * cos(b);
+    }
+}
+
+/* phi across faces for E (ElectrostaticSolver.cpp:134-203).  The Dirichlet
+ * ghost is 2*phi_b - phi_0 (what MLMG leaves in the aliased solution) or phi_b
+ * (face value), per ghost_mode. */
+struct PhiView {
+    const double* __restrict__ p;
+    int n[3];
+    int lo[3], hi[3];          /* BC kinds */
+    double v[6];               /* Dirichlet values x,y,z lo then hi */
+    int ghost_face;
+    __device__ __forceinline__ double at(int i, int j, int k) const {
+        const int nx = n[0], ny = n[1], nz = n[2];
+        if (i < 0 || i >= nx) {
+            const bool l = i < 0; const int t = l ? lo[0] : hi[0];
+            if (t == BC_PER) i = l ? i + nx : i - nx;
+            else { const double p0 = p[(l ? 0 : nx - 1) + nx * (j + ny * k)];
+                   if (t == BC_NEU) return p0; const double pb = v[l ? 0 : 3];
+                   return ghost_face ? pb : 2.0 * pb - p0; }
+        }
+        if (j < 0 || j >= ny) {
+            const bool l = j < 0; const int t = l ? lo[1] : hi[1];
+            if (t == BC_PER) j = l ? j + ny : j - ny;
+            else { const double p0 = p[i + nx * ((l ? 0 : ny - 1) + ny * k)];
+                   if (t == BC_NEU) return p0; const double pb = v[l ? 1 : 4];
+                   return ghost_face ? pb : 2.0 * pb - p0; }
+        }
+        if (k < 0 || k >= nz) {
+            const bool l = k < 0; const int t = l ? lo[2] : hi[2];
+            if (t == BC_PER) k = l ? k + nz : k - nz;
+            else { const double p0 = p[i + nx * (j + ny * (l ? 0 : nz - 1))];
+                   if (t == BC_NEU) return p0; const double pb = v[l ? 2 : 5];
+                   return ghost_face ? pb : 2.0 * pb - p0; }
+        }
+        return p[i + nx * (j + ny * k)];
+    }
+};
+
+/* grad(phi) with the reference's one-sided z formula at the metal faces
+ * (DphiDz, DFDx, DFDy: DerivativeAlgorithm.H:8-40). */
+__device__ __forceinline__ void grad_phi(const PhiView& P, const Geom& g, int i, int j, int k,
+                                         double& dfdx, double& dfdy, double& dfdz) {
+    const double* dx = g.dx;
+    dfdx = (P.at(i + 1, j, k) - P.at(i - 1, j, k)) / (2. * dx[0]);
+    dfdy = (P.at(i, j + 1, k) - P.at(i, j - 1, k)) / (2. * dx[1]);
+    const double z_hi = g.prob_lo[2] + (k + 1.5) * dx[2];
+    const double z_lo = g.prob_lo[2] + (k - 0.5) * dx[2];
+    if (z_lo < g.prob_lo[2]) dfdz = (-4. * P.at(i, j, k - 1) + 3. * P.at(i, j, k) + P.at(i, j, k + 1)) / (3. * dx[2]);
+    else if (z_hi > g.prob_hi[2]) dfdz = (4. * P.at(i, j, k + 1) - 3. * P.at(i, j, k) - P.at(i, j, k - 1)) / (3. * dx[2]);
+    else dfdz = (P.at(i, j, k + 1) - P.at(i, j, k - 1)) / (2. * dx[2]);
+}
+
+/* ------------------------------ multigrid -------------------------------- */
+struct DLev {
+    int n0, n1, n2;
+    double ih2[3];
+    double* phi;
+    double* b;
+    double* bx;   /* (n0+1)*n1*n2 */
+    double* by;   /* n0*(n1+1)*n2 */
+    double* bz;   /* n0*n1*(n2+1) */
+    double* a;
+    int crs[3];
+};
+
+struct DBC {
+    int lo[3], hi[3];
+    double v[6];
+};
+
+/* (A phi)_c = D*phi_c - S for the operator
+ *   -a*phi - div(beta grad phi)
+ * with face coefficients beta, periodic / Dirichlet (ghost 2*phi_b - phi_0) /
+ * homogeneous Neumann faces.  Accumulation order: -a, x lo, x hi, y lo, y hi,
+ * z lo, z hi. */
+__device__ __forceinline__ void mg_row(const DLev& l, const double* __restrict__ phi, int i, int j, int k,
+                                       const DBC& bc, bool inh, double& D, double& S) {
+    const int n0 = l.n0, n1 = l.n1, n2 = l.n2;
+    const int c = i + n0 * (j + n1 * k);
+    D = -l.a[c];
+    S = 0.0;
+    {
+        const int f = i + (n0 + 1) * (j + n1 * k);
+        const double wlo = l.bx[f] * l.ih2[0], whi = l.bx[f + 1] * l.ih2[0];
+        if (i > 0 || bc.lo[0] == BC_PER) { D += wlo; S += wlo * phi[i > 0 ? c - 1 : c + (n0 - 1)]; }
+        else if (bc.lo[0] == BC_DIR) { D += 2.0 * wlo; if (inh) S += 2.0 * wlo * bc.v[0]; }
+        if (i < n0 - 1 || bc.hi[0] == BC_PER) { D += whi; S += whi * phi[i < n0 - 1 ? c + 1 : c - (n0 - 1)]; }
+        else if (bc.hi[0] == BC_DIR) { D += 2.0 * whi; if (inh) S += 2.0 * whi * bc.v[3]; }
+    }
+    {
+        const int f = i + n0 * (j + (n1 + 1) * k);
+        const double wlo = l.by[f] * l.ih2[1], whi = l.by[f + n0] * l.ih2[1];
+        if (j > 0 || bc.lo[1] == BC_PER) { D += wlo; S += wlo * phi[j > 0 ? c - n0 : c + n0 * (n1 - 1)]; }
+        else if (bc.lo[1] == BC_DIR) { D += 2.0 * wlo; if (inh) S += 2.0 * wlo * bc.v[1]; }
+        if (j < n1 - 1 || bc.hi[1] == BC_PER) { D += whi; S += whi * phi[j < n1 - 1 ? c + n0 : c - n0 * (n1 - 1)]; }
+        else if (bc.hi[1] == BC_DIR) { D += 2.0 * whi; if (inh) S += 2.0 * whi * bc.v[4]; }
+    }
+    {
+        const int s = n0 * n1;
+        const double wlo = l.bz[c] * l.ih2[2], whi = l.bz[c + s] * l.ih2[2];
+        if (k > 0 || bc.lo[2] == BC_PER) { D += wlo; S += wlo * phi[k > 0 ? c - s : c + s * (n2 - 1)]; }
+        else if (bc.lo[2] == BC_DIR) { D += 2.0 * wlo; if (inh) S += 2.0 * wlo * bc.v[2]; }
+        if (k < n2 - 1 || bc.hi[2] == BC_PER) { D += whi; S += whi * phi[k < n2 - 1 ? c + s : c - s * (n2 - 1)]; }
+        else if (bc.hi[2] == BC_DIR) { D += 2.0 * whi; if (inh) S += 2.0 * whi * bc.v[5]; }
+    }
+}
+
+/* Block-wide reductions (wavefront shuffle, then LDS across waves). */
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) { for (int i = 0; i < BS / 64; ++i) r += sh[i]; sh[BS / 64] = r; }
+    __syncthreads();
+    r = sh[BS / 64];
+    __syncthreads();
+    return r;
+}
+/* max that propagates NaN (fmax would drop it and hide a diverged solve) */
+__device__ __forceinline__ double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+template <int BS>
+__device__ __forceinline__ double block_max(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v = nmax(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) { for (int i = 0; i < BS / 64; ++i) r = nmax(r, sh[i]); sh[BS / 64] = r; }
+    __syncthreads();
+    r = sh[BS / 64];
+    __syncthreads();
+    return r;
+}
+
+}  // namespace fxd

@@ -1,0 +1,419 @@
+/*
+ * Inputs-file reader for the FerroX hot path: a ParmParse-compatible subset.
+ *
+ * Replaces amrex::ParmParse + InitializeFerroXNamespace (Source/FerroX.cpp:273-572),
+ * c_GeometryProperties::ParseBasicDomainInput (Source/Input/GeometryProperties/
+ * GeometryProperties.cpp:87-165) and the boundary-token parsing of
+ * Source/Input/BoundaryConditions/BoundaryConditions.cpp:52-214.
+ *
+ * Format: "key = v1 v2 ...", '#' starts a comment, a later definition of a key
+ * replaces an earlier one, double-quoted values may span lines.  Numeric
+ * values may be arithmetic expressions (+ - * / ^ **, parentheses, e-notation,
+ * sqrt/exp/log/sin/cos/abs, pi) over names defined as my_constants.<name>,
+ * following the parser the reference uses for getWithParser keys
+ * (Source/Utils/SelectWarpXUtils/WarpXUtil.cpp:101-315).  Unknown keys are
+ * accepted and ignored, as ParmParse does.
+ */
+#include "../../include/ferrox_mi.h"
+
+#include <cctype>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace {
+
+const double UNSET = NAN;
+const int IUNSET = INT_MIN;
+
+void set_err(char* err, int errlen, const std::string& m) {
+    if (err && errlen > 0) {
+        std::snprintf(err, (size_t)errlen, "%s", m.c_str());
+    }
+}
+
+/* ---------------- expression evaluator (recursive descent) ---------------- */
+struct Expr {
+    const std::string& s;
+    size_t p = 0;
+    const std::map<std::string, double>& consts;
+    bool ok = true;
+    std::string why;
+    Expr(const std::string& str, const std::map<std::string, double>& c) : s(str), consts(c) {}
+    void ws() { while (p < s.size() && std::isspace((unsigned char)s[p])) ++p; }
+    double parse() {
+        double v = sum();
+        ws();
+        if (p != s.size()) { ok = false; why = "trailing characters in '" + s + "'"; }
+        return v;
+    }
+    double sum() {
+        double v = prod();
+        for (;;) {
+            ws();
+            if (p < s.size() && (s[p] == '+' || s[p] == '-')) { char o = s[p++]; double r = prod(); v = (o == '+') ? v + r : v - r; }
+            else return v;
+        }
+    }
+    double prod() {
+        double v = unary();
+        for (;;) {
+            ws();
+            if (p + 1 < s.size() && s[p] == '*' && s[p + 1] == '*') return v;   /* handled in power */
+            if (p < s.size() && (s[p] == '*' || s[p] == '/')) { char o = s[p++]; double r = unary(); v = (o == '*') ? v * r : v / r; }
+            else return v;
+        }
+    }
+    double unary() {
+        ws();
+        if (p < s.size() && s[p] == '-') { ++p; return -unary(); }
+        if (p < s.size() && s[p] == '+') { ++p; return unary(); }
+        return power();
+    }
+    double power() {
+        double b = atom();
+        ws();
+        if (p < s.size() && s[p] == '^') { ++p; return std::pow(b, unary()); }
+        if (p + 1 < s.size() && s[p] == '*' && s[p + 1] == '*') { p += 2; return std::pow(b, unary()); }
+        return b;
+    }
+    double atom() {
+        ws();
+        if (p >= s.size()) { ok = false; why = "unexpected end of '" + s + "'"; return 0; }
+        if (s[p] == '(') { ++p; double v = sum(); ws(); if (p < s.size() && s[p] == ')') ++p; else { ok = false; why = "missing ')'"; } return v; }
+        if (std::isdigit((unsigned char)s[p]) || s[p] == '.') {
+            const char* b = s.c_str() + p;
+            char* e = nullptr;
+            double v = std::strtod(b, &e);
+            if (e == b) { ok = false; why = "bad number"; return 0; }
+            p += (size_t)(e - b);
+            return v;
+        }
+        if (std::isalpha((unsigned char)s[p]) || s[p] == '_') {
+            size_t b = p;
+            while (p < s.size() && (std::isalnum((unsigned char)s[p]) || s[p] == '_' || s[p] == '.')) ++p;
+            std::string name = s.substr(b, p - b);
+            ws();
+            if (p < s.size() && s[p] == '(') {
+                ++p; double a = sum(); ws();
+                if (p < s.size() && s[p] == ')') ++p; else { ok = false; why = "missing ')'"; }
+                if (name == "sqrt") return std::sqrt(a);
+                if (name == "exp") return std::exp(a);
+                if (name == "log") return std::log(a);
+                if (name == "sin") return std::sin(a);
+                if (name == "cos") return std::cos(a);
+                if (name == "abs" || name == "fabs") return std::fabs(a);
+                ok = false; why = "unknown function " + name; return 0;
+            }
+            if (name == "pi") return 3.14159265358979323846;
+            auto it = consts.find(name);
+            if (it != consts.end()) return it->second;
+            ok = false; why = "unknown name '" + name + "'"; return 0;
+        }
+        ok = false; why = std::string("unexpected '") + s[p] + "'";
+        return 0;
+    }
+};
+
+/* --------------------------- key/value table ------------------------------ */
+struct Table {
+    std::map<std::string, std::vector<std::string>> kv;   /* last definition wins */
+};
+
+/* Split a value string into whitespace-separated tokens, keeping "dir(0.0)"
+ * and quoted strings whole. */
+std::vector<std::string> tokens(const std::string& v) {
+    std::vector<std::string> out;
+    size_t i = 0;
+    while (i < v.size()) {
+        while (i < v.size() && std::isspace((unsigned char)v[i])) ++i;
+        if (i >= v.size()) break;
+        if (v[i] == '"') {
+            size_t j = v.find('"', i + 1);
+            if (j == std::string::npos) j = v.size();
+            out.push_back(v.substr(i + 1, j - i - 1));
+            i = j + 1;
+            continue;
+        }
+        size_t j = i;
+        int depth = 0;
+        while (j < v.size() && (depth > 0 || !std::isspace((unsigned char)v[j]))) {
+            if (v[j] == '(') ++depth;
+            if (v[j] == ')') --depth;
+            ++j;
+        }
+        out.push_back(v.substr(i, j - i));
+        i = j;
+    }
+    return out;
+}
+
+bool parse_text(const std::string& text, Table& t, std::string& why) {
+    /* strip comments outside quotes, join quoted multi-line values */
+    std::string clean;
+    bool inq = false;
+    for (size_t i = 0; i < text.size(); ++i) {
+        char c = text[i];
+        if (c == '"') inq = !inq;
+        if (c == '#' && !inq) { while (i < text.size() && text[i] != '\n') ++i; clean += '\n'; continue; }
+        if (c == '\n' && inq) c = ' ';
+        clean += c;
+    }
+    std::istringstream is(clean);
+    std::string line;
+    int ln = 0;
+    while (std::getline(is, line)) {
+        ++ln;
+        size_t eq = line.find('=');
+        std::string key = line.substr(0, eq == std::string::npos ? line.size() : eq);
+        while (!key.empty() && std::isspace((unsigned char)key.back())) key.pop_back();
+        size_t b = 0;
+        while (b < key.size() && std::isspace((unsigned char)key[b])) ++b;
+        key = key.substr(b);
+        if (key.empty()) continue;
+        if (eq == std::string::npos) { why = "line " + std::to_string(ln) + ": expected 'key = value'"; return false; }
+        t.kv[key] = tokens(line.substr(eq + 1));
+    }
+    return true;
+}
+
+struct Applier {
+    fx_inputs* in;
+    std::map<std::string, double> consts;
+    std::string why;
+
+    bool num(const std::string& tok, double& v) {
+        Expr e(tok, consts);
+        v = e.parse();
+        if (!e.ok) { why = e.why; return false; }
+        return true;
+    }
+    bool dbl(const std::vector<std::string>& vals, double* dst, int n, const std::string& key) {
+        if ((int)vals.size() < n) { why = key + ": expected " + std::to_string(n) + " value(s)"; return false; }
+        for (int i = 0; i < n; ++i) if (!num(vals[i], dst[i])) { why = key + ": " + why; return false; }
+        return true;
+    }
+    bool integer(const std::vector<std::string>& vals, int* dst, int n, const std::string& key) {
+        double tmp[3];
+        if (!dbl(vals, tmp, n, key)) return false;
+        for (int i = 0; i < n; ++i) {
+            if (tmp[i] != std::floor(tmp[i])) { why = key + ": expected an integer"; return false; }
+            dst[i] = (int)tmp[i];
+        }
+        return true;
+    }
+    bool bc(const std::vector<std::string>& vals, int* dst, const std::string& key) {
+        if (vals.size() < 3) { why = key + ": expected 3 tokens"; return false; }
+        for (int d = 0; d < 3; ++d) {
+            const std::string& s = vals[d];
+            if (s == "per") dst[d] = FX_BC_PERIODIC;
+            else if (s.rfind("dir", 0) == 0) dst[d] = FX_BC_DIRICHLET;   /* dir(value): value parsed but never applied (ElectrostaticSolver.cpp:574-596) */
+            else if (s.rfind("neu", 0) == 0) dst[d] = FX_BC_NEUMANN;
+            else { why = key + ": unknown boundary token '" + s + "'"; return false; }
+        }
+        return true;
+    }
+
+    bool apply(const std::string& key, const std::vector<std::string>& v) {
+        fx_inputs& I = *in;
+#define D1(name) if (key == #name) return dbl(v, &I.name, 1, key);
+#define D3(name) if (key == #name) return dbl(v, I.name, 3, key);
+#define I1(name) if (key == #name) return integer(v, &I.name, 1, key);
+#define I3(name) if (key == #name) return integer(v, I.name, 3, key);
+        if (key == "domain.n_cell") return integer(v, I.n_cell, 3, key);
+        if (key == "domain.prob_lo") return dbl(v, I.prob_lo, 3, key);
+        if (key == "domain.prob_hi") return dbl(v, I.prob_hi, 3, key);
+        if (key == "domain.is_periodic") return integer(v, I.is_periodic, 3, key);
+        if (key == "domain.max_grid_size") return integer(v, I.max_grid_size, 3, key);
+        if (key == "domain.blocking_factor" || key == "domain.coord_sys") return true;
+        if (key == "domain.embedded_boundary") {
+            int eb = 0; if (!integer(v, &eb, 1, key)) return false;
+            if (eb) { why = "domain.embedded_boundary = 1 is not supported (embedded boundaries are out of scope)"; return false; }
+            return true;
+        }
+        if (key == "boundary.lo") return bc(v, I.bc_lo, key);
+        if (key == "boundary.hi") return bc(v, I.bc_hi, key);
+        I1(TimeIntegratorOrder) I1(prob_type) I1(is_polarization_scalar) I1(nsteps) I1(plot_int)
+        D1(dt) D1(delta) I1(random_seed) I1(voltage_sweep) I1(inc_step) I1(num_Vapp_max)
+        D1(Phi_Bc_lo) D1(Phi_Bc_hi) D1(Phi_Bc_inc) D1(Phi_Bc_hi_max) D1(phi_tolerance)
+        I1(include_Landau) I1(include_Grad) I1(include_Elec)
+        I3(P_BC_flag_lo) I3(P_BC_flag_hi) D1(lambda)
+        D1(epsilon_0) D1(epsilonX_fe) D1(epsilonX_fe_tphase) D1(epsilonZ_fe) D1(epsilon_de) D1(epsilon_si)
+        D1(alpha) D1(beta) D1(gamma) D1(alpha_12) D1(alpha_112) D1(alpha_123) D1(BigGamma)
+        D1(g11) D1(g44) D1(g12) D1(g44_p)
+        D3(FE_lo) D3(FE_hi) D3(DE_lo) D3(DE_hi) D3(SC_lo) D3(SC_hi) D3(Channel_lo) D3(Channel_hi) D3(Remnant_P)
+        D1(acceptor_doping) D1(donor_doping) I1(use_Fermi_Dirac) I1(use_work_function) D1(metal_work_function)
+        I1(Coordinate_Transformation) I1(use_Euler_angles)
+        if (key == "use_sundials") {
+            int u = 0; if (!integer(v, &u, 1, key)) return false;
+            if (u) { why = "use_sundials = 1 is not supported (SUNDIALS integrators are out of scope)"; return false; }
+            return true;
+        }
+        /* solver controls added by this build */
+        D1(mg_tol_rel) I1(mg_max_iter) I1(mg_nu1) I1(mg_nu2) I1(mg_warm_start) I1(outer_max_iter)
+        I1(phi_ghost_mode) I1(precision) I1(bias_ramp_steps) D1(bias_ramp_start) D1(bias_ramp_end) I1(verbose)
+#undef D1
+#undef D3
+#undef I1
+#undef I3
+        return true;   /* unknown keys are ignored, as ParmParse does */
+    }
+};
+
+bool apply_table(Table& t, fx_inputs* in, std::string& why) {
+    Applier a{in, {}, {}};
+    /* my_constants first (they may refer to each other in file order; a simple
+     * fixed-point pass resolves forward references) */
+    for (int pass = 0; pass < 4; ++pass)
+        for (auto& kv : t.kv)
+            if (kv.first.rfind("my_constants.", 0) == 0 && !kv.second.empty()) {
+                double v;
+                if (a.num(kv.second[0], v)) a.consts[kv.first.substr(13)] = v;
+            }
+    for (auto& kv : t.kv) {
+        if (kv.first.rfind("my_constants.", 0) == 0) continue;
+        if (!a.apply(kv.first, kv.second)) { why = a.why; return false; }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fx_version(void) { return "ferrox-mi355x 0.1"; }
+
+int fx_inputs_default(fx_inputs* in) {
+    if (!in) return FX_EINVAL;
+    std::memset(in, 0, sizeof(*in));
+    for (int d = 0; d < 3; ++d) {
+        in->n_cell[d] = IUNSET;
+        in->prob_lo[d] = UNSET; in->prob_hi[d] = UNSET;
+        in->is_periodic[d] = 0;                       /* GeometryProperties.cpp:100 */
+        in->max_grid_size[d] = 128;                   /* :98 */
+        in->bc_lo[d] = -1; in->bc_hi[d] = -1;          /* from is_periodic when absent */
+        in->P_BC_flag_lo[d] = 0; in->P_BC_flag_hi[d] = 0;
+        in->FE_lo[d] = UNSET; in->FE_hi[d] = UNSET;
+        in->DE_lo[d] = UNSET; in->DE_hi[d] = UNSET;    /* prob_lo-1 / prob_hi+1, FerroX.cpp:445-456 */
+        in->SC_lo[d] = UNSET; in->SC_hi[d] = UNSET;
+        in->Channel_lo[d] = UNSET; in->Channel_hi[d] = UNSET;
+    }
+    in->Remnant_P[0] = 0.0; in->Remnant_P[1] = 0.0; in->Remnant_P[2] = 0.002;   /* :462-464 */
+    in->TimeIntegratorOrder = IUNSET;
+    in->prob_type = IUNSET;
+    in->is_polarization_scalar = 1;                    /* :333 */
+    in->nsteps = 10;                                   /* :365 */
+    in->plot_int = -1;                                 /* :370 */
+    in->dt = UNSET;
+    in->delta = 1.e-6;                                 /* :376 */
+    in->random_seed = 1;                               /* :409 */
+    in->voltage_sweep = 0;                             /* :388 */
+    in->inc_step = -1;                                 /* :391 */
+    in->num_Vapp_max = 1;                              /* :412 */
+    in->Phi_Bc_lo = 0.; in->Phi_Bc_hi = 0.; in->Phi_Bc_inc = 0.; in->Phi_Bc_hi_max = 0.;
+    in->phi_tolerance = 1.e-7;                         /* :406 */
+    in->include_Landau = 1; in->include_Grad = 1; in->include_Elec = 1;
+    in->lambda = UNSET;
+    in->epsilon_0 = UNSET; in->epsilonX_fe = UNSET; in->epsilonX_fe_tphase = UNSET;
+    in->epsilonZ_fe = UNSET; in->epsilon_de = UNSET; in->epsilon_si = UNSET;
+    in->alpha = UNSET; in->beta = UNSET; in->gamma = UNSET;
+    in->alpha_12 = UNSET; in->alpha_112 = UNSET; in->alpha_123 = UNSET;
+    in->BigGamma = UNSET; in->g11 = UNSET; in->g44 = UNSET; in->g12 = UNSET; in->g44_p = UNSET;
+    in->acceptor_doping = 0.0; in->donor_doping = 0.0;
+    in->use_Fermi_Dirac = 1;                           /* :556 */
+    in->use_work_function = 0;
+    in->metal_work_function = 4.85;
+    in->Coordinate_Transformation = 0;
+    in->use_Euler_angles = 0;
+    in->mg_tol_rel = 1.e-10;                           /* ElectrostaticSolver.cpp:719 */
+    in->mg_max_iter = 200;
+    in->mg_nu1 = 2; in->mg_nu2 = 2;
+    in->mg_warm_start = 0;
+    in->outer_max_iter = 1000;
+    in->phi_ghost_mode = FX_GHOST_AMREX;
+    in->precision = FX_PREC_FP64;
+    in->bias_ramp_steps = 0;
+    in->bias_ramp_start = 0.0; in->bias_ramp_end = 0.0;
+    in->verbose = 0;
+    return FX_OK;
+}
+
+/* Derived defaults and the required-key check (pp.get aborts in the
+ * reference when a required key is missing).  Idempotent. */
+int fx_inputs_finalize(fx_inputs* in, char* err, int errlen) {
+    if (!in) return FX_EINVAL;
+    struct { const char* name; double v; } req[] = {
+        {"dt", in->dt}, {"lambda", in->lambda}, {"epsilon_0", in->epsilon_0}, {"epsilonX_fe", in->epsilonX_fe},
+        {"epsilonZ_fe", in->epsilonZ_fe}, {"epsilon_de", in->epsilon_de}, {"epsilon_si", in->epsilon_si},
+        {"alpha", in->alpha}, {"beta", in->beta}, {"gamma", in->gamma}, {"alpha_12", in->alpha_12},
+        {"alpha_112", in->alpha_112}, {"alpha_123", in->alpha_123}, {"BigGamma", in->BigGamma},
+        {"g11", in->g11}, {"g44", in->g44}, {"g12", in->g12}, {"g44_p", in->g44_p},
+        {"domain.prob_lo", in->prob_lo[0]}, {"domain.prob_hi", in->prob_hi[0]},
+        {"FE_lo", in->FE_lo[0]}, {"FE_hi", in->FE_hi[0]},
+    };
+    for (auto& r : req)
+        if (std::isnan(r.v)) { set_err(err, errlen, std::string("missing required key ") + r.name); return FX_EINVAL; }
+    if (in->n_cell[0] == IUNSET) { set_err(err, errlen, "missing required key domain.n_cell"); return FX_EINVAL; }
+    if (in->TimeIntegratorOrder == IUNSET) { set_err(err, errlen, "missing required key TimeIntegratorOrder"); return FX_EINVAL; }
+    if (in->prob_type == IUNSET) { set_err(err, errlen, "missing required key prob_type"); return FX_EINVAL; }
+    for (int d = 0; d < 3; ++d) {
+        if (std::isnan(in->DE_lo[d])) in->DE_lo[d] = in->prob_lo[d] - 1.0;
+        if (std::isnan(in->SC_lo[d])) in->SC_lo[d] = in->prob_lo[d] - 1.0;
+        if (std::isnan(in->Channel_lo[d])) in->Channel_lo[d] = in->prob_lo[d] - 1.0;
+        if (std::isnan(in->DE_hi[d])) in->DE_hi[d] = in->prob_hi[d] + 1.0;
+        if (std::isnan(in->SC_hi[d])) in->SC_hi[d] = in->prob_hi[d] + 1.0;
+        if (std::isnan(in->Channel_hi[d])) in->Channel_hi[d] = in->prob_hi[d] + 1.0;
+        if (in->bc_lo[d] < 0) in->bc_lo[d] = in->is_periodic[d] ? FX_BC_PERIODIC : FX_BC_DIRICHLET;
+        if (in->bc_hi[d] < 0) in->bc_hi[d] = in->is_periodic[d] ? FX_BC_PERIODIC : FX_BC_DIRICHLET;
+        if ((in->bc_lo[d] == FX_BC_PERIODIC) != (in->is_periodic[d] != 0) ||
+            (in->bc_hi[d] == FX_BC_PERIODIC) != (in->is_periodic[d] != 0)) {
+            set_err(err, errlen, "boundary.lo/hi 'per' tokens must match domain.is_periodic");
+            return FX_EINVAL;
+        }
+        if (in->n_cell[d] < 1) { set_err(err, errlen, "domain.n_cell must be positive"); return FX_EINVAL; }
+        if (!(in->prob_hi[d] > in->prob_lo[d])) { set_err(err, errlen, "domain.prob_hi must exceed domain.prob_lo"); return FX_EINVAL; }
+        if (in->P_BC_flag_lo[d] < 0 || in->P_BC_flag_lo[d] > 4 || in->P_BC_flag_hi[d] < 0 || in->P_BC_flag_hi[d] > 4) {
+            set_err(err, errlen, "P_BC_flag_lo/hi must be in 0..4");
+            return FX_EINVAL;
+        }
+    }
+    if (std::isnan(in->epsilonX_fe_tphase)) in->epsilonX_fe_tphase = in->epsilonX_fe;   /* FerroX.cpp:343-345 */
+    if (in->prob_type < 1 || in->prob_type > 3) { set_err(err, errlen, "Invalid prob_type (1, 2 or 3)"); return FX_EINVAL; }
+    if (in->TimeIntegratorOrder != 1 && in->TimeIntegratorOrder != 2) {
+        set_err(err, errlen, "TimeIntegratorOrder must be 1 (forward Euler) or 2 (Heun)");
+        return FX_EINVAL;
+    }
+    if (in->mg_nu1 < 0 || in->mg_nu2 < 0 || in->mg_max_iter < 1 || in->outer_max_iter < 1) {
+        set_err(err, errlen, "bad multigrid controls");
+        return FX_EINVAL;
+    }
+    return FX_OK;
+}
+
+int fx_inputs_parse_file(const char* path, fx_inputs* in, char* err, int errlen) {
+    if (!path || !in) return FX_EINVAL;
+    std::ifstream f(path);
+    if (!f) { set_err(err, errlen, std::string("cannot open ") + path); return FX_EINVAL; }
+    std::stringstream ss;
+    ss << f.rdbuf();
+    Table t;
+    std::string why;
+    if (!parse_text(ss.str(), t, why) || !apply_table(t, in, why)) { set_err(err, errlen, why); return FX_EINVAL; }
+    return FX_OK;
+}
+
+int fx_inputs_set(fx_inputs* in, const char* kv, char* err, int errlen) {
+    if (!in || !kv) return FX_EINVAL;
+    Table t;
+    std::string why;
+    if (!parse_text(kv, t, why) || !apply_table(t, in, why)) { set_err(err, errlen, why); return FX_EINVAL; }
+    return FX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+/*
+ * Kernel argument blocks and host launchers (fx_kernels.hip).
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include "fx_device.h"
+
+/* Semiconductor constants (FerroX.cpp:534-554) and doping. */
+struct SCParams {
+    double q, kb, T, Nc, Nv, Eg, Chi, phi_ref, Ea, Ed, gA, gD, NA, ND;
+    int fermi_dirac;
+};
+
+/* Rotation: uniform R, or a table of R indexed per cell. */
+struct RotArgs {
+    const int* idx;        /* null: uniform */
+    const double* tab;
+    double Ru[9];
+};
+
+struct RhsArgs {
+    fxd::Geom g;
+    fxd::MView M;
+    fxd::PBC pbc;
+    RotArgs rot;
+    const double* P[3];
+    const double* phi;
+    const double* rho;
+    double* rhs;
+    double* alpha;
+    double delta;
+    int pxy_zero;
+    SCParams sc;
+};
+
+struct RhoArgs {
+    int n[3];
+    fxd::MView M;
+    const double* phi;
+    double* rho;
+    double* e_den;
+    double* h_den;
+    double* phi_prev;
+    double* part;
+    int want_err;
+    SCParams sc;
+};
+
+struct TdglArgs {
+    fxd::Geom g;
+    fxd::MView M;
+    fxd::PBC pbc;
+    fxd::PhiView phv;
+    RotArgs rot;
+    const double* P[3];
+    double* Pout[3];
+    double* GLout[3];
+    double alpha, beta, gamma, a12, a112, a123, BigGamma, g11, g44, g12, g44_p, dt;
+    int incL, incG, incE, scalar;
+};
+
+struct HeunArgs {
+    double* P[3];
+    const double* G0[3];
+    const double* G1[3];
+    double dt;
+    long long N;
+};
+
+struct EArgs {
+    fxd::Geom g;
+    fxd::PhiView phv;
+    RotArgs rot;
+    double* E[3];
+};
+
+namespace fxk {
+void fill(hipStream_t s, double* p, long long n, double v);
+void copy(hipStream_t s, double* d, const double* src, long long n);
+void add(hipStream_t s, double* d, const double* src, long long n);
+void final_reduce(hipStream_t s, const double* part, int nb, int nq, int is_max, double* out);
+void maxabs2(hipStream_t s, const double* a, const double* b, long long n, double* part, int nb);
+void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, double phi_max);
+void rhs_newton(hipStream_t s, const RhsArgs& a);
+void rho_update(hipStream_t s, const RhoArgs& a, int nb);
+void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast);
+void heun(hipStream_t s, const HeunArgs& a);
+void compute_E(hipStream_t s, const EArgs& a);
+void mg_smooth(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int color);
+void mg_resid_restrict(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc, int inh);
+void mg_prolong(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c);
+void mg_restrict_cells(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const double* src, double* dst);
+void mg_resid_norm(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, double* r, double* part, int nb);
+void bottom_cg(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* rhs, double* e, double* work,
+               double tol, int maxit, int* iters);
+}  // namespace fxk

@@ -1,0 +1,589 @@
+/*
+ * HIP kernels (gfx950) of the FerroX per-timestep hot path.
+ *
+ *   k_rhs_newton   ComputePoissonRHS + dF_dPhi + ComputePoissonRHS_Newton
+ *                  (ElectrostaticSolver.cpp:8-132) fused: one pass writes the
+ *                  Newton right-hand side and the A-coefficient.
+ *   k_rho_update   ComputeRho (ChargeDensity.cpp:15-102) fused with the L1
+ *                  norms of the outer-loop error (ElectrostaticSolver.cpp:731-738).
+ *   k_tdgl<...>    Calculate_Landau + Calculate_Grad + Calculate_Elec +
+ *                  CalculateTDGL_RHS (TotalEnergyDensity.cpp) with E = -R grad(phi)
+ *                  formed on the fly (ComputeEfromPhi, ElectrostaticSolver.cpp:134)
+ *                  and the forward-Euler update (main.cpp:293-306) fused, so P and
+ *                  phi cross HBM once per step.
+ *   k_compute_E    ComputeEfromPhi for output.
+ *   k_mg_*         geometric multigrid: red-black Gauss-Seidel, residual fused
+ *                  with restriction, prolongation fused with correction,
+ *                  single-workgroup Jacobi-PCG bottom solve, residual norms.
+ */
+#include "fx_device.h"
+#include "fx_kernels.h"
+
+using namespace fxd;
+
+/* --------------------------------- utils --------------------------------- */
+__global__ void k_fill(double* __restrict__ p, long long n, double v) {
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) p[c] = v;
+}
+__global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, long long n) {
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) d[c] = s[c];
+}
+__global__ void k_add(double* __restrict__ d, const double* __restrict__ s, long long n) {
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) d[c] += s[c];
+}
+
+/* Final reduction of per-block partials: out[q] = reduce_b partial[q*nb + b]. */
+__global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, int nb, int nq, int is_max, double* __restrict__ out) {
+    __shared__ double sh[256 / 64 + 1];
+    for (int q = 0; q < nq; ++q) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) v = is_max ? nmax(v, part[q * nb + b]) : v + part[q * nb + b];
+        v = is_max ? block_max<256>(v, sh) : block_sum<256>(v, sh);
+        if (threadIdx.x == 0) out[q] = v;
+    }
+}
+
+/* max|a| and max|a-b| (b may be null) per block. */
+__global__ __launch_bounds__(256) void k_maxabs2(const double* __restrict__ a, const double* __restrict__ b, long long n,
+                                                 double* __restrict__ part) {
+    __shared__ double sh[256 / 64 + 1];
+    double m1 = 0.0, m2 = 0.0;
+    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+        m1 = nmax(m1, fabs(a[c]));
+        if (b) m2 = nmax(m2, fabs(a[c] - b[c]));
+    }
+    m1 = block_max<256>(m1, sh);
+    m2 = block_max<256>(m2, sh);
+    if (threadIdx.x == 0) { part[blockIdx.x] = m1; part[gridDim.x + blockIdx.x] = m2; }
+}
+
+/* PhiDiff = |phi - phi_old| / phi_max, then phi_old = phi (CheckSteadyState,
+ * ElectrostaticSolver.cpp:531-556). */
+__global__ void k_phidiff(const double* __restrict__ phi, double* __restrict__ phi_old, double* __restrict__ diff,
+                          long long n, double phi_max) {
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
+        diff[c] = fabs(phi[c] - phi_old[c]) / phi_max;
+        phi_old[c] = phi[c];
+    }
+}
+
+/* --------------------------- charge density ------------------------------- */
+__device__ __forceinline__ double fd_half(double eta) {
+    const double nu = pow(eta, 4.0) + 50.0 + 33.6 * eta * (1.0 - 0.68 * exp(-0.17 * pow((eta + 1.0), 2.0)));
+    const double xi = 3.0 * sqrt(3.14) / (4.0 * pow(nu, 3. / 8.));
+    return pow(exp(-eta) + xi, -1.0);
+}
+
+/* n, p and rho at potential phi in a semiconductor cell (ChargeDensity.cpp:46-93). */
+__device__ __forceinline__ double rho_of(const SCParams& s, double phi, double& n, double& p) {
+    const double q = s.q, kb = s.kb, T = s.T, Ef = 0.0, Eg = s.Eg, Chi = s.Chi;
+    const double phi_ref = s.phi_ref;
+    const double Ec_corr = -q * (phi - phi_ref) - Chi * q;
+    const double Ev_corr = Ec_corr - q * Eg;
+    if (s.fermi_dirac == 1) {
+        const double eta_n = -(Ec_corr - q * Ef) / (kb * T);
+        const double eta_p = -(q * Ef - Ev_corr) / (kb * T);
+        n = s.Nc * fd_half(eta_n);
+        p = s.Nv * fd_half(eta_p);
+    } else {
+        n = s.Nc * exp(-(Ec_corr - q * Ef) / (kb * T));
+        p = s.Nv * exp(-(q * Ef - Ev_corr) / (kb * T));
+    }
+    const double NA = s.NA / (1.0 + s.gA * exp((-q * Ef + q * s.Ea + q * phi_ref - q * Chi - q * Eg - q * phi) / (kb * T)));
+    const double ND = s.ND / (1.0 + s.gD * exp((q * Ef + q * s.Ed - q * phi_ref + q * Chi + q * phi) / (kb * T)));
+    return q * (p - n - NA + ND);
+}
+
+__device__ __forceinline__ void rot_of(const RotArgs& r, int c, double R[9]) {
+    if (r.idx) {
+        const double* t = r.tab + 9 * r.idx[c];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) R[q] = t[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) R[q] = r.Ru[q];
+    }
+}
+
+/* Poisson RHS + Newton linearisation, one thread per cell. */
+__global__ __launch_bounds__(256) void k_rhs_newton(RhsArgs a) {
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int c = i + nx * (j + ny * k);
+    const int code = a.M.code(i, j, k);
+    double rhs, al;
+    if (code >= 2) {
+        const double rho = a.rho[c], ph = a.phi[c];
+        double n, p;
+        const double rho_d = rho_of(a.sc, ph + a.delta, n, p);
+        al = (1. / a.delta) * rho_d + (-1. / a.delta) * rho;
+        rhs = rho - al * ph;
+    } else if (code == 1) {
+        rhs = 0.;
+        al = 0.;
+    } else {
+        double R[9];
+        rot_of(a.rot, c, R);
+        const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+        const double* dx = a.g.dx;
+        double x = 0.0;
+        if (a.pxy_zero) {
+            if (R[6] != 0.0) x += R[6] * dpd<0>(Fr, a.M, i, j, k, dx, a.pbc);
+            if (R[7] != 0.0) x += R[7] * dpd<1>(Fr, a.M, i, j, k, dx, a.pbc);
+            x += R[8] * dpd<2>(Fr, a.M, i, j, k, dx, a.pbc);
+            rhs = -x;
+        } else {
+            const PView Fp{a.P[0], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+            const PView Fq{a.P[1], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+            rhs = -(R[0] * dpd<0>(Fp, a.M, i, j, k, dx, a.pbc) + R[1] * dpd<1>(Fp, a.M, i, j, k, dx, a.pbc) + R[2] * dpd<2>(Fp, a.M, i, j, k, dx, a.pbc))
+                  - (R[3] * dpd<0>(Fq, a.M, i, j, k, dx, a.pbc) + R[4] * dpd<1>(Fq, a.M, i, j, k, dx, a.pbc) + R[5] * dpd<2>(Fq, a.M, i, j, k, dx, a.pbc))
+                  - (R[6] * dpd<0>(Fr, a.M, i, j, k, dx, a.pbc) + R[7] * dpd<1>(Fr, a.M, i, j, k, dx, a.pbc) + R[8] * dpd<2>(Fr, a.M, i, j, k, dx, a.pbc));
+        }
+        al = 0.0;
+    }
+    a.rhs[c] = rhs;
+    a.alpha[c] = al;
+}
+
+/* rho(phi) in SC cells (0 elsewhere), optional L1 partial sums of
+ * |phi - phi_prev| and |phi| with phi_prev <- phi. */
+__global__ __launch_bounds__(256) void k_rho_update(RhoArgs a) {
+    __shared__ double sh[256 / 64 + 1];
+    const int nx = a.n[0], ny = a.n[1];
+    const long long N = (long long)nx * ny * a.n[2];
+    double s_num = 0.0, s_den = 0.0;
+    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < N; c += (long long)gridDim.x * 256) {
+        const int i = (int)(c % nx);
+        const int j = (int)((c / nx) % ny);
+        const int k = (int)(c / ((long long)nx * ny));
+        const double ph = a.phi[c];
+        if (a.M.code(i, j, k) >= 2) {
+            double n, p;
+            a.rho[c] = rho_of(a.sc, ph, n, p);
+            a.e_den[c] = n;
+            a.h_den[c] = p;
+        } else {
+            a.rho[c] = 0.0;
+        }
+        if (a.want_err) {
+            s_num += fabs(ph - a.phi_prev[c]);
+            s_den += fabs(ph);
+            a.phi_prev[c] = ph;
+        }
+    }
+    if (a.want_err) {
+        s_num = block_sum<256>(s_num, sh);
+        s_den = block_sum<256>(s_den, sh);
+        if (threadIdx.x == 0) { a.part[blockIdx.x] = s_num; a.part[gridDim.x + blockIdx.x] = s_den; }
+    }
+}
+
+/* ------------------------------ TDGL right-hand side ---------------------- */
+/* dF_grad/dP_r with the rotated tensor, TotalEnergyDensity.cpp:209-231. */
+__device__ __forceinline__ double grad_r(const PView& Fr, const MView& M, int i, int j, int k, const double* dx,
+                                         const PBC& bc, const double R[9], double g11, double g44m) {
+    const double R11 = R[0], R12 = R[1], R13 = R[2], R21 = R[3], R22 = R[4], R23 = R[5], R31 = R[6], R32 = R[7], R33 = R[8];
+    const double Hxx = ddpd<0>(Fr, M, i, j, k, dx, bc);
+    const double Hyy = ddpd<1>(Fr, M, i, j, k, dx, bc);
+    const double Hzz = ddpd<2>(Fr, M, i, j, k, dx, bc);
+    const bool nxy = (R31 * R32 != 0.0) || (R11 * R12 != 0.0) || (R21 * R22 != 0.0);
+    const bool nyz = (R32 * R33 != 0.0) || (R12 * R13 != 0.0) || (R22 * R23 != 0.0);
+    const bool nxz = (R33 * R31 != 0.0) || (R13 * R11 != 0.0) || (R23 * R21 != 0.0);
+    const double Hxy = nxy ? ddpd_mixed<0, 1>(Fr, M, i, j, k, dx, bc) : 0.0;
+    const double Hyz = nyz ? ddpd_mixed<1, 2>(Fr, M, i, j, k, dx, bc) : 0.0;
+    const double Hxz = nxz ? ddpd_mixed<0, 2>(Fr, M, i, j, k, dx, bc) : 0.0;
+    return -g11 * (R31 * R31 * Hxx + R32 * R32 * Hyy + R33 * R33 * Hzz + 2. * R31 * R32 * Hxy + 2. * R32 * R33 * Hyz + 2. * R33 * R31 * Hxz)
+           - g44m * (R11 * R11 * Hxx + R12 * R12 * Hyy + R13 * R13 * Hzz + 2. * R11 * R12 * Hxy + 2. * R12 * R13 * Hyz + 2. * R13 * R11 * Hxz)
+           - g44m * (R21 * R21 * Hxx + R22 * R22 * Hyy + R23 * R23 * Hzz + 2. * R21 * R22 * Hxy + 2. * R22 * R23 * Hyz + 2. * R23 * R21 * Hxz);
+}
+
+/* SCALAR_FAST: is_polarization_scalar with P_p = P_q = 0 everywhere (only P_r
+ * is read or written).  Non-FE cells have Gamma = 0, so P never changes there;
+ * both P buffers hold the same values in those cells and they are skipped. */
+template <bool SCALAR_FAST>
+__global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int c = i + nx * (j + ny * k);
+    const int raw = a.M.raw(i, j, k);
+    if ((raw & 3) != 0) {
+        if (a.GLout[2]) {
+            if (!SCALAR_FAST) { a.GLout[0][c] = 0.0; a.GLout[1][c] = 0.0; }
+            a.GLout[2][c] = 0.0;
+        }
+        return;
+    }
+    const bool tph = (raw & 4) != 0;
+    const double Gam = a.BigGamma;
+    const double* dx = a.g.dx;
+    double R[9];
+    rot_of(a.rot, c, R);
+    const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+    double GL[3] = {0.0, 0.0, 0.0};
+
+    /* electric field E = -R grad(phi) */
+    double E[3] = {0.0, 0.0, 0.0};
+    if (a.incE == 1) {
+        double fx, fy, fz;
+        grad_phi(a.phv, a.g, i, j, k, fx, fy, fz);
+        if (SCALAR_FAST) {
+            double s = 0.0;
+            if (R[6] != 0.0) s += R[6] * fx;
+            if (R[7] != 0.0) s += R[7] * fy;
+            s += R[8] * fz;
+            E[2] = -s;
+        } else {
+            E[0] = -(R[0] * fx + R[1] * fy + R[2] * fz);
+            E[1] = -(R[3] * fx + R[4] * fy + R[5] * fz);
+            E[2] = -(R[6] * fx + R[7] * fy + R[8] * fz);
+        }
+    }
+
+    if (SCALAR_FAST) {
+        const double r = a.P[2][c];
+        double L = 0.0, G = 0.0, El = 0.0;
+        if (a.incL == 1) {
+            const double r2 = r * r;
+            const double dF = a.alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
+            L = -1.0 * Gam * dF;
+        }
+        if (a.incG == 1) G = -1.0 * Gam * grad_r(Fr, a.M, i, j, k, dx, a.pbc, R, a.g11, a.g44 - a.g44_p);
+        if (a.incE == 1) El = -1.0 * Gam * (-E[2]);
+        if (tph) { L = 0.0; G = 0.0; El = 0.0; }
+        double g = 0.0;
+        if (a.incL == 1) g += L;
+        if (a.incG == 1) g += G;
+        if (a.incE == 1) g += El;
+        GL[2] = g;
+        if (a.GLout[2]) a.GLout[2][c] = g;
+        if (a.Pout[2]) a.Pout[2][c] = r + a.dt * g;
+        return;
+    }
+
+    const PView Fp{a.P[0], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+    const PView Fq{a.P[1], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+    const double p = a.P[0][c], q = a.P[1][c], r = a.P[2][c];
+    double L[3] = {0, 0, 0}, G[3] = {0, 0, 0}, El[3] = {0, 0, 0};
+    if (a.incL == 1) {
+        const double p2 = p * p, q2 = q * q, r2 = r * r;
+        const double a12 = a.a12, a112 = a.a112, a123 = a.a123;
+        const double dFp = a.alpha * p + a.beta * (p2 * p) + a.gamma * (p2 * p2 * p)
+                         + 2. * a12 * p * q2 + 2. * a12 * p * r2
+                         + 4. * a112 * (p2 * p) * (q2 + r2)
+                         + 2. * a112 * p * (q2 * q2) + 2. * a112 * p * (r2 * r2)
+                         + 2. * a123 * p * q2 * r2;
+        const double dFq = a.alpha * q + a.beta * (q2 * q) + a.gamma * (q2 * q2 * q)
+                         + 2. * a12 * q * p2 + 2. * a12 * q * r2
+                         + 4. * a112 * (q2 * q) * (p2 + r2)
+                         + 2. * a112 * q * (p2 * p2) + 2. * a112 * q * (r2 * r2)
+                         + 2. * a123 * q * p2 * r2;
+        const double dFr = a.alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r)
+                         + 2. * a12 * r * p2 + 2. * a12 * r * q2
+                         + 4. * a112 * (r2 * r) * (p2 + q2)
+                         + 2. * a112 * r * (p2 * p2) + 2. * a112 * r * (q2 * q2)
+                         + 2. * a123 * r * p2 * q2;
+        L[0] = -1.0 * Gam * dFp; L[1] = -1.0 * Gam * dFq; L[2] = -1.0 * Gam * dFr;
+        if (a.scalar) { L[0] = 0.0; L[1] = 0.0; }
+        if (tph) { L[0] = L[1] = L[2] = 0.0; }
+    }
+    if (a.incG == 1) {
+        const double g11 = a.g11, g44 = a.g44, g12 = a.g12, g44_p = a.g44_p;
+        double dFp = 0.0, dFq = 0.0;
+        if (!a.scalar) {
+            dFp = -g11 * ddpd<0>(Fp, a.M, i, j, k, dx, a.pbc)
+                  - (g44 + g44_p) * ddpd<1>(Fp, a.M, i, j, k, dx, a.pbc)
+                  - (g44 + g44_p) * ddpd<2>(Fp, a.M, i, j, k, dx, a.pbc)
+                  - (g12 + g44 - g44_p) * ddpd_mixed<0, 1>(Fq, a.M, i, j, k, dx, a.pbc)
+                  - (g12 + g44 - g44_p) * ddpd_mixed<0, 2>(Fr, a.M, i, j, k, dx, a.pbc);
+            dFq = -g11 * ddpd<1>(Fq, a.M, i, j, k, dx, a.pbc)
+                  - (g44 - g44_p) * ddpd<0>(Fq, a.M, i, j, k, dx, a.pbc)
+                  - (g44 - g44_p) * ddpd<2>(Fq, a.M, i, j, k, dx, a.pbc)
+                  - (g12 + g44 + g44_p) * ddpd_mixed<0, 1>(Fp, a.M, i, j, k, dx, a.pbc)
+                  - (g12 + g44 - g44_p) * ddpd_mixed<1, 2>(Fr, a.M, i, j, k, dx, a.pbc);
+        }
+        double dFr = grad_r(Fr, a.M, i, j, k, dx, a.pbc, R, g11, g44 - g44_p);
+        dFr = dFr - (g44 + g44_p + g12) * ddpd_mixed<1, 2>(Fq, a.M, i, j, k, dx, a.pbc)
+                  - (g44 + g44_p + g12) * ddpd_mixed<0, 2>(Fp, a.M, i, j, k, dx, a.pbc);
+        G[0] = -1.0 * Gam * dFp; G[1] = -1.0 * Gam * dFq; G[2] = -1.0 * Gam * dFr;
+        if (a.scalar) { G[0] = 0.0; G[1] = 0.0; }
+        if (tph) { G[0] = G[1] = G[2] = 0.0; }
+    }
+    if (a.incE == 1) {
+        El[0] = -1.0 * Gam * (-E[0]); El[1] = -1.0 * Gam * (-E[1]); El[2] = -1.0 * Gam * (-E[2]);
+        if (a.scalar) { El[0] = 0.0; El[1] = 0.0; }
+        if (tph) { El[0] = El[1] = El[2] = 0.0; }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double g = 0.0;
+        if (a.incL == 1) g += L[d];
+        if (a.incG == 1) g += G[d];
+        if (a.incE == 1) g += El[d];
+        GL[d] = g;
+        if (a.GLout[d]) a.GLout[d][c] = g;
+    }
+    if (a.Pout[0]) {
+        a.Pout[0][c] = p + a.dt * GL[0];
+        a.Pout[1][c] = q + a.dt * GL[1];
+        a.Pout[2][c] = r + a.dt * GL[2];
+    }
+}
+
+/* Heun combination, main.cpp:336-341: P <- P + dt*(0.5*f0 + 0.5*f1) in FE cells. */
+__global__ void k_heun(HeunArgs a) {
+    for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < a.N; c += (long long)gridDim.x * blockDim.x) {
+        for (int d = 0; d < 3; ++d) {
+            if (!a.P[d]) continue;
+            const double avg = 0.5 * a.G0[d][c] + 0.5 * a.G1[d][c];
+            a.P[d][c] = a.P[d][c] + a.dt * avg;
+        }
+    }
+}
+
+/* ComputeEfromPhi for all cells (output field). */
+__global__ __launch_bounds__(256) void k_compute_E(EArgs a) {
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny || k >= nz) return;
+    const int c = i + nx * (j + ny * k);
+    double R[9];
+    rot_of(a.rot, c, R);
+    double fx, fy, fz;
+    grad_phi(a.phv, a.g, i, j, k, fx, fy, fz);
+    a.E[0][c] = -(R[0] * fx + R[1] * fy + R[2] * fz);
+    a.E[1][c] = -(R[3] * fx + R[4] * fy + R[5] * fz);
+    a.E[2][c] = -(R[6] * fx + R[7] * fy + R[8] * fz);
+}
+
+/* ------------------------------- multigrid -------------------------------- */
+/* One colour of red-black Gauss-Seidel; thread per same-colour cell. */
+__global__ __launch_bounds__(256) void k_mg_smooth(DLev l, DBC bc, int inh, int color) {
+    const int half = (l.n0 + 1) >> 1;
+    const long long tot = (long long)half * l.n1 * l.n2;
+    const long long t = blockIdx.x * 256LL + threadIdx.x;
+    if (t >= tot) return;
+    const int ih = (int)(t % half);
+    const long long jk = t / half;
+    const int j = (int)(jk % l.n1);
+    const int k = (int)(jk / l.n1);
+    const int i = 2 * ih + ((j + k + color) & 1);
+    if (i >= l.n0) return;
+    double D, S;
+    mg_row(l, l.phi, i, j, k, bc, inh != 0, D, S);
+    const int c = i + l.n0 * (j + l.n1 * k);
+    l.phi[c] = (l.b[c] + S) / D;
+}
+
+/* Residual of the fine level averaged into the coarse right-hand side; the
+ * coarse solution is zeroed.  Thread per coarse cell. */
+__global__ __launch_bounds__(256) void k_mg_resid_restrict(DLev f, DLev cl, DBC bc, int inh) {
+    const long long tot = (long long)cl.n0 * cl.n1 * cl.n2;
+    const long long t = blockIdx.x * 256LL + threadIdx.x;
+    if (t >= tot) return;
+    const int I = (int)(t % cl.n0);
+    const int J = (int)((t / cl.n0) % cl.n1);
+    const int K = (int)(t / ((long long)cl.n0 * cl.n1));
+    const int cx = f.crs[0], cy = f.crs[1], cz = f.crs[2];
+    double s = 0.0;
+    for (int kk = 0; kk <= cz; ++kk)
+        for (int jj = 0; jj <= cy; ++jj)
+            for (int ii = 0; ii <= cx; ++ii) {
+                const int i = cx ? 2 * I + ii : I, j = cy ? 2 * J + jj : J, k = cz ? 2 * K + kk : K;
+                double D, S;
+                mg_row(f, f.phi, i, j, k, bc, inh != 0, D, S);
+                const int c = i + f.n0 * (j + f.n1 * k);
+                s += f.b[c] - (D * f.phi[c] - S);
+            }
+    const int cnt = (1 + cx) * (1 + cy) * (1 + cz);
+    cl.b[t] = s / cnt;
+    cl.phi[t] = 0.0;
+}
+
+/* Piecewise-constant prolongation added to the fine solution. */
+__global__ __launch_bounds__(256) void k_mg_prolong(DLev f, DLev cl) {
+    const long long tot = (long long)f.n0 * f.n1 * f.n2;
+    const long long t = blockIdx.x * 256LL + threadIdx.x;
+    if (t >= tot) return;
+    const int i = (int)(t % f.n0);
+    const int j = (int)((t / f.n0) % f.n1);
+    const int k = (int)(t / ((long long)f.n0 * f.n1));
+    const int I = f.crs[0] ? i >> 1 : i, J = f.crs[1] ? j >> 1 : j, K = f.crs[2] ? k >> 1 : k;
+    f.phi[t] += cl.phi[I + cl.n0 * (J + cl.n1 * K)];
+}
+
+/* Cell average of a fine array into the coarse level (A-coefficient). */
+__global__ __launch_bounds__(256) void k_mg_restrict_cells(DLev f, DLev cl, const double* __restrict__ src, double* __restrict__ dst) {
+    const long long tot = (long long)cl.n0 * cl.n1 * cl.n2;
+    const long long t = blockIdx.x * 256LL + threadIdx.x;
+    if (t >= tot) return;
+    const int I = (int)(t % cl.n0);
+    const int J = (int)((t / cl.n0) % cl.n1);
+    const int K = (int)(t / ((long long)cl.n0 * cl.n1));
+    const int cx = f.crs[0], cy = f.crs[1], cz = f.crs[2];
+    double s = 0.0;
+    for (int kk = 0; kk <= cz; ++kk)
+        for (int jj = 0; jj <= cy; ++jj)
+            for (int ii = 0; ii <= cx; ++ii)
+                s += src[(cx ? 2 * I + ii : I) + f.n0 * ((cy ? 2 * J + jj : J) + f.n1 * (cz ? 2 * K + kk : K))];
+    dst[t] = s / ((1 + cx) * (1 + cy) * (1 + cz));
+}
+
+/* Residual r = b - A phi into r (may be null) with per-block max|r| and max|b|. */
+__global__ __launch_bounds__(256) void k_mg_resid_norm(DLev l, DBC bc, int inh, double* __restrict__ r, double* __restrict__ part) {
+    __shared__ double sh[256 / 64 + 1];
+    const long long tot = (long long)l.n0 * l.n1 * l.n2;
+    double mr = 0.0, mb = 0.0;
+    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < tot; t += (long long)gridDim.x * 256) {
+        const int i = (int)(t % l.n0);
+        const int j = (int)((t / l.n0) % l.n1);
+        const int k = (int)(t / ((long long)l.n0 * l.n1));
+        double D, S;
+        mg_row(l, l.phi, i, j, k, bc, inh != 0, D, S);
+        const double rr = l.b[t] - (D * l.phi[t] - S);
+        if (r) r[t] = rr;
+        mr = nmax(mr, fabs(rr));
+        mb = nmax(mb, fabs(l.b[t]));
+    }
+    mr = block_max<256>(mr, sh);
+    mb = block_max<256>(mb, sh);
+    if (threadIdx.x == 0) { part[blockIdx.x] = mr; part[gridDim.x + blockIdx.x] = mb; }
+}
+
+/* Jacobi-preconditioned CG for A e = rhs (homogeneous BCs) in one workgroup.
+ * work holds 5*N doubles (r, z, p, q, diag). */
+__global__ __launch_bounds__(1024) void k_bottom_cg(DLev l, DBC bc, const double* __restrict__ rhs, double* __restrict__ e,
+                                                    double* __restrict__ work, double tol, int maxit, int* __restrict__ iters) {
+    constexpr int BS = 1024;
+    __shared__ double sh[BS / 64 + 1];
+    const int N = l.n0 * l.n1 * l.n2;
+    double* r = work;
+    double* z = work + N;
+    double* p = work + 2 * N;
+    double* q = work + 3 * N;
+    double* Dg = work + 4 * N;
+    const int tid = threadIdx.x;
+    double rr = 0.0;
+    for (int c = tid; c < N; c += BS) {
+        const int i = c % l.n0, j = (c / l.n0) % l.n1, k = c / (l.n0 * l.n1);
+        double D, S;
+        mg_row(l, rhs, i, j, k, bc, false, D, S);
+        Dg[c] = D;
+        r[c] = rhs[c];
+        e[c] = 0.0;
+        rr += rhs[c] * rhs[c];
+    }
+    const double rr0 = block_sum<BS>(rr, sh);
+    if (rr0 == 0.0) { if (tid == 0) *iters += 0; return; }
+    double rz = 0.0;
+    for (int c = tid; c < N; c += BS) { z[c] = r[c] / Dg[c]; p[c] = z[c]; rz += r[c] * z[c]; }
+    rz = block_sum<BS>(rz, sh);
+    const double tol2 = tol * tol * rr0;
+    int it = 0;
+    for (; it < maxit; ++it) {
+        double pq = 0.0;
+        for (int c = tid; c < N; c += BS) {
+            const int i = c % l.n0, j = (c / l.n0) % l.n1, k = c / (l.n0 * l.n1);
+            double D, S;
+            mg_row(l, p, i, j, k, bc, false, D, S);
+            const double qq = D * p[c] - S;
+            q[c] = qq;
+            pq += p[c] * qq;
+        }
+        pq = block_sum<BS>(pq, sh);
+        const double al = rz / pq;
+        rr = 0.0;
+        for (int c = tid; c < N; c += BS) {
+            e[c] += al * p[c];
+            r[c] -= al * q[c];
+            rr += r[c] * r[c];
+        }
+        rr = block_sum<BS>(rr, sh);
+        if (rr <= tol2) { ++it; break; }
+        double rzn = 0.0;
+        for (int c = tid; c < N; c += BS) { z[c] = r[c] / Dg[c]; rzn += r[c] * z[c]; }
+        rzn = block_sum<BS>(rzn, sh);
+        const double be = rzn / rz;
+        rz = rzn;
+        for (int c = tid; c < N; c += BS) p[c] = z[c] + be * p[c];
+        __syncthreads();
+    }
+    if (tid == 0) *iters += it;
+}
+
+/* ------------------------------ launchers -------------------------------- */
+namespace fxk {
+
+static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
+static inline dim3 grid_cells(const int n[3]) { return dim3((n[0] + 63) / 64, (n[1] + 3) / 4, n[2]); }
+
+void fill(hipStream_t s, double* p, long long n, double v) {
+    unsigned b = nblk(n, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
+    hipLaunchKernelGGL(k_fill, dim3(b), dim3(256), 0, s, p, n, v);
+}
+void copy(hipStream_t s, double* d, const double* src, long long n) {
+    unsigned b = nblk(n, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
+    hipLaunchKernelGGL(k_copy, dim3(b), dim3(256), 0, s, d, src, n);
+}
+void add(hipStream_t s, double* d, const double* src, long long n) {
+    unsigned b = nblk(n, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
+    hipLaunchKernelGGL(k_add, dim3(b), dim3(256), 0, s, d, src, n);
+}
+void final_reduce(hipStream_t s, const double* part, int nb, int nq, int is_max, double* out) {
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, part, nb, nq, is_max, out);
+}
+void maxabs2(hipStream_t s, const double* a, const double* b, long long n, double* part, int nb) {
+    hipLaunchKernelGGL(k_maxabs2, dim3(nb), dim3(256), 0, s, a, b, n, part);
+}
+void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, double phi_max) {
+    unsigned b = nblk(n, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
+    hipLaunchKernelGGL(k_phidiff, dim3(b), dim3(256), 0, s, phi, phi_old, diff, n, phi_max);
+}
+void rhs_newton(hipStream_t s, const RhsArgs& a) {
+    hipLaunchKernelGGL(k_rhs_newton, grid_cells(a.g.n), dim3(256), 0, s, a);
+}
+void rho_update(hipStream_t s, const RhoArgs& a, int nb) {
+    hipLaunchKernelGGL(k_rho_update, dim3(nb), dim3(256), 0, s, a);
+}
+void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast) {
+    if (scalar_fast) hipLaunchKernelGGL(k_tdgl<true>, grid_cells(a.g.n), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_tdgl<false>, grid_cells(a.g.n), dim3(256), 0, s, a);
+}
+void heun(hipStream_t s, const HeunArgs& a) {
+    unsigned b = nblk(a.N, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
+    hipLaunchKernelGGL(k_heun, dim3(b), dim3(256), 0, s, a);
+}
+void compute_E(hipStream_t s, const EArgs& a) {
+    hipLaunchKernelGGL(k_compute_E, grid_cells(a.g.n), dim3(256), 0, s, a);
+}
+void mg_smooth(hipStream_t s, const DLev& l, const DBC& bc, int inh, int color) {
+    const long long tot = (long long)((l.n0 + 1) >> 1) * l.n1 * l.n2;
+    hipLaunchKernelGGL(k_mg_smooth, dim3(nblk(tot, 256)), dim3(256), 0, s, l, bc, inh, color);
+}
+void mg_resid_restrict(hipStream_t s, const DLev& f, const DLev& c, const DBC& bc, int inh) {
+    const long long tot = (long long)c.n0 * c.n1 * c.n2;
+    hipLaunchKernelGGL(k_mg_resid_restrict, dim3(nblk(tot, 256)), dim3(256), 0, s, f, c, bc, inh);
+}
+void mg_prolong(hipStream_t s, const DLev& f, const DLev& c) {
+    const long long tot = (long long)f.n0 * f.n1 * f.n2;
+    hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(tot, 256)), dim3(256), 0, s, f, c);
+}
+void mg_restrict_cells(hipStream_t s, const DLev& f, const DLev& c, const double* src, double* dst) {
+    const long long tot = (long long)c.n0 * c.n1 * c.n2;
+    hipLaunchKernelGGL(k_mg_restrict_cells, dim3(nblk(tot, 256)), dim3(256), 0, s, f, c, src, dst);
+}
+void mg_resid_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, double* r, double* part, int nb) {
+    hipLaunchKernelGGL(k_mg_resid_norm, dim3(nb), dim3(256), 0, s, l, bc, inh, r, part);
+}
+void bottom_cg(hipStream_t s, const DLev& l, const DBC& bc, const double* rhs, double* e, double* work, double tol, int maxit, int* iters) {
+    hipLaunchKernelGGL(k_bottom_cg, dim3(1), dim3(1024), 0, s, l, bc, rhs, e, work, tol, maxit, iters);
+}
+
+}  // namespace fxk

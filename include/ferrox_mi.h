@@ -1,0 +1,266 @@
+/*
+ * ferrox_mi.h -- C ABI of the MI355X-native FerroX TDGL + Poisson hot path.
+ *
+ * The reference has no plugin or FFI layer: its hot path is a set of free C++
+ * functions over amrex::MultiFab, called only from main_main's time loop
+ * (Source/main.cpp:261-623).  This header is the drop-in boundary that replaces
+ * them.  Each entry point names the reference function(s) it stands for.
+ *
+ * Conventions
+ *   - One context = one device (one rectangular box of cells), one HIP stream.
+ *     Calls on one context are not thread-safe; different contexts are
+ *     independent.
+ *   - Host buffers are dense over the valid region, i fastest, then j, then k
+ *     (the order of an AMReX plotfile FAB of a single box), FP64.
+ *   - Every call returns an fx_status; fx_last_error() gives the message.
+ *   - Results are deterministic: reductions run in a fixed order.
+ */
+#ifndef FERROX_MI_H
+#define FERROX_MI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the reference aborts or loops forever in these cases). */
+typedef enum fx_status {
+    FX_OK = 0,
+    FX_EINVAL = 1,          /* bad key, flag or argument (amrex::Abort in the reference, e.g. DerivativeAlgorithm.H:72) */
+    FX_ENOCONV_MG = 2,      /* multigrid did not reach its tolerance in mg_max_iter V-cycles (MLMG abort) */
+    FX_ENOCONV_OUTER = 3,   /* semiconductor outer loop exceeded outer_max_iter (reference loops forever, ElectrostaticSolver.cpp:703,742) */
+    FX_ENAN = 4,            /* non-finite value in a reduction */
+    FX_EHIP = 5,            /* HIP runtime error */
+    FX_ENOMEM = 6
+} fx_status;
+
+/* Boundary-condition kinds for the Poisson problem (boundary.lo / boundary.hi tokens,
+ * BoundaryConditions.cpp:52-214): per, dir(...), neu. */
+enum { FX_BC_PERIODIC = 0, FX_BC_DIRICHLET = 1, FX_BC_NEUMANN = 2 };
+
+/* Where the ghost value of phi at a Dirichlet face sits when E is formed
+ * (ElectrostaticSolver.cpp:134-203 reads the ghost after MLMG::solve). */
+enum {
+    FX_GHOST_AMREX = 0,   /* ghost = 2*phi_b - phi_0, the linear extrapolation the solver leaves behind */
+    FX_GHOST_FACE = 1     /* ghost = phi_b, the face value */
+};
+
+/* Arithmetic of the Poisson solve. */
+enum { FX_PREC_FP64 = 0, FX_PREC_FP32_REFINE = 1 };
+
+/*
+ * Parsed inputs.  Field names are the reference's inputs keys
+ * (FerroX.cpp:273-572, GeometryProperties.cpp:107-124); the trailing block
+ * holds the solver controls this build adds.  Fill with fx_inputs_default()
+ * then fx_inputs_parse_file() / fx_inputs_set().
+ */
+typedef struct fx_inputs {
+    /* domain.* */
+    int    n_cell[3];
+    double prob_lo[3];
+    double prob_hi[3];
+    int    is_periodic[3];
+    int    max_grid_size[3];      /* accepted, not used: one box per context */
+    /* boundary.lo / boundary.hi */
+    int    bc_lo[3];
+    int    bc_hi[3];
+    /* control */
+    int    TimeIntegratorOrder;
+    int    prob_type;
+    int    is_polarization_scalar;
+    int    nsteps;
+    int    plot_int;
+    double dt;
+    double delta;
+    int    random_seed;
+    int    voltage_sweep;
+    int    inc_step;
+    int    num_Vapp_max;
+    double Phi_Bc_lo;
+    double Phi_Bc_hi;
+    double Phi_Bc_inc;
+    double Phi_Bc_hi_max;
+    double phi_tolerance;
+    int    include_Landau;
+    int    include_Grad;
+    int    include_Elec;
+    int    P_BC_flag_lo[3];
+    int    P_BC_flag_hi[3];
+    double lambda;
+    /* materials */
+    double epsilon_0;
+    double epsilonX_fe;
+    double epsilonX_fe_tphase;
+    double epsilonZ_fe;           /* read but unused by the reference (FerroX.cpp:347) */
+    double epsilon_de;
+    double epsilon_si;
+    double alpha;
+    double beta;
+    double gamma;
+    double alpha_12;
+    double alpha_112;
+    double alpha_123;
+    double BigGamma;
+    double g11;
+    double g44;
+    double g12;
+    double g44_p;
+    /* stack geometry */
+    double FE_lo[3], FE_hi[3];
+    double DE_lo[3], DE_hi[3];
+    double SC_lo[3], SC_hi[3];
+    double Channel_lo[3], Channel_hi[3];
+    double Remnant_P[3];
+    /* semiconductor */
+    double acceptor_doping;
+    double donor_doping;
+    int    use_Fermi_Dirac;
+    int    use_work_function;
+    double metal_work_function;
+    /* rotated frame */
+    int    Coordinate_Transformation;
+    int    use_Euler_angles;
+    /* ---- solver controls added by this build ---- */
+    double mg_tol_rel;            /* 1e-10 = the reference's MLMG::solve(…, 1e-10, -1) */
+    int    mg_max_iter;           /* 200 */
+    int    mg_nu1;                /* pre-smoothing RB-GS sweeps (2) */
+    int    mg_nu2;                /* post-smoothing RB-GS sweeps (2) */
+    int    mg_warm_start;         /* 0: zero initial guess as the reference (ElectrostaticSolver.cpp:716) */
+    int    outer_max_iter;        /* cap on the semiconductor outer loop */
+    int    phi_ghost_mode;        /* FX_GHOST_AMREX (default) or FX_GHOST_FACE */
+    int    precision;             /* FX_PREC_FP64 (default) or FX_PREC_FP32_REFINE */
+    int    bias_ramp_steps;       /* >0: Phi_Bc_hi(step) = Phi_Bc_hi_start + step/bias_ramp_steps*(Phi_Bc_hi_end-start) */
+    double bias_ramp_start;
+    double bias_ramp_end;
+    int    verbose;
+} fx_inputs;
+
+/* Field identifiers.  0..17 follow the plotfile variable order of the
+ * reference (Plotfile.cpp:30-106). */
+typedef enum fx_field {
+    FX_FIELD_PX = 0, FX_FIELD_PY, FX_FIELD_PZ,
+    FX_FIELD_PHI, FX_FIELD_POISSON_RHS,
+    FX_FIELD_EX, FX_FIELD_EY, FX_FIELD_EZ,
+    FX_FIELD_HOLES, FX_FIELD_ELECTRONS, FX_FIELD_CHARGE,
+    FX_FIELD_EPSILON, FX_FIELD_MASK, FX_FIELD_TPHASE,
+    FX_FIELD_ANGLE_ALPHA, FX_FIELD_ANGLE_BETA, FX_FIELD_ANGLE_THETA,
+    FX_FIELD_PHIDIFF,
+    FX_FIELD_ALPHA_CC,            /* Newton A-coefficient of the last solve */
+    FX_FIELD_GAMMA,               /* BigGamma in FE, 0 elsewhere */
+    FX_FIELD_GL_RHS_X, FX_FIELD_GL_RHS_Y, FX_FIELD_GL_RHS_Z,   /* TDGL right-hand side, last fx_tdgl_rhs */
+    FX_FIELD_COUNT
+} fx_field;
+
+typedef struct fx_solve_stats {
+    int    outer_iters;           /* iterations of the semiconductor outer loop (1 without SC) */
+    int    vcycles;               /* V-cycles summed over the outer loop */
+    double outer_err;             /* last ||phi_k - phi_{k-1}||_1 / ||phi_k||_1 (0 without SC) */
+    double mg_resid0;             /* ||r0||_inf of the last solve */
+    double mg_resid;              /* ||r||_inf at exit of the last solve */
+    double mg_bnorm;              /* ||b||_inf of the last solve */
+} fx_solve_stats;
+
+typedef struct fx_step_stats {
+    int    steps_done;
+    int    stopped_steady;        /* 1 if the steady-state test ended the loop (main.cpp:606-609) */
+    int    total_vcycles;
+    int    total_outer_iters;
+    int    total_solves;
+    double last_phi_change;       /* CheckSteadyState metric of the last step (ElectrostaticSolver.cpp:546) */
+    double time;                  /* simulated time after the loop */
+    double Phi_Bc_hi;             /* current top bias */
+    int    num_Vapp;
+    double wall_seconds;          /* host wall time of the call (device synchronized) */
+} fx_step_stats;
+
+typedef struct fx_ctx fx_ctx;
+
+/* Library / inputs -------------------------------------------------------- */
+const char* fx_version(void);
+/* Fills every key with the reference default (FerroX.cpp:273-572; Appendix of keys). */
+int fx_inputs_default(fx_inputs* in);
+/* ParmParse-compatible reader for a FerroX inputs file: key = v1 v2 ..., '#'
+ * comments, last-wins duplicates, "key=value" overrides in argv style
+ * (replaces amrex::ParmParse + InitializeFerroXNamespace, FerroX.cpp:273). */
+int fx_inputs_parse_file(const char* path, fx_inputs* in, char* err, int errlen);
+int fx_inputs_set(fx_inputs* in, const char* key_equals_value, char* err, int errlen);
+/* Required-key check (pp.get aborts when a key is missing) and the defaults
+ * that depend on other keys (DE/SC/Channel boxes outside the domain,
+ * epsilonX_fe_tphase = epsilonX_fe, BCs from domain.is_periodic).  fx_create
+ * calls it on its own copy; idempotent. */
+int fx_inputs_finalize(fx_inputs* in, char* err, int errlen);
+
+/* Context ----------------------------------------------------------------- */
+/* Builds the material mask, permittivity, multigrid hierarchy and initial P
+ * and rho on the device.  Replaces main_main setup, main.cpp:65-226:
+ * InitializeMaterialMask (Initialization.cpp:169), InitializePermittivity
+ * (ElectrostaticSolver.cpp:205), AverageCellCenteredMultiFabToCellFaces
+ * (eXstaticUtil.cpp:105), SetupMLMG (ElectrostaticSolver.cpp:562),
+ * InitializePandRho (Initialization.cpp:6). */
+int fx_create(const fx_inputs* in, fx_ctx** out);
+int fx_destroy(fx_ctx* ctx);
+const char* fx_last_error(fx_ctx* ctx);
+int fx_get_dims(fx_ctx* ctx, int dims[3]);
+
+/* Host <-> device copies of one field (dense valid region, i fastest). */
+int fx_get_field(fx_ctx* ctx, int field, double* host);
+/* Settable: PX, PY, PZ, PHI, CHARGE, TPHASE, ANGLE_ALPHA/BETA/THETA, MASK.
+ * Setting MASK/TPHASE/angles rebuilds permittivity and the multigrid
+ * coefficients (Initialize_tphase_Mask / Initialize_Euler_angles,
+ * Initialization.cpp:253-383). */
+int fx_set_field(fx_ctx* ctx, int field, const double* host);
+
+/* Top/bottom Dirichlet values of phi.  Replaces SetPhiBC_z + setLevelBC
+ * (ElectrostaticSolver.cpp:502-524, main.cpp:585-592). */
+int fx_set_bias(fx_ctx* ctx, double phi_bc_lo, double phi_bc_hi);
+
+/* Self-consistent phi and rho for the current P: the outer loop of
+ * ComputePhi_Rho (ElectrostaticSolver.cpp:678-747) with ComputePoissonRHS
+ * (:8), dF_dPhi (:85), ComputePoissonRHS_Newton (:114), MLMG solve
+ * (:713-720) and ComputeRho (ChargeDensity.cpp:15). */
+int fx_compute_phi_rho(fx_ctx* ctx, fx_solve_stats* stats);
+
+/* E = -R grad(phi) into the E fields.  ComputeEfromPhi,
+ * ElectrostaticSolver.cpp:134-203. */
+int fx_compute_E(fx_ctx* ctx);
+
+/* GL right-hand side of the current P and phi into the GL_RHS fields.
+ * Calculate_Landau + Calculate_Grad + Calculate_Elec + CalculateTDGL_RHS,
+ * TotalEnergyDensity.cpp:6-294. */
+int fx_tdgl_rhs(fx_ctx* ctx);
+
+/* One forward-Euler (order 1) or Heun (order 2) update of P using the current
+ * phi for the first stage: main.cpp:293-348. */
+int fx_update(fx_ctx* ctx, int order);
+
+/* nsteps iterations of the reference time loop, main.cpp:261-623: solve,
+ * update, CheckSteadyState (ElectrostaticSolver.cpp:526), voltage sweep and
+ * the stop tests.  nsteps <= 0 means the inputs' nsteps. */
+int fx_step(fx_ctx* ctx, int nsteps, fx_step_stats* stats);
+
+/* Multigrid solve of  -alpha_cc*phi - div(eps grad phi) = rhs  on the
+ * context's geometry and BCs, from host arrays (test and tooling entry point;
+ * replaces MLMG::solve, ElectrostaticSolver.cpp:719).  acoef may be NULL (0). */
+int fx_poisson_solve(fx_ctx* ctx, const double* rhs, const double* acoef,
+                     double* phi, fx_solve_stats* stats);
+
+/* Per-step top bias (extension: the reference changes Phi_Bc_hi only at
+ * steady-state / inc_step events, main.cpp:572-604).  Step s (1-based, counted
+ * over the context's lifetime) uses v[min(s, n) - 1]; n = 0 clears it. */
+int fx_set_bias_waveform(fx_ctx* ctx, const double* v, int n);
+
+/* Multigrid hierarchy (diagnostics). */
+int fx_num_levels(fx_ctx* ctx);
+int fx_level_dims(fx_ctx* ctx, int level, int dims[3]);
+
+/* Device-resident field pointer (FP64, dense, i fastest) for zero-copy
+ * callers such as the torch layer.  Valid until fx_destroy. */
+int fx_field_device_ptr(fx_ctx* ctx, int field, void** dptr);
+/* Device synchronisation of the context's stream. */
+int fx_sync(fx_ctx* ctx);
+/* HIP stream of the context (hipStream_t). */
+void* fx_stream(fx_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,150 @@
+"""Parity of the HIP path (through the C ABI) with the FP64 CPU oracle.
+
+Bar: FP64 results within 1e-10 relative (max-norm ratio) of the oracle after
+N steps, with both Poisson solves converged far below that (mg_tol_rel 1e-13),
+and the reference's own stopping rule (1e-10) also exercised.
+"""
+import numpy as np
+import pytest
+
+import ferrox_amd as fx
+from ferrox_amd import configs
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def pair(text, **kw):
+    kw.setdefault("mg_tol_rel", 1e-13)
+    inp = fx.from_text(text, **kw)
+    return fx.FerroX(inp), Oracle(inp)
+
+
+def small_mfim(**kw):
+    return configs.mfim(nx=32, ny=32, nz=36, lx=16e-9, ly=16e-9, **kw)
+
+
+def test_level_hierarchy_matches_oracle():
+    g, o = pair(configs.mfim_64(nsteps=1))
+    assert g.levels() == o.levels()
+    print("levels", g.levels())
+
+
+def test_setup_fields_match(relerr):
+    g, o = pair(configs.mfim_64(nsteps=1))
+    for name in ("Pz", "Px", "Py", "mask", "epsilon", "charge", "Gamma"):
+        a, b = g.get(name), o.get(name)
+        assert np.array_equal(a, b), name
+
+
+def test_poisson_solve_matches_oracle(relerr):
+    g, o = pair(configs.mfim_64(nsteps=1), mg_tol_rel=1e-13)
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(g.shape) * 1e6
+    pg, sg = g.poisson_solve(b)
+    po, so = o.poisson_solve(b)
+    print(sg, so)
+    assert sg["mg_resid"] <= 1e-13 * max(sg["mg_bnorm"], sg["mg_resid0"])
+    assert relerr(pg, po) < 1e-12
+
+
+def test_poisson_solve_reference_tolerance(relerr):
+    """The reference's stopping rule (1e-10): both sides stop at the same V-cycle."""
+    g, o = pair(configs.mfim_64(nsteps=1), mg_tol_rel=1e-10)
+    rng = np.random.default_rng(4)
+    b = rng.standard_normal(g.shape) * 1e6
+    pg, sg = g.poisson_solve(b)
+    po, so = o.poisson_solve(b)
+    assert sg["vcycles"] == so["vcycles"]
+    assert relerr(pg, po) < 1e-10
+
+
+def test_phi_rho_and_rhs_one_step(relerr):
+    g, o = pair(small_mfim(nsteps=1))
+    sg, so = g.ComputePhi_Rho(), o.ComputePhi_Rho()
+    assert sg["outer_iters"] == so["outer_iters"] == 1
+    assert relerr(g.get("Phi"), o.get("Phi")) < 1e-12
+    assert relerr(g.get("PoissonRHS"), o.get("PoissonRHS")) < 1e-13
+    g.CalculateTDGL_RHS(); o.CalculateTDGL_RHS()
+    assert relerr(g.get("GL_rhs_z"), o.get("GL_rhs_z")) < 1e-11
+    for e in ("Ex", "Ey", "Ez"):
+        assert relerr(g.get(e), o.get(e)) < 1e-11, e
+
+
+@pytest.mark.parametrize("ghost", [0, 1])
+def test_mfim_steps(relerr, ghost):
+    g, o = pair(small_mfim(nsteps=20), phi_ghost_mode=ghost)
+    sg, so = g.step(20), o.step(20)
+    assert sg["steps_done"] == so["steps_done"] == 20
+    for name in ("Pz", "Phi"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < TOL, (name, e)
+
+
+def test_mfim_reference_stopping_rule(relerr):
+    g, o = pair(small_mfim(nsteps=20), mg_tol_rel=1e-10)
+    sg, so = g.step(20), o.step(20)
+    assert sg["total_vcycles"] == so["total_vcycles"]
+    for name in ("Pz", "Phi"):
+        assert relerr(g.get(name), o.get(name)) < TOL, name
+
+
+def test_mfism_semiconductor_steps(relerr):
+    text = configs.mfism(nx=16, ny=16, nz=64, lx=8e-9, ly=8e-9, nsteps=5, ramp_steps=5, v_end=1.0)
+    g, o = pair(text)
+    sg, so = g.step(5), o.step(5)
+    print(sg, so)
+    assert sg["total_outer_iters"] == so["total_outer_iters"]
+    for name in ("Pz", "Phi", "electrons", "holes", "charge"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < TOL, (name, e)
+
+
+def test_vector_polarization_rotated(relerr):
+    text = small_mfim(nsteps=3) + "\nis_polarization_scalar = 0\nRemnant_P = 1.0e-3 2.0e-3 1.0e-3\n" \
+        "alpha_12 = 1.0e8\nalpha_112 = 2.0e9\nalpha_123 = 3.0e9\ng12 = 2.0e-11\ng44_p = 1.0e-11\n"
+    g, o = pair(text)
+    rng = np.random.default_rng(7)
+    ang = [rng.uniform(0, 40, g.shape) for _ in range(3)]
+    for nm, a in zip(("alpha", "beta", "theta"), ang):
+        g.set(nm, a); o.set(nm, a)
+    g.ComputePhi_Rho(); o.ComputePhi_Rho()
+    g.CalculateTDGL_RHS(); o.CalculateTDGL_RHS()
+    for name in ("PoissonRHS", "Phi", "GL_rhs_x", "GL_rhs_y", "GL_rhs_z"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < 1e-11, (name, e)
+    g.step(3); o.step(3)
+    for name in ("Px", "Py", "Pz", "Phi"):
+        assert relerr(g.get(name), o.get(name)) < TOL, name
+
+
+def test_heun(relerr):
+    g, o = pair(small_mfim(nsteps=5) + "\nTimeIntegratorOrder = 2\n")
+    g.step(5); o.step(5)
+    for name in ("Pz", "Phi"):
+        assert relerr(g.get(name), o.get(name)) < TOL, name
+
+
+def test_voltage_sweep_events(relerr):
+    text = small_mfim(nsteps=12) + "\nvoltage_sweep = 1\ninc_step = 3\nPhi_Bc_inc = 0.1\nPhi_Bc_hi_max = 0.2\nnum_Vapp_max = 5\n"
+    g, o = pair(text)
+    sg, so = g.step(12), o.step(12)
+    print(sg, so)
+    for k in ("steps_done", "num_Vapp", "total_solves"):
+        assert sg[k] == so[k], k
+    assert abs(sg["Phi_Bc_hi"] - so["Phi_Bc_hi"]) < 1e-15
+    for name in ("Pz", "Phi"):
+        assert relerr(g.get(name), o.get(name)) < TOL, name
+
+
+def test_steady_state_stop():
+    text = small_mfim(nsteps=50) + "\nphi_tolerance = 1.0\n"
+    g, o = pair(text)
+    sg, so = g.step(50), o.step(50)
+    assert sg["stopped_steady"] == so["stopped_steady"] == 1
+    assert sg["steps_done"] == so["steps_done"] == 2

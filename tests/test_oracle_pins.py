@@ -1,0 +1,489 @@
+"""Pins of the FP64 CPU oracle (runs without a GPU).
+
+The reference (C++ on AMReX) cannot be built here and holds no golden
+vectors, so each oracle function is checked against what the mathematics
+fixes, chosen so that a dropped term, a wrong sign or a wrong index fails:
+  - the Poisson operator against an independent sparse assembly, its
+    symmetry, a direct sparse solve, a 1-D series-capacitor solution and
+    second-order convergence on a manufactured solution;
+  - the masked stencils on profiles they differentiate exactly (quadratics
+    that vanish on the face for flag 0, Robin-linear profiles for flag 1,
+    cosines for the lateral Laplacian);
+  - the Landau term against a finite difference of the free energy and its
+    analytic equilibrium P0 = 0.195058 C/m^2;
+  - rho(phi) against the Boltzmann relations n = n_i exp(q phi/kT),
+    n p = n_i^2, the FD_half asymptotes, and the Newton coefficient against
+    the analytic derivative;
+  - E against linear potentials, for both ghost conventions;
+  - the time loop's steady-state and sweep control against main.cpp's rules.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import ferrox_amd as fx
+from ferrox_amd import configs
+from oracle.oracle import Oracle, uniform01
+
+EPS0 = 8.85e-12
+
+
+def mk(text, **kw):
+    inp = fx.from_text(text, **kw)
+    return inp, Oracle(inp)
+
+
+def small_mfim(nx=8, ny=8, nz=12, lx=4e-9, ly=4e-9, lz=3e-9, de_top=1e-9, fe_top=3e-9, **kw):
+    return configs.mfim(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz, de_top=de_top, fe_top=fe_top, **kw)
+
+
+# ---------------------------------------------------------------- operator
+def assemble(eps, h, bc_lo, bc_hi, acoef=None, bvals=(0, 0, 0, 0, 0, 0)):
+    """Independent assembly of  -a phi - div(beta grad phi)  with arithmetic
+    face means, Dirichlet value on the face (ghost 2 phi_b - phi_0), and
+    homogeneous Neumann / periodic faces.  Returns (A, bc_rhs)."""
+    nz, ny, nx = eps.shape
+    n = (nx, ny, nz)
+    N = eps.size
+    idx = lambda i, j, k: i + nx * (j + ny * k)
+    rows, cols, vals = [], [], []
+    rhs = np.zeros(N)
+    a = np.zeros(N) if acoef is None else acoef.ravel()
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = idx(i, j, k)
+                diag = -a[c]
+                for d in range(3):
+                    ijk = [i, j, k]
+                    w0 = 1.0 / h[d] ** 2
+                    for side in (-1, 1):
+                        m = list(ijk)
+                        m[d] += side
+                        inside = 0 <= m[d] < n[d]
+                        bct = bc_lo[d] if side < 0 else bc_hi[d]
+                        if inside or bct == 0:
+                            m[d] %= n[d]
+                            beta = 0.5 * (eps[k, j, i] + eps[m[2], m[1], m[0]])
+                            diag += beta * w0
+                            rows.append(c); cols.append(idx(*m)); vals.append(-beta * w0)
+                        elif bct == 1:
+                            beta = eps[k, j, i]
+                            diag += 2 * beta * w0
+                            rhs[c] += 2 * beta * w0 * bvals[d + (0 if side < 0 else 3)]
+                rows.append(c); cols.append(c); vals.append(diag)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(N, N)), rhs
+
+
+def layered_eps(o, inp):
+    nz = inp.n_cell[2]
+    dz = (inp.prob_hi[2] - inp.prob_lo[2]) / nz
+    z = inp.prob_lo[2] + (np.arange(nz) + 0.5) * dz
+    mask = o.get("mask")
+    fe = (z >= inp.FE_lo[2]) & (z <= inp.FE_hi[2])
+    de = (z >= inp.DE_lo[2]) & (z <= inp.DE_hi[2]) & ~fe
+    sc = (z >= inp.SC_lo[2]) & (z <= inp.SC_hi[2]) & ~fe & ~de
+    expect_mask = np.where(fe, 0.0, np.where(de, 1.0, np.where(sc, 2.0, 1.0)))
+    assert np.array_equal(mask, np.broadcast_to(expect_mask[:, None, None], mask.shape))
+    epsr = np.where(fe, inp.epsilonX_fe, np.where(sc, inp.epsilon_si, inp.epsilon_de))
+    return np.broadcast_to((epsr * inp.epsilon_0)[:, None, None], mask.shape).copy()
+
+
+def test_faces_are_arithmetic_means():
+    inp, o = mk(small_mfim())
+    eps = o.get("epsilon")
+    fx_, fy_, fz_ = o.faces()
+    assert np.allclose(fz_[1:-1], 0.5 * (eps[1:] + eps[:-1]), rtol=0, atol=0)
+    assert np.array_equal(fz_[0], eps[0]) and np.array_equal(fz_[-1], eps[-1])
+    assert np.array_equal(fx_[:, :, 0], 0.5 * (eps[:, :, 0] + eps[:, :, -1]))
+    assert np.array_equal(fy_[:, 1:-1, :], 0.5 * (eps[:, 1:, :] + eps[:, :-1, :]))
+
+
+def test_operator_matches_independent_assembly_and_is_symmetric():
+    inp, o = mk(small_mfim(), Phi_Bc_hi=0.3, Phi_Bc_lo=-0.1)
+    eps = layered_eps(o, inp)
+    h = [(inp.prob_hi[d] - inp.prob_lo[d]) / inp.n_cell[d] for d in range(3)]
+    rng = np.random.default_rng(0)
+    a = -np.abs(rng.standard_normal(eps.shape)) * 1e8
+    A, brhs = assemble(eps, h, [0, 0, 1], [0, 0, 1], a, bvals=(0, 0, -0.1, 0, 0, 0.3))
+    x = rng.standard_normal(eps.shape)
+    y = o.apply_op(x, acoef=a)
+    assert np.allclose(y.ravel(), A @ x.ravel(), rtol=1e-13, atol=1e-13 * np.abs(A @ x.ravel()).max())
+    yi = o.apply_op(x, acoef=a, inhomog=True)
+    assert np.allclose(yi.ravel(), A @ x.ravel() - brhs, rtol=1e-12, atol=1e-12 * np.abs(yi).max())
+    z = rng.standard_normal(eps.shape)
+    lhs = np.dot(o.apply_op(x, acoef=a).ravel(), z.ravel())
+    rhs = np.dot(x.ravel(), o.apply_op(z, acoef=a).ravel())
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("nz,with_a", [(12, False), (16, True)])
+def test_multigrid_matches_sparse_direct_solve(nz, with_a):
+    inp, o = mk(small_mfim(nz=nz, lz=nz * 0.25e-9, fe_top=nz * 0.25e-9), Phi_Bc_hi=0.25)
+    eps = layered_eps(o, inp)
+    h = [(inp.prob_hi[d] - inp.prob_lo[d]) / inp.n_cell[d] for d in range(3)]
+    rng = np.random.default_rng(1)
+    b = rng.standard_normal(eps.shape) * 1e6
+    a = -np.abs(rng.standard_normal(eps.shape)) * 1e9 if with_a else None
+    A, brhs = assemble(eps, h, [0, 0, 1], [0, 0, 1], a, bvals=(0, 0, 0, 0, 0, 0.25))
+    exact = spla.spsolve(A.tocsc(), b.ravel() + brhs).reshape(eps.shape)
+    phi, st = o.poisson_solve(b, acoef=a, tol=1e-14)
+    assert st["mg_resid"] <= 1e-14 * max(st["mg_bnorm"], st["mg_resid0"])
+    assert np.abs(phi - exact).max() <= 1e-12 * np.abs(exact).max()
+
+
+def test_multigrid_neumann_and_dirichlet_lateral():
+    text = small_mfim() .replace("domain.is_periodic = 1 1 0", "domain.is_periodic = 0 1 0") \
+        .replace("boundary.lo = per per dir(0.0)", "boundary.lo = neu per dir(0.0)") \
+        .replace("boundary.hi = per per dir(0.0)", "boundary.hi = dir(0.0) per dir(0.0)")
+    inp, o = mk(text, Phi_Bc_hi=0.2)
+    eps = layered_eps(o, inp)
+    h = [(inp.prob_hi[d] - inp.prob_lo[d]) / inp.n_cell[d] for d in range(3)]
+    b = np.random.default_rng(2).standard_normal(eps.shape) * 1e6
+    A, brhs = assemble(eps, h, [2, 0, 1], [1, 0, 1], None, bvals=(0, 0, 0, 0, 0, 0.2))
+    exact = spla.spsolve(A.tocsc(), b.ravel() + brhs).reshape(eps.shape)
+    phi, _ = o.poisson_solve(b, tol=1e-14)
+    assert np.abs(phi - exact).max() <= 1e-12 * np.abs(exact).max()
+
+
+def test_series_capacitor_1d():
+    """Laterally uniform Pz in the FE: phi depends on z only and solves the
+    1-D three-point problem with the bound charge of the FE faces."""
+    inp, o = mk(small_mfim(nx=4, ny=4, nz=12), Phi_Bc_hi=0.5, mg_tol_rel=1e-14)
+    P0 = 0.1
+    mask = o.get("mask")
+    o.set("Pz", np.where(mask == 0, P0, 0.0))
+    o.ComputePhi_Rho()
+    phi = o.get("Phi")
+    assert np.abs(phi - phi[:, :1, :1]).max() <= 1e-12 * np.abs(phi).max()
+    nz = inp.n_cell[2]
+    dz = (inp.prob_hi[2] - inp.prob_lo[2]) / nz
+    epsz = layered_eps(o, inp)[:, 0, 0]
+    fe = mask[:, 0, 0] == 0
+    # -dPz/dz with flag 0 at the FE bottom face (P=0 there) and flag 1 at the top
+    lam = inp.lambda_
+    rhs = np.zeros(nz)
+    for k in range(nz):
+        if not fe[k]:
+            continue
+        lo_b = not fe[k - 1] if k > 0 else True
+        hi_b = (k == nz - 1) or not fe[k + 1]
+        F = P0
+        if lo_b:
+            d = (3 * F + P0) / (3 * dz)                    # (-4*0 + 3F + F(k+1)) / (3 dz)
+        elif hi_b:
+            Fhi = F / (1 - dz / 2 / lam)
+            d = (dz * Fhi / lam + F - P0) / (2 * dz)
+        else:
+            d = 0.0
+        rhs[k] = -d
+    Az = np.zeros((nz, nz))
+    b = rhs.copy()
+    for k in range(nz):
+        for side in (-1, 1):
+            m = k + side
+            if 0 <= m < nz:
+                be = 0.5 * (epsz[k] + epsz[m]) / dz**2
+                Az[k, k] += be; Az[k, m] -= be
+            else:
+                be = 2 * epsz[k] / dz**2
+                Az[k, k] += be
+                b[k] += be * (0.5 if side > 0 else 0.0)
+    ref = np.linalg.solve(Az, b)
+    assert np.abs(phi[:, 0, 0] - ref).max() <= 1e-11 * np.abs(ref).max()
+
+
+def test_manufactured_solution_second_order():
+    """phi = sin(pi z / L) (1 + 0.3 cos(2 pi x / Lx)), homogeneous eps, phi = 0
+    on both z faces: the discrete error falls ~4x per refinement."""
+    errs = []
+    for n in (8, 16, 32):
+        text = small_mfim(nx=n, ny=4, nz=n, lx=4e-9, ly=4e-9, lz=4e-9, de_top=4e-9, fe_top=4e-9)
+        inp, o = mk(text, epsilon_de=10.0)
+        e = 10.0 * EPS0
+        L = 4e-9
+        dx = L / n
+        x = -L / 2 + (np.arange(n) + 0.5) * dx
+        z = (np.arange(n) + 0.5) * dx
+        X = x[None, None, :]
+        Z = z[:, None, None]
+        u = np.sin(np.pi * Z / L) * (1 + 0.3 * np.cos(2 * np.pi * X / L))
+        f = e * (np.pi / L) ** 2 * np.sin(np.pi * Z / L) * (1 + 0.3 * np.cos(2 * np.pi * X / L)) \
+            + e * 0.3 * (2 * np.pi / L) ** 2 * np.sin(np.pi * Z / L) * np.cos(2 * np.pi * X / L)
+        b = np.broadcast_to(f, (n, 4, n)).copy()
+        phi, _ = o.poisson_solve(b, tol=1e-13)
+        errs.append(np.abs(phi - np.broadcast_to(u, phi.shape)).max())
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert 3.3 < r1 < 4.7 and 3.3 < r2 < 4.7, errs
+
+
+# ---------------------------------------------------------------- stencils
+def fe_cells_z(inp):
+    nz = inp.n_cell[2]
+    dz = (inp.prob_hi[2] - inp.prob_lo[2]) / nz
+    z = inp.prob_lo[2] + (np.arange(nz) + 0.5) * dz
+    fe = (z >= inp.FE_lo[2]) & (z <= inp.FE_hi[2])
+    k0 = int(np.argmax(fe))
+    return z, dz, fe, k0
+
+
+def grad_only(**kw):
+    return mk(small_mfim(), include_Landau=0, include_Elec=0, **kw)
+
+
+def test_second_derivative_exact_for_quadratic_vanishing_on_bottom_face():
+    inp, o = grad_only()
+    z, dz, fe, k0 = fe_cells_z(inp)
+    zf = z[k0] - dz / 2
+    C, B = 3.0e16, 2.0e7
+    prof = np.where(fe, (z - zf) * (B + C * (z - zf)), 0.0)
+    o.set("Pz", np.broadcast_to(prof[:, None, None], o.shape))
+    o.CalculateTDGL_RHS()
+    gl = o.get("GL_rhs_z")[:, 0, 0]
+    expect = inp.BigGamma * inp.g11 * 2 * C       # -Gamma * (-g11 * d2P/dz2)
+    top = len(z) - 1
+    for k in range(k0, top):
+        assert abs(gl[k] - expect) <= 1e-9 * abs(expect), (k, gl[k], expect)
+    assert np.all(gl[~fe] == 0.0)
+
+
+def test_robin_linear_profile_on_top_face():
+    inp, o = grad_only()
+    z, dz, fe, k0 = fe_cells_z(inp)
+    lam = inp.lambda_
+    zt = inp.prob_hi[2]          # FE top = domain top: flag_hi[2] = 1 branch
+    A = 0.05
+    prof = np.where(fe, A * (1 + (z - zt) / lam), 0.0)
+    o.set("Pz", np.broadcast_to(prof[:, None, None], o.shape))
+    o.CalculateTDGL_RHS()
+    gl = o.get("GL_rhs_z")[:, 0, 0]
+    scale = inp.BigGamma * inp.g11 * A / dz**2
+    assert abs(gl[-1]) <= 1e-9 * scale
+    o.ComputePhi_Rho()
+    rhs = o.get("PoissonRHS")[:, 0, 0]
+    assert abs(rhs[-1] - (-A / lam)) <= 1e-9 * A / lam     # flag-1 first derivative = P_face / lambda
+
+
+def test_lateral_laplacian_eigenvalue():
+    inp, o = grad_only()
+    nx = inp.n_cell[0]
+    Lx = inp.prob_hi[0] - inp.prob_lo[0]
+    dx = Lx / nx
+    x = inp.prob_lo[0] + (np.arange(nx) + 0.5) * dx
+    z, dz, fe, k0 = fe_cells_z(inp)
+    P = np.where(fe[:, None, None], 0.01 * np.cos(2 * np.pi * x / Lx)[None, None, :], 0.0)
+    P = np.broadcast_to(P, o.shape).copy()
+    o.set("Pz", P)
+    o.CalculateTDGL_RHS()
+    gl = o.get("GL_rhs_z")
+    lam_x = -(2 / dx**2) * (1 - np.cos(2 * np.pi * dx / Lx))
+    k = k0 + 2        # interior FE cell: d2/dz2 of a z-constant profile is 0
+    expect = -inp.BigGamma * (-inp.g44 * lam_x * P[k])
+    assert np.allclose(gl[k], expect, rtol=1e-9, atol=1e-9 * np.abs(expect).max())
+
+
+def test_poisson_rhs_first_derivative_flag0_exact():
+    inp, o = mk(small_mfim())
+    z, dz, fe, k0 = fe_cells_z(inp)
+    zf = z[k0] - dz / 2
+    C, B = 3.0e16, 2.0e7
+    prof = np.where(fe, (z - zf) * (B + C * (z - zf)), 0.0)
+    o.set("Pz", np.broadcast_to(prof[:, None, None], o.shape))
+    o.ComputePhi_Rho()
+    rhs = o.get("PoissonRHS")[:, 0, 0]
+    for k in range(k0, len(z) - 1):
+        expect = -(B + 2 * C * (z[k] - zf))
+        assert abs(rhs[k] - expect) <= 1e-9 * abs(expect), k
+    assert np.all(rhs[~fe] == 0.0)
+
+
+# ---------------------------------------------------------------- Landau
+def test_landau_matches_free_energy_derivative_and_equilibrium():
+    inp, o = mk(small_mfim(), include_Grad=0, include_Elec=0)
+    a, b, g = inp.alpha, inp.beta, inp.gamma
+    f = lambda P: a / 2 * P**2 + b / 4 * P**4 + g / 6 * P**6
+    mask = o.get("mask")
+    vals = np.linspace(-0.3, 0.3, mask.size).reshape(mask.shape)
+    o.set("Pz", np.where(mask == 0, vals, 0.0))
+    o.CalculateTDGL_RHS()
+    gl = o.get("GL_rhs_z")
+    h = 1e-6
+    dfdP = (f(vals + h) - f(vals - h)) / (2 * h)
+    sel = mask == 0
+    assert np.allclose(gl[sel], -inp.BigGamma * dfdP[sel], rtol=1e-6, atol=1e-6 * np.abs(gl).max())
+    # analytic non-trivial equilibrium: P0^2 = (-b + sqrt(b^2 - 4 a g)) / (2 g)
+    P0 = math.sqrt((-b + math.sqrt(b * b - 4 * a * g)) / (2 * g))
+    assert abs(P0 - 0.195058) < 5e-7
+    o.set("Pz", np.where(mask == 0, P0, 0.0))
+    o.CalculateTDGL_RHS()
+    assert np.abs(o.get("GL_rhs_z")).max() <= 1e-12 * inp.BigGamma * abs(a) * P0
+
+
+def test_landau_coupling_terms_vector_mode():
+    text = small_mfim() + "\nis_polarization_scalar = 0\n"
+    inp, o = mk(text, include_Grad=0, include_Elec=0, alpha_12=1.1e9, alpha_112=2.3e9, alpha_123=3.7e9)
+    a, b, g = inp.alpha, inp.beta, inp.gamma
+    a12, a112, a123 = inp.alpha_12, inp.alpha_112, inp.alpha_123
+
+    def F(p, q, r):   # free energy whose gradient the reference's dF/dP expressions are
+        return (a / 2 * (p**2 + q**2 + r**2) + b / 4 * (p**4 + q**4 + r**4) + g / 6 * (p**6 + q**6 + r**6)
+                + a12 * (p**2 * q**2 + q**2 * r**2 + r**2 * p**2)
+                + a112 * (p**4 * (q**2 + r**2) + q**4 * (p**2 + r**2) + r**4 * (p**2 + q**2))
+                + a123 * p**2 * q**2 * r**2)
+    mask = o.get("mask")
+    rng = np.random.default_rng(5)
+    Ps = [np.where(mask == 0, rng.uniform(-0.3, 0.3, mask.shape), 0.0) for _ in range(3)]
+    for nm, v in zip(("Px", "Py", "Pz"), Ps):
+        o.set(nm, v)
+    o.CalculateTDGL_RHS()
+    h = 1e-6
+    sel = mask == 0
+    for d, nm in enumerate(("GL_rhs_x", "GL_rhs_y", "GL_rhs_z")):
+        up = [P.copy() for P in Ps]; dn = [P.copy() for P in Ps]
+        up[d] = up[d] + h; dn[d] = dn[d] - h
+        dF = (F(*up) - F(*dn)) / (2 * h)
+        gl = o.get(nm)
+        assert np.allclose(gl[sel], -inp.BigGamma * dF[sel], rtol=1e-5, atol=1e-6 * np.abs(gl).max()), nm
+
+
+# ---------------------------------------------------------------- charge
+def mis(**kw):
+    text = configs.mfism(nx=4, ny=4, nz=32, lx=2e-9, ly=2e-9, lz=16e-9, sc_top=10e-9, de_top=11e-9,
+                         nsteps=1, ramp_steps=0)
+    return mk(text, **kw)
+
+
+def test_boltzmann_relations_and_mass_action():
+    inp, o = mis(Phi_Bc_hi=0.4)
+    q, kb, T = 1.602e-19, 1.38e-23, 300.0
+    Nc, Nv, Eg = 2.8e25, 1.83e25, 1.12
+    ni = math.sqrt(Nc * Nv) * math.exp(-0.5 * q * Eg / (kb * T))
+    assert abs(ni - 8.7868e15) / 8.7868e15 < 1e-4
+    st = o.ComputePhi_Rho()
+    assert st["outer_iters"] >= 2 and st["outer_err"] <= 1e-5
+    mask = o.get("mask")
+    sc = mask >= 2
+    phi, n, p = o.get("Phi"), o.get("electrons"), o.get("holes")
+    assert np.allclose(n[sc] * p[sc], ni * ni, rtol=1e-12)
+    assert np.allclose(n[sc], ni * np.exp(q * phi[sc] / (kb * T)), rtol=1e-11)
+    # acceptor ionisation within (0, N_A); rho = q (p - n - N_A^-)
+    NAm = p[sc] - n[sc] - o.get("charge")[sc] / q
+    assert np.all(NAm > 0) and np.all(NAm < inp.acceptor_doping)
+
+
+def test_newton_coefficient_is_derivative_of_rho():
+    inp, o = mis(Phi_Bc_hi=0.3)
+    o.ComputePhi_Rho()
+    o.ComputePhi_Rho()    # the last outer iteration linearises at the converged phi
+    q, kb, T = 1.602e-19, 1.38e-23, 300.0
+    mask = o.get("mask")
+    sc = mask >= 2
+    phi, n, p, rho = o.get("Phi"), o.get("electrons"), o.get("holes"), o.get("charge")
+    a = o.get("alpha_cc")
+    NAm = p - n - rho / q
+    NA = inp.acceptor_doping
+    # d/dphi of q (p - n - N_A^-): p' = -p q/kT, n' = n q/kT, N_A^-' = N_A^- (N_A - N_A^-)/N_A * q/kT
+    drho = q * (-p * q / (kb * T) - n * q / (kb * T) - NAm * (NA - NAm) / NA * q / (kb * T))
+    assert np.allclose(a[sc], drho[sc], rtol=5e-4)
+    assert np.all(a[~sc] == 0.0)
+
+
+def test_fermi_dirac_carriers_nondegenerate_limit():
+    """FD_half (ChargeDensity.cpp:5-11) tends to exp(eta) for eta -> -inf: in the
+    p-Si bulk the electron quasi-level sits ~20 kT below the band edge, so the
+    Fermi-Dirac electron density must equal the Boltzmann one to ~1e-8."""
+    q, kb, T = 1.602e-19, 1.38e-23, 300.0
+    inp, o = mis(use_Fermi_Dirac=1, Phi_Bc_hi=0.0)
+    o.ComputePhi_Rho()
+    mask = o.get("mask")
+    sc = mask >= 2
+    phi, n = o.get("Phi"), o.get("electrons")
+    phi_ref = 4.05 + 0.5 * 1.12 + 0.5 * kb * T * math.log(2.8e25 / 1.83e25) / q
+    eta_n = -(-q * (phi - phi_ref) - 4.05 * q) / (kb * T)
+    assert eta_n[sc].max() < -15
+    assert np.allclose(n[sc] / 2.8e25, np.exp(eta_n[sc]), rtol=1e-6)
+    nu = lambda e: e**4 + 50 + 33.6 * e * (1 - 0.68 * math.exp(-0.17 * (e + 1) ** 2))
+    fd = lambda e: 1.0 / (math.exp(-e) + 3 * math.sqrt(3.14) / (4 * nu(e) ** 0.375))
+    assert abs(fd(30.0) / (4 / (3 * math.sqrt(math.pi)) * 30.0**1.5) - 1) < 0.02
+
+
+# ---------------------------------------------------------------- E field
+@pytest.mark.parametrize("ghost", [0, 1])
+def test_E_of_linear_potential(ghost):
+    inp, o = mk(small_mfim(), phi_ghost_mode=ghost, Phi_Bc_lo=0.2, Phi_Bc_hi=0.7, mg_tol_rel=1e-14,
+                epsilonX_fe=10.0, epsilon_de=10.0)
+    nz = inp.n_cell[2]
+    dz = (inp.prob_hi[2] - inp.prob_lo[2]) / nz
+    z = inp.prob_lo[2] + (np.arange(nz) + 0.5) * dz
+    s = 0.5 / (inp.prob_hi[2] - inp.prob_lo[2])
+    lin = 0.2 + s * (z - inp.prob_lo[2])
+    # a pure solve of the inhomogeneous problem with no charge gives exactly this phi
+    o.set("Pz", np.zeros(o.shape))
+    o.ComputePhi_Rho()
+    phi = o.get("Phi")
+    assert np.allclose(phi[:, 0, 0], lin, rtol=1e-12, atol=1e-14)
+    o.ComputeEfromPhi()
+    Ez = o.get("Ez")[:, 0, 0]
+    assert np.allclose(Ez[1:-1], -s, rtol=1e-9)
+    if ghost == 1:      # face value in the ghost: the one-sided formula is exact
+        assert np.allclose(Ez[[0, -1]], -s, rtol=1e-9)
+    else:               # ghost 2 phi_b - phi_0 -> slope * (1 + 2/3) at both faces
+        assert np.allclose(Ez[[0, -1]], -s * 5.0 / 3.0, rtol=1e-9)
+    assert np.abs(o.get("Ex")).max() <= 1e-12 * s and np.abs(o.get("Ey")).max() <= 1e-12 * s
+
+
+# ---------------------------------------------------------------- time loop
+def test_euler_update_is_P_plus_dt_GL():
+    inp, o = mk(small_mfim())
+    P0 = o.get("Pz")
+    o.ComputePhi_Rho()
+    o.CalculateTDGL_RHS()
+    gl = o.get("GL_rhs_z")
+    o.update(1)
+    assert np.array_equal(o.get("Pz"), P0 + inp.dt * gl)
+
+
+def test_heun_update_matches_two_stage_formula():
+    inp, o = mk(small_mfim() + "\nTimeIntegratorOrder = 2\n")
+    P0 = o.get("Pz")
+    o.ComputePhi_Rho(); o.CalculateTDGL_RHS()
+    g0 = o.get("GL_rhs_z")
+    o2 = Oracle(inp)
+    o2.set("Pz", P0 + inp.dt * g0)
+    o2.ComputePhi_Rho(); o2.CalculateTDGL_RHS()
+    g1 = o2.get("GL_rhs_z")
+    o.update(2)
+    assert np.array_equal(o.get("Pz"), P0 + inp.dt * (0.5 * g0 + 0.5 * g1))
+
+
+def test_random_initial_P_and_gamma():
+    inp, o = mk(small_mfim(), random_seed=7)
+    mask = o.get("mask")
+    pz = o.get("Pz")
+    assert np.all(pz[mask != 0] == 0.0) and np.all(o.get("Px") == 0) and np.all(o.get("Py") == 0)
+    assert np.abs(pz).max() <= 1e-3 and abs(pz[mask == 0].mean()) < 1e-4
+    assert np.all(o.get("Gamma") == np.where(mask == 0, inp.BigGamma, 0.0))
+    nx, ny, nz = inp.n_cell
+    k, j, i = np.argwhere(mask == 0)[3]
+    c = i + nx * (j + ny * k)
+    assert pz[k, j, i] == (-1.0 + 2.0 * uniform01(7, 3 * c + 2)) * 1e-3
+    u = np.array([uniform01(1, t) for t in range(20000)])
+    assert 0 <= u.min() and u.max() < 1 and abs(u.mean() - 0.5) < 0.01
+
+
+def test_steady_state_and_sweep_control():
+    inp, o = mk(small_mfim(nsteps=50), phi_tolerance=1.0)
+    st = o.step(50)
+    assert st["stopped_steady"] == 1 and st["steps_done"] == 2      # test applies from step 2 (main.cpp / :548)
+    text = small_mfim(nsteps=40) + "\nvoltage_sweep = 1\ninc_step = 2\nPhi_Bc_inc = 0.1\nPhi_Bc_hi_max = 0.2\nnum_Vapp_max = 3\n"
+    inp, o = mk(text)
+    st = o.step(40)
+    # bias raised at step 2 -> 0.1; no steady state with phi_tolerance 0, so no further events
+    assert st["num_Vapp"] == 1 and abs(st["Phi_Bc_hi"] - 0.1) < 1e-15 and st["steps_done"] == 40
+    inp, o = mk(text.replace("phi_tolerance = 0", "phi_tolerance = 10.0"))
+    st = o.step(40)
+    # steady from step 2 on: increments at steps 2, 3, 4 -> 0.1, 0.2 (reverse), 0.1; stop at num_Vapp_max
+    assert st["num_Vapp"] == 3 and st["steps_done"] == 4 and abs(st["Phi_Bc_hi"] - 0.1) < 1e-12
