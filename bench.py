@@ -1,0 +1,133 @@
+#!/usr/bin/env python3
+"""TDGL+Poisson throughput of the MI355X FerroX hot path on one GPU.
+
+Workload: 256x256x128-cell MFIM (dx = dy = 0.5 nm, dz = 0.1 nm; 4 nm Al2O3
+under 8 nm HZO-like FE and a 0.8 nm spacer; 0.5 V bias; random initial Pz),
+FP64 throughout, forward Euler, the reference's Poisson semantics (zero
+initial guess each step, MLMG stopping rule ||r||_inf <= 1e-10 max(||b||, ||r0||)).
+A step is one pass of main.cpp:261-623: Newton RHS, multigrid solve, fused
+TDGL update, steady-state reduction.
+
+Prints one JSON line.  `value` is steps/s with every field resident in HBM;
+`roofline` is measured live on the dominant kernel with HIP events on the
+stream it runs on; `cpu_baseline` times the FP64 CPU oracle (the build's
+restatement of the reference algorithm, OpenMP) on a bounded sample of the
+same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--warm-start", action="store_true", help="previous phi as the multigrid initial guess")
+    ap.add_argument("--no-kernel-timing", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
+              "(multi-GPU slab decomposition is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+    import ferrox_amd as fx
+    from ferrox_amd import configs
+
+    text = configs.mfim_256(nsteps=a.steps + a.warmup)
+    inp = fx.from_text(text, mg_warm_start=int(a.warm_start))
+    nx, ny, nz = inp.n_cell
+    N = nx * ny * nz
+    g = fx.FerroX(inp)
+    print(f"# levels {g.levels()}", file=sys.stderr, flush=True)
+    if a.warmup:
+        g.step(a.warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    st = g.step(a.steps)
+    g.sync()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    steps_per_s = a.steps / secs
+    n_v = st["total_vcycles"] / max(1, st["steps_done"])
+    n_outer = st["total_outer_iters"] / max(1, st["steps_done"])
+    print(f"# {a.steps} steps in {secs:.3f} s, {n_v:.2f} V-cycles/step, {n_outer:.2f} outer/step",
+          file=sys.stderr, flush=True)
+
+    # algorithmic bytes per step (FP64, scalar P, 1-byte material id)
+    B_step = N * (25 + 17 + n_v * 154 + n_outer * 58)
+    roof = None
+    if not a.no_kernel_timing:
+        reps = 50
+        t_s = g.bench_kernel("smooth0", reps)         # ms per launch of the finest-level smoother
+        t_t = g.bench_kernel("tdgl", reps)
+        t_c = g.bench_kernel("copy", reps)
+        # algorithmic bytes of one red-black sweep: read phi + rhs + mat, write phi = 25 B/cell;
+        # one launch of k_mg_smooth is one colour, i.e. half a sweep
+        fused = bool(inp.mg_fused_sweep)
+        alg = (25.0 if fused else 12.5) * N
+        ach = alg / (t_s * 1e-3) / 1e9
+        kname = "k_mg_gsrb_fused (finest level, red+black)" if fused else "k_mg_smooth (finest level, one colour)"
+        roof = {"bound": "hbm", "kernel": kname,
+                "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "alg_bytes_per_launch": alg, "avg_launch_ms": t_s,
+                "tdgl_kernel_GBs": round(25.0 * N / (t_t * 1e-3) / 1e9, 1), "tdgl_avg_launch_ms": t_t,
+                "copy_GBs": round(16.0 * N / (t_c * 1e-3) / 1e9, 1)}
+    g.close()
+
+    cpu = None
+    if a.cpu_steps > 0:
+        from oracle.oracle import Oracle, threads
+        o = Oracle(fx.from_text(text))
+        c0 = time.perf_counter()
+        o.step(a.cpu_steps)
+        c1 = time.perf_counter()
+        o.close()
+        cpu = {"value": round(a.cpu_steps / (c1 - c0), 5), "unit": "steps/s", "cores": threads(),
+               "kind": "port",
+               "sample": f"{a.cpu_steps} step(s) of the same 256x256x128 workload, FP64 oracle (OpenMP), "
+                         f"{c1 - c0:.1f} s"}
+
+    out = {
+        "metric": "TDGL+Poisson steps/s at 256x256x128 (FP64)",
+        "value": round(steps_per_s, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1e3 * secs / a.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded random initial Pz)",
+        "config": {"workload": "mfim_256: 256x256x128 MFIM, 4 nm Al2O3 / 8 nm FE / 0.8 nm spacer, 0.5 V, "
+                               "zero-guess Poisson to 1e-10" + (" (warm start)" if a.warm_start else ""),
+                   "cells": N, "vcycles_per_step": round(n_v, 3), "outer_iters_per_step": round(n_outer, 3)},
+        "cell_updates_per_s": round(N * steps_per_s, 1),
+        "step_bytes_algorithmic": B_step,
+        "step_hbm_GBs_algorithmic": round(B_step * steps_per_s / 1e9, 1),
+        "step_roofline_frac": round(B_step * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

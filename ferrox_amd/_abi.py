@@ -52,7 +52,7 @@ class FxInputs(C.Structure):
         ("mg_tol_rel", C.c_double), ("mg_max_iter", C.c_int), ("mg_nu1", C.c_int), ("mg_nu2", C.c_int),
         ("mg_warm_start", C.c_int), ("outer_max_iter", C.c_int), ("phi_ghost_mode", C.c_int),
         ("precision", C.c_int), ("bias_ramp_steps", C.c_int), ("bias_ramp_start", C.c_double),
-        ("bias_ramp_end", C.c_double), ("verbose", C.c_int),
+        ("bias_ramp_end", C.c_double), ("verbose", C.c_int), ("mg_fused_sweep", C.c_int),
     ]
 
 
@@ -98,6 +98,7 @@ SIGNATURES = {
     "fx_field_device_ptr": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "fx_sync": (C.c_int, [_P]),
     "fx_stream": (_P, [_P]),
+    "fx_bench_kernel": (C.c_int, [_P, C.c_char_p, C.c_int, C.POINTER(C.c_double)]),
     "fx_num_levels": (C.c_int, [_P]),
     "fx_level_dims": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
 }

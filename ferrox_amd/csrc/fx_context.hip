@@ -31,7 +31,7 @@ using namespace fxd;
 
 namespace {
 
-constexpr int RED_NB = 512;          /* blocks of the fixed-grid reductions */
+constexpr int RED_NB = 2048;         /* blocks of the fixed-grid reductions */
 constexpr double OUTER_TOL = 1.e-5;  /* ElectrostaticSolver.cpp:697 */
 constexpr double BOTTOM_TOL = 1.e-6;
 constexpr int BOTTOM_MAXIT = 400;
@@ -84,6 +84,7 @@ struct fx_ctx {
     double Ru[9];
     /* reductions */
     double* red = nullptr;
+    int nred = 0;                  /* capacity of red, in partials per quantity */
     double* red_out = nullptr;
     double* h_red = nullptr;       /* pinned */
     int* d_iters = nullptr;
@@ -216,8 +217,18 @@ int build_levels(fx_ctx* x) {
         if ((rc = dalloc(x, (void**)&d.bx, sizeof(double) * (size_t)(d.n0 + 1) * d.n1 * d.n2))) return rc;
         if ((rc = dalloc(x, (void**)&d.by, sizeof(double) * (size_t)d.n0 * (d.n1 + 1) * d.n2))) return rc;
         if ((rc = dalloc(x, (void**)&d.bz, sizeof(double) * (size_t)d.n0 * d.n1 * (d.n2 + 1)))) return rc;
+        d.mat = nullptr; d.msx = 0; d.msy = 0; d.matcoef = 0; d.read_a = 0;
+        d.layered = 0; d.bxk = d.byk = d.bzk = nullptr;
+        for (int q = 0; q < 8; ++q) d.eps[q] = 0.0;
         if (l == 0) {
             d.phi = x->phi; d.b = x->rhs; d.a = x->alpha_cc;
+            const fx_inputs& in = x->in;
+            d.mat = x->mat; d.msx = x->msx; d.msy = x->msy; d.matcoef = 1;
+            for (int q = 0; q < 8; ++q) {
+                const int m = q & 3, tph = (q >> 2) & 1;
+                d.eps[q] = m == 0 ? (tph ? in.epsilonX_fe_tphase * in.epsilon_0 : in.epsilonX_fe * in.epsilon_0)
+                         : m == 1 ? in.epsilon_de * in.epsilon_0 : in.epsilon_si * in.epsilon_0;
+            }
         } else {
             if ((rc = dalloc(x, (void**)&d.phi, sizeof(double) * L.N))) return rc;
             if ((rc = dalloc(x, (void**)&d.b, sizeof(double) * L.N))) return rc;
@@ -333,6 +344,39 @@ int build_coefficients(fx_ctx* x) {
     }
     for (size_t l = 0; l < x->lev.size(); ++l) {
         DLev& d = x->lev[l].d;
+        /* layered level: every face coefficient depends on k only */
+        bool lay = true;
+        for (int k = 0; k < d.n2 && lay; ++k)
+            for (int j = 0; j < d.n1 && lay; ++j)
+                for (int i = 0; i <= d.n0 && lay; ++i)
+                    lay = fx[l][(size_t)i + (size_t)(d.n0 + 1) * (j + (size_t)d.n1 * k)] == fx[l][(size_t)(d.n0 + 1) * ((size_t)d.n1 * k)];
+        for (int k = 0; k < d.n2 && lay; ++k)
+            for (int j = 0; j <= d.n1 && lay; ++j)
+                for (int i = 0; i < d.n0 && lay; ++i)
+                    lay = fy[l][(size_t)i + (size_t)d.n0 * (j + (size_t)(d.n1 + 1) * k)] == fy[l][(size_t)d.n0 * ((size_t)(d.n1 + 1) * k)];
+        for (int k = 0; k <= d.n2 && lay; ++k)
+            for (int j = 0; j < d.n1 && lay; ++j)
+                for (int i = 0; i < d.n0 && lay; ++i)
+                    lay = fz[l][(size_t)i + (size_t)d.n0 * (j + (size_t)d.n1 * k)] == fz[l][(size_t)d.n0 * ((size_t)d.n1 * k)];
+        if (d.bxk) { (void)hipFree((void*)d.bxk); (void)hipFree((void*)d.byk); (void)hipFree((void*)d.bzk); d.bxk = d.byk = d.bzk = nullptr; }
+        d.layered = lay ? 1 : 0;
+        if (lay) {
+            std::vector<double> kx(d.n2), ky(d.n2), kz(d.n2 + 1);
+            for (int k = 0; k < d.n2; ++k) {
+                kx[k] = fx[l][(size_t)(d.n0 + 1) * ((size_t)d.n1 * k)];
+                ky[k] = fy[l][(size_t)d.n0 * ((size_t)(d.n1 + 1) * k)];
+            }
+            for (int k = 0; k <= d.n2; ++k) kz[k] = fz[l][(size_t)d.n0 * ((size_t)d.n1 * k)];
+            double *pbx, *pby, *pbz;
+            HIPCHK(x, hipMalloc((void**)&pbx, kx.size() * sizeof(double)));
+            HIPCHK(x, hipMalloc((void**)&pby, ky.size() * sizeof(double)));
+            HIPCHK(x, hipMalloc((void**)&pbz, kz.size() * sizeof(double)));
+            HIPCHK(x, hipMemcpyAsync(pbx, kx.data(), kx.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
+            HIPCHK(x, hipMemcpyAsync(pby, ky.data(), ky.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
+            HIPCHK(x, hipMemcpyAsync(pbz, kz.data(), kz.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
+            HIPCHK(x, hipStreamSynchronize(x->s));
+            d.bxk = pbx; d.byk = pby; d.bzk = pbz;
+        }
         HIPCHK(x, hipMemcpyAsync(d.bx, fx[l].data(), fx[l].size() * sizeof(double), hipMemcpyHostToDevice, x->s));
         HIPCHK(x, hipMemcpyAsync(d.by, fy[l].data(), fy[l].size() * sizeof(double), hipMemcpyHostToDevice, x->s));
         HIPCHK(x, hipMemcpyAsync(d.bz, fz[l].data(), fz[l].size() * sizeof(double), hipMemcpyHostToDevice, x->s));
@@ -446,7 +490,7 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
     const int inh = li == 0;
     if (li + 1 == x->lev.size()) {
         if (li == 0) {
-            fxk::mg_resid_norm(x->s, L.d, bc, 1, x->tmp, x->red, RED_NB);
+            fxk::mg_resid_norm(x->s, L.d, bc, 1, x->tmp, x->red, x->red_out + 4);
             fxk::bottom_cg(x->s, L.d, bc, x->tmp, x->tmp + L.N, x->cg_work, BOTTOM_TOL, BOTTOM_MAXIT, x->d_iters);
             fxk::add(x->s, L.d.phi, x->tmp + L.N, (long long)L.N);
         } else {
@@ -454,18 +498,18 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
         }
         return;
     }
-    for (int s = 0; s < x->in.mg_nu1; ++s) {
-        fxk::mg_smooth(x->s, L.d, bc, inh, 0);
-        fxk::mg_smooth(x->s, L.d, bc, inh, 1);
-    }
+    fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu1, x->in.mg_fused_sweep);
     Level& C = x->lev[li + 1];
     fxk::mg_resid_restrict(x->s, L.d, C.d, bc, inh);
     vcycle(x, li + 1, bc);
-    fxk::mg_prolong(x->s, L.d, C.d);
-    for (int s = 0; s < x->in.mg_nu2; ++s) {
-        fxk::mg_smooth(x->s, L.d, bc, inh, 0);
-        fxk::mg_smooth(x->s, L.d, bc, inh, 1);
-    }
+    fxk::mg_prolong(x->s, L.d, C.d, bc);
+    fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu2, x->in.mg_fused_sweep);
+}
+
+/* Which levels read the A-coefficient: 0 none, 1 the finest level only in
+ * SC cells (it comes from the Newton linearisation), 2 anywhere. */
+void set_read_a(fx_ctx* x, int mode) {
+    for (size_t l = 0; l < x->lev.size(); ++l) x->lev[l].d.read_a = (l == 0) ? mode : (mode ? 2 : 0);
 }
 
 /* MLMG-style solve (stopping rule ||r||_inf <= tol*max(||b||_inf, ||r0||_inf),
@@ -475,8 +519,7 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     const DBC bc = dbc_of(x, bv);
     Level& L0 = x->lev[0];
     double nr[2];
-    fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, RED_NB);
-    fxk::final_reduce(x->s, x->red, RED_NB, 2, 1, x->red_out);
+    fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
     int rc = read_red(x, 2, nr);
     if (rc) return rc;
     const double r0 = nr[0], bn = nr[1];
@@ -486,8 +529,7 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     if (r0 <= target) return FX_OK;
     for (int it = 1; it <= maxit; ++it) {
         vcycle(x, 0, bc);
-        fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, RED_NB);
-        fxk::final_reduce(x->s, x->red, RED_NB, 2, 1, x->red_out);
+        fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
         if ((rc = read_red(x, 2, nr))) return rc;
         st->mg_resid = nr[0]; st->vcycles = it;
         if (!(nr[0] == nr[0])) return fail(x, FX_ENAN, "multigrid produced a non-finite residual");
@@ -520,6 +562,7 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
     st->outer_iters = 0; st->vcycles = 0; st->outer_err = 0.0;
     while (err > OUTER_TOL) {
         fxk::rhs_newton(x->s, rhs_args(x, P));
+        set_read_a(x, x->has_sc ? 1 : 0);
         if (x->has_sc) restrict_acoef(x);
         if (!x->in.mg_warm_start) fxk::fill(x->s, x->phi, (long long)x->N, 0.0);
         fx_solve_stats one{};
@@ -531,12 +574,12 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
         for (int d = 0; d < 3; ++d) ra.n[d] = x->n[d];
         ra.M = mview_of(x); ra.phi = x->phi; ra.rho = x->rho; ra.e_den = x->e_den; ra.h_den = x->h_den;
         ra.phi_prev = x->phi_prev; ra.part = x->red; ra.want_err = x->has_sc; ra.sc = x->sc;
-        fxk::rho_update(x->s, ra, RED_NB);
+        const int nbr = fxk::rho_update(x->s, ra);
         if (!x->has_sc) {
             err = 0.;
             st->outer_iters = 1;
         } else {
-            fxk::final_reduce(x->s, x->red, RED_NB, 2, 0, x->red_out);
+            fxk::final1024(x->s, x->red, nbr, 2, 0, x->red_out);
             double nd[2];
             if ((rc = read_red(x, 2, nd))) return rc;
             if (iter > 0) err = nd[0] / nd[1];
@@ -599,8 +642,7 @@ int update(fx_ctx* x, int order, fx_solve_stats* st2) {
 
 /* CheckSteadyState, ElectrostaticSolver.cpp:526-560. */
 int steady_check(fx_ctx* x, int step, double* metric) {
-    fxk::maxabs2(x->s, x->phi, x->phi_old, (long long)x->N, x->red, RED_NB);
-    fxk::final_reduce(x->s, x->red, RED_NB, 2, 1, x->red_out);
+    fxk::maxabs2(x->s, x->phi, x->phi_old, (long long)x->N, x->red, x->red_out);
     double m[2];
     int rc = read_red(x, 2, m);
     if (rc) return rc;
@@ -625,6 +667,7 @@ void free_all(fx_ctx* x) {
         if (l > 0) { F(d.phi); F(d.b); F(d.a); }
     }
     if (x->h_red) hipHostFree(x->h_red);
+    for (size_t l = 0; l < x->lev.size(); ++l) { F((void*)x->lev[l].d.bxk); F((void*)x->lev[l].d.byk); F((void*)x->lev[l].d.bzk); }
     if (x->s) hipStreamDestroy(x->s);
 }
 
@@ -689,7 +732,8 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
     ALLOC(x->e_den, B); ALLOC(x->h_den, B); ALLOC(x->alpha_cc, B); ALLOC(x->phidiff, B); ALLOC(x->eps, B);
     ALLOC(x->tmp, 2 * B);
     ALLOC(x->mat, (size_t)x->msy * (x->n[2] + 2));
-    ALLOC(x->red, sizeof(double) * 4 * RED_NB);
+    x->nred = std::max(RED_NB, ((x->n[0] + 63) / 64) * ((x->n[1] + 3) / 4) * x->n[2]);
+    ALLOC(x->red, sizeof(double) * 4 * (size_t)x->nred);
     ALLOC(x->red_out, sizeof(double) * 8);
     ALLOC(x->d_iters, sizeof(int) * 4);
 #undef ALLOC
@@ -988,6 +1032,7 @@ int fx_poisson_solve(fx_ctx* x, const double* rhs, const double* acoef, double* 
     HIPCHK(x, hipMemcpyAsync(x->rhs, rhs, B, hipMemcpyHostToDevice, x->s));
     if (acoef) HIPCHK(x, hipMemcpyAsync(x->alpha_cc, acoef, B, hipMemcpyHostToDevice, x->s));
     else fxk::fill(x->s, x->alpha_cc, (long long)x->N, 0.0);
+    set_read_a(x, acoef ? 2 : 0);
     restrict_acoef(x);
     HIPCHK(x, hipMemcpyAsync(x->phi, phi, B, hipMemcpyHostToDevice, x->s));
     double bv[6];
@@ -1015,6 +1060,60 @@ int fx_sync(fx_ctx* x) {
 }
 
 void* fx_stream(fx_ctx* x) { return x ? (void*)x->s : nullptr; }
+
+/* Average duration of one launch of a hot-path kernel, timed with HIP events
+ * on the context's stream (the stream the kernel runs on).  which:
+ *   "smooth0"  red-black Gauss-Seidel on the finest level: one colour
+ *              (k_mg_smooth), or one full sweep when mg_fused_sweep is set
+ *   "resid0"   residual + restriction from the finest level
+ *   "tdgl"     fused TDGL right-hand side + forward-Euler update (state unchanged)
+ *   "rhs"      Poisson right-hand side + Newton coefficient
+ *   "copy"     plain copy of N doubles (bandwidth calibration)
+ * Kernels that write state write it into scratch or re-smooth phi; use after
+ * the measured run. */
+int fx_bench_kernel(fx_ctx* x, const char* which, int reps, double* avg_ms) {
+    if (!x || !which || reps < 1 || !avg_ms) return FX_EINVAL;
+    const std::string w(which);
+    double bv[6];
+    bc_values(x, bv);
+    const DBC bc = dbc_of(x, bv);
+    auto launch = [&]() -> bool {
+        if (w == "smooth0") {
+            if (x->in.mg_fused_sweep && fxk::fused_ok(x->lev[0].d, bc)) fxk::mg_sweep(x->s, x->lev[0].d, bc, 1, 1, 1);
+            else fxk::mg_smooth(x->s, x->lev[0].d, bc, 1, 0);
+        }
+        else if (w == "resid0") {
+            if (x->lev.size() < 2) return false;
+            fxk::mg_resid_restrict(x->s, x->lev[0].d, x->lev[1].d, bc, 1);
+        } else if (w == "tdgl") {
+            const bool fast = scalar_fast(x);
+            TdglArgs a = tdgl_args(x, x->P);
+            for (int d = 0; d < 3; ++d) a.Pout[d] = (fast && d < 2) ? nullptr : x->Pn[d];
+            fxk::tdgl(x->s, a, fast);
+        } else if (w == "rhs") {
+            RhsArgs a = rhs_args(x, x->P);
+            a.rhs = x->tmp; a.alpha = x->tmp + x->N;
+            fxk::rhs_newton(x->s, a);
+        } else if (w == "copy") fxk::copy(x->s, x->tmp, x->phi, (long long)x->N);
+        else return false;
+        return true;
+    };
+    if (!launch()) return fail(x, FX_EINVAL, "unknown kernel name");
+    hipEvent_t e0, e1;
+    HIPCHK(x, hipEventCreate(&e0));
+    HIPCHK(x, hipEventCreate(&e1));
+    HIPCHK(x, hipEventRecord(e0, x->s));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCHK(x, hipEventRecord(e1, x->s));
+    HIPCHK(x, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(x, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHK(x, hipGetLastError());
+    *avg_ms = (double)ms / reps;
+    return FX_OK;
+}
 
 int fx_num_levels(fx_ctx* x) { return x ? (int)x->lev.size() : 0; }
 

@@ -258,6 +258,7 @@ struct Applier {
         /* solver controls added by this build */
         D1(mg_tol_rel) I1(mg_max_iter) I1(mg_nu1) I1(mg_nu2) I1(mg_warm_start) I1(outer_max_iter)
         I1(phi_ghost_mode) I1(precision) I1(bias_ramp_steps) D1(bias_ramp_start) D1(bias_ramp_end) I1(verbose)
+        I1(mg_fused_sweep)
 #undef D1
 #undef D3
 #undef I1
@@ -341,6 +342,7 @@ int fx_inputs_default(fx_inputs* in) {
     in->bias_ramp_steps = 0;
     in->bias_ramp_start = 0.0; in->bias_ramp_end = 0.0;
     in->verbose = 0;
+    in->mg_fused_sweep = 0;
     return FX_OK;
 }
 

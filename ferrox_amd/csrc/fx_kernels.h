@@ -79,18 +79,28 @@ void fill(hipStream_t s, double* p, long long n, double v);
 void copy(hipStream_t s, double* d, const double* src, long long n);
 void add(hipStream_t s, double* d, const double* src, long long n);
 void final_reduce(hipStream_t s, const double* part, int nb, int nq, int is_max, double* out);
-void maxabs2(hipStream_t s, const double* a, const double* b, long long n, double* part, int nb);
+/* out[0] = max|a|, out[1] = max|a-b| (b may be null); part holds 2*2048 doubles */
+void maxabs2(hipStream_t s, const double* a, const double* b, long long n, double* part, double* out);
 void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, double phi_max);
 void rhs_newton(hipStream_t s, const RhsArgs& a);
-void rho_update(hipStream_t s, const RhoArgs& a, int nb);
+/* returns the number of per-block partials written ([num...][den...]) */
+int rho_update(hipStream_t s, const RhoArgs& a);
 void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast);
 void heun(hipStream_t s, const HeunArgs& a);
 void compute_E(hipStream_t s, const EArgs& a);
 void mg_smooth(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int color);
+/* nsweeps full red-black sweeps (fused kernel where the level allows it) */
+void mg_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused);
+bool fused_ok(const fxd::DLev& l, const fxd::DBC& bc);
 void mg_resid_restrict(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc, int inh);
-void mg_prolong(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c);
+void mg_prolong(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc);
 void mg_restrict_cells(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const double* src, double* dst);
-void mg_resid_norm(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, double* r, double* part, int nb);
+/* r = b - A phi (r may be null); out[0] = max|r|, out[1] = max|b|; part holds
+ * 2*resid_norm_blocks(l) doubles */
+void mg_resid_norm(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, double* r, double* part, double* out);
+int resid_norm_blocks(const fxd::DLev& l);
+void final1024(hipStream_t s, const double* part, int nb, int nq, int is_max, double* out);
+int resid_blocks(const fxd::DLev& l);
 void bottom_cg(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* rhs, double* e, double* work,
                double tol, int maxit, int* iters);
 }  // namespace fxk

@@ -12,9 +12,7 @@
  *                  and the forward-Euler update (main.cpp:293-306) fused, so P and
  *                  phi cross HBM once per step.
  *   k_compute_E    ComputeEfromPhi for output.
- *   k_mg_*         geometric multigrid: red-black Gauss-Seidel, residual fused
- *                  with restriction, prolongation fused with correction,
- *                  single-workgroup Jacobi-PCG bottom solve, residual norms.
+ * The multigrid kernels live in fx_mg.hip.
  */
 #include "fx_device.h"
 #include "fx_kernels.h"
@@ -43,7 +41,7 @@ __global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, 
     }
 }
 
-/* max|a| and max|a-b| (b may be null) per block. */
+/* max|a| and max|a-b| (b may be null) per workgroup. */
 __global__ __launch_bounds__(256) void k_maxabs2(const double* __restrict__ a, const double* __restrict__ b, long long n,
                                                  double* __restrict__ part) {
     __shared__ double sh[256 / 64 + 1];
@@ -149,16 +147,16 @@ __global__ __launch_bounds__(256) void k_rhs_newton(RhsArgs a) {
 }
 
 /* rho(phi) in SC cells (0 elsewhere), optional L1 partial sums of
- * |phi - phi_prev| and |phi| with phi_prev <- phi. */
+ * |phi - phi_prev| and |phi| with phi_prev <- phi.  grid: 64x4 tiles, z. */
 __global__ __launch_bounds__(256) void k_rho_update(RhoArgs a) {
     __shared__ double sh[256 / 64 + 1];
     const int nx = a.n[0], ny = a.n[1];
-    const long long N = (long long)nx * ny * a.n[2];
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
     double s_num = 0.0, s_den = 0.0;
-    for (long long c = blockIdx.x * 256LL + threadIdx.x; c < N; c += (long long)gridDim.x * 256) {
-        const int i = (int)(c % nx);
-        const int j = (int)((c / nx) % ny);
-        const int k = (int)(c / ((long long)nx * ny));
+    if (i < nx && j < ny) {
+        const int c = i + nx * (j + ny * k);
         const double ph = a.phi[c];
         if (a.M.code(i, j, k) >= 2) {
             double n, p;
@@ -169,15 +167,17 @@ __global__ __launch_bounds__(256) void k_rho_update(RhoArgs a) {
             a.rho[c] = 0.0;
         }
         if (a.want_err) {
-            s_num += fabs(ph - a.phi_prev[c]);
-            s_den += fabs(ph);
+            s_num = fabs(ph - a.phi_prev[c]);
+            s_den = fabs(ph);
             a.phi_prev[c] = ph;
         }
     }
     if (a.want_err) {
         s_num = block_sum<256>(s_num, sh);
         s_den = block_sum<256>(s_den, sh);
-        if (threadIdx.x == 0) { a.part[blockIdx.x] = s_num; a.part[gridDim.x + blockIdx.x] = s_den; }
+        const int nb = gridDim.x * gridDim.y * gridDim.z;
+        const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        if (threadIdx.x == 0) { a.part[bid] = s_num; a.part[nb + bid] = s_den; }
     }
 }
 
@@ -363,161 +363,6 @@ __global__ __launch_bounds__(256) void k_compute_E(EArgs a) {
     a.E[2][c] = -(R[6] * fx + R[7] * fy + R[8] * fz);
 }
 
-/* ------------------------------- multigrid -------------------------------- */
-/* One colour of red-black Gauss-Seidel; thread per same-colour cell. */
-__global__ __launch_bounds__(256) void k_mg_smooth(DLev l, DBC bc, int inh, int color) {
-    const int half = (l.n0 + 1) >> 1;
-    const long long tot = (long long)half * l.n1 * l.n2;
-    const long long t = blockIdx.x * 256LL + threadIdx.x;
-    if (t >= tot) return;
-    const int ih = (int)(t % half);
-    const long long jk = t / half;
-    const int j = (int)(jk % l.n1);
-    const int k = (int)(jk / l.n1);
-    const int i = 2 * ih + ((j + k + color) & 1);
-    if (i >= l.n0) return;
-    double D, S;
-    mg_row(l, l.phi, i, j, k, bc, inh != 0, D, S);
-    const int c = i + l.n0 * (j + l.n1 * k);
-    l.phi[c] = (l.b[c] + S) / D;
-}
-
-/* Residual of the fine level averaged into the coarse right-hand side; the
- * coarse solution is zeroed.  Thread per coarse cell. */
-__global__ __launch_bounds__(256) void k_mg_resid_restrict(DLev f, DLev cl, DBC bc, int inh) {
-    const long long tot = (long long)cl.n0 * cl.n1 * cl.n2;
-    const long long t = blockIdx.x * 256LL + threadIdx.x;
-    if (t >= tot) return;
-    const int I = (int)(t % cl.n0);
-    const int J = (int)((t / cl.n0) % cl.n1);
-    const int K = (int)(t / ((long long)cl.n0 * cl.n1));
-    const int cx = f.crs[0], cy = f.crs[1], cz = f.crs[2];
-    double s = 0.0;
-    for (int kk = 0; kk <= cz; ++kk)
-        for (int jj = 0; jj <= cy; ++jj)
-            for (int ii = 0; ii <= cx; ++ii) {
-                const int i = cx ? 2 * I + ii : I, j = cy ? 2 * J + jj : J, k = cz ? 2 * K + kk : K;
-                double D, S;
-                mg_row(f, f.phi, i, j, k, bc, inh != 0, D, S);
-                const int c = i + f.n0 * (j + f.n1 * k);
-                s += f.b[c] - (D * f.phi[c] - S);
-            }
-    const int cnt = (1 + cx) * (1 + cy) * (1 + cz);
-    cl.b[t] = s / cnt;
-    cl.phi[t] = 0.0;
-}
-
-/* Piecewise-constant prolongation added to the fine solution. */
-__global__ __launch_bounds__(256) void k_mg_prolong(DLev f, DLev cl) {
-    const long long tot = (long long)f.n0 * f.n1 * f.n2;
-    const long long t = blockIdx.x * 256LL + threadIdx.x;
-    if (t >= tot) return;
-    const int i = (int)(t % f.n0);
-    const int j = (int)((t / f.n0) % f.n1);
-    const int k = (int)(t / ((long long)f.n0 * f.n1));
-    const int I = f.crs[0] ? i >> 1 : i, J = f.crs[1] ? j >> 1 : j, K = f.crs[2] ? k >> 1 : k;
-    f.phi[t] += cl.phi[I + cl.n0 * (J + cl.n1 * K)];
-}
-
-/* Cell average of a fine array into the coarse level (A-coefficient). */
-__global__ __launch_bounds__(256) void k_mg_restrict_cells(DLev f, DLev cl, const double* __restrict__ src, double* __restrict__ dst) {
-    const long long tot = (long long)cl.n0 * cl.n1 * cl.n2;
-    const long long t = blockIdx.x * 256LL + threadIdx.x;
-    if (t >= tot) return;
-    const int I = (int)(t % cl.n0);
-    const int J = (int)((t / cl.n0) % cl.n1);
-    const int K = (int)(t / ((long long)cl.n0 * cl.n1));
-    const int cx = f.crs[0], cy = f.crs[1], cz = f.crs[2];
-    double s = 0.0;
-    for (int kk = 0; kk <= cz; ++kk)
-        for (int jj = 0; jj <= cy; ++jj)
-            for (int ii = 0; ii <= cx; ++ii)
-                s += src[(cx ? 2 * I + ii : I) + f.n0 * ((cy ? 2 * J + jj : J) + f.n1 * (cz ? 2 * K + kk : K))];
-    dst[t] = s / ((1 + cx) * (1 + cy) * (1 + cz));
-}
-
-/* Residual r = b - A phi into r (may be null) with per-block max|r| and max|b|. */
-__global__ __launch_bounds__(256) void k_mg_resid_norm(DLev l, DBC bc, int inh, double* __restrict__ r, double* __restrict__ part) {
-    __shared__ double sh[256 / 64 + 1];
-    const long long tot = (long long)l.n0 * l.n1 * l.n2;
-    double mr = 0.0, mb = 0.0;
-    for (long long t = blockIdx.x * 256LL + threadIdx.x; t < tot; t += (long long)gridDim.x * 256) {
-        const int i = (int)(t % l.n0);
-        const int j = (int)((t / l.n0) % l.n1);
-        const int k = (int)(t / ((long long)l.n0 * l.n1));
-        double D, S;
-        mg_row(l, l.phi, i, j, k, bc, inh != 0, D, S);
-        const double rr = l.b[t] - (D * l.phi[t] - S);
-        if (r) r[t] = rr;
-        mr = nmax(mr, fabs(rr));
-        mb = nmax(mb, fabs(l.b[t]));
-    }
-    mr = block_max<256>(mr, sh);
-    mb = block_max<256>(mb, sh);
-    if (threadIdx.x == 0) { part[blockIdx.x] = mr; part[gridDim.x + blockIdx.x] = mb; }
-}
-
-/* Jacobi-preconditioned CG for A e = rhs (homogeneous BCs) in one workgroup.
- * work holds 5*N doubles (r, z, p, q, diag). */
-__global__ __launch_bounds__(1024) void k_bottom_cg(DLev l, DBC bc, const double* __restrict__ rhs, double* __restrict__ e,
-                                                    double* __restrict__ work, double tol, int maxit, int* __restrict__ iters) {
-    constexpr int BS = 1024;
-    __shared__ double sh[BS / 64 + 1];
-    const int N = l.n0 * l.n1 * l.n2;
-    double* r = work;
-    double* z = work + N;
-    double* p = work + 2 * N;
-    double* q = work + 3 * N;
-    double* Dg = work + 4 * N;
-    const int tid = threadIdx.x;
-    double rr = 0.0;
-    for (int c = tid; c < N; c += BS) {
-        const int i = c % l.n0, j = (c / l.n0) % l.n1, k = c / (l.n0 * l.n1);
-        double D, S;
-        mg_row(l, rhs, i, j, k, bc, false, D, S);
-        Dg[c] = D;
-        r[c] = rhs[c];
-        e[c] = 0.0;
-        rr += rhs[c] * rhs[c];
-    }
-    const double rr0 = block_sum<BS>(rr, sh);
-    if (rr0 == 0.0) { if (tid == 0) *iters += 0; return; }
-    double rz = 0.0;
-    for (int c = tid; c < N; c += BS) { z[c] = r[c] / Dg[c]; p[c] = z[c]; rz += r[c] * z[c]; }
-    rz = block_sum<BS>(rz, sh);
-    const double tol2 = tol * tol * rr0;
-    int it = 0;
-    for (; it < maxit; ++it) {
-        double pq = 0.0;
-        for (int c = tid; c < N; c += BS) {
-            const int i = c % l.n0, j = (c / l.n0) % l.n1, k = c / (l.n0 * l.n1);
-            double D, S;
-            mg_row(l, p, i, j, k, bc, false, D, S);
-            const double qq = D * p[c] - S;
-            q[c] = qq;
-            pq += p[c] * qq;
-        }
-        pq = block_sum<BS>(pq, sh);
-        const double al = rz / pq;
-        rr = 0.0;
-        for (int c = tid; c < N; c += BS) {
-            e[c] += al * p[c];
-            r[c] -= al * q[c];
-            rr += r[c] * r[c];
-        }
-        rr = block_sum<BS>(rr, sh);
-        if (rr <= tol2) { ++it; break; }
-        double rzn = 0.0;
-        for (int c = tid; c < N; c += BS) { z[c] = r[c] / Dg[c]; rzn += r[c] * z[c]; }
-        rzn = block_sum<BS>(rzn, sh);
-        const double be = rzn / rz;
-        rz = rzn;
-        for (int c = tid; c < N; c += BS) p[c] = z[c] + be * p[c];
-        __syncthreads();
-    }
-    if (tid == 0) *iters += it;
-}
-
 /* ------------------------------ launchers -------------------------------- */
 namespace fxk {
 
@@ -539,8 +384,9 @@ void add(hipStream_t s, double* d, const double* src, long long n) {
 void final_reduce(hipStream_t s, const double* part, int nb, int nq, int is_max, double* out) {
     hipLaunchKernelGGL(k_final, dim3(1), dim3(256), 0, s, part, nb, nq, is_max, out);
 }
-void maxabs2(hipStream_t s, const double* a, const double* b, long long n, double* part, int nb) {
-    hipLaunchKernelGGL(k_maxabs2, dim3(nb), dim3(256), 0, s, a, b, n, part);
+void maxabs2(hipStream_t s, const double* a, const double* b, long long n, double* part, double* out) {
+    hipLaunchKernelGGL(k_maxabs2, dim3(2048), dim3(256), 0, s, a, b, n, part);
+    final1024(s, part, 2048, 2, 1, out);
 }
 void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, double phi_max) {
     unsigned b = nblk(n, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
@@ -549,8 +395,10 @@ void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, lo
 void rhs_newton(hipStream_t s, const RhsArgs& a) {
     hipLaunchKernelGGL(k_rhs_newton, grid_cells(a.g.n), dim3(256), 0, s, a);
 }
-void rho_update(hipStream_t s, const RhoArgs& a, int nb) {
-    hipLaunchKernelGGL(k_rho_update, dim3(nb), dim3(256), 0, s, a);
+int rho_update(hipStream_t s, const RhoArgs& a) {
+    const dim3 g = grid_cells(a.n);
+    hipLaunchKernelGGL(k_rho_update, g, dim3(256), 0, s, a);
+    return (int)(g.x * g.y * g.z);
 }
 void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast) {
     if (scalar_fast) hipLaunchKernelGGL(k_tdgl<true>, grid_cells(a.g.n), dim3(256), 0, s, a);
@@ -563,27 +411,4 @@ void heun(hipStream_t s, const HeunArgs& a) {
 void compute_E(hipStream_t s, const EArgs& a) {
     hipLaunchKernelGGL(k_compute_E, grid_cells(a.g.n), dim3(256), 0, s, a);
 }
-void mg_smooth(hipStream_t s, const DLev& l, const DBC& bc, int inh, int color) {
-    const long long tot = (long long)((l.n0 + 1) >> 1) * l.n1 * l.n2;
-    hipLaunchKernelGGL(k_mg_smooth, dim3(nblk(tot, 256)), dim3(256), 0, s, l, bc, inh, color);
-}
-void mg_resid_restrict(hipStream_t s, const DLev& f, const DLev& c, const DBC& bc, int inh) {
-    const long long tot = (long long)c.n0 * c.n1 * c.n2;
-    hipLaunchKernelGGL(k_mg_resid_restrict, dim3(nblk(tot, 256)), dim3(256), 0, s, f, c, bc, inh);
-}
-void mg_prolong(hipStream_t s, const DLev& f, const DLev& c) {
-    const long long tot = (long long)f.n0 * f.n1 * f.n2;
-    hipLaunchKernelGGL(k_mg_prolong, dim3(nblk(tot, 256)), dim3(256), 0, s, f, c);
-}
-void mg_restrict_cells(hipStream_t s, const DLev& f, const DLev& c, const double* src, double* dst) {
-    const long long tot = (long long)c.n0 * c.n1 * c.n2;
-    hipLaunchKernelGGL(k_mg_restrict_cells, dim3(nblk(tot, 256)), dim3(256), 0, s, f, c, src, dst);
-}
-void mg_resid_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, double* r, double* part, int nb) {
-    hipLaunchKernelGGL(k_mg_resid_norm, dim3(nb), dim3(256), 0, s, l, bc, inh, r, part);
-}
-void bottom_cg(hipStream_t s, const DLev& l, const DBC& bc, const double* rhs, double* e, double* work, double tol, int maxit, int* iters) {
-    hipLaunchKernelGGL(k_bottom_cg, dim3(1), dim3(1024), 0, s, l, bc, rhs, e, work, tol, maxit, iters);
-}
-
 }  // namespace fxk

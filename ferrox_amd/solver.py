@@ -125,6 +125,12 @@ class FerroX:
         self._check(rc)
         return phi, stats_dict(st)
 
+    def bench_kernel(self, which: str, reps: int = 20) -> float:
+        """Average ms per launch of a hot-path kernel (HIP events, context stream)."""
+        ms = C.c_double()
+        self._check(self._lib.fx_bench_kernel(self._h, which.encode(), reps, C.byref(ms)))
+        return ms.value
+
     def levels(self) -> list[tuple[int, int, int]]:
         out = []
         d = (C.c_int * 3)()

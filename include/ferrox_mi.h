@@ -132,6 +132,7 @@ typedef struct fx_inputs {
     double bias_ramp_start;
     double bias_ramp_end;
     int    verbose;
+    int    mg_fused_sweep;        /* 1: one-pass red+black sweep kernel on large levels (default 0) */
 } fx_inputs;
 
 /* Field identifiers.  0..17 follow the plotfile variable order of the
@@ -247,6 +248,11 @@ int fx_poisson_solve(fx_ctx* ctx, const double* rhs, const double* acoef,
  * steady-state / inc_step events, main.cpp:572-604).  Step s (1-based, counted
  * over the context's lifetime) uses v[min(s, n) - 1]; n = 0 clears it. */
 int fx_set_bias_waveform(fx_ctx* ctx, const double* v, int n);
+
+/* Measurement: average duration in ms of one launch of a named hot-path
+ * kernel ("smooth0", "resid0", "tdgl", "rhs", "copy"), timed with HIP events
+ * on the context's stream.  No reference counterpart (instrumentation). */
+int fx_bench_kernel(fx_ctx* ctx, const char* which, int reps, double* avg_ms);
 
 /* Multigrid hierarchy (diagnostics). */
 int fx_num_levels(fx_ctx* ctx);

@@ -329,6 +329,43 @@ struct MG {
         }
     }
 
+    /* Cell-centred linear prolongation of the coarse correction: in each
+     * coarsened direction the fine cell takes 3/4 of its parent and 1/4 of the
+     * parent's neighbour on its side; across a domain face that neighbour is
+     * the homogeneous ghost (-parent for Dirichlet, parent for Neumann,
+     * periodic image).  Terms are summed z-outer, x-inner, parent first. */
+    void prolong(Lev& l, const Lev& c) {
+        #pragma omp parallel for schedule(static)
+        for (int k = 0; k < l.n[2]; ++k)
+            for (int j = 0; j < l.n[1]; ++j)
+                for (int i = 0; i < l.n[0]; ++i) {
+                    const int f[3] = {i, j, k};
+                    int id[3][2];
+                    double w[3][2];
+                    int cnt[3];
+                    for (int d = 0; d < 3; ++d) {
+                        if (!l.crs[d]) { id[d][0] = f[d]; w[d][0] = 1.0; cnt[d] = 1; continue; }
+                        const int I = f[d] >> 1;
+                        int J = (f[d] & 1) ? I + 1 : I - 1;
+                        double sg = 1.0;
+                        if (J < 0 || J >= c.n[d]) {
+                            const int bct = J < 0 ? bclo[d] : bchi[d];
+                            if (bct == FX_BC_PERIODIC) J = (J + c.n[d]) % c.n[d];
+                            else { sg = (bct == FX_BC_DIRICHLET) ? -1.0 : 1.0; J = I; }
+                        }
+                        id[d][0] = I; w[d][0] = 0.75;
+                        id[d][1] = J; w[d][1] = 0.25 * sg;
+                        cnt[d] = 2;
+                    }
+                    double e = 0.0;
+                    for (int a = 0; a < cnt[2]; ++a)
+                        for (int b = 0; b < cnt[1]; ++b)
+                            for (int q = 0; q < cnt[0]; ++q)
+                                e += (w[2][a] * w[1][b]) * w[0][q] * c.phi[c.c(id[0][q], id[1][b], id[2][a])];
+                    l.phi[l.c(i, j, k)] += e;
+                }
+    }
+
     void vcycle(size_t li, const double* bv0) {
         Lev& l = L[li];
         const double* bv = (li == 0) ? bv0 : nullptr;
@@ -349,12 +386,7 @@ struct MG {
         restrict_cells(l, c, l.r.data(), c.b.data());
         std::fill(c.phi.begin(), c.phi.end(), 0.0);
         vcycle(li + 1, bv0);
-        const int cx = l.crs[0], cy = l.crs[1], cz = l.crs[2];
-        #pragma omp parallel for schedule(static)
-        for (int k = 0; k < l.n[2]; ++k)
-            for (int j = 0; j < l.n[1]; ++j)
-                for (int i = 0; i < l.n[0]; ++i)
-                    l.phi[l.c(i, j, k)] += c.phi[c.c(cx ? i >> 1 : i, cy ? j >> 1 : j, cz ? k >> 1 : k)];
+        prolong(l, c);
         for (int s = 0; s < nu2; ++s) { smooth(l, 0, bv); smooth(l, 1, bv); }
     }
 
