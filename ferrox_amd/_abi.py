@@ -77,6 +77,7 @@ _P = C.c_void_p
 _DP = C.POINTER(C.c_double)
 SIGNATURES = {
     "fx_version": (C.c_char_p, []),
+    "fx_inputs_size": (C.c_int, []),
     "fx_inputs_default": (C.c_int, [C.POINTER(FxInputs)]),
     "fx_inputs_parse_file": (C.c_int, [C.c_char_p, C.POINTER(FxInputs), C.c_char_p, C.c_int]),
     "fx_inputs_set": (C.c_int, [C.POINTER(FxInputs), C.c_char_p, C.c_char_p, C.c_int]),

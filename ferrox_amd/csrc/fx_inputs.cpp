@@ -290,6 +290,8 @@ extern "C" {
 
 const char* fx_version(void) { return "ferrox-mi355x 0.1"; }
 
+int fx_inputs_size(void) { return (int)sizeof(fx_inputs); }
+
 int fx_inputs_default(fx_inputs* in) {
     if (!in) return FX_EINVAL;
     std::memset(in, 0, sizeof(*in));

@@ -177,6 +177,8 @@ typedef struct fx_ctx fx_ctx;
 
 /* Library / inputs -------------------------------------------------------- */
 const char* fx_version(void);
+/* sizeof(fx_inputs), for bindings that mirror the struct. */
+int fx_inputs_size(void);
 /* Fills every key with the reference default (FerroX.cpp:273-572; Appendix of keys). */
 int fx_inputs_default(fx_inputs* in);
 /* ParmParse-compatible reader for a FerroX inputs file: key = v1 v2 ..., '#'
