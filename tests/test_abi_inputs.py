@@ -109,7 +109,7 @@ def test_last_definition_wins_and_expressions():
              "domain.prob_hi = L L nz*1.0e-9\ndt = 2**-3 * 1.0e-12\n")
     fx.finalize(inp)
     assert inp.nsteps == 7
-    assert inp.prob_hi[0] == 16e-9 and abs(inp.prob_hi[2] - 9e-9) < 1e-24
+    assert inp.prob_hi[0] == 16e-9 and abs(inp.prob_hi[2] - 9e-9) < 1e-20
     assert inp.dt == 0.125e-12
 
 
