@@ -106,7 +106,7 @@ def test_reference_style_inputs_file(tmp_path):
 def test_last_definition_wins_and_expressions():
     inp = fx.default_inputs()
     fx.apply(inp, REF_LIKE + "\nnsteps = 7\nmy_constants.L = 16.e-9\nmy_constants.nz = 9\n"
-             "domain.prob_hi = L L nz*1.0e-9\ndt = 2**-3 * 1.0e-12\n")
+             "domain.prob_hi = L L nz*1.0e-9\ndt = 2**-3*1.0e-12\n")
     fx.finalize(inp)
     assert inp.nsteps == 7
     assert inp.prob_hi[0] == 16e-9 and abs(inp.prob_hi[2] - 9e-9) < 1e-20
