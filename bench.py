@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=1, help="steps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--warm-start", action="store_true", help="previous phi as the multigrid initial guess")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+FP64-refinement leg")
     return ap.parse_args()
 
 
@@ -88,7 +89,29 @@ def main():
                 "alg_bytes_per_launch": alg, "avg_launch_ms": t_s,
                 "tdgl_kernel_GBs": round(25.0 * N / (t_t * 1e-3) / 1e9, 1), "tdgl_avg_launch_ms": t_t,
                 "copy_GBs": round(16.0 * N / (t_c * 1e-3) / 1e9, 1)}
+    P64, F64 = g.get("Pz"), g.get("Phi")
     g.close()
+
+    # precision form: FP32 V-cycles inside FP64 iterative refinement, same
+    # workload and same number of steps, compared with the FP64 run
+    fp32 = None
+    if not a.no_fp32:
+        g2 = fx.FerroX(fx.from_text(text, mg_warm_start=int(a.warm_start), precision=1))
+        if a.warmup:
+            g2.step(a.warmup)
+        g2.sync()
+        t0 = time.perf_counter()
+        st2 = g2.step(a.steps)
+        g2.sync()
+        s2 = time.perf_counter() - t0
+        import numpy as np
+        eP = float(np.abs(g2.get("Pz") - P64).max() / np.abs(P64).max())
+        eF = float(np.abs(g2.get("Phi") - F64).max() / np.abs(F64).max())
+        g2.close()
+        fp32 = {"steps_per_s": round(a.steps / s2, 3), "speedup_vs_fp64": round((a.steps / s2) / steps_per_s, 3),
+                "vcycles_per_step": round(st2["total_vcycles"] / max(1, st2["steps_done"]), 3),
+                "max_rel_err_Pz": eP, "max_rel_err_phi": eF}
+        print(f"# fp32+refine: {fp32}", file=sys.stderr, flush=True)
 
     cpu = None
     if a.cpu_steps > 0:
@@ -124,6 +147,7 @@ def main():
         "step_hbm_GBs_algorithmic": round(B_step * steps_per_s / 1e9, 1),
         "step_roofline_frac": round(B_step * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roof,
+        "fp32_refine": fp32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -91,6 +91,10 @@ struct fx_ctx {
     double* cg_work = nullptr;
     double* tmp = nullptr;         /* N doubles: residual of a one-level solve */
     std::vector<Level> lev;
+    /* FP32 correction hierarchy of the mixed-precision solve (precision = FP32_REFINE) */
+    std::vector<DLevF> lev32;
+    float* cg_work32 = nullptr;
+    double* phi_alt = nullptr;     /* second phi buffer of the refinement update */
     /* host shadows */
     std::vector<uint8_t> h_mat;
     std::vector<double> h_tphase, h_ang[3];
@@ -237,6 +241,25 @@ int build_levels(fx_ctx* x) {
     }
     int rc;
     if ((rc = dalloc(x, (void**)&x->cg_work, sizeof(double) * 5 * x->lev.back().N))) return rc;
+    if (x->in.precision == FX_PREC_FP32_REFINE) {
+        x->lev32.resize(x->lev.size());
+        for (size_t l = 0; l < x->lev.size(); ++l) {
+            const DLev& d = x->lev[l].d;
+            DLevF& f = x->lev32[l];
+            std::memset(&f, 0, sizeof(f));
+            f.n0 = d.n0; f.n1 = d.n1; f.n2 = d.n2;
+            for (int q = 0; q < 3; ++q) { f.ih2[q] = (float)d.ih2[q]; f.crs[q] = d.crs[q]; }
+            const size_t Nl = x->lev[l].N;
+            if ((rc = dalloc(x, (void**)&f.phi, sizeof(float) * Nl))) return rc;
+            if ((rc = dalloc(x, (void**)&f.b, sizeof(float) * Nl))) return rc;
+            if ((rc = dalloc(x, (void**)&f.a, sizeof(float) * Nl))) return rc;
+            if ((rc = dalloc(x, (void**)&f.bx, sizeof(float) * (size_t)(d.n0 + 1) * d.n1 * d.n2))) return rc;
+            if ((rc = dalloc(x, (void**)&f.by, sizeof(float) * (size_t)d.n0 * (d.n1 + 1) * d.n2))) return rc;
+            if ((rc = dalloc(x, (void**)&f.bz, sizeof(float) * (size_t)d.n0 * d.n1 * (d.n2 + 1)))) return rc;
+        }
+        if ((rc = dalloc(x, (void**)&x->cg_work32, sizeof(float) * 5 * x->lev.back().N))) return rc;
+        if ((rc = dalloc(x, (void**)&x->phi_alt, sizeof(double) * x->N))) return rc;
+    }
     return FX_OK;
 }
 
@@ -383,6 +406,36 @@ int build_coefficients(fx_ctx* x) {
     }
     HIPCHK(x, hipMemcpyAsync(x->eps, cell.data(), cell.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
     HIPCHK(x, hipStreamSynchronize(x->s));
+    /* FP32 copies of every level's coefficients (rounded from FP64) */
+    for (size_t l = 0; l < x->lev32.size(); ++l) {
+        DLevF& f = x->lev32[l];
+        const DLev& d = x->lev[l].d;
+        auto up = [&](float* dst, const std::vector<double>& src) -> int {
+            std::vector<float> t(src.size());
+            for (size_t q = 0; q < src.size(); ++q) t[q] = (float)src[q];
+            HIPCHK(x, hipMemcpy(dst, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+            return FX_OK;
+        };
+        int rc;
+        if ((rc = up(f.bx, fx[l])) || (rc = up(f.by, fy[l])) || (rc = up(f.bz, fz[l]))) return rc;
+        if (f.bxk) { (void)hipFree((void*)f.bxk); (void)hipFree((void*)f.byk); (void)hipFree((void*)f.bzk); f.bxk = f.byk = f.bzk = nullptr; }
+        f.layered = d.layered;
+        f.matcoef = d.matcoef;
+        f.mat = d.mat; f.msx = d.msx; f.msy = d.msy;
+        for (int q = 0; q < 8; ++q) f.eps[q] = (float)d.eps[q];
+        if (d.layered) {
+            std::vector<double> kx(d.n2), ky(d.n2), kz(d.n2 + 1);
+            HIPCHK(x, hipMemcpy(kx.data(), d.bxk, kx.size() * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(x, hipMemcpy(ky.data(), d.byk, ky.size() * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(x, hipMemcpy(kz.data(), d.bzk, kz.size() * sizeof(double), hipMemcpyDeviceToHost));
+            float *pbx, *pby, *pbz;
+            HIPCHK(x, hipMalloc((void**)&pbx, kx.size() * sizeof(float)));
+            HIPCHK(x, hipMalloc((void**)&pby, ky.size() * sizeof(float)));
+            HIPCHK(x, hipMalloc((void**)&pbz, kz.size() * sizeof(float)));
+            if ((rc = up(pbx, kx)) || (rc = up(pby, ky)) || (rc = up(pbz, kz))) return rc;
+            f.bxk = pbx; f.byk = pby; f.bzk = pbz;
+        }
+    }
     return FX_OK;
 }
 
@@ -510,6 +563,23 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
  * SC cells (it comes from the Newton linearisation), 2 anywhere. */
 void set_read_a(fx_ctx* x, int mode) {
     for (size_t l = 0; l < x->lev.size(); ++l) x->lev[l].d.read_a = (l == 0) ? mode : (mode ? 2 : 0);
+    for (size_t l = 0; l < x->lev32.size(); ++l) x->lev32[l].read_a = (l == 0) ? mode : (mode ? 2 : 0);
+}
+
+/* FP32 V-cycle on the correction equation A e = r (homogeneous BCs); level 0
+ * starts from e = 0 through the zero-initial first red pass. */
+void vcycle32(fx_ctx* x, size_t li, const DBC& bc) {
+    DLevF& L = x->lev32[li];
+    if (li + 1 == x->lev32.size()) {
+        fxk::bottom_cg(x->s, L, bc, L.b, L.phi, x->cg_work32, 1e-5, BOTTOM_MAXIT, x->d_iters);
+        return;
+    }
+    fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, 1);
+    DLevF& C = x->lev32[li + 1];
+    fxk::mg_resid_restrict(x->s, L, C, bc);
+    vcycle32(x, li + 1, bc);
+    fxk::mg_prolong(x->s, L, C, bc);
+    fxk::mg_sweep(x->s, L, bc, x->in.mg_nu2, 0);
 }
 
 /* MLMG-style solve (stopping rule ||r||_inf <= tol*max(||b||_inf, ||r0||_inf),
@@ -536,6 +606,58 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
         if (nr[0] <= target) return FX_OK;
     }
     return fail(x, FX_ENOCONV_MG, "multigrid did not converge in mg_max_iter V-cycles");
+}
+
+/* Mixed precision: FP64 iterative refinement around FP32 V-cycles.  Each
+ * iteration solves A e = r approximately with one FP32 V-cycle, then one FP64
+ * pass forms phi + e into the second buffer and the new residual (stored as
+ * the next FP32 right-hand side) with the same stopping rule as mg_solve. */
+int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+    const DBC bc = dbc_of(x, bv);
+    const DBC bc0 = dbc_of(x, bv);
+    double* home = x->phi;
+    auto swap_phi = [&]() { std::swap(x->phi, x->phi_alt); x->lev[0].d.phi = x->phi; };
+    double nr[2];
+    fxk::refine_resid(x->s, x->lev[0].d, bc0, nullptr, x->phi_alt, x->lev32[0].b, x->red, x->red_out);
+    int rc = read_red(x, 2, nr);
+    if (rc) return rc;
+    swap_phi();
+    const double r0 = nr[0], bn = nr[1];
+    const double target = tol * std::max(bn, r0);
+    st->mg_resid0 = r0; st->mg_bnorm = bn; st->mg_resid = r0; st->vcycles = 0;
+    rc = FX_OK;
+    if (!(r0 == r0) || !(bn == bn)) rc = fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
+    else if (r0 > target) {
+        rc = fail(x, FX_ENOCONV_MG, "mixed-precision multigrid did not converge in mg_max_iter iterations");
+        for (int it = 1; it <= maxit; ++it) {
+            vcycle32(x, 0, bc);
+            fxk::refine_resid(x->s, x->lev[0].d, bc0, x->lev32[0].phi, x->phi_alt, x->lev32[0].b, x->red, x->red_out);
+            int rr = read_red(x, 2, nr);
+            if (rr) return rr;
+            swap_phi();
+            st->mg_resid = nr[0]; st->vcycles = it;
+            if (!(nr[0] == nr[0])) { rc = fail(x, FX_ENAN, "mixed-precision multigrid produced a non-finite residual"); break; }
+            if (nr[0] <= target) { rc = FX_OK; break; }
+        }
+    }
+    if (x->phi != home) {       /* keep the field's device address stable */
+        fxk::copy(x->s, x->phi_alt, x->phi, (long long)x->N);
+        swap_phi();
+    }
+    return rc;
+}
+
+int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+    if (x->in.precision == FX_PREC_FP32_REFINE) {
+        /* FP32 copy of the A-coefficient for the correction levels */
+        if (x->lev32[0].read_a) {
+            fxk::d2f(x->s, x->lev32[0].a, x->alpha_cc, (long long)x->N);
+            for (size_t l = 0; l + 1 < x->lev32.size(); ++l)
+                fxk::mg_restrict_cells(x->s, x->lev32[l], x->lev32[l + 1], x->lev32[l].a, x->lev32[l + 1].a);
+        }
+        return mg_solve_mixed(x, bv, tol, maxit, st);
+    }
+    return mg_solve(x, bv, tol, maxit, st);
 }
 
 void restrict_acoef(fx_ctx* x) {
@@ -566,7 +688,7 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
         if (x->has_sc) restrict_acoef(x);
         if (!x->in.mg_warm_start) fxk::fill(x->s, x->phi, (long long)x->N, 0.0);
         fx_solve_stats one{};
-        int rc = mg_solve(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one);
+        int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one);
         st->vcycles += one.vcycles;
         st->mg_resid0 = one.mg_resid0; st->mg_resid = one.mg_resid; st->mg_bnorm = one.mg_bnorm;
         if (rc) { st->outer_iters = iter + 1; return rc; }
@@ -666,6 +788,11 @@ void free_all(fx_ctx* x) {
         F(d.bx); F(d.by); F(d.bz);
         if (l > 0) { F(d.phi); F(d.b); F(d.a); }
     }
+    for (size_t l = 0; l < x->lev32.size(); ++l) {
+        DLevF& f = x->lev32[l];
+        F(f.phi); F(f.b); F(f.a); F(f.bx); F(f.by); F(f.bz); F((void*)f.bxk); F((void*)f.byk); F((void*)f.bzk);
+    }
+    F(x->cg_work32); F(x->phi_alt);
     if (x->h_red) hipHostFree(x->h_red);
     for (size_t l = 0; l < x->lev.size(); ++l) { F((void*)x->lev[l].d.bxk); F((void*)x->lev[l].d.byk); F((void*)x->lev[l].d.bzk); }
     if (x->s) hipStreamDestroy(x->s);
@@ -1039,7 +1166,7 @@ int fx_poisson_solve(fx_ctx* x, const double* rhs, const double* acoef, double* 
     bc_values(x, bv);
     std::memcpy(x->bc_solve, bv, sizeof(bv));
     fx_solve_stats tmp{};
-    int rc = mg_solve(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, st ? st : &tmp);
+    int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, st ? st : &tmp);
     HIPCHK(x, hipMemcpyAsync(phi, x->phi, B, hipMemcpyDeviceToHost, x->s));
     HIPCHK(x, hipStreamSynchronize(x->s));
     return rc;

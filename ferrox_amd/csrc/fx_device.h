@@ -205,33 +205,35 @@ __device__ __forceinline__ void grad_phi(const PhiView& P, const Geom& g, int i,
 }
 
 /* ------------------------------ multigrid -------------------------------- */
-/* One level of the hierarchy.  Face coefficients come from arrays, or, on
- * the finest level, from the material id: beta_face = 0.5*(eps_c + eps_nb)
- * with eps from a table indexed by (mat & 7) and eps_nb = eps_c across a
- * non-periodic face (the reference copies the adjacent cell into the ghost,
- * ElectrostaticSolver.cpp:257-298) -- the same values the arrays hold. */
-struct DLev {
+/* One level of the hierarchy, in FP64 or FP32.  Face coefficients come from
+ * per-k arrays (layered level: the stack is laterally uniform), from the
+ * material id on the finest level (beta_face = 0.5*(eps_c + eps_nb), eps_nb =
+ * eps_c across a non-periodic face, as the reference copies the adjacent cell
+ * into the ghost, ElectrostaticSolver.cpp:257-298), or from face arrays.  The
+ * three hold the same values. */
+template <class T>
+struct DLevT {
     int n0, n1, n2;
-    double ih2[3];
-    double* phi;
-    double* b;
-    double* bx;   /* (n0+1)*n1*n2 */
-    double* by;   /* n0*(n1+1)*n2 */
-    double* bz;   /* n0*n1*(n2+1) */
-    double* a;
+    T ih2[3];
+    T* phi;
+    T* b;
+    T* bx;   /* (n0+1)*n1*n2 */
+    T* by;   /* n0*(n1+1)*n2 */
+    T* bz;   /* n0*n1*(n2+1) */
+    T* a;
     int crs[3];
     const uint8_t* mat;   /* grown box, finest level */
     int msx, msy;
     int matcoef;          /* faces from mat + eps table */
     int read_a;           /* 0: a == 0; 1: a nonzero only in SC cells; 2: a anywhere */
-    double eps[8];
-    /* Layered level: every face coefficient depends on k only (the stack is
-     * laterally uniform), stored as bxk[k], byk[k], bzk[k] (bzk has n2+1). */
+    T eps[8];
     int layered;
-    const double* bxk;
-    const double* byk;
-    const double* bzk;
+    const T* bxk;
+    const T* byk;
+    const T* bzk;
 };
+using DLev = DLevT<double>;
+using DLevF = DLevT<float>;
 
 struct DBC {
     int lo[3], hi[3];
@@ -240,25 +242,25 @@ struct DBC {
 
 /* Cell permittivity of a material id: FE (t-phase or not), DE, SC; chosen
  * with selects on kernel-argument scalars rather than an indexed table. */
-__device__ __forceinline__ double eps_of(const DLev& l, int m) {
+template <class T>
+__device__ __forceinline__ T eps_of(const DLevT<T>& l, int m) {
     const int code = m & 3;
-    const double fe = (m & 4) ? l.eps[4] : l.eps[0];
+    const T fe = (m & 4) ? l.eps[4] : l.eps[0];
     return code == 0 ? fe : (code == 1 ? l.eps[1] : l.eps[2]);
 }
 
 /* Coefficients of row (i,j,k): A-coefficient and the six face weights
- * beta/h^2 in the order x lo, x hi, y lo, y hi, z lo, z hi.  Sources, by
- * precedence: per-k arrays (layered level), the material id (finest level),
- * face arrays.  All three hold the same values. */
-__device__ __forceinline__ void mg_coef(const DLev& l, const DBC& bc, int i, int j, int k, int c,
-                                        double& a, double w[6]) {
+ * beta/h^2 in the order x lo, x hi, y lo, y hi, z lo, z hi. */
+template <class T>
+__device__ __forceinline__ void mg_coef(const DLevT<T>& l, const DBC& bc, int i, int j, int k, int c,
+                                        T& a, T w[6]) {
     const int n0 = l.n0, n1 = l.n1;
     const int g = (i + 1) + l.msx * (j + 1) + l.msy * (k + 1);   /* finest level only */
-    if (l.read_a == 0) a = 0.0;
+    if (l.read_a == 0) a = T(0);
     else if (l.read_a == 2) a = l.a[c];
-    else a = ((l.mat[g] & 3) >= 2) ? l.a[c] : 0.0;
+    else a = ((l.mat[g] & 3) >= 2) ? l.a[c] : T(0);
     if (l.layered) {
-        const double bxv = l.bxk[k], byv = l.byk[k];
+        const T bxv = l.bxk[k], byv = l.byk[k];
         w[0] = bxv * l.ih2[0];
         w[1] = bxv * l.ih2[0];
         w[2] = byv * l.ih2[1];
@@ -267,19 +269,20 @@ __device__ __forceinline__ void mg_coef(const DLev& l, const DBC& bc, int i, int
         w[5] = l.bzk[k + 1] * l.ih2[2];
     } else if (l.matcoef) {
         const int mc = l.mat[g];
-        const double ec = eps_of(l, mc);
-        const double exl = (i > 0 || bc.lo[0] == BC_PER) ? eps_of(l, l.mat[g - 1]) : ec;
-        const double exh = (i < n0 - 1 || bc.hi[0] == BC_PER) ? eps_of(l, l.mat[g + 1]) : ec;
-        const double eyl = (j > 0 || bc.lo[1] == BC_PER) ? eps_of(l, l.mat[g - l.msx]) : ec;
-        const double eyh = (j < n1 - 1 || bc.hi[1] == BC_PER) ? eps_of(l, l.mat[g + l.msx]) : ec;
-        const double ezl = (k > 0 || bc.lo[2] == BC_PER) ? eps_of(l, l.mat[g - l.msy]) : ec;
-        const double ezh = (k < l.n2 - 1 || bc.hi[2] == BC_PER) ? eps_of(l, l.mat[g + l.msy]) : ec;
-        w[0] = (0.5 * (ec + exl)) * l.ih2[0];
-        w[1] = (0.5 * (exh + ec)) * l.ih2[0];
-        w[2] = (0.5 * (ec + eyl)) * l.ih2[1];
-        w[3] = (0.5 * (eyh + ec)) * l.ih2[1];
-        w[4] = (0.5 * (ec + ezl)) * l.ih2[2];
-        w[5] = (0.5 * (ezh + ec)) * l.ih2[2];
+        const T ec = eps_of(l, mc);
+        const T exl = (i > 0 || bc.lo[0] == BC_PER) ? eps_of(l, l.mat[g - 1]) : ec;
+        const T exh = (i < n0 - 1 || bc.hi[0] == BC_PER) ? eps_of(l, l.mat[g + 1]) : ec;
+        const T eyl = (j > 0 || bc.lo[1] == BC_PER) ? eps_of(l, l.mat[g - l.msx]) : ec;
+        const T eyh = (j < n1 - 1 || bc.hi[1] == BC_PER) ? eps_of(l, l.mat[g + l.msx]) : ec;
+        const T ezl = (k > 0 || bc.lo[2] == BC_PER) ? eps_of(l, l.mat[g - l.msy]) : ec;
+        const T ezh = (k < l.n2 - 1 || bc.hi[2] == BC_PER) ? eps_of(l, l.mat[g + l.msy]) : ec;
+        const T h = T(0.5);
+        w[0] = (h * (ec + exl)) * l.ih2[0];
+        w[1] = (h * (exh + ec)) * l.ih2[0];
+        w[2] = (h * (ec + eyl)) * l.ih2[1];
+        w[3] = (h * (eyh + ec)) * l.ih2[1];
+        w[4] = (h * (ec + ezl)) * l.ih2[2];
+        w[5] = (h * (ezh + ec)) * l.ih2[2];
     } else {
         const int fx = i + (n0 + 1) * (j + n1 * k);
         const int fy = i + n0 * (j + (n1 + 1) * k);
@@ -296,25 +299,27 @@ __device__ __forceinline__ void mg_coef(const DLev& l, const DBC& bc, int i, int
  * Dirichlet (ghost 2*phi_b - phi_0, value on the face) / homogeneous Neumann
  * faces.  nb[6] are the neighbour values (read only for interior or periodic
  * faces).  Accumulation order: -a, x lo, x hi, y lo, y hi, z lo, z hi. */
-__device__ __forceinline__ void mg_acc(const DLev& l, const DBC& bc, bool inh, int i, int j, int k,
-                                       double a, const double w[6], const double nb[6], double& D, double& S) {
+template <class T>
+__device__ __forceinline__ void mg_acc(const DLevT<T>& l, const DBC& bc, bool inh, int i, int j, int k,
+                                       T a, const T w[6], const T nb[6], T& D, T& S) {
     const int n[3] = {l.n0, l.n1, l.n2};
     const int ijk[3] = {i, j, k};
     D = -a;
-    S = 0.0;
+    S = T(0);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double wlo = w[2 * d], whi = w[2 * d + 1];
+        const T wlo = w[2 * d], whi = w[2 * d + 1];
         if (ijk[d] > 0 || bc.lo[d] == BC_PER) { D += wlo; S += wlo * nb[2 * d]; }
-        else if (bc.lo[d] == BC_DIR) { D += 2.0 * wlo; if (inh) S += 2.0 * wlo * bc.v[d]; }
+        else if (bc.lo[d] == BC_DIR) { D += T(2) * wlo; if (inh) S += T(2) * wlo * T(bc.v[d]); }
         if (ijk[d] < n[d] - 1 || bc.hi[d] == BC_PER) { D += whi; S += whi * nb[2 * d + 1]; }
-        else if (bc.hi[d] == BC_DIR) { D += 2.0 * whi; if (inh) S += 2.0 * whi * bc.v[3 + d]; }
+        else if (bc.hi[d] == BC_DIR) { D += T(2) * whi; if (inh) S += T(2) * whi * T(bc.v[3 + d]); }
     }
 }
 
 /* Neighbour values from a global array (periodic wrap). */
-__device__ __forceinline__ void mg_nb_global(const DLev& l, const double* __restrict__ phi, int i, int j, int k,
-                                             int c, double nb[6]) {
+template <class T>
+__device__ __forceinline__ void mg_nb_global(const DLevT<T>& l, const T* __restrict__ phi, int i, int j, int k,
+                                             int c, T nb[6]) {
     const int n0 = l.n0, n1 = l.n1, n2 = l.n2, s = n0 * n1;
     nb[0] = phi[i > 0 ? c - 1 : c + (n0 - 1)];
     nb[1] = phi[i < n0 - 1 ? c + 1 : c - (n0 - 1)];
@@ -324,23 +329,24 @@ __device__ __forceinline__ void mg_nb_global(const DLev& l, const double* __rest
     nb[5] = phi[k < n2 - 1 ? c + s : c - s * (n2 - 1)];
 }
 
-__device__ __forceinline__ void mg_row(const DLev& l, const double* __restrict__ phi, int i, int j, int k,
-                                       const DBC& bc, bool inh, double& D, double& S) {
+template <class T>
+__device__ __forceinline__ void mg_row(const DLevT<T>& l, const T* __restrict__ phi, int i, int j, int k,
+                                       const DBC& bc, bool inh, T& D, T& S) {
     const int c = i + l.n0 * (j + l.n1 * k);
-    double a, w[6], nb[6];
+    T a, w[6], nb[6];
     mg_coef(l, bc, i, j, k, c, a, w);
     mg_nb_global(l, phi, i, j, k, c, nb);
     mg_acc(l, bc, inh, i, j, k, a, w, nb, D, S);
 }
 
 /* Block-wide reductions (wavefront shuffle, then LDS across waves). */
-template <int BS>
-__device__ __forceinline__ double block_sum(double v, double* sh) {
+template <int BS, class T>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 0) sh[w] = v;
     __syncthreads();
-    double r = 0.0;
+    T r = T(0);
     if (threadIdx.x == 0) { for (int i = 0; i < BS / 64; ++i) r += sh[i]; sh[BS / 64] = r; }
     __syncthreads();
     r = sh[BS / 64];

@@ -103,4 +103,16 @@ void final1024(hipStream_t s, const double* part, int nb, int nq, int is_max, do
 int resid_blocks(const fxd::DLev& l);
 void bottom_cg(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* rhs, double* e, double* work,
                double tol, int maxit, int* iters);
+/* FP32 correction hierarchy (homogeneous BCs) */
+void mg_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int nsweeps, int zero_first);
+void mg_resid_restrict(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc);
+void mg_prolong(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc);
+void mg_restrict_cells(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const float* src, float* dst);
+void bottom_cg(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, const float* rhs, float* e, float* work,
+               double tol, int maxit, int* iters);
+/* phi_new = phi + e (e may be null), r32 = (float)(b - A phi_new) in FP64;
+ * out[0] = max|r|, out[1] = max|b| */
+void refine_resid(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const float* e, double* phi_new, float* r32,
+                  double* part, double* out);
+void d2f(hipStream_t s, float* d, const double* src, long long n);
 }  // namespace fxk

@@ -165,3 +165,36 @@ def test_large_layered_solve_matches_oracle(relerr):
         sg, so = g.ComputePhi_Rho(), o.ComputePhi_Rho()
         print(fused, sg["vcycles"], so["vcycles"])
         assert relerr(g.get("Phi"), o.get("Phi")) < 1e-11
+
+
+# ----------------------------------------------------------- precision form
+def test_mixed_precision_solve_reaches_fp64_accuracy(relerr):
+    """FP32 V-cycles inside FP64 iterative refinement converge to the same
+    FP64 tolerance as the plain solve."""
+    g, o = pair(configs.mfim_64(nsteps=1), mg_tol_rel=1e-12, precision=1)
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(g.shape) * 1e6
+    pg, sg = g.poisson_solve(b)
+    po, so = o.poisson_solve(b)
+    print(sg, so)
+    assert sg["mg_resid"] <= 1e-12 * max(sg["mg_bnorm"], sg["mg_resid0"])
+    assert relerr(pg, po) < 1e-10
+
+
+def test_mixed_precision_steps_mfim(relerr):
+    g, o = pair(small_mfim(nsteps=20), precision=1, mg_tol_rel=1e-10)
+    sg, so = g.step(20), o.step(20)
+    for name in ("Pz", "Phi"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e, sg["total_vcycles"], so["total_vcycles"])
+        assert e < 1e-6, (name, e)
+
+
+def test_mixed_precision_steps_semiconductor(relerr):
+    text = configs.mfism(nx=16, ny=16, nz=64, lx=8e-9, ly=8e-9, nsteps=5, ramp_steps=5, v_end=1.0)
+    g, o = pair(text, precision=1, mg_tol_rel=1e-10)
+    g.step(5); o.step(5)
+    for name in ("Pz", "Phi", "electrons", "holes"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < 1e-6, (name, e)
