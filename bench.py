@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--warm-start", action="store_true", help="previous phi as the multigrid initial guess")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+FP64-refinement leg")
+    ap.add_argument("--fused", type=int, default=0, help="one-pass red+black sweep kernel (1) or per-colour kernels (0)")
     return ap.parse_args()
 
 
@@ -50,7 +51,7 @@ def main():
     from ferrox_amd import configs
 
     text = configs.mfim_256(nsteps=a.steps + a.warmup)
-    inp = fx.from_text(text, mg_warm_start=int(a.warm_start))
+    inp = fx.from_text(text, mg_warm_start=int(a.warm_start), mg_fused_sweep=a.fused)
     nx, ny, nz = inp.n_cell
     N = nx * ny * nz
     g = fx.FerroX(inp)
@@ -96,7 +97,7 @@ def main():
     # workload and same number of steps, compared with the FP64 run
     fp32 = None
     if not a.no_fp32:
-        g2 = fx.FerroX(fx.from_text(text, mg_warm_start=int(a.warm_start), precision=1))
+        g2 = fx.FerroX(fx.from_text(text, mg_warm_start=int(a.warm_start), precision=1, mg_fused_sweep=a.fused))
         if a.warmup:
             g2.step(a.warmup)
         g2.sync()

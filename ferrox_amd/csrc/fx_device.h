@@ -231,6 +231,7 @@ struct DLevT {
     const T* bxk;
     const T* byk;
     const T* bzk;
+    T* phi2;   /* second solution buffer of the one-pass sweep (null if not allocated) */
 };
 using DLev = DLevT<double>;
 using DLevF = DLevT<float>;

@@ -104,7 +104,7 @@ int resid_blocks(const fxd::DLev& l);
 void bottom_cg(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* rhs, double* e, double* work,
                double tol, int maxit, int* iters);
 /* FP32 correction hierarchy (homogeneous BCs) */
-void mg_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int nsweeps, int zero_first);
+void mg_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int nsweeps, int zero_first, int allow_fused);
 void mg_resid_restrict(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc);
 void mg_prolong(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc);
 void mg_restrict_cells(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const float* src, float* dst);
@@ -115,4 +115,12 @@ void bottom_cg(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, const flo
 void refine_resid(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const float* e, double* phi_new, float* r32,
                   double* part, double* out);
 void d2f(hipStream_t s, float* d, const double* src, long long n);
+void add_f2d(hipStream_t s, double* phi, const float* e, long long n);
+/* whole sub-V-cycle of the levels levs[0..nl-1] (device array) in one workgroup */
+void mg_tail(hipStream_t s, const fxd::DLev* levs, int nl, const fxd::DBC& bc, int read_a, int nu1, int nu2, double* work,
+             double tol, int maxit, int* iters);
+void mg_tail(hipStream_t s, const fxd::DLevF* levs, int nl, const fxd::DBC& bc, int read_a, int nu1, int nu2, float* work,
+             double tol, int maxit, int* iters);
+/* r32 = (float)(b - A phi) in FP64 (inhomogeneous BCs); out[0] = max|r|, out[1] = max|b| */
+void resid_to32(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, float* r32, double* part, double* out);
 }  // namespace fxk

@@ -78,9 +78,9 @@ def test_phi_rho_and_rhs_one_step(relerr):
         assert relerr(g.get(e), o.get(e)) < 1e-11, e
 
 
-@pytest.mark.parametrize("ghost", [0, 1])
-def test_mfim_steps(relerr, ghost):
-    g, o = pair(small_mfim(nsteps=20), phi_ghost_mode=ghost)
+@pytest.mark.parametrize("ghost,fused", [(0, 0), (1, 0), (0, 1)])
+def test_mfim_steps(relerr, ghost, fused):
+    g, o = pair(small_mfim(nsteps=20), phi_ghost_mode=ghost, mg_fused_sweep=fused)
     sg, so = g.step(20), o.step(20)
     assert sg["steps_done"] == so["steps_done"] == 20
     for name in ("Pz", "Phi"):
@@ -168,10 +168,11 @@ def test_large_layered_solve_matches_oracle(relerr):
 
 
 # ----------------------------------------------------------- precision form
-def test_mixed_precision_solve_reaches_fp64_accuracy(relerr):
+@pytest.mark.parametrize("fused", [0, 1])
+def test_mixed_precision_solve_reaches_fp64_accuracy(relerr, fused):
     """FP32 V-cycles inside FP64 iterative refinement converge to the same
     FP64 tolerance as the plain solve."""
-    g, o = pair(configs.mfim_64(nsteps=1), mg_tol_rel=1e-12, precision=1)
+    g, o = pair(configs.mfim_64(nsteps=1), mg_tol_rel=1e-12, precision=1, mg_fused_sweep=fused)
     rng = np.random.default_rng(11)
     b = rng.standard_normal(g.shape) * 1e6
     pg, sg = g.poisson_solve(b)
@@ -181,8 +182,9 @@ def test_mixed_precision_solve_reaches_fp64_accuracy(relerr):
     assert relerr(pg, po) < 1e-10
 
 
-def test_mixed_precision_steps_mfim(relerr):
-    g, o = pair(small_mfim(nsteps=20), precision=1, mg_tol_rel=1e-10)
+@pytest.mark.parametrize("fused", [0, 1])
+def test_mixed_precision_steps_mfim(relerr, fused):
+    g, o = pair(small_mfim(nsteps=20), precision=1, mg_tol_rel=1e-10, mg_fused_sweep=fused)
     sg, so = g.step(20), o.step(20)
     for name in ("Pz", "Phi"):
         e = relerr(g.get(name), o.get(name))
