@@ -19,7 +19,7 @@ FX_GHOST_AMREX, FX_GHOST_FACE = 0, 1
 
 FIELDS = ["Px", "Py", "Pz", "Phi", "PoissonRHS", "Ex", "Ey", "Ez", "holes", "electrons",
           "charge", "epsilon", "mask", "tphase", "alpha", "beta", "theta", "PhiDiff",
-          "alpha_cc", "Gamma", "GL_rhs_x", "GL_rhs_y", "GL_rhs_z"]
+          "alpha_cc", "Gamma", "GL_rhs_x", "GL_rhs_y", "GL_rhs_z", "landau_alpha"]
 FIELD_ID = {name: i for i, name in enumerate(FIELDS)}
 # plotfile variable order of the reference (Plotfile.cpp:30-106)
 PLOT_VARS = FIELDS[:18]

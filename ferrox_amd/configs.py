@@ -114,3 +114,102 @@ def mfism_128(nsteps=2000) -> str:
 def mfim_256(nsteps=500, seed=1) -> str:
     return mfim(nx=256, ny=256, nz=128, lx=128e-9, ly=128e-9, lz=12.8e-9, de_top=4e-9, fe_top=12e-9,
                 nsteps=nsteps, seed=seed, bias=0.5)
+
+
+# --------------------------------------------------------------- polycrystal
+def euler_from_matrix(Q):
+    """Euler angles in degrees (angle_alpha, angle_beta, angle_theta) whose
+    rotation matrix under use_Euler_angles = 1 (TotalEnergyDensity.cpp:167-178,
+    R33 = cos(beta), R31 = sin(alpha) sin(beta), R32 = -cos(alpha) sin(beta),
+    R13 = sin(beta) sin(theta), R23 = sin(beta) cos(theta)) equals Q.
+    Q has shape (..., 3, 3) and is a proper rotation."""
+    import numpy as np
+    Q = np.asarray(Q, dtype=np.float64)
+    b = np.arccos(np.clip(Q[..., 2, 2], -1.0, 1.0))
+    a = np.arctan2(Q[..., 2, 0], -Q[..., 2, 1])
+    t = np.arctan2(Q[..., 0, 2], Q[..., 1, 2])
+    return np.degrees(a), np.degrees(b), np.degrees(t)
+
+
+def random_rotations(rng, n):
+    """n Haar-uniform rotations of SO(3) from uniformly random unit
+    quaternions (normalised 4-D Gaussians), as (n, 3, 3) matrices."""
+    import numpy as np
+    q = rng.standard_normal((n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def polycrystal_fields(inp, n_grains=400, fe_fraction=0.7, alpha_pe=1.0e9, seed=7):
+    """Per-cell fields of a polycrystalline FE layer: the layer (cells whose
+    centres lie inside FE_lo..FE_hi) is tiled by the Voronoi cells of
+    n_grains seeded points, nearest point by physical distance with the
+    periodic image in x and y.  Each grain takes a Haar-random rotation,
+    written as the reference's Euler-angle fields in degrees
+    (use_Euler_angles = 1), and is ferroelectric (Landau alpha from the inputs)
+    or paraelectric (alpha_pe); round(fe_fraction * n_grains) grains are
+    ferroelectric.  Cells outside the layer keep zero angles and the inputs'
+    alpha.
+
+    Returns {"angle_alpha", "angle_beta", "angle_theta", "landau_alpha",
+    "grain"} as (nz, ny, nx) arrays (grain = -1 outside the layer)."""
+    import numpy as np
+    from scipy.spatial import cKDTree
+    nx, ny, nz = inp.n_cell
+    lo = np.array(inp.prob_lo, dtype=np.float64)
+    hi = np.array(inp.prob_hi, dtype=np.float64)
+    L = hi - lo
+    h = L / np.array([nx, ny, nz])
+    xc = lo[0] + (np.arange(nx) + 0.5) * h[0]
+    yc = lo[1] + (np.arange(ny) + 0.5) * h[1]
+    zc = lo[2] + (np.arange(nz) + 0.5) * h[2]
+    inz = (zc >= inp.FE_lo[2]) & (zc <= inp.FE_hi[2])
+    iny = (yc >= inp.FE_lo[1]) & (yc <= inp.FE_hi[1])
+    inx = (xc >= inp.FE_lo[0]) & (xc <= inp.FE_hi[0])
+    rng = np.random.default_rng(seed)
+    z0, z1 = max(inp.FE_lo[2], lo[2]), min(inp.FE_hi[2], hi[2])
+    pts = np.column_stack([rng.uniform(0.0, L[0], n_grains), rng.uniform(0.0, L[1], n_grains),
+                           rng.uniform(z0 - lo[2], z1 - lo[2], n_grains)])
+    Q = random_rotations(rng, n_grains)
+    n_fe = int(round(fe_fraction * n_grains))
+    is_fe = np.zeros(n_grains, dtype=bool)
+    is_fe[rng.permutation(n_grains)[:n_fe]] = True
+    ga, gb, gt = euler_from_matrix(Q)
+    galpha = np.where(is_fe, inp.alpha, alpha_pe)
+
+    # z is not periodic: a box far taller than the domain turns its wrap off
+    tree = cKDTree(pts, boxsize=[L[0], L[1], 4.0 * L[2] + 1.0])
+    grain = np.full((nz, ny, nx), -1, dtype=np.int32)
+    X, Y = np.meshgrid(xc - lo[0], yc - lo[1], indexing="xy")
+    sel2 = np.outer(iny, inx)
+    for k in np.nonzero(inz)[0]:
+        q = np.column_stack([X[sel2], Y[sel2], np.full(int(sel2.sum()), zc[k] - lo[2])])
+        _, g = tree.query(q, k=1)
+        layer = grain[k]
+        layer[sel2] = g
+    out = {"grain": grain}
+    inside = grain >= 0
+    gi = np.where(inside, grain, 0)
+    out["angle_alpha"] = np.where(inside, ga[gi], 0.0)
+    out["angle_beta"] = np.where(inside, gb[gi], 0.0)
+    out["angle_theta"] = np.where(inside, gt[gi], 0.0)
+    out["landau_alpha"] = np.where(inside, galpha[gi], inp.alpha)
+    return out
+
+
+def apply_fields(sim, fields):
+    """Sets the per-cell fields of polycrystal_fields on a FerroX (or oracle)
+    instance; P is left as initialised."""
+    for key, field in (("angle_alpha", "alpha"), ("angle_beta", "beta"), ("angle_theta", "theta"),
+                       ("landau_alpha", "landau_alpha")):
+        sim.set(field, fields[key])
+
+
+def mfim_poly_256(nsteps=500, seed=1) -> str:
+    """The polycrystalline precision-form geometry: mfim_256 with the Euler
+    convention of the grain rotations."""
+    return mfim_256(nsteps=nsteps, seed=seed) + "\nuse_Euler_angles = 1\n"

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <string>
 #include <vector>
+#include <unordered_map>
 
 #include "../../include/ferrox_mi.h"
 #include "fx_device.h"
@@ -82,6 +83,7 @@ struct fx_ctx {
     int msx = 0, msy = 0;
     int* rot_idx = nullptr;
     double* rot_tab = nullptr;
+    double* rot_atab = nullptr;
     double Ru[9];
     /* reductions */
     double* red = nullptr;
@@ -96,13 +98,15 @@ struct fx_ctx {
     int tail0 = -1;
     DLev* d_tail64 = nullptr;
     DLevF* d_tail32 = nullptr;
+    int tail_ts64 = 0, tail_ts32 = 0;          /* first tail level staged in LDS */
+    size_t tail_lds64 = 0, tail_lds32 = 0;     /* dynamic LDS of the tail kernel */
     /* FP32 correction hierarchy of the mixed-precision solve (precision = FP32_REFINE) */
     std::vector<DLevF> lev32;
     float* cg_work32 = nullptr;
     double* phi_alt = nullptr;     /* second phi buffer of the refinement update */
     /* host shadows */
     std::vector<uint8_t> h_mat;
-    std::vector<double> h_tphase, h_ang[3];
+    std::vector<double> h_tphase, h_ang[3], h_lalpha;
     bool has_sc = false;
     bool pxy_zero = true;
     /* run state (main.cpp:238-245) */
@@ -245,7 +249,9 @@ int build_levels(fx_ctx* x) {
         }
     }
     int rc;
-    if ((rc = dalloc(x, (void**)&x->cg_work, sizeof(double) * 5 * x->lev.back().N))) return rc;
+    const size_t Nb = x->lev.back().N;
+    const size_t bottom_work = std::max<size_t>(5 * Nb, Nb <= (size_t)fxk::BOTTOM_DIRECT_CELLS ? Nb * Nb + Nb : 0);
+    if ((rc = dalloc(x, (void**)&x->cg_work, sizeof(double) * bottom_work))) return rc;
     x->tail0 = -1;
     for (int l = (int)x->lev.size() - 1; l >= 1 && x->lev[l].N <= TAIL_MAX_CELLS; --l) x->tail0 = l;
     if (x->tail0 >= 0 && (int)x->lev.size() - x->tail0 < 2) x->tail0 = -1;
@@ -273,7 +279,7 @@ int build_levels(fx_ctx* x) {
             if ((rc = dalloc(x, (void**)&f.bz, sizeof(float) * (size_t)d.n0 * d.n1 * (d.n2 + 1)))) return rc;
             if (x->in.mg_fused_sweep && d.n0 >= 64 && d.n1 >= 8 && (rc = dalloc(x, (void**)&f.phi2, sizeof(float) * Nl))) return rc;
         }
-        if ((rc = dalloc(x, (void**)&x->cg_work32, sizeof(float) * 5 * x->lev.back().N))) return rc;
+        if ((rc = dalloc(x, (void**)&x->cg_work32, sizeof(float) * bottom_work))) return rc;
         if ((rc = dalloc(x, (void**)&x->phi_alt, sizeof(double) * x->N))) return rc;
     }
     return FX_OK;
@@ -454,6 +460,26 @@ int build_coefficients(fx_ctx* x) {
     }
     /* descriptors of the tail levels, read by the single-workgroup kernel */
     if (x->tail0 >= 0) {
+        /* stage as many of the tail's levels (the smallest ones) in LDS as
+         * fit, with the bottom solve's work arrays */
+        const size_t lmax = fxk::tail_lds_max();
+        const int nl = (int)x->lev.size() - x->tail0;
+        const size_t Nb = x->lev.back().N;
+        const size_t bottom_work = std::max<size_t>(5 * Nb, Nb <= (size_t)fxk::BOTTOM_DIRECT_CELLS ? Nb * Nb + Nb : 0);
+        auto values = [&](int ts) {
+            size_t v = bottom_work;
+            for (int t = ts; t < nl; ++t) v += fxk::tail_level_values(x->lev[x->tail0 + t].d);
+            return v;
+        };
+        x->tail_ts64 = nl; x->tail_lds64 = 0;
+        x->tail_ts32 = nl; x->tail_lds32 = 0;
+        for (int ts = nl - 1; ts >= 0; --ts) {
+            if (values(ts) * sizeof(double) <= lmax) { x->tail_ts64 = ts; x->tail_lds64 = values(ts) * sizeof(double); }
+            if (values(ts) * sizeof(float) <= lmax) { x->tail_ts32 = ts; x->tail_lds32 = values(ts) * sizeof(float); }
+        }
+        if (x->in.verbose)
+            std::printf("multigrid tail: levels %d.. in one workgroup, LDS limit %zu B, staged from tail level %d (%zu B FP64)\n",
+                        x->tail0, lmax, x->tail_ts64, x->tail_lds64);
         const int nt = (int)x->lev.size() - x->tail0;
         std::vector<DLev> t64(nt);
         for (int q = 0; q < nt; ++q) t64[q] = x->lev[x->tail0 + q].d;
@@ -478,39 +504,55 @@ int upload_mat(fx_ctx* x) {
     return FX_OK;
 }
 
-/* Rotation: a uniform matrix, or a table of the distinct (alpha,beta,theta)
- * triples and a per-cell index.  R is evaluated on the host with the
- * reference formula. */
+/* Rotation and Landau alpha: uniform, or a table of the distinct
+ * (angle_alpha, angle_beta, angle_theta, landau_alpha) rows with a per-cell
+ * index.  R is evaluated on the host with the reference formula. */
 int refresh_rotation(fx_ctx* x) {
     const size_t N = x->N;
     bool uniform = true;
     for (size_t c = 1; c < N && uniform; ++c)
-        uniform = x->h_ang[0][c] == x->h_ang[0][0] && x->h_ang[1][c] == x->h_ang[1][0] && x->h_ang[2][c] == x->h_ang[2][0];
+        uniform = x->h_ang[0][c] == x->h_ang[0][0] && x->h_ang[1][c] == x->h_ang[1][0] && x->h_ang[2][c] == x->h_ang[2][0] &&
+                  x->h_lalpha[c] == x->h_lalpha[0];
     rotation(x->h_ang[0][0], x->h_ang[1][0], x->h_ang[2][0], x->in.use_Euler_angles, x->Ru);
-    if (x->rot_idx) { hipFree(x->rot_idx); x->rot_idx = nullptr; }
-    if (x->rot_tab) { hipFree(x->rot_tab); x->rot_tab = nullptr; }
-    if (uniform) return FX_OK;
-    std::vector<int> idx(N);
-    std::vector<double> tab;
-    struct Key { double a, b, t; bool operator<(const Key& o) const { return a < o.a || (a == o.a && (b < o.b || (b == o.b && t < o.t))); } };
-    std::vector<std::pair<Key, int>> seen;
-    std::vector<std::pair<Key, size_t>> order(N);
-    for (size_t c = 0; c < N; ++c) order[c] = {Key{x->h_ang[0][c], x->h_ang[1][c], x->h_ang[2][c]}, c};
-    std::sort(order.begin(), order.end(), [](const std::pair<Key, size_t>& p, const std::pair<Key, size_t>& q) { return p.first < q.first; });
-    int next = -1;
-    for (size_t t = 0; t < N; ++t) {
-        if (t == 0 || order[t - 1].first < order[t].first) {
-            ++next;
-            double R[9];
-            rotation(order[t].first.a, order[t].first.b, order[t].first.t, x->in.use_Euler_angles, R);
-            tab.insert(tab.end(), R, R + 9);
+    if (x->rot_idx) { (void)hipFree(x->rot_idx); x->rot_idx = nullptr; }
+    if (x->rot_tab) { (void)hipFree(x->rot_tab); x->rot_tab = nullptr; }
+    if (x->rot_atab) { (void)hipFree(x->rot_atab); x->rot_atab = nullptr; }
+    if (uniform && x->h_lalpha[0] == x->in.alpha) return FX_OK;
+    /* rows in order of first appearance, keyed on the bit patterns */
+    struct Key {
+        uint64_t w[4];
+        bool operator==(const Key& o) const { return w[0] == o.w[0] && w[1] == o.w[1] && w[2] == o.w[2] && w[3] == o.w[3]; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key& k) const {
+            uint64_t h = 0x9e3779b97f4a7c15ull;
+            for (int q = 0; q < 4; ++q) { h ^= k.w[q] + 0x9e3779b97f4a7c15ull + (h << 6) + (h >> 2); }
+            return (size_t)h;
         }
-        idx[order[t].second] = next;
+    };
+    std::unordered_map<Key, int, KeyHash> rows;
+    std::vector<int> idx(N);
+    std::vector<double> tab, atab;
+    for (size_t c = 0; c < N; ++c) {
+        const double v[4] = {x->h_ang[0][c], x->h_ang[1][c], x->h_ang[2][c], x->h_lalpha[c]};
+        Key k;
+        std::memcpy(k.w, v, sizeof v);
+        auto it = rows.find(k);
+        if (it == rows.end()) {
+            it = rows.emplace(k, (int)atab.size()).first;
+            double R[9];
+            rotation(v[0], v[1], v[2], x->in.use_Euler_angles, R);
+            tab.insert(tab.end(), R, R + 9);
+            atab.push_back(v[3]);
+        }
+        idx[c] = it->second;
     }
     HIPCHK(x, hipMalloc((void**)&x->rot_idx, N * sizeof(int)));
     HIPCHK(x, hipMalloc((void**)&x->rot_tab, tab.size() * sizeof(double)));
+    HIPCHK(x, hipMalloc((void**)&x->rot_atab, atab.size() * sizeof(double)));
     HIPCHK(x, hipMemcpyAsync(x->rot_idx, idx.data(), N * sizeof(int), hipMemcpyHostToDevice, x->s));
     HIPCHK(x, hipMemcpyAsync(x->rot_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
+    HIPCHK(x, hipMemcpyAsync(x->rot_atab, atab.data(), atab.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
     HIPCHK(x, hipStreamSynchronize(x->s));
     return FX_OK;
 }
@@ -533,7 +575,7 @@ PBC pbc_of(const fx_ctx* x) {
 }
 RotArgs rot_of(const fx_ctx* x) {
     RotArgs r;
-    r.idx = x->rot_idx; r.tab = x->rot_tab;
+    r.idx = x->rot_idx; r.tab = x->rot_tab; r.atab = x->rot_atab;
     std::memcpy(r.Ru, x->Ru, sizeof(r.Ru));
     return r;
 }
@@ -571,7 +613,7 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
     const int inh = li == 0;
     if ((int)li == x->tail0) {
         fxk::mg_tail(x->s, x->d_tail64, (int)(x->lev.size() - li), bc, L.d.read_a, x->in.mg_nu1, x->in.mg_nu2,
-                     x->cg_work, BOTTOM_TOL, BOTTOM_MAXIT, x->d_iters);
+                     x->cg_work, BOTTOM_TOL, BOTTOM_MAXIT, x->d_iters, x->tail_ts64, x->tail_lds64);
         return;
     }
     if (li + 1 == x->lev.size()) {
@@ -588,8 +630,7 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
     Level& C = x->lev[li + 1];
     fxk::mg_resid_restrict(x->s, L.d, C.d, bc, inh);
     vcycle(x, li + 1, bc);
-    fxk::mg_prolong(x->s, L.d, C.d, bc);
-    fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu2, x->in.mg_fused_sweep);
+    fxk::mg_prolong_sweep(x->s, L.d, C.d, bc, inh, x->in.mg_nu2, x->in.mg_fused_sweep);
 }
 
 /* Which levels read the A-coefficient: 0 none, 1 the finest level only in
@@ -606,7 +647,7 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
     DLevF& L = x->lev32[li];
     if ((int)li == x->tail0) {
         fxk::mg_tail(x->s, x->d_tail32, (int)(x->lev32.size() - li), bc, L.read_a, x->in.mg_nu1, x->in.mg_nu2,
-                     x->cg_work32, 1e-5, BOTTOM_MAXIT, x->d_iters);
+                     x->cg_work32, 1e-5, BOTTOM_MAXIT, x->d_iters, x->tail_ts32, x->tail_lds32);
         return;
     }
     if (li + 1 == x->lev32.size()) {
@@ -617,8 +658,7 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
     DLevF& C = x->lev32[li + 1];
     fxk::mg_resid_restrict(x->s, L, C, bc);
     vcycle32(x, li + 1, bc, true);
-    fxk::mg_prolong(x->s, L, C, bc);
-    fxk::mg_sweep(x->s, L, bc, x->in.mg_nu2, 0, x->in.mg_fused_sweep);
+    fxk::mg_prolong_sweep(x->s, L, C, bc, x->in.mg_nu2, x->in.mg_fused_sweep);
 }
 
 /* MLMG-style solve (stopping rule ||r||_inf <= tol*max(||b||_inf, ||r0||_inf),
@@ -822,7 +862,7 @@ void free_all(fx_ctx* x) {
     auto F = [](void* p) { if (p) hipFree(p); };
     for (int d = 0; d < 3; ++d) { F(x->P[d]); F(x->Pn[d]); F(x->E[d]); F(x->GL[d]); F(x->GL1[d]); }
     F(x->phi); F(x->phi_old); F(x->phi_prev); F(x->rhs); F(x->rho); F(x->e_den); F(x->h_den);
-    F(x->alpha_cc); F(x->phidiff); F(x->eps); F(x->mat); F(x->rot_idx); F(x->rot_tab);
+    F(x->alpha_cc); F(x->phidiff); F(x->eps); F(x->mat); F(x->rot_idx); F(x->rot_tab); F(x->rot_atab);
     F(x->red); F(x->red_out); F(x->d_iters); F(x->cg_work); F(x->tmp);
     for (size_t l = 0; l < x->lev.size(); ++l) {
         DLev& d = x->lev[l].d;
@@ -909,6 +949,7 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
     if ((rc = build_levels(x))) { std::fprintf(stderr, "fx_create: %s\n", x->err.c_str()); free_all(x); delete x; return rc; }
     x->h_tphase.assign(x->N, 0.0);
     for (int d = 0; d < 3; ++d) x->h_ang[d].assign(x->N, 0.0);
+    x->h_lalpha.assign(x->N, in.alpha);
     mask_from_boxes(x, x->h_mat);
     periodic_fill_u8(x, x->h_mat);
     if ((rc = upload_mat(x)) || (rc = build_coefficients(x)) || (rc = refresh_rotation(x))) {
@@ -996,6 +1037,10 @@ int fx_get_field(fx_ctx* x, int f, double* host) {
     }
     if (f == FX_FIELD_ANGLE_ALPHA || f == FX_FIELD_ANGLE_BETA || f == FX_FIELD_ANGLE_THETA) {
         std::memcpy(host, x->h_ang[f - FX_FIELD_ANGLE_ALPHA].data(), N * sizeof(double));
+        return FX_OK;
+    }
+    if (f == FX_FIELD_LANDAU_ALPHA) {
+        std::memcpy(host, x->h_lalpha.data(), N * sizeof(double));
         return FX_OK;
     }
     if (f == FX_FIELD_GAMMA) {
@@ -1086,6 +1131,9 @@ int fx_set_field(fx_ctx* x, int f, const double* host) {
     }
     case FX_FIELD_ANGLE_ALPHA: case FX_FIELD_ANGLE_BETA: case FX_FIELD_ANGLE_THETA:
         std::memcpy(x->h_ang[f - FX_FIELD_ANGLE_ALPHA].data(), host, B);
+        return refresh_rotation(x);
+    case FX_FIELD_LANDAU_ALPHA:
+        std::memcpy(x->h_lalpha.data(), host, B);
         return refresh_rotation(x);
     default:
         return fail(x, FX_EINVAL, "field is not settable");

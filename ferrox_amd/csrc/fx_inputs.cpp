@@ -344,7 +344,7 @@ int fx_inputs_default(fx_inputs* in) {
     in->bias_ramp_steps = 0;
     in->bias_ramp_start = 0.0; in->bias_ramp_end = 0.0;
     in->verbose = 0;
-    in->mg_fused_sweep = 0;
+    in->mg_fused_sweep = 1;
     return FX_OK;
 }
 

@@ -11,10 +11,12 @@ struct SCParams {
     int fermi_dirac;
 };
 
-/* Rotation: uniform R, or a table of R indexed per cell. */
+/* Rotation and Landau alpha: uniform, or a table of (R, alpha) rows indexed
+ * per cell. */
 struct RotArgs {
     const int* idx;        /* null: uniform */
-    const double* tab;
+    const double* tab;     /* 9 doubles of R per row */
+    const double* atab;    /* alpha per row */
     double Ru[9];
 };
 
@@ -91,6 +93,12 @@ void compute_E(hipStream_t s, const EArgs& a);
 void mg_smooth(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int color);
 /* nsweeps full red-black sweeps (fused kernel where the level allows it) */
 void mg_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused);
+/* prolongation of the coarse correction c into l followed by nsweeps sweeps
+ * (the prolongation folded into the first one-pass sweep where it applies) */
+void mg_prolong_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, const fxd::DBC& bc, int inh, int nsweeps,
+                      int allow_fused);
+void mg_prolong_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nsweeps,
+                      int allow_fused);
 bool fused_ok(const fxd::DLev& l, const fxd::DBC& bc);
 void mg_resid_restrict(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc, int inh);
 void mg_prolong(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc);
@@ -110,17 +118,19 @@ void mg_prolong(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const f
 void mg_restrict_cells(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const float* src, float* dst);
 void bottom_cg(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, const float* rhs, float* e, float* work,
                double tol, int maxit, int* iters);
-/* phi_new = phi + e (e may be null), r32 = (float)(b - A phi_new) in FP64;
- * out[0] = max|r|, out[1] = max|b| */
-void refine_resid(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const float* e, double* phi_new, float* r32,
-                  double* part, double* out);
 void d2f(hipStream_t s, float* d, const double* src, long long n);
 void add_f2d(hipStream_t s, double* phi, const float* e, long long n);
 /* whole sub-V-cycle of the levels levs[0..nl-1] (device array) in one workgroup */
 void mg_tail(hipStream_t s, const fxd::DLev* levs, int nl, const fxd::DBC& bc, int read_a, int nu1, int nu2, double* work,
-             double tol, int maxit, int* iters);
+             double tol, int maxit, int* iters, int ts, size_t lds_bytes);
 void mg_tail(hipStream_t s, const fxd::DLevF* levs, int nl, const fxd::DBC& bc, int read_a, int nu1, int nu2, float* work,
-             double tol, int maxit, int* iters);
+             double tol, int maxit, int* iters, int ts, size_t lds_bytes);
+/* largest dynamic LDS per workgroup on the current device */
+size_t tail_lds_max();
+/* LDS values (of the level's type) the tail kernel stages for a level */
+size_t tail_level_values(const fxd::DLev& l);
+/* bottom solves of more cells than this use Jacobi-PCG, below it the exact dense solve */
+constexpr int BOTTOM_DIRECT_CELLS = 64;
 /* r32 = (float)(b - A phi) in FP64 (inhomogeneous BCs); out[0] = max|r|, out[1] = max|b| */
 void resid_to32(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, float* r32, double* part, double* out);
 }  // namespace fxk

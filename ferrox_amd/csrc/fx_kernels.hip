@@ -250,7 +250,8 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
         double L = 0.0, G = 0.0, El = 0.0;
         if (a.incL == 1) {
             const double r2 = r * r;
-            const double dF = a.alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
+            const double alpha = a.rot.idx ? a.rot.atab[a.rot.idx[c]] : a.alpha;
+            const double dF = alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
             L = -1.0 * Gam * dF;
         }
         if (a.incG == 1) G = -1.0 * Gam * grad_r(Fr, a.M, i, j, k, dx, a.pbc, R, a.g11, a.g44 - a.g44_p);
@@ -273,17 +274,18 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
     if (a.incL == 1) {
         const double p2 = p * p, q2 = q * q, r2 = r * r;
         const double a12 = a.a12, a112 = a.a112, a123 = a.a123;
-        const double dFp = a.alpha * p + a.beta * (p2 * p) + a.gamma * (p2 * p2 * p)
+        const double al = a.rot.idx ? a.rot.atab[a.rot.idx[c]] : a.alpha;
+        const double dFp = al * p + a.beta * (p2 * p) + a.gamma * (p2 * p2 * p)
                          + 2. * a12 * p * q2 + 2. * a12 * p * r2
                          + 4. * a112 * (p2 * p) * (q2 + r2)
                          + 2. * a112 * p * (q2 * q2) + 2. * a112 * p * (r2 * r2)
                          + 2. * a123 * p * q2 * r2;
-        const double dFq = a.alpha * q + a.beta * (q2 * q) + a.gamma * (q2 * q2 * q)
+        const double dFq = al * q + a.beta * (q2 * q) + a.gamma * (q2 * q2 * q)
                          + 2. * a12 * q * p2 + 2. * a12 * q * r2
                          + 4. * a112 * (q2 * q) * (p2 + r2)
                          + 2. * a112 * q * (p2 * p2) + 2. * a112 * q * (r2 * r2)
                          + 2. * a123 * q * p2 * r2;
-        const double dFr = a.alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r)
+        const double dFr = al * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r)
                          + 2. * a12 * r * p2 + 2. * a12 * r * q2
                          + 4. * a112 * (r2 * r) * (p2 + q2)
                          + 2. * a112 * r * (p2 * p2) + 2. * a112 * r * (q2 * q2)

@@ -3,18 +3,18 @@
  * from-scratch replacement of AMReX MLABecLaplacian + MLMG
  * (ElectrostaticSolver.cpp:562-611, 713-720).
  *
- *   k_mg_gsrb_fused  a full red-black Gauss-Seidel sweep in one pass: each
- *                    workgroup owns a 64x4 column tile and a z-chunk, keeps four
+ *   k_mg_gsrb_stream a full red-black Gauss-Seidel sweep in one pass: each
+ *                    workgroup owns a 64x8 column tile and a z-chunk, keeps five
  *                    z-planes of phi (with a 2-cell in-plane halo) in LDS and
- *                    streams upward, updating red on plane t and black on plane
- *                    t-1 -- the same values as a red sweep followed by a black
- *                    sweep, with phi, rhs and the material id read once and phi
- *                    written once.
+ *                    streams upward, updating red on plane p and black on plane
+ *                    p-1 -- the same values as a red sweep followed by a black
+ *                    sweep, with phi and rhs read once and phi written once.
  *   k_mg_smooth      one colour of the sweep (small levels, periodic z)
  *   k_mg_resid_restrict   residual averaged into the coarse rhs; coarse phi = 0
  *   k_mg_prolong     cell-centred linear prolongation added to the fine phi
- *   k_mg_resid_norm  max|r| and max|b| per workgroup
- *   k_bottom_cg      Jacobi-PCG on the coarsest level in one workgroup
+ *   k_mg_resid_max   residual with max|r| and max|b| per workgroup (fixed grid)
+ *   k_bottom         exact (dense) or Jacobi-PCG bottom solve in one workgroup
+ *   k_mg_tail        the coarse end of the V-cycle in one workgroup, staged in LDS
  * Launches use 3-D grids (x tiles of 64, y tiles of 4, z) so no thread does
  * 64-bit index division.
  */
@@ -22,6 +22,7 @@
 #include "fx_kernels.h"
 
 #include <algorithm>
+#include <type_traits>
 
 using namespace fxd;
 
@@ -49,97 +50,322 @@ __global__ __launch_bounds__(256) void k_mg_smooth(DLevT<T> l, DBC bc, int inh, 
     l.phi[c] = (l.b[c] + S) / D;
 }
 
-/* ---------------------- fused red + black sweep --------------------------- */
+/* ------------------- one-pass red + black sweep (streaming) --------------- */
 namespace {
-constexpr int FTX = 64, FTY = 4, FKZ = 8;          /* interior block: 64 x 4 columns, 8 planes */
-constexpr int HX = FTX + 4, HY = FTY + 4, HZ = FKZ + 4;
+constexpr int STX = 64, STY = 8;                  /* output tile of columns */
+constexpr int SHX = STX + 4, SHY = STY + 4;       /* with the 2-cell in-plane halo */
+constexpr int SPL = SHX * SHY;                    /* LDS values per plane */
+constexpr int SNS = 5;                            /* plane slots in LDS */
+constexpr int SLD = (SPL + 255) / 256;            /* plane loads per thread */
+constexpr int SCZ_MAX = 64;                       /* largest z-chunk */
+constexpr int NRED = (STY + 2) * (STX / 2 + 1);   /* red cells of the tile and its ring */
+constexpr int NPW = 6;                            /* per-plane table: wx wy wzl wzh D 1/D */
 }
 
-/* One full red-black sweep in one pass, from phi_in to phi_out (two buffers,
- * so no workgroup can read a neighbour's already-updated values).  A
- * workgroup loads its 64x4x8 block of phi with a 2-cell halo in x, y and z
- * into LDS (every load in flight at once), updates red over the block and a
- * 1-cell ring (planes k0-1..k1) from the old black values, then black over
- * the interior from the new red values, and stores the interior.  Each cell
- * is computed from the same values as by a red launch followed by a black
- * launch; the ring's red values are recomputed, identically, by the
- * neighbouring block.  Requires z not periodic, n0 >= 64, n1 >= 8. */
+template <class T> struct Vec2;
+template <> struct Vec2<double> { using type = double2; };
+template <> struct Vec2<float> { using type = float2; };
+
+/* Arguments of the streaming sweep: only what it reads, so the kernel
+ * argument block stays small. */
 template <class T>
-__global__ __launch_bounds__(256) void k_mg_gsrb_fused(DLevT<T> l, DBC bc, int inh, int zero_init,
-                                                       const T* __restrict__ phi_in, T* __restrict__ phi_out) {
-    __shared__ T sp[HZ * HY * HX];
-    const int n0 = l.n0, n1 = l.n1, n2 = l.n2;
-    const int tx0 = blockIdx.x * FTX, ty0 = blockIdx.y * FTY, k0 = blockIdx.z * FKZ;
-    const int k1 = min(k0 + FKZ, n2);
+struct StreamArgs {
+    int n0, n1, n2, msx, msy;
+    const T* __restrict__ b;
+    const T* __restrict__ a;
+    const uint8_t* __restrict__ mat;
+    const T* __restrict__ bxk;
+    const T* __restrict__ byk;
+    const T* __restrict__ bzk;
+    T ih2[3];
+    DBC bc;
+    int inh, zero_init;
+    const T* __restrict__ phi_in;
+    T* __restrict__ phi_out;
+    int ntx, nty, cz, remap;
+    const T* __restrict__ ec;   /* PROLONG: coarse correction (z-coarsened level, same n0 x n1) */
+    int cn2;                    /* its number of planes */
+};
+
+namespace {
+template <class T>
+struct StreamRegs {
+    T phi[SLD];      /* next phi plane (raw loads; out-of-domain lanes masked when stored to LDS) */
+    T ck[SLD], cj[SLD];   /* PROLONG: its parent coarse plane and the parent's neighbour on its side */
+    T b[3];          /* rhs of red cells u = 0, 1 and of the black cell */
+    T a[3];          /* Helmholtz term of the same cells */
+    uint8_t m[3];    /* their material id */
+};
+/* geometry of one of this thread's cells, fixed for the whole sweep */
+struct StreamCell {
+    int q;           /* LDS index in a plane */
+    int c2;          /* in-plane global index (wrapped) */
+    int g2;          /* in-plane index on the grown material box */
+    int i, j;        /* wrapped coordinates */
+    bool ok;         /* a cell of the domain this thread updates */
+    bool lat;        /* away from every non-periodic lateral face */
+};
+}
+
+/* One full red-black Gauss-Seidel sweep, phi_in -> phi_out (two buffers, so a
+ * workgroup never reads values another workgroup has already updated).
+ *
+ * A workgroup owns a 64x8 column tile and the planes [z0, z1) (z0 even), and
+ * marches upward: at step p it updates red on plane p over the tile and a
+ * 1-cell ring (from old black values), then black on plane p-1 over the tile
+ * (from the new red values of planes p-2, p-1, p), and stores plane p-1.  The
+ * ring's red values are the ones the neighbouring workgroup computes, from
+ * the same old values, so every cell gets the value of a red launch followed
+ * by a black launch (to rounding: interior cells multiply by a per-plane 1/D).
+ * Old phi is staged in LDS one plane at a time with a 2-cell in-plane halo
+ * (five plane slots, so two barriers per plane suffice).  Every global load
+ * of step p+1 (the next phi plane, rhs and the Helmholtz term of this
+ * thread's three cells) is issued during step p into the other half of a
+ * register ping-pong; the loop is unrolled by two so the plane parity, and
+ * with it each thread's red and black cells, is known at compile time.
+ * Black values are never read again by this workgroup, so they go straight
+ * to HBM with the red value of the same pair.
+ *
+ * AMODE: 0 no Helmholtz term; 1 a[c] in semiconductor cells (material id);
+ * 2 a[c] everywhere.  Coefficients must be layered (per-plane face values).
+ * Tiles are numbered x fastest, then y, then z-chunk; with remap the
+ * workgroup id is permuted so that each of the eight XCDs (which receive
+ * workgroups round-robin) works on one contiguous range of tiles and the
+ * in-plane halos are shared in that XCD's L2.  Requires: z not periodic, n0
+ * even, n0 >= 64, n1 >= 8, periodic extents even, cz even. */
+template <class T, int AMODE, int PROLONG>
+__global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
+    __shared__ T sp[SNS * SPL];
+    __shared__ T scw[NPW * (SCZ_MAX + 2)];
+    const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
+    int w = blockIdx.x;
+    if (A.remap) w = (w & 7) * (gridDim.x >> 3) + (w >> 3);
+    const int tx = w % A.ntx, rest = w / A.ntx, ty = rest % A.nty, tz = rest / A.nty;
+    const int tx0 = tx * STX, ty0 = ty * STY, z0 = tz * A.cz, z1 = min(z0 + A.cz, n2);
     const int tid = threadIdx.x;
-    const bool perx = bc.lo[0] == BC_PER, pery = bc.lo[1] == BC_PER;
-    auto wrapx = [&](int i) { return i < 0 ? i + n0 : (i >= n0 ? i - n0 : i); };
-    auto wrapy = [&](int j) { return j < 0 ? j + n1 : (j >= n1 ? j - n1 : j); };
-    /* 1. load phi (zero-initial: nothing to load, every value is 0) */
-    for (int p = tid; p < HZ * HY * HX; p += 256) {
-        const int hx = p % HX, hy = (p / HX) % HY, hz = p / (HX * HY);
-        const int i = tx0 + hx - 2, j = ty0 + hy - 2, k = k0 + hz - 2;
-        T v = T(0);
-        if (!zero_init && k >= 0 && k < n2 && (perx || (i >= 0 && i < n0)) && (pery || (j >= 0 && j < n1)))
-            v = phi_in[wrapx(i) + n0 * (wrapy(j) + n1 * k)];
-        sp[p] = v;
+    const bool perx = A.bc.lo[0] == BC_PER, pery = A.bc.lo[1] == BC_PER;
+    const int plane = n0 * n1;
+    const int gplane = A.msy;
+
+    /* per-plane weights beta/h^2 (the products of mg_coef's layered branch),
+     * the interior diagonal accumulated in mg_acc's order, and its inverse */
+    for (int e = tid; e < z1 - z0 + 2; e += 256) {
+        const int k = z0 - 1 + e;
+        T wx = T(0), wy = T(0), wzl = T(0), wzh = T(0);
+        if (k >= 0 && k < n2) {
+            wx = A.bxk[k] * A.ih2[0];
+            wy = A.byk[k] * A.ih2[1];
+            wzl = A.bzk[k] * A.ih2[2];
+            wzh = A.bzk[k + 1] * A.ih2[2];
+        }
+        T D = T(0);
+        D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+        T* t = scw + NPW * e;
+        t[0] = wx; t[1] = wy; t[2] = wzl; t[3] = wzh; t[4] = D;
+        t[5] = D != T(0) ? T(1) / D : T(0);
     }
-    __syncthreads();
-    auto update = [&](int hx, int hy, int hz) {
-        const int i = tx0 + hx - 2, j = ty0 + hy - 2, k = k0 + hz - 2;
-        const int iw = wrapx(i), jw = wrapy(j);
-        const int c = iw + n0 * (jw + n1 * k);
-        const int p = hx + HX * (hy + HY * hz);
-        T a, w[6], nb[6];
-        mg_coef(l, bc, iw, jw, k, c, a, w);
-        nb[0] = sp[p - 1];
-        nb[1] = sp[p + 1];
-        nb[2] = sp[p - HX];
-        nb[3] = sp[p + HX];
-        nb[4] = sp[p - HX * HY];
-        nb[5] = sp[p + HX * HY];
-        T D, S;
-        mg_acc(l, bc, inh != 0, iw, jw, k, a, w, nb, D, S);
-        sp[p] = (l.b[c] + S) / D;
+
+    /* plane loads: LDS index tid + 256 s */
+    int goff[SLD];
+#pragma unroll
+    for (int s = 0; s < SLD; ++s) {
+        const int q = tid + 256 * s;
+        goff[s] = -1;
+        if (q < SPL) {
+            int i = tx0 + q % SHX - 2, j = ty0 + q / SHX - 2;
+            bool ok = true;
+            if (i < 0) { if (perx) i += n0; else ok = false; }
+            else if (i >= n0) { if (perx) i -= n0; else ok = false; }
+            if (j < 0) { if (pery) j += n1; else ok = false; }
+            else if (j >= n1) { if (pery) j -= n1; else ok = false; }
+            if (ok) goff[s] = i + n0 * j;
+        }
+    }
+    auto make_cell = [&](int hx, int hy, bool valid) {
+        StreamCell c;
+        int i = tx0 + hx - 2, j = ty0 + hy - 2;
+        bool ok = valid && i <= n0 && j <= n1;
+        if (i < 0) { if (perx) i += n0; else ok = false; }
+        else if (i >= n0) { if (perx) i -= n0; else ok = false; }
+        if (j < 0) { if (pery) j += n1; else ok = false; }
+        else if (j >= n1) { if (pery) j -= n1; else ok = false; }
+        c.q = hx + SHX * hy;
+        c.i = i; c.j = j;
+        c.c2 = ok ? i + n0 * j : 0;
+        c.g2 = ok ? (i + 1) + A.msx * (j + 1) : 0;
+        c.ok = ok;
+        c.lat = (perx || (i > 0 && i < n0 - 1)) && (pery || (j > 0 && j < n1 - 1));
+        return c;
     };
-    /* 2. red over planes k0-1..k1 and the 1-cell ring: 33 red cells per ring row */
-    {
-        const int rows = (FTY + 2) * (FKZ + 2);
-        for (int q = tid; q < rows * 33; q += 256) {
-            const int r = q / 33, x2 = q % 33;
-            const int hy = 1 + r % (FTY + 2), hz = 1 + r / (FTY + 2);
-            const int j = ty0 + hy - 2, k = k0 + hz - 2;
-            const int i0 = tx0 - 1;                           /* hx = 1 */
-            const int hx = 1 + 2 * x2 + ((i0 + j + k) & 1);
-            const int i = tx0 + hx - 2;
-            if (hx > FTX + 2 || k < 0 || k >= n2) continue;
-            if (!perx && (i < 0 || i >= n0)) continue;
-            if (!pery && (j < 0 || j >= n1)) continue;
-            if (i > n0 || j > n1) continue;
-            update(hx, hy, hz);
+    /* red cells u = 0, 1 (positions tid and tid + 256 of the 10 rows x 33 red
+     * cells of the tile and its ring) for planes of parity 0 and 1; the black
+     * cell (one of the pair 2*x2, 2*x2+1 of tile row tid/32) for planes of
+     * parity 0 and 1 */
+    StreamCell red[2][2], blk[2];
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int qn = tid + 256 * u;
+            const bool valid = qn < NRED;
+            const int qq = valid ? qn : 0;
+            const int hy = 1 + qq / (STX / 2 + 1), x2 = qq % (STX / 2 + 1);
+            const int hx = 1 + 2 * x2 + ((tx0 - 1 + ty0 + hy - 2 + par) & 1);
+            red[par][u] = make_cell(hx, hy, valid && hx <= STX + 2);
         }
+        const int bhy = 2 + (tid >> 5), bx2 = tid & 31;
+        const int off = (ty0 + bhy - 2 + par + 1) & 1;
+        blk[par] = make_cell(2 + 2 * bx2 + off, bhy, ty0 + bhy - 2 < n1 && tx0 + 2 * bx2 < n0);
     }
-    __syncthreads();
-    /* 3. black over the interior planes k0..k1-1: 32 black cells per row */
-    {
-        const int rows = FTY * FKZ;
-        for (int q = tid; q < rows * 32; q += 256) {
-            const int r = q / 32, x2 = q % 32;
-            const int hy = 2 + r % FTY, hz = 2 + r / FTY;
-            const int j = ty0 + hy - 2, k = k0 + hz - 2;
-            const int hx = 2 + 2 * x2 + ((tx0 + j + k + 1) & 1);
-            const int i = tx0 + hx - 2;
-            if (k >= k1 || i >= n0 || j >= n1) continue;
-            update(hx, hy, hz);
+    const int bhy = 2 + (tid >> 5), bx2 = tid & 31;
+    const int bj = ty0 + bhy - 2, bi0 = tx0 + 2 * bx2;
+    const bool bstore = bj < n1 && bi0 < n0;
+
+    const T* ap = AMODE ? A.a : A.b;
+    const uint8_t* mp = AMODE == 1 ? A.mat : reinterpret_cast<const uint8_t*>(A.b);
+    /* the loads of step p (plane parity PAR): phi plane p+1, red cells of
+     * plane p, black cell of plane p-1.  Every load is unconditional (clamped
+     * to a valid address, masked where used). */
+    auto issue = [&](StreamRegs<T>& R, int p, auto par_tag) {
+        constexpr int PAR = decltype(par_tag)::value;
+        const int kf = min(max(p + 1, 0), n2 - 1);
+        const int kof = plane * kf;
+#pragma unroll
+        for (int s = 0; s < SLD; ++s) R.phi[s] = A.phi_in[(goff[s] >= 0 ? goff[s] : 0) + kof];
+        if (PROLONG) {
+            const int K = kf >> 1;
+            const int J = min(max((kf & 1) ? K + 1 : K - 1, 0), A.cn2 - 1);
+#pragma unroll
+            for (int s = 0; s < SLD; ++s) {
+                const int g = goff[s] >= 0 ? goff[s] : 0;
+                R.ck[s] = A.ec[g + plane * K];
+                R.cj[s] = A.ec[g + plane * J];
+            }
         }
+        const int kr = min(max(p, 0), n2 - 1), kb = min(max(p - 1, 0), n2 - 1);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int c = red[PAR][u].c2 + plane * kr;
+            R.b[u] = A.b[c];
+            if (AMODE) R.a[u] = ap[c];
+            if (AMODE == 1) R.m[u] = mp[red[PAR][u].g2 + gplane * (kr + 1)];
+        }
+        const int c = blk[1 - PAR].c2 + plane * kb;
+        R.b[2] = A.b[c];
+        if (AMODE) R.a[2] = ap[c];
+        if (AMODE == 1) R.m[2] = mp[blk[1 - PAR].g2 + gplane * (kb + 1)];
+    };
+    auto a_use = [&](const StreamRegs<T>& R, int u) -> T {
+        if (AMODE == 0) return T(0);
+        if (AMODE == 2) return R.a[u];
+        return ((R.m[u] & 3) >= 2) ? R.a[u] : T(0);
+    };
+    auto put_plane = [&](const StreamRegs<T>& R, int k) {
+        T* d = sp + ((k - z0 + 2) % SNS) * SPL;
+        const bool kin = !A.zero_init && k >= 0 && k < n2;
+        /* PROLONG: plane k of phi + P e, with the weights and sum order of
+         * k_mg_prolong for a z-only coarsening (parent 3/4, neighbour 1/4;
+         * across a z face the neighbour is the homogeneous ghost) */
+        T wj = T(0);
+        if (PROLONG) {
+            const int K = k >> 1;
+            const int J = (k & 1) ? K + 1 : K - 1;
+            T sg = T(1);
+            if (J < 0 || J >= A.cn2) sg = ((J < 0 ? A.bc.lo[2] : A.bc.hi[2]) == BC_DIR) ? T(-1) : T(1);
+            wj = T(0.25) * sg;
+        }
+#pragma unroll
+        for (int s = 0; s < SLD; ++s) {
+            const int q = tid + 256 * s;
+            T v = R.phi[s];
+            if (PROLONG) {
+                T e = T(0);
+                e += T(0.75) * R.ck[s];
+                e += wj * R.cj[s];
+                v += e;
+            }
+            if (q < SPL) d[q] = (kin && goff[s] >= 0) ? v : T(0);
+        }
+    };
+    auto slot = [&](int k) { return sp + ((k - z0 + 2) % SNS) * SPL; };
+    /* relaxation of one cell of plane k at LDS index q */
+    auto relax = [&](const StreamCell& cl, int k, const T* pl, const T* pm, const T* pu, T bv, T av) -> T {
+        const int q = cl.q;
+        const T* t = scw + NPW * (k - z0 + 1);
+        const T wx = t[0], wy = t[1], wzl = t[2], wzh = t[3];
+        const T n0v = pm[q - 1], n1v = pm[q + 1], n2v = pm[q - SHX], n3v = pm[q + SHX], n4v = pl[q], n5v = pu[q];
+        if (cl.lat && k > 0 && k < n2 - 1) {
+            T S = T(0);
+            S += wx * n0v; S += wx * n1v; S += wy * n2v; S += wy * n3v; S += wzl * n4v; S += wzh * n5v;
+            if (AMODE == 0) return (bv + S) * t[5];
+            T D = -av;
+            D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+            return (bv + S) / D;
+        }
+        DLevT<T> lv;
+        lv.n0 = n0; lv.n1 = n1; lv.n2 = n2;
+        const T wc[6] = {wx, wx, wy, wy, wzl, wzh};
+        const T nb[6] = {n0v, n1v, n2v, n3v, n4v, n5v};
+        T D, S;
+        mg_acc(lv, A.bc, A.inh != 0, cl.i, cl.j, k, av, wc, nb, D, S);
+        return (bv + S) / D;
+    };
+    /* the black plane computed by the previous step, stored one step late so
+     * that its completion is never waited for right after it is issued */
+    using T2 = typename Vec2<T>::type;
+    T2 pv;
+    T* pptr = nullptr;
+    bool pend = false;
+    auto step = [&](int p, StreamRegs<T>& cur, StreamRegs<T>& nxt, auto par_tag) {
+        constexpr int PAR = decltype(par_tag)::value;
+        /* cur's loads and the store were issued a step ago: wait for them
+         * here, before this step issues its own loads.  (s_waitcnt immediate:
+         * vmcnt [3:0] = 0, expcnt [6:4] = 7 and lgkmcnt [11:8] = 15 leave those
+         * counters unconstrained.) */
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        put_plane(cur, p + 1);
+        issue(nxt, p + 1, std::integral_constant<int, 1 - PAR>{});
+        if (pend) *reinterpret_cast<T2*>(pptr) = pv;
+        __syncthreads();
+        if (p >= 0 && p < n2) {       /* red on plane p */
+            const T* pl = slot(p - 1);
+            T* pm = slot(p);
+            const T* pu = slot(p + 1);
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const StreamCell& cl = red[PAR][u];
+                if (cl.ok) pm[cl.q] = relax(cl, p, pl, pm, pu, cur.b[u], a_use(cur, u));
+            }
+        }
+        __syncthreads();
+        const int k = p - 1;          /* black on plane p-1 (parity 1 - PAR) */
+        pend = k >= z0 && bstore;
+        if (pend) {
+            const StreamCell& cl = blk[1 - PAR];
+            const T* pm = slot(k);
+            const T vb = relax(cl, k, slot(k - 1), pm, slot(k + 1), cur.b[2], a_use(cur, 2));
+            const int ob = (bj + k + 1) & 1;
+            const T vr = pm[cl.q + (ob ? -1 : 1)];
+            pv.x = ob ? vr : vb;
+            pv.y = ob ? vb : vr;
+            pptr = A.phi_out + bi0 + n0 * bj + (long long)plane * k;
+        }
+    };
+
+    StreamRegs<T> R0, R1;
+    /* planes z0-2 and z0-1 straight into LDS; step z0-1's loads into R0 */
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    issue(R0, z0 - 3, P1{});
+    put_plane(R0, z0 - 2);
+    issue(R0, z0 - 2, P0{});
+    put_plane(R0, z0 - 1);
+    issue(R0, z0 - 1, P1{});
+    for (int p = z0 - 1; p <= z1; p += 2) {    /* z0 even: p odd, p + 1 even */
+        step(p, R0, R1, P1{});
+        if (p + 1 <= z1) step(p + 1, R1, R0, P0{});
     }
-    __syncthreads();
-    /* 4. store the interior */
-    for (int q = tid; q < FTX * FTY * FKZ; q += 256) {
-        const int hx = 2 + q % FTX, hy = 2 + (q / FTX) % FTY, hz = 2 + q / (FTX * FTY);
-        const int i = tx0 + hx - 2, j = ty0 + hy - 2, k = k0 + hz - 2;
-        if (i < n0 && j < n1 && k < k1) phi_out[i + n0 * (j + n1 * k)] = sp[hx + HX * (hy + HY * hz)];
-    }
+    if (pend) *reinterpret_cast<T2*>(pptr) = pv;
 }
 
 /* --------------------- residual + restriction ----------------------------- */
@@ -225,28 +451,46 @@ __global__ __launch_bounds__(256) void k_mg_restrict_cells(DLevT<T> f, DLevT<T> 
     dst[I + cl.n0 * (J + cl.n1 * K)] = s / T((1 + cx) * (1 + cy) * (1 + cz));
 }
 
-/* Residual (optionally stored) and per-workgroup max|r| and max|b|, one
- * cell per thread.  Partials: [max|r| of every block][max|b| of every block]. */
-__global__ __launch_bounds__(256) void k_mg_resid_norm(DLev l, DBC bc, int inh, double* __restrict__ r, double* __restrict__ part) {
+/* r = b - A phi (inhomogeneous BCs) over the finest level with max|r| and
+ * max|b| per workgroup.  A fixed grid of workgroups strides over bricks of
+ * 64 x 4 columns x ZC planes (the ZC planes of a column unrolled, so their
+ * loads are in flight together); the few partials are finished by
+ * k_final1024.  (A last-workgroup-done atomic would serialise thousands of
+ * same-address atomics at the L2.)  STORE: 0 nothing, 1 r in FP64, 2 r in
+ * FP32 (the right-hand side of an FP32 correction solve). */
+template <int STORE, int ZC>
+__global__ __launch_bounds__(256) void k_mg_resid_max(DLev l, DBC bc, int inh, double* __restrict__ r,
+                                                      float* __restrict__ r32, double* __restrict__ part) {
     __shared__ double sh[256 / 64 + 1];
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int bx = (l.n0 + 63) / 64, by = (l.n1 + 3) / 4, bz = (l.n2 + ZC - 1) / ZC;
+    const int nbrick = bx * by * bz;
     double mr = 0.0, mb = 0.0;
-    const int k = blockIdx.z;
-    if (i < l.n0 && j < l.n1) {
-        double D, S;
-        mg_row(l, l.phi, i, j, k, bc, inh != 0, D, S);
-        const int c = i + l.n0 * (j + l.n1 * k);
-        const double rr = l.b[c] - (D * l.phi[c] - S);
-        if (r) r[c] = rr;
-        mr = fabs(rr);
-        mb = fabs(l.b[c]);
+    for (int t = blockIdx.x; t < nbrick; t += gridDim.x) {
+        const int tx = t % bx, rest = t / bx, ty = rest % by, tz = rest / by;
+        const int i = tx * 64 + (threadIdx.x & 63);
+        const int j = ty * 4 + (threadIdx.x >> 6);
+        const int k0 = tz * ZC;
+        if (i < l.n0 && j < l.n1) {
+#pragma unroll
+            for (int kk = 0; kk < ZC; ++kk) {
+                const int k = k0 + kk;
+                if (k < l.n2) {
+                    double D, S;
+                    mg_row(l, l.phi, i, j, k, bc, inh != 0, D, S);
+                    const int c = i + l.n0 * (j + l.n1 * k);
+                    const double bb = l.b[c];
+                    const double rr = bb - (D * l.phi[c] - S);
+                    if (STORE == 1) r[c] = rr;
+                    if (STORE == 2) r32[c] = (float)rr;
+                    mr = nmax(mr, fabs(rr));
+                    mb = nmax(mb, fabs(bb));
+                }
+            }
+        }
     }
     mr = block_max<256>(mr, sh);
     mb = block_max<256>(mb, sh);
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if (threadIdx.x == 0) { part[bid] = mr; part[nb + bid] = mb; }
+    if (threadIdx.x == 0) { part[blockIdx.x] = mr; part[gridDim.x + blockIdx.x] = mb; }
 }
 
 /* out[q] = max (or sum) over b of part[q*nb + b], one workgroup of 1024. */
@@ -260,12 +504,92 @@ __global__ __launch_bounds__(1024) void k_final1024(const double* __restrict__ p
     }
 }
 
-/* Jacobi-preconditioned CG for A e = rhs (homogeneous BCs) in one workgroup.
- * work holds 5*N doubles (r, z, p, q, diag). */
-template <class T>
-__global__ __launch_bounds__(1024) void k_bottom_cg(DLevT<T> l, DBC bc, const T* __restrict__ rhs, T* __restrict__ e,
-                                                    T* __restrict__ work, double tol, int maxit, int* __restrict__ iters) {
-    constexpr int BS = 1024;
+/* ------------------------------- bottom solve ----------------------------- */
+/* Bottom solve of A e = rhs (homogeneous BCs) by one workgroup of BS threads,
+ * the device form of the oracle's bottom_solve: a bottom of at most
+ * BOTTOM_DIRECT_MAX cells is solved exactly -- the dense matrix assembled
+ * column by column from the row operator, Gaussian elimination with partial
+ * pivoting (first largest |pivot|; f = M[i][k]/M[k][k]; M[i][j] -= f*M[k][j]
+ * for j > k; y[i] -= f*y[k]) and back substitution summed in increasing
+ * column order -- and anything larger, or a singular operator (a pivot below
+ * a small fraction of the largest diagonal entry: no Dirichlet face), by
+ * Jacobi-PCG.  work: N*N + N values for the direct solve, 5*N for CG. */
+constexpr int BOTTOM_DIRECT_MAX = 64;
+constexpr int TAIL_MAXL = 16;
+
+template <class T, int BS>
+__device__ bool bottom_direct(const DLevT<T>& l, const DBC& bc, const T* rhs, T* e, T* work) {
+    __shared__ int piv_sh, ok_sh;
+    const int n0 = l.n0, n1 = l.n1, n2 = l.n2, s = n0 * n1;
+    const int N = s * n2;
+    T* M = work;
+    T* y = work + N * N;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < N * N; idx += BS) {
+        const int r = idx / N, col = idx % N;
+        const int i = r % n0, j = (r / n0) % n1, k = r / s;
+        const int nbi[6] = {i > 0 ? r - 1 : r + (n0 - 1), i < n0 - 1 ? r + 1 : r - (n0 - 1),
+                            j > 0 ? r - n0 : r + n0 * (n1 - 1), j < n1 - 1 ? r + n0 : r - n0 * (n1 - 1),
+                            k > 0 ? r - s : r + s * (n2 - 1), k < n2 - 1 ? r + s : r - s * (n2 - 1)};
+        T a, w[6], nb[6], D, S;
+        mg_coef(l, bc, i, j, k, r, a, w);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nb[q] = nbi[q] == col ? T(1) : T(0);
+        mg_acc(l, bc, false, i, j, k, a, w, nb, D, S);
+        M[idx] = (r == col ? D : T(0)) - S;
+    }
+    for (int r = tid; r < N; r += BS) y[r] = rhs[r];
+    __syncthreads();
+    T dmax = T(0);
+    if (tid == 0)
+        for (int r = 0; r < N; ++r) dmax = fmax(dmax, fabs(M[r * N + r]));
+    const T small = sizeof(T) == 8 ? T(1e-14) : T(1e-6);
+    for (int kk = 0; kk < N; ++kk) {
+        if (tid == 0) {
+            int p = kk;
+            T best = fabs(M[kk * N + kk]);
+            for (int i = kk + 1; i < N; ++i) {
+                const T v = fabs(M[i * N + kk]);
+                if (v > best) { best = v; p = i; }
+            }
+            piv_sh = p;
+            ok_sh = best > small * dmax;
+        }
+        __syncthreads();
+        if (!ok_sh) return false;
+        const int p = piv_sh;
+        if (p != kk) {
+            for (int jj = kk + tid; jj < N; jj += BS) {
+                const T t = M[kk * N + jj];
+                M[kk * N + jj] = M[p * N + jj];
+                M[p * N + jj] = t;
+            }
+            if (tid == 0) { const T t = y[kk]; y[kk] = y[p]; y[p] = t; }
+        }
+        __syncthreads();
+        const int rows = N - kk - 1, cols = N - kk;    /* column N stands for y */
+        for (int idx = tid; idx < rows * cols; idx += BS) {
+            const int i = kk + 1 + idx / cols, jj = kk + 1 + idx % cols;
+            const T f = M[i * N + kk] / M[kk * N + kk];
+            if (jj < N) M[i * N + jj] = M[i * N + jj] - f * M[kk * N + jj];
+            else y[i] = y[i] - f * y[kk];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        for (int i = N - 1; i >= 0; --i) {
+            T sum = y[i];
+            for (int jj = i + 1; jj < N; ++jj) sum -= M[i * N + jj] * e[jj];
+            e[i] = sum / M[i * N + i];
+        }
+    }
+    __syncthreads();
+    return true;
+}
+
+template <class T, int BS>
+__device__ void bottom_cg_dev(const DLevT<T>& l, const DBC& bc, const T* rhs, T* e, T* work, double tol, int maxit,
+                              int* iters) {
     __shared__ T sh[BS / 64 + 1];
     const int N = l.n0 * l.n1 * l.n2;
     T* r = work;
@@ -305,11 +629,7 @@ __global__ __launch_bounds__(1024) void k_bottom_cg(DLevT<T> l, DBC bc, const T*
         pq = block_sum<BS>(pq, sh);
         const T al = rz / pq;
         rr = T(0);
-        for (int c = tid; c < N; c += BS) {
-            e[c] += al * p[c];
-            r[c] -= al * q[c];
-            rr += r[c] * r[c];
-        }
+        for (int c = tid; c < N; c += BS) { e[c] += al * p[c]; r[c] -= al * q[c]; rr += r[c] * r[c]; }
         rr = block_sum<BS>(rr, sh);
         if (rr <= tol2) { ++it; break; }
         T rzn = T(0);
@@ -320,23 +640,105 @@ __global__ __launch_bounds__(1024) void k_bottom_cg(DLevT<T> l, DBC bc, const T*
         for (int c = tid; c < N; c += BS) p[c] = z[c] + be * p[c];
         __syncthreads();
     }
-    if (tid == 0) *iters += it;
+    if (tid == 0 && iters) *iters += it;
+    __syncthreads();
+}
+
+template <class T, int BS>
+__device__ void bottom_solve_dev(const DLevT<T>& l, const DBC& bc, const T* rhs, T* e, T* work, double tol, int maxit,
+                                 int* iters) {
+    if (l.n0 * l.n1 * l.n2 <= BOTTOM_DIRECT_MAX && bottom_direct<T, BS>(l, bc, rhs, e, work)) return;
+    bottom_cg_dev<T, BS>(l, bc, rhs, e, work, tol, maxit, iters);
+}
+
+/* Bottom solve as a kernel (hierarchies without a tail). */
+template <class T>
+__global__ __launch_bounds__(1024) void k_bottom(DLevT<T> l, DBC bc, const T* __restrict__ rhs, T* __restrict__ e,
+                                                 T* __restrict__ work, double tol, int maxit, int* __restrict__ iters) {
+    bottom_solve_dev<T, 1024>(l, bc, rhs, e, work, tol, maxit, iters);
 }
 
 /* ------------------- coarse tail of the V-cycle in one workgroup ----------- */
+/* LDS values a staged tail level needs: phi, b, its Helmholtz term and its
+ * face coefficients (per-plane values when layered, face arrays otherwise). */
+template <class T>
+__host__ __device__ inline size_t fxd_tail_level_values(const DLevT<T>& l) {
+    const size_t n = (size_t)l.n0 * l.n1 * l.n2;
+    const size_t faces = l.layered ? (size_t)(3 * l.n2 + 1)
+                                   : (size_t)(l.n0 + 1) * l.n1 * l.n2 + (size_t)l.n0 * (l.n1 + 1) * l.n2 +
+                                         (size_t)l.n0 * l.n1 * (l.n2 + 1);
+    return 3 * n + faces;
+}
+
 /* Levels with few cells are launch-latency bound: the whole sub-V-cycle below
- * a given level (smoothing, residual + restriction, Jacobi-PCG bottom,
+ * a given level (smoothing, residual + restriction, bottom solve,
  * prolongation) runs here in one 1024-thread workgroup, with the same per-cell
  * arithmetic as the separate kernels.  levs[0] is the first tail level (its b
- * already restricted, its phi zero); levs[nl-1] is the bottom. */
+ * already restricted, its phi zero); levs[nl-1] is the bottom.  The level
+ * descriptors live in LDS, and levels ts..nl-1 are staged there entirely
+ * (phi, b, Helmholtz term, face coefficients; level 0's b copied in and its
+ * phi copied back out when ts == 0), as are the bottom solve's work arrays,
+ * so that the phases between barriers cost LDS rather than memory latency.
+ * Levels below ts stay in global memory. */
 template <class T>
 __global__ __launch_bounds__(1024) void k_mg_tail(const DLevT<T>* __restrict__ levs, int nl, DBC bc, int read_a,
-                                                  int nu1, int nu2, T* __restrict__ work, double tol, int maxit,
-                                                  int* __restrict__ iters) {
+                                                  int nu1, int nu2, T* __restrict__ gwork, double tol, int maxit,
+                                                  int* __restrict__ iters, int ts) {
     constexpr int BS = 1024;
-    __shared__ T sh[BS / 64 + 1];
+    extern __shared__ __align__(16) unsigned char tail_lds[];
+    __shared__ DLevT<T> sl[TAIL_MAXL];
+    __shared__ T* swork;
     const int tid = threadIdx.x;
-    auto lev = [&](int t) { DLevT<T> l = levs[t]; l.read_a = read_a ? 2 : 0; return l; };
+    if (tid == 0) {
+        T* q = reinterpret_cast<T*>(tail_lds);
+        for (int t = 0; t < nl; ++t) {
+            DLevT<T> l = levs[t];
+            l.read_a = read_a ? 2 : 0;
+            if (t >= ts) {
+                const int n = l.n0 * l.n1 * l.n2;
+                l.phi = q; q += n;
+                l.b = q; q += n;
+                l.a = q; q += n;
+                if (l.layered) {
+                    l.bxk = q; q += l.n2;
+                    l.byk = q; q += l.n2;
+                    l.bzk = q; q += l.n2 + 1;
+                } else {
+                    l.bx = q; q += (l.n0 + 1) * l.n1 * l.n2;
+                    l.by = q; q += l.n0 * (l.n1 + 1) * l.n2;
+                    l.bz = q; q += l.n0 * l.n1 * (l.n2 + 1);
+                }
+            }
+            sl[t] = l;
+        }
+        swork = ts < nl ? q : gwork;
+    }
+    __syncthreads();
+    /* copy the staged levels' coefficients in (and level 0's rhs) */
+    for (int t = ts; t < nl; ++t) {
+        const DLevT<T>& g = levs[t];
+        const DLevT<T>& l = sl[t];
+        const int n = l.n0 * l.n1 * l.n2;
+        for (int c = tid; c < n; c += BS) {
+            if (read_a) l.a[c] = g.a[c];
+            if (t == 0) { l.b[c] = g.b[c]; l.phi[c] = T(0); }
+        }
+        if (l.layered) {
+            /* the per-plane arrays are read-only everywhere else; here they are this kernel's LDS copies */
+            T* bxk = const_cast<T*>(l.bxk);
+            T* byk = const_cast<T*>(l.byk);
+            T* bzk = const_cast<T*>(l.bzk);
+            for (int c = tid; c < l.n2; c += BS) { bxk[c] = g.bxk[c]; byk[c] = g.byk[c]; }
+            for (int c = tid; c <= l.n2; c += BS) bzk[c] = g.bzk[c];
+        } else {
+            const int nx = (l.n0 + 1) * l.n1 * l.n2, ny = l.n0 * (l.n1 + 1) * l.n2, nz = l.n0 * l.n1 * (l.n2 + 1);
+            for (int c = tid; c < nx; c += BS) l.bx[c] = g.bx[c];
+            for (int c = tid; c < ny; c += BS) l.by[c] = g.by[c];
+            for (int c = tid; c < nz; c += BS) l.bz[c] = g.bz[c];
+        }
+    }
+    __syncthreads();
+    auto lev = [&](int t) -> const DLevT<T>& { return sl[t]; };
     auto half_sweep = [&](const DLevT<T>& l, int color) {
         const int half = (l.n0 + 1) >> 1;
         const int tot = half * l.n1 * l.n2;
@@ -404,135 +806,30 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const DLevT<T>* __restrict__ l
         __syncthreads();
     };
     for (int t = 0; t + 1 < nl; ++t) {
-        const DLevT<T> l = lev(t);
+        const DLevT<T>& l = lev(t);
         for (int s = 0; s < nu1; ++s) { half_sweep(l, 0); half_sweep(l, 1); }
         restrict_to(l, lev(t + 1));
     }
-    {   /* Jacobi-PCG bottom, as k_bottom_cg */
-        const DLevT<T> l = lev(nl - 1);
-        const int N = l.n0 * l.n1 * l.n2;
-        T* r = work; T* z = work + N; T* p = work + 2 * N; T* q = work + 3 * N; T* Dg = work + 4 * N;
-        const T* rhs = l.b;
-        T* e = l.phi;
-        T rr = T(0);
-        for (int c = tid; c < N; c += BS) {
-            const int i = c % l.n0, j = (c / l.n0) % l.n1, k = c / (l.n0 * l.n1);
-            T a, w[6], nb[6] = {T(0), T(0), T(0), T(0), T(0), T(0)}, D, S;
-            mg_coef(l, bc, i, j, k, c, a, w);
-            mg_acc(l, bc, false, i, j, k, a, w, nb, D, S);
-            Dg[c] = D; r[c] = rhs[c]; e[c] = T(0);
-            rr += rhs[c] * rhs[c];
-        }
-        const T rr0 = block_sum<BS>(rr, sh);
-        if (rr0 != T(0)) {
-            T rz = T(0);
-            for (int c = tid; c < N; c += BS) { z[c] = r[c] / Dg[c]; p[c] = z[c]; rz += r[c] * z[c]; }
-            rz = block_sum<BS>(rz, sh);
-            const T tol2 = T(tol * tol) * rr0;
-            int it = 0;
-            for (; it < maxit; ++it) {
-                T pq = T(0);
-                for (int c = tid; c < N; c += BS) {
-                    const int i = c % l.n0, j = (c / l.n0) % l.n1, k = c / (l.n0 * l.n1);
-                    T D, S;
-                    mg_row(l, p, i, j, k, bc, false, D, S);
-                    const T qq = D * p[c] - S;
-                    q[c] = qq;
-                    pq += p[c] * qq;
-                }
-                pq = block_sum<BS>(pq, sh);
-                const T al = rz / pq;
-                rr = T(0);
-                for (int c = tid; c < N; c += BS) { e[c] += al * p[c]; r[c] -= al * q[c]; rr += r[c] * r[c]; }
-                rr = block_sum<BS>(rr, sh);
-                if (rr <= tol2) { ++it; break; }
-                T rzn = T(0);
-                for (int c = tid; c < N; c += BS) { z[c] = r[c] / Dg[c]; rzn += r[c] * z[c]; }
-                rzn = block_sum<BS>(rzn, sh);
-                const T be = rzn / rz;
-                rz = rzn;
-                for (int c = tid; c < N; c += BS) p[c] = z[c] + be * p[c];
-                __syncthreads();
-            }
-            if (tid == 0) *iters += it;
-        }
-        __syncthreads();
+    {
+        const DLevT<T>& l = lev(nl - 1);
+        bottom_solve_dev<T, BS>(l, bc, l.b, l.phi, swork, tol, maxit, iters);
     }
     for (int t = nl - 2; t >= 0; --t) {
-        const DLevT<T> l = lev(t);
+        const DLevT<T>& l = lev(t);
         prolong_from(l, lev(t + 1));
         for (int s = 0; s < nu2; ++s) { half_sweep(l, 0); half_sweep(l, 1); }
+    }
+    if (ts == 0) {
+        const int n = levs[0].n0 * levs[0].n1 * levs[0].n2;
+        for (int c = tid; c < n; c += BS) levs[0].phi[c] = sl[0].phi[c];
     }
 }
 
 /* --------------------- FP64 refinement of an FP32 correction -------------- */
-/* phi_new = phi + e (e may be null), r = b - A(phi_new) in FP64 with the
- * inhomogeneous BCs, stored as the FP32 right-hand side of the next
- * correction solve; per-workgroup max|r| and max|b|.  phi_new is a separate
- * buffer: neighbours are read as phi + e from the old arrays. */
-__global__ __launch_bounds__(256) void k_refine_resid(DLev l, DBC bc, const float* __restrict__ e, double* __restrict__ phi_new,
-                                                      float* __restrict__ r32, double* __restrict__ part) {
-    __shared__ double sh[256 / 64 + 1];
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    double mr = 0.0, mb = 0.0;
-    const int k = blockIdx.z;
-    if (i < l.n0 && j < l.n1) {
-        const int n0 = l.n0, n1 = l.n1, n2 = l.n2, s = n0 * n1;
-        {
-            const int c = i + n0 * (j + n1 * k);
-            const int nbi[6] = {i > 0 ? c - 1 : c + (n0 - 1), i < n0 - 1 ? c + 1 : c - (n0 - 1),
-                                j > 0 ? c - n0 : c + n0 * (n1 - 1), j < n1 - 1 ? c + n0 : c - n0 * (n1 - 1),
-                                k > 0 ? c - s : c + s * (n2 - 1), k < n2 - 1 ? c + s : c - s * (n2 - 1)};
-            double nb[6];
-#pragma unroll
-            for (int q = 0; q < 6; ++q) nb[q] = e ? l.phi[nbi[q]] + (double)e[nbi[q]] : l.phi[nbi[q]];
-            const double pc = e ? l.phi[c] + (double)e[c] : l.phi[c];
-            double a, w[6], D, S;
-            mg_coef(l, bc, i, j, k, c, a, w);
-            mg_acc(l, bc, true, i, j, k, a, w, nb, D, S);
-            const double rr = l.b[c] - (D * pc - S);
-            phi_new[c] = pc;
-            r32[c] = (float)rr;
-            mr = nmax(mr, fabs(rr));
-            mb = nmax(mb, fabs(l.b[c]));
-        }
-    }
-    mr = block_max<256>(mr, sh);
-    mb = block_max<256>(mb, sh);
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if (threadIdx.x == 0) { part[bid] = mr; part[nb + bid] = mb; }
-}
-
 /* phi += e (FP32 correction added into the FP64 solution). */
 __global__ void k_add_f2d(double* __restrict__ phi, const float* __restrict__ e, long long n) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
         phi[c] += (double)e[c];
-}
-
-/* FP64 residual of the current solution, stored as the FP32 right-hand side
- * of the next correction solve, with per-workgroup max|r| and max|b|. */
-__global__ __launch_bounds__(256) void k_resid_to32(DLev l, DBC bc, float* __restrict__ r32, double* __restrict__ part) {
-    __shared__ double sh[256 / 64 + 1];
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    double mr = 0.0, mb = 0.0;
-    if (i < l.n0 && j < l.n1) {
-        double D, S;
-        mg_row(l, l.phi, i, j, k, bc, true, D, S);
-        const int c = i + l.n0 * (j + l.n1 * k);
-        const double rr = l.b[c] - (D * l.phi[c] - S);
-        r32[c] = (float)rr;
-        mr = fabs(rr);
-        mb = fabs(l.b[c]);
-    }
-    mr = block_max<256>(mr, sh);
-    mb = block_max<256>(mb, sh);
-    const int nb = gridDim.x * gridDim.y * gridDim.z;
-    const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-    if (threadIdx.x == 0) { part[bid] = mr; part[nb + bid] = mb; }
 }
 
 __global__ void k_d2f(float* __restrict__ d, const double* __restrict__ s, long long n) {
@@ -546,24 +843,79 @@ static inline dim3 grid3(int n0, int n1, int n2) { return dim3((n0 + 63) / 64, (
 
 int resid_blocks(const DLev& l) { return ((l.n0 + 63) / 64) * ((l.n1 + 3) / 4) * l.n2; }
 
-bool fused_ok(const DLev& l, const DBC& bc) {
-    return bc.lo[2] != BC_PER && l.n0 >= 64 && l.n1 >= 8 && l.phi2 != nullptr;
+template <class T>
+static bool stream_ok(const DLevT<T>& l, const DBC& bc) {
+    return l.phi2 != nullptr && l.layered && bc.lo[2] != BC_PER && l.n0 >= 64 && l.n1 >= 8 && (l.n0 % 2) == 0 &&
+           (bc.lo[1] != BC_PER || (l.n1 % 2) == 0) && l.n2 >= 2 && (l.read_a != 1 || l.mat != nullptr);
 }
+bool fused_ok(const DLev& l, const DBC& bc) { return stream_ok(l, bc); }
 
 template <class T>
 __global__ void k_copyT(T* __restrict__ d, const T* __restrict__ s, long long n) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) d[c] = s[c];
 }
 
+/* Below this size the two per-colour launches are faster: a streaming
+ * workgroup's planes are sequential, so a small level has too few of them
+ * in flight to cover the memory latency of each plane step. */
+static constexpr long long STREAM_MIN_CELLS = 1LL << 21;
+
+/* z-chunk of the streaming sweep: even, about 1024 workgroups, 8..64 planes */
+static int stream_chunk(int tiles, int n2) {
+    int cz = (int)(((long long)n2 * tiles + 1023) / 1024);
+    cz = std::max(cz, 8);
+    cz = std::min(std::min(cz, n2), SCZ_MAX);
+    return (cz + 1) & ~1;
+}
+
 template <class T>
-static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh, int nsweeps, int allow_fused, int zero_first) {
+static bool stream_use(const DLevT<T>& l, const DBC& bc, int allow_fused) {
+    return allow_fused && stream_ok(l, bc) && (long long)l.n0 * l.n1 * l.n2 >= STREAM_MIN_CELLS;
+}
+
+/* nsweeps RB-GS sweeps of level l.  With coarse != null the coarse
+ * correction is first prolongated into l: folded into the first streaming
+ * sweep's plane loads when l is z-coarsened (the sweep reads phi + P e), else
+ * by k_mg_prolong. */
+template <class T>
+static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh, int nsweeps, int allow_fused, int zero_first,
+                       const DLevT<T>* coarse = nullptr) {
+    const bool zonly = coarse && l.crs[0] == 0 && l.crs[1] == 0 && l.crs[2] == 1 && coarse->n0 == l.n0 && coarse->n1 == l.n1;
+    if (coarse && !(zonly && nsweeps > 0 && stream_use(l, bc, allow_fused))) {
+        hipLaunchKernelGGL(k_mg_prolong<T>, grid3(l.n0, l.n1, l.n2), dim3(256), 0, s, l, *coarse, bc);
+        coarse = nullptr;
+    }
     if (nsweeps <= 0) return;
-    if (allow_fused && l.phi2 && bc.lo[2] != BC_PER && l.n0 >= 64 && l.n1 >= 8) {
-        const dim3 g((l.n0 + FTX - 1) / FTX, (l.n1 + FTY - 1) / FTY, (l.n2 + FKZ - 1) / FKZ);
+    if (stream_use(l, bc, allow_fused)) {
+        StreamArgs<T> A;
+        A.n0 = l.n0; A.n1 = l.n1; A.n2 = l.n2; A.msx = l.msx; A.msy = l.msy;
+        A.b = l.b; A.a = l.a; A.mat = l.mat;
+        A.bxk = l.bxk; A.byk = l.byk; A.bzk = l.bzk;
+        for (int d = 0; d < 3; ++d) A.ih2[d] = l.ih2[d];
+        A.bc = bc;
+        A.inh = inh;
+        A.ntx = (l.n0 + STX - 1) / STX;
+        A.nty = (l.n1 + STY - 1) / STY;
+        A.cz = stream_chunk(A.ntx * A.nty, l.n2);
+        const int nz = (l.n2 + A.cz - 1) / A.cz;
+        const int nblk = A.ntx * A.nty * nz;
+        A.remap = (nblk % 8) == 0 ? 1 : 0;
         T* cur = l.phi;
         T* nxt = l.phi2;
         for (int it = 0; it < nsweeps; ++it) {
-            hipLaunchKernelGGL(k_mg_gsrb_fused<T>, g, dim3(256), 0, s, l, bc, inh, (zero_first && it == 0) ? 1 : 0, cur, nxt);
+            const bool pro = coarse && it == 0;
+            A.zero_init = (zero_first && it == 0 && !pro) ? 1 : 0;
+            A.phi_in = cur;
+            A.phi_out = nxt;
+            A.ec = pro ? coarse->phi : nullptr;
+            A.cn2 = pro ? coarse->n2 : 0;
+#define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr>), dim3(nblk), dim3(256), 0, s, A)
+            if (pro) {
+                switch (l.read_a) { case 0: SWEEP(0, 1); break; case 1: SWEEP(1, 1); break; default: SWEEP(2, 1); break; }
+            } else {
+                switch (l.read_a) { case 0: SWEEP(0, 0); break; case 1: SWEEP(1, 0); break; default: SWEEP(2, 0); break; }
+            }
+#undef SWEEP
             std::swap(cur, nxt);
         }
         if (cur != l.phi) {   /* odd number of sweeps: the result goes back to phi */
@@ -579,10 +931,16 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
     }
 }
 void mg_sweep(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps, int allow_fused) {
-    sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused && fused_ok(l, bc), 0);
+    sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, 0);
 }
 void mg_sweep(hipStream_t s, const DLevF& l, const DBC& bc, int nsweeps, int zero_first, int allow_fused) {
     sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, zero_first);
+}
+void mg_prolong_sweep(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, int inh, int nsweeps, int allow_fused) {
+    sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, 0, &c);
+}
+void mg_prolong_sweep(hipStream_t s, const DLevF& l, const DLevF& c, const DBC& bc, int nsweeps, int allow_fused) {
+    sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, 0, &c);
 }
 void mg_smooth(hipStream_t s, const DLev& l, const DBC& bc, int inh, int color) {
     const dim3 g(((l.n0 + 1) / 2 + 63) / 64, (l.n1 + 3) / 4, l.n2);
@@ -623,33 +981,50 @@ void mg_restrict_cells(hipStream_t s, const DLev& f, const DLev& c, const double
 void mg_restrict_cells(hipStream_t s, const DLevF& f, const DLevF& c, const float* src, float* dst) {
     hipLaunchKernelGGL(k_mg_restrict_cells<float>, grid3(c.n0, c.n1, c.n2), dim3(256), 0, s, f, c, src, dst);
 }
-int resid_norm_blocks(const DLev& l) { return ((l.n0 + 63) / 64) * ((l.n1 + 3) / 4) * l.n2; }
 
+static constexpr int RESID_ZC = 4;
+static constexpr int RESID_NB = 1024;   /* workgroups of the residual kernels */
+int resid_norm_blocks(const DLev&) { return RESID_NB; }
 void mg_resid_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, double* r, double* part, double* out) {
-    hipLaunchKernelGGL(k_mg_resid_norm, dim3((l.n0 + 63) / 64, (l.n1 + 3) / 4, l.n2), dim3(256), 0, s,
-                       l, bc, inh, r, part);
-    hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, resid_norm_blocks(l), 2, 1, out);
-}
-void refine_resid(hipStream_t s, const DLev& l, const DBC& bc, const float* e, double* phi_new, float* r32, double* part, double* out) {
-    hipLaunchKernelGGL(k_refine_resid, dim3((l.n0 + 63) / 64, (l.n1 + 3) / 4, l.n2), dim3(256), 0, s,
-                       l, bc, e, phi_new, r32, part);
-    hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, resid_norm_blocks(l), 2, 1, out);
+    if (r) hipLaunchKernelGGL((k_mg_resid_max<1, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, inh, r, nullptr, part);
+    else hipLaunchKernelGGL((k_mg_resid_max<0, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, inh, nullptr, nullptr, part);
+    hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, RESID_NB, 2, 1, out);
 }
 void add_f2d(hipStream_t s, double* phi, const float* e, long long n) {
     unsigned b = (unsigned)((n + 255) / 256); if (b > 8192) b = 8192; if (b == 0) b = 1;
     hipLaunchKernelGGL(k_add_f2d, dim3(b), dim3(256), 0, s, phi, e, n);
 }
 void resid_to32(hipStream_t s, const DLev& l, const DBC& bc, float* r32, double* part, double* out) {
-    hipLaunchKernelGGL(k_resid_to32, dim3((l.n0 + 63) / 64, (l.n1 + 3) / 4, l.n2), dim3(256), 0, s, l, bc, r32, part);
-    hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, resid_norm_blocks(l), 2, 1, out);
+    hipLaunchKernelGGL((k_mg_resid_max<2, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, 1, nullptr, r32, part);
+    hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, RESID_NB, 2, 1, out);
+}
+size_t tail_level_values(const DLev& l) { return fxd_tail_level_values(l); }
+/* largest dynamic LDS a workgroup may use on the current device */
+size_t tail_lds_max() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 65536;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0) return 65536;
+    return (size_t)v;
+}
+template <class T>
+static void tail_impl(hipStream_t s, const DLevT<T>* levs, int nl, const DBC& bc, int read_a, int nu1, int nu2, T* work,
+                      double tol, int maxit, int* iters, int ts, size_t lds_bytes) {
+    static size_t attr_set = 0;      /* largest dynamic LDS requested so far for this instantiation */
+    if (lds_bytes > attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail<T>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds_bytes);
+        attr_set = lds_bytes;
+    }
+    hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(1024), lds_bytes, s, levs, nl, bc, read_a, nu1, nu2, work, tol, maxit,
+                       iters, ts);
 }
 void mg_tail(hipStream_t s, const DLev* levs, int nl, const DBC& bc, int read_a, int nu1, int nu2, double* work,
-             double tol, int maxit, int* iters) {
-    hipLaunchKernelGGL(k_mg_tail<double>, dim3(1), dim3(1024), 0, s, levs, nl, bc, read_a, nu1, nu2, work, tol, maxit, iters);
+             double tol, int maxit, int* iters, int ts, size_t lds_bytes) {
+    tail_impl<double>(s, levs, nl, bc, read_a, nu1, nu2, work, tol, maxit, iters, ts, lds_bytes);
 }
 void mg_tail(hipStream_t s, const DLevF* levs, int nl, const DBC& bc, int read_a, int nu1, int nu2, float* work,
-             double tol, int maxit, int* iters) {
-    hipLaunchKernelGGL(k_mg_tail<float>, dim3(1), dim3(1024), 0, s, levs, nl, bc, read_a, nu1, nu2, work, tol, maxit, iters);
+             double tol, int maxit, int* iters, int ts, size_t lds_bytes) {
+    tail_impl<float>(s, levs, nl, bc, read_a, nu1, nu2, work, tol, maxit, iters, ts, lds_bytes);
 }
 void d2f(hipStream_t s, float* d, const double* src, long long n) {
     unsigned b = (unsigned)((n + 255) / 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
@@ -659,10 +1034,10 @@ void final1024(hipStream_t s, const double* part, int nb, int nq, int is_max, do
     hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, nb, nq, is_max, out);
 }
 void bottom_cg(hipStream_t s, const DLev& l, const DBC& bc, const double* rhs, double* e, double* work, double tol, int maxit, int* iters) {
-    hipLaunchKernelGGL(k_bottom_cg<double>, dim3(1), dim3(1024), 0, s, l, bc, rhs, e, work, tol, maxit, iters);
+    hipLaunchKernelGGL(k_bottom<double>, dim3(1), dim3(1024), 0, s, l, bc, rhs, e, work, tol, maxit, iters);
 }
 void bottom_cg(hipStream_t s, const DLevF& l, const DBC& bc, const float* rhs, float* e, float* work, double tol, int maxit, int* iters) {
-    hipLaunchKernelGGL(k_bottom_cg<float>, dim3(1), dim3(1024), 0, s, l, bc, rhs, e, work, tol, maxit, iters);
+    hipLaunchKernelGGL(k_bottom<float>, dim3(1), dim3(1024), 0, s, l, bc, rhs, e, work, tol, maxit, iters);
 }
 
 }  // namespace fxk

@@ -132,7 +132,7 @@ typedef struct fx_inputs {
     double bias_ramp_start;
     double bias_ramp_end;
     int    verbose;
-    int    mg_fused_sweep;        /* 1: one-pass red+black sweep kernel on large levels (default 0) */
+    int    mg_fused_sweep;        /* 1: one-pass streaming red+black sweep on levels of >= 2M cells (default 1); 0: two launches per sweep */
 } fx_inputs;
 
 /* Field identifiers.  0..17 follow the plotfile variable order of the
@@ -148,6 +148,8 @@ typedef enum fx_field {
     FX_FIELD_ALPHA_CC,            /* Newton A-coefficient of the last solve */
     FX_FIELD_GAMMA,               /* BigGamma in FE, 0 elsewhere */
     FX_FIELD_GL_RHS_X, FX_FIELD_GL_RHS_Y, FX_FIELD_GL_RHS_Z,   /* TDGL right-hand side, last fx_tdgl_rhs */
+    FX_FIELD_LANDAU_ALPHA,        /* per-cell Landau alpha (extension: the reference's alpha is global,
+                                     FerroX.cpp:350); defaults to the inputs' alpha everywhere */
     FX_FIELD_COUNT
 } fx_field;
 

@@ -19,7 +19,7 @@
  * The Poisson solve (AMReX MLMG in the reference, not available here) is a
  * geometric multigrid written to the same specification as the device solver
  * (semi-coarsening rule, RB Gauss-Seidel, face-averaged coefficients,
- * piecewise-constant prolongation, Jacobi-PCG bottom, MLMG stopping rule
+ * cell-centred linear prolongation, exact (dense) bottom solve, MLMG stopping rule
  * ||r||_inf <= tol*max(||b||_inf, ||r0||_inf)); the tests pin it against a
  * sparse direct solve of the assembled operator.
  *
@@ -290,6 +290,61 @@ struct MG {
         return m;
     }
 
+    /* Bottom solve of A e = rhs (homogeneous BCs).  A bottom of at most
+     * BOTTOM_DIRECT_MAX cells is solved exactly: the dense matrix is assembled
+     * column by column from the row operator, then Gaussian elimination with
+     * partial pivoting (first largest |pivot| in row order; factor
+     * f = M[i][k]/M[k][k]; M[i][j] -= f*M[k][j] for j > k; y[i] -= f*y[k]) and
+     * back substitution with the sum in increasing column order.  A pivot
+     * below 1e-14 of the largest diagonal entry (singular operator: no
+     * Dirichlet face) falls back to Jacobi-PCG, as do larger bottoms. */
+    static constexpr int BOTTOM_DIRECT_MAX = 64;
+    void bottom_solve(Lev& l, const double* rhs, double* e) {
+        const int N = (int)l.N;
+        if (N <= BOTTOM_DIRECT_MAX && bottom_direct(l, rhs, e)) return;
+        bottom_cg(l, rhs, e);
+    }
+    bool bottom_direct(Lev& l, const double* rhs, double* e) {
+        const int N = (int)l.N;
+        std::vector<double> M((size_t)N * N), y(rhs, rhs + N), unit(N, 0.0);
+        double dmax = 0.0;
+        for (int col = 0; col < N; ++col) {
+            unit[col] = 1.0;
+            for (int k = 0; k < l.n[2]; ++k)
+                for (int j = 0; j < l.n[1]; ++j)
+                    for (int i = 0; i < l.n[0]; ++i) {
+                        double D, S;
+                        row(l, unit.data(), i, j, k, nullptr, D, S);
+                        const int r = (int)l.c(i, j, k);
+                        M[(size_t)r * N + col] = (r == col ? D : 0.0) - S;
+                    }
+            unit[col] = 0.0;
+        }
+        for (int r = 0; r < N; ++r) dmax = std::max(dmax, std::fabs(M[(size_t)r * N + r]));
+        for (int k = 0; k < N; ++k) {
+            int p = k;
+            double best = std::fabs(M[(size_t)k * N + k]);
+            for (int i = k + 1; i < N; ++i)
+                if (std::fabs(M[(size_t)i * N + k]) > best) { best = std::fabs(M[(size_t)i * N + k]); p = i; }
+            if (!(best > 1e-14 * dmax)) return false;
+            if (p != k) {
+                for (int j = k; j < N; ++j) std::swap(M[(size_t)k * N + j], M[(size_t)p * N + j]);
+                std::swap(y[k], y[p]);
+            }
+            for (int i = k + 1; i < N; ++i) {
+                const double f = M[(size_t)i * N + k] / M[(size_t)k * N + k];
+                for (int j = k + 1; j < N; ++j) M[(size_t)i * N + j] = M[(size_t)i * N + j] - f * M[(size_t)k * N + j];
+                y[i] = y[i] - f * y[k];
+            }
+        }
+        for (int i = N - 1; i >= 0; --i) {
+            double s = y[i];
+            for (int j = i + 1; j < N; ++j) s -= M[(size_t)i * N + j] * e[j];
+            e[i] = s / M[(size_t)i * N + i];
+        }
+        return true;
+    }
+
     /* Jacobi-preconditioned CG on level l for A e = rhs (homogeneous BCs). */
     void bottom_cg(Lev& l, const double* rhs, double* e) {
         const size_t N = l.N;
@@ -373,10 +428,10 @@ struct MG {
             if (li == 0) {     /* single-level hierarchy: correct the solution */
                 residual(l, bv);
                 std::vector<double> e(l.N);
-                bottom_cg(l, l.r.data(), e.data());
+                bottom_solve(l, l.r.data(), e.data());
                 for (size_t c = 0; c < l.N; ++c) l.phi[c] += e[c];
             } else {
-                bottom_cg(l, l.b.data(), l.phi.data());
+                bottom_solve(l, l.b.data(), l.phi.data());
             }
             return;
         }
@@ -421,7 +476,7 @@ struct fxo_state {
     size_t N;
     double dx[3];
     G3 mask, eps;
-    std::vector<double> tphase, ang[3];
+    std::vector<double> tphase, ang[3], lalpha;
     std::vector<double> P[3], Gam, rho, e_den, h_den, phi, phi_old, phi_prev, alpha_cc, rhs, phidiff;
     std::vector<double> E[3], GL[3];
     MG mg;
@@ -694,7 +749,7 @@ void tdgl_rhs(fxo_state* s, std::vector<double>* const P, std::vector<double>* G
     PAcc Fp{P[0].data(), {s->n[0], s->n[1], s->n[2]}, {in.is_periodic[0], in.is_periodic[1], in.is_periodic[2]}};
     PAcc Fq = Fp, Fr = Fp; Fq.p = P[1].data(); Fr.p = P[2].data();
     MAcc M{&s->mask};
-    const double alpha = in.alpha, beta = in.beta, gamma = in.gamma;
+    const double beta = in.beta, gamma = in.gamma;
     const double a12 = in.alpha_12, a112 = in.alpha_112, a123 = in.alpha_123;
     const double g11 = in.g11, g44 = in.g44, g12 = in.g12, g44_p = in.g44_p;
     const double* dx = s->dx;
@@ -706,6 +761,7 @@ void tdgl_rhs(fxo_state* s, std::vector<double>* const P, std::vector<double>* G
                 size_t c = s->c(i, j, k);
                 const double Gam = s->Gam[c];
                 const double tph = s->tphase[c];
+                const double alpha = s->lalpha[c];   /* per-cell Landau alpha (extension; default: inputs' alpha) */
                 const double p = P[0][c], q = P[1][c], r = P[2][c];
                 double L[3] = {0, 0, 0}, G[3] = {0, 0, 0}, El[3] = {0, 0, 0};
                 if (in.include_Landau == 1) {
@@ -910,6 +966,7 @@ fxo_state* fxo_create(const fx_inputs* in) {
     s->N = (size_t)s->n[0] * s->n[1] * s->n[2];
     const size_t N = s->N;
     s->tphase.assign(N, 0.0);
+    s->lalpha.assign(N, in->alpha);
     for (int d = 0; d < 3; ++d) {
         s->ang[d].assign(N, 0.0); s->P[d].assign(N, 0.0); s->E[d].assign(N, 0.0); s->GL[d].assign(N, 0.0);
         s->pbc.lo[d] = in->P_BC_flag_lo[d]; s->pbc.hi[d] = in->P_BC_flag_hi[d];
@@ -1015,6 +1072,7 @@ static std::vector<double>* field_ptr(fxo_state* s, int f) {
     case FX_FIELD_GL_RHS_X: return &s->GL[0];
     case FX_FIELD_GL_RHS_Y: return &s->GL[1];
     case FX_FIELD_GL_RHS_Z: return &s->GL[2];
+    case FX_FIELD_LANDAU_ALPHA: return &s->lalpha;
     default: return nullptr;
     }
 }

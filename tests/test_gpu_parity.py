@@ -200,3 +200,40 @@ def test_mixed_precision_steps_semiconductor(relerr):
         e = relerr(g.get(name), o.get(name))
         print(name, e)
         assert e < 1e-6, (name, e)
+
+
+def poly_case(**kw):
+    text = configs.mfim(nx=48, ny=40, nz=48, lx=24e-9, ly=20e-9, lz=4.8e-9, de_top=1.2e-9, fe_top=4.0e-9,
+                        nsteps=1, bias=0.5) + "\nuse_Euler_angles = 1\n"
+    inp = fx.from_text(text, **kw)
+    return inp, configs.polycrystal_fields(inp, n_grains=16, seed=5)
+
+
+def test_polycrystal_steps(relerr):
+    """Per-grain rotations (mixed second derivatives in the gradient term,
+    rotated Poisson RHS and E) and per-grain Landau alpha."""
+    inp, f = poly_case(mg_tol_rel=1e-13)
+    g, o = fx.FerroX(inp), Oracle(inp)
+    configs.apply_fields(g, f)
+    configs.apply_fields(o, f)
+    for name in ("alpha", "beta", "theta", "landau_alpha"):
+        assert np.array_equal(g.get(name), o.get(name)), name
+    g.step(20); o.step(20)
+    for name in ("Pz", "Phi", "Ez", "Ex"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < TOL, name
+
+
+def test_polycrystal_mixed_precision_vs_fp64(relerr):
+    inp, f = poly_case()
+    g64 = fx.FerroX(inp)
+    inp32, _ = poly_case(precision=1)
+    g32 = fx.FerroX(inp32)
+    configs.apply_fields(g64, f)
+    configs.apply_fields(g32, f)
+    g64.step(40); g32.step(40)
+    for name in ("Pz", "Phi"):
+        e = relerr(g32.get(name), g64.get(name))
+        print(name, e)
+        assert e < 1e-6, name
