@@ -90,6 +90,10 @@ SIGNATURES = {
     "fx_set_field": (C.c_int, [_P, C.c_int, _DP]),
     "fx_set_bias": (C.c_int, [_P, C.c_double, C.c_double]),
     "fx_set_bias_waveform": (C.c_int, [_P, _DP, C.c_int]),
+    "fx_diff_set_params": (C.c_int, [_P, _DP]),
+    "fx_diff_get_params": (C.c_int, [_P, _DP]),
+    "fx_diff_forward": (C.c_int, [_P, _DP, C.c_int, C.c_int, _DP]),
+    "fx_diff_backward": (C.c_int, [_P, _DP, _DP, _DP]),
     "fx_compute_phi_rho": (C.c_int, [_P, C.POINTER(FxSolveStats)]),
     "fx_compute_E": (C.c_int, [_P]),
     "fx_tdgl_rhs": (C.c_int, [_P]),

@@ -112,6 +112,22 @@ struct fx_ctx {
     /* run state (main.cpp:238-245) */
     double Phi_Bc_lo = 0, Phi_Bc_hi = 0;
     double bc_solve[6] = {0, 0, 0, 0, 0, 0};
+    /* differentiation form: checkpoints of the last fx_diff_forward */
+    struct Diff {
+        int nsteps = 0, K = 0, nck = 0, nfe = 0;
+        bool valid = false;
+        std::vector<double> V;
+        double* ck = nullptr;        /* nck x N: P at steps 0, K, 2K, ... */
+        double* segP = nullptr;      /* K x N: P_n of the segment being reversed */
+        double* segPhi = nullptr;    /* K x N: phi_n of that segment */
+        double* lam[2] = {nullptr, nullptr};
+        double* s = nullptr;         /* Fz^T nu */
+        double* fin = nullptr;       /* P after the last forward step */
+        double* acc = nullptr;       /* 6 parameter gradients, then dL/dV per step */
+        double* m = nullptr;         /* FE sums per step */
+        double* part = nullptr;      /* 7 x adj_blocks partials */
+        size_t cap_ck = 0, cap_segP = 0, cap_segPhi = 0, cap_acc = 0, cap_m = 0;   /* bytes allocated */
+    } diff;
     int step = 0, sign = 1, num_Vapp = 0, steady_state_step = 1000000, inc_step = -1;
     double time = 0.0;
     std::vector<double> waveform;
@@ -863,6 +879,8 @@ void free_all(fx_ctx* x) {
     for (int d = 0; d < 3; ++d) { F(x->P[d]); F(x->Pn[d]); F(x->E[d]); F(x->GL[d]); F(x->GL1[d]); }
     F(x->phi); F(x->phi_old); F(x->phi_prev); F(x->rhs); F(x->rho); F(x->e_den); F(x->h_den);
     F(x->alpha_cc); F(x->phidiff); F(x->eps); F(x->mat); F(x->rot_idx); F(x->rot_tab); F(x->rot_atab);
+    F(x->diff.ck); F(x->diff.segP); F(x->diff.segPhi); F(x->diff.lam[0]); F(x->diff.lam[1]); F(x->diff.s);
+    F(x->diff.fin); F(x->diff.acc); F(x->diff.m); F(x->diff.part);
     F(x->red); F(x->red_out); F(x->d_iters); F(x->cg_work); F(x->tmp);
     for (size_t l = 0; l < x->lev.size(); ++l) {
         DLev& d = x->lev[l].d;
@@ -1276,6 +1294,193 @@ int fx_sync(fx_ctx* x) {
 }
 
 void* fx_stream(fx_ctx* x) { return x ? (void*)x->s : nullptr; }
+
+/* ------------------------- differentiation form --------------------------- */
+/* Discrete adjoint of the forward-Euler step with its Poisson solve; the
+ * kernels and the derivation are in fx_adjoint.hip.  The forward stores P
+ * every K steps; the backward re-runs each K-step segment from its checkpoint
+ * (the same kernels in the same order, so the same values), keeping P_n and
+ * phi_n of the segment, and reverses it. */
+static int diff_check(fx_ctx* x) {
+    const fx_inputs& in = x->in;
+    if (in.TimeIntegratorOrder != 1) return fail(x, FX_EINVAL, "differentiation form needs TimeIntegratorOrder = 1");
+    if (x->has_sc) return fail(x, FX_EINVAL, "differentiation form does not cover semiconductor regions");
+    if (!scalar_fast(x)) return fail(x, FX_EINVAL, "differentiation form needs is_polarization_scalar = 1 with Px = Py = 0");
+    if (x->rot_idx) return fail(x, FX_EINVAL, "differentiation form needs uniform angles and Landau alpha");
+    for (int q = 0; q < 9; ++q)
+        if (x->Ru[q] != ((q % 4) == 0 ? 1.0 : 0.0)) return fail(x, FX_EINVAL, "differentiation form needs zero Euler angles");
+    if (in.include_Landau != 1 || in.include_Grad != 1 || in.include_Elec != 1)
+        return fail(x, FX_EINVAL, "differentiation form needs include_Landau = include_Grad = include_Elec = 1");
+    return FX_OK;
+}
+
+static int diff_reserve(fx_ctx* x, int nsteps, int K) {
+    fx_ctx::Diff& D = x->diff;
+    const size_t N = x->N, B = N * sizeof(double);
+    const int nck = nsteps / K + 1;
+    int rc;
+    auto grow = [&](double*& p, size_t& cap, size_t need) -> int {
+        if (need <= cap) return FX_OK;
+        if (p) { (void)hipFree(p); p = nullptr; }
+        cap = 0;
+        int r = dalloc(x, (void**)&p, need);
+        if (!r) cap = need;
+        return r;
+    };
+    if ((rc = grow(D.ck, D.cap_ck, (size_t)nck * B))) return rc;
+    if ((rc = grow(D.segP, D.cap_segP, (size_t)K * B))) return rc;
+    if ((rc = grow(D.segPhi, D.cap_segPhi, (size_t)K * B))) return rc;
+    if (!D.lam[0]) {
+        if ((rc = dalloc(x, (void**)&D.lam[0], B)) || (rc = dalloc(x, (void**)&D.lam[1], B)) ||
+            (rc = dalloc(x, (void**)&D.s, B)) || (rc = dalloc(x, (void**)&D.fin, B)) ||
+            (rc = dalloc(x, (void**)&D.part, sizeof(double) * 7 * (size_t)fxk::adj_blocks(geom_of(x)))))
+            return rc;
+    }
+    if ((rc = grow(D.acc, D.cap_acc, sizeof(double) * (size_t)(6 + nsteps)))) return rc;
+    if ((rc = grow(D.m, D.cap_m, sizeof(double) * (size_t)nsteps))) return rc;
+    D.nck = nck;
+    int nfe = 0;
+    for (int k = 0; k < x->n[2]; ++k)
+        for (int j = 0; j < x->n[1]; ++j)
+            for (int i = 0; i < x->n[0]; ++i) {
+                const int r = x->h_mat[(i + 1) + x->msx * (j + 1) + x->msy * (k + 1)];
+                if ((r & 3) == 0 && !(r & 4)) ++nfe;
+            }
+    D.nfe = nfe;
+    if (nfe == 0) return fail(x, FX_EINVAL, "differentiation form: no ferroelectric cells");
+    return FX_OK;
+}
+
+/* restore P (both buffers: the scalar step leaves non-FE cells in place) */
+static void diff_set_P(fx_ctx* x, const double* src) {
+    fxk::copy(x->s, x->P[2], src, (long long)x->N);
+    fxk::copy(x->s, x->Pn[2], src, (long long)x->N);
+}
+
+static int diff_step(fx_ctx* x, double V) {
+    x->Phi_Bc_hi = V;
+    fx_solve_stats ss{}, ss2{};
+    int rc = compute_phi_rho(x, x->P, &ss);
+    if (rc) return rc;
+    return update(x, 1, &ss2);
+}
+
+int fx_diff_set_params(fx_ctx* x, const double* th) {
+    if (!x || !th) return FX_EINVAL;
+    fx_inputs& in = x->in;
+    const double a_old = in.alpha;
+    in.alpha = th[0]; in.beta = th[1]; in.gamma = th[2]; in.g11 = th[3]; in.g44 = th[4];
+    for (double& v : x->h_lalpha) if (v == a_old) v = in.alpha;
+    x->diff.valid = false;
+    if (th[5] != in.epsilonX_fe) {
+        if (in.epsilonX_fe_tphase == in.epsilonX_fe) in.epsilonX_fe_tphase = th[5];
+        in.epsilonX_fe = th[5];
+        return build_coefficients(x);
+    }
+    return FX_OK;
+}
+
+int fx_diff_get_params(fx_ctx* x, double* th) {
+    if (!x || !th) return FX_EINVAL;
+    const fx_inputs& in = x->in;
+    th[0] = in.alpha; th[1] = in.beta; th[2] = in.gamma; th[3] = in.g11; th[4] = in.g44; th[5] = in.epsilonX_fe;
+    return FX_OK;
+}
+
+int fx_diff_forward(fx_ctx* x, const double* V, int nsteps, int K, double* m_out) {
+    if (!x || !V || nsteps < 1 || K < 1) return FX_EINVAL;
+    int rc;
+    if ((rc = diff_check(x)) || (rc = diff_reserve(x, nsteps, K))) return rc;
+    fx_ctx::Diff& D = x->diff;
+    D.valid = false;
+    D.nsteps = nsteps; D.K = K;
+    D.V.assign(V, V + nsteps);
+    const size_t N = x->N;
+    const Geom g = geom_of(x);
+    const MView M = mview_of(x);
+    fxk::copy(x->s, D.ck, x->P[2], (long long)N);
+    for (int n = 0; n < nsteps; ++n) {
+        if ((rc = diff_step(x, V[n]))) return rc;
+        fxk::fe_sum(x->s, x->P[2], M, g, D.part, D.m + n);
+        if ((n + 1) % K == 0) fxk::copy(x->s, D.ck + (size_t)((n + 1) / K) * N, x->P[2], (long long)N);
+    }
+    fxk::copy(x->s, D.fin, x->P[2], (long long)N);
+    std::vector<double> m(nsteps);
+    HIPCHK(x, hipMemcpyAsync(m.data(), D.m, sizeof(double) * nsteps, hipMemcpyDeviceToHost, x->s));
+    HIPCHK(x, hipStreamSynchronize(x->s));
+    for (int n = 0; n < nsteps; ++n) m_out[n] = m[n] / D.nfe;
+    x->step += nsteps;
+    x->time += nsteps * x->in.dt;
+    D.valid = true;
+    return FX_OK;
+}
+
+int fx_diff_backward(fx_ctx* x, const double* dL_dm, double* g_theta, double* g_V) {
+    if (!x || !dL_dm) return FX_EINVAL;
+    fx_ctx::Diff& D = x->diff;
+    if (!D.valid) return fail(x, FX_EINVAL, "fx_diff_backward needs a preceding fx_diff_forward with the same parameters");
+    const int N_ = D.nsteps, K = D.K;
+    const size_t N = x->N;
+    const Geom g = geom_of(x);
+    const MView M = mview_of(x);
+    const fx_inputs& in = x->in;
+    int rc;
+    fxk::fill(x->s, D.acc, 6 + N_, 0.0);
+    fxk::adj_init(x->s, D.lam[0], M, g, dL_dm[N_ - 1] / D.nfe);
+    int cur = 0;
+    const int nb = fxk::adj_blocks(g);
+    for (int seg = (N_ - 1) / K; seg >= 0; --seg) {
+        const int n0 = seg * K, n1 = std::min(n0 + K, N_);
+        diff_set_P(x, D.ck + (size_t)seg * N);
+        for (int n = n0; n < n1; ++n) {
+            fxk::copy(x->s, D.segP + (size_t)(n - n0) * N, x->P[2], (long long)N);
+            x->Phi_Bc_hi = D.V[n];
+            fx_solve_stats ss{}, ss2{};
+            if ((rc = compute_phi_rho(x, x->P, &ss))) return rc;
+            fxk::copy(x->s, D.segPhi + (size_t)(n - n0) * N, x->phi, (long long)N);
+            if ((rc = update(x, 1, &ss2))) return rc;
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            x->Phi_Bc_hi = D.V[n];
+            double bv[6];
+            bc_values(x, bv);
+            fxk::AdjArgs a;
+            a.g = g; a.M = M; a.pbc = pbc_of(x);
+            a.lam = D.lam[cur]; a.lam_out = D.lam[1 - cur];
+            a.P = D.segP + (size_t)(n - n0) * N;
+            a.phi = D.segPhi + (size_t)(n - n0) * N;
+            a.psi = x->phi;
+            a.s = D.s;
+            a.dtG = in.dt * in.BigGamma;
+            a.alpha = in.alpha; a.beta = in.beta; a.gamma = in.gamma; a.g11 = in.g11; a.g44m = in.g44 - in.g44_p;
+            a.loss_w = n >= 1 ? dL_dm[n - 1] / D.nfe : 0.0;
+            a.eps0 = in.epsilon_0;
+            for (int d = 0; d < 3; ++d) { a.bclo[d] = in.bc_lo[d]; a.bchi[d] = in.bc_hi[d]; a.ih2[d] = x->lev[0].d.ih2[d]; }
+            std::memcpy(a.bcv, bv, sizeof(bv));
+            a.eps = x->eps;
+            a.ghost_face = in.phi_ghost_mode == FX_GHOST_FACE;
+            a.part = D.part;
+            fxk::adj_fzT(x->s, a);
+            /* psi = A^-1 (Fz^T nu): the symmetric operator, homogeneous BCs */
+            fxk::copy(x->s, x->rhs, D.s, (long long)N);
+            fxk::fill(x->s, x->phi, (long long)N, 0.0);
+            set_read_a(x, 0);
+            const double bz[6] = {0, 0, 0, 0, 0, 0};
+            fx_solve_stats st{};
+            if ((rc = solve_any(x, bz, in.mg_tol_rel, in.mg_max_iter, &st))) return rc;
+            fxk::adj_step(x->s, a);
+            fxk::adj_accum(x->s, D.part, nb, D.acc, D.acc + 6, n);
+            cur = 1 - cur;
+        }
+    }
+    diff_set_P(x, D.fin);
+    std::vector<double> acc(6 + N_);
+    HIPCHK(x, hipMemcpyAsync(acc.data(), D.acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, x->s));
+    HIPCHK(x, hipStreamSynchronize(x->s));
+    if (g_theta) for (int q = 0; q < 6; ++q) g_theta[q] = acc[q];
+    if (g_V) for (int n = 0; n < N_; ++n) g_V[n] = acc[6 + n];
+    return FX_OK;
+}
 
 /* Average duration of one launch of a hot-path kernel, timed with HIP events
  * on the context's stream (the stream the kernel runs on).  which:

@@ -58,8 +58,8 @@ struct PBC {
 
 /* DPDx/DPDy/DPDz, DerivativeAlgorithm.H:45-252.  Flags outside 0..4 are
  * rejected before any launch. */
-template <int d>
-__device__ __forceinline__ double dpd(const PView& F, const MView& M, int i, int j, int k,
+template <int d, class V = PView>
+__device__ __forceinline__ double dpd(const V& F, const MView& M, int i, int j, int k,
                                       const double* dx, const PBC& bc) {
     constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
     const double h = dx[d];
@@ -89,8 +89,8 @@ __device__ __forceinline__ double dpd(const PView& F, const MView& M, int i, int
 }
 
 /* DoubleDPDx/y/z, DerivativeAlgorithm.H:257-467. */
-template <int d>
-__device__ __forceinline__ double ddpd(const PView& F, const MView& M, int i, int j, int k,
+template <int d, class V = PView>
+__device__ __forceinline__ double ddpd(const V& F, const MView& M, int i, int j, int k,
                                        const double* dx, const PBC& bc) {
     constexpr int di = (d == 0), dj = (d == 1), dk = (d == 2);
     const double h = dx[d];
@@ -120,8 +120,8 @@ __device__ __forceinline__ double ddpd(const PView& F, const MView& M, int i, in
 }
 
 /* DoubleDPDxDy / DxDz / DyDz, DerivativeAlgorithm.H:471-497. */
-template <int a, int b>
-__device__ __forceinline__ double ddpd_mixed(const PView& F, const MView& M, int i, int j, int k,
+template <int a, int b, class V = PView>
+__device__ __forceinline__ double ddpd_mixed(const V& F, const MView& M, int i, int j, int k,
                                              const double* dx, const PBC& bc) {
     constexpr int ai = (a == 0), aj = (a == 1), ak = (a == 2);
     return (dpd<b>(F, M, i + ai, j + aj, k + ak, dx, bc) - dpd<b>(F, M, i - ai, j - aj, k - ak, dx, bc)) / 2. / dx[a];
@@ -192,7 +192,8 @@ struct PhiView {
 
 /* grad(phi) with the reference's one-sided z formula at the metal faces
  * (DphiDz, DFDx, DFDy: DerivativeAlgorithm.H:8-40). */
-__device__ __forceinline__ void grad_phi(const PhiView& P, const Geom& g, int i, int j, int k,
+template <class PV = PhiView>
+__device__ __forceinline__ void grad_phi(const PV& P, const Geom& g, int i, int j, int k,
                                          double& dfdx, double& dfdy, double& dfdz) {
     const double* dx = g.dx;
     dfdx = (P.at(i + 1, j, k) - P.at(i - 1, j, k)) / (2. * dx[0]);

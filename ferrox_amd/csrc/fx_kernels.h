@@ -133,4 +133,34 @@ size_t tail_level_values(const fxd::DLev& l);
 constexpr int BOTTOM_DIRECT_CELLS = 64;
 /* r32 = (float)(b - A phi) in FP64 (inhomogeneous BCs); out[0] = max|r|, out[1] = max|b| */
 void resid_to32(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, float* r32, double* part, double* out);
+/* ------------------------------- adjoint ---------------------------------- */
+/* One backward step of the discrete adjoint (fx_adjoint.hip). */
+struct AdjArgs {
+    fxd::Geom g;
+    fxd::MView M;
+    fxd::PBC pbc;
+    const double* lam;       /* lambda_{n+1} */
+    double* lam_out;         /* lambda_n */
+    const double* P;         /* P_n */
+    const double* phi;       /* phi_n */
+    const double* psi;       /* A^-1 Fz^T nu */
+    double* s;               /* Fz^T nu */
+    double dtG;              /* dt * BigGamma */
+    double alpha, beta, gamma, g11, g44m;
+    double loss_w;           /* loss weight added to lambda_n on FE cells */
+    double eps0;
+    int bclo[3], bchi[3];    /* potential BC kinds */
+    double bcv[6];           /* Dirichlet values of step n */
+    const double* eps;       /* cell permittivity */
+    double ih2[3];
+    int ghost_face;
+    double* part;            /* 7 * adj_blocks partial sums */
+};
+int adj_blocks(const fxd::Geom& g);
+void adj_fzT(hipStream_t s, const AdjArgs& a);
+void adj_step(hipStream_t s, const AdjArgs& a);
+void adj_accum(hipStream_t s, const double* part, int nb, double* acc, double* gV, int step);
+void adj_init(hipStream_t s, double* lam, const fxd::MView& M, const fxd::Geom& g, double w);
+/* out[0] = sum of P over the FE (non t-phase) cells; part: adj_blocks doubles */
+void fe_sum(hipStream_t s, const double* P, const fxd::MView& M, const fxd::Geom& g, double* part, double* out);
 }  // namespace fxk

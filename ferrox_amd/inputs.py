@@ -60,8 +60,13 @@ def from_text(text: str, **overrides) -> FxInputs:
 
 
 def _fmt(v) -> str:
-    if isinstance(v, (list, tuple)):
+    import numbers
+    if isinstance(v, (list, tuple)) or (hasattr(v, "ndim") and getattr(v, "ndim", 0) == 1):
         return " ".join(_fmt(x) for x in v)
-    if isinstance(v, float):
-        return repr(v)
+    if isinstance(v, bool):
+        return str(int(v))
+    if isinstance(v, numbers.Integral):
+        return str(int(v))
+    if isinstance(v, numbers.Real):
+        return repr(float(v))        # shortest round-trip text, numpy scalars included
     return str(v)

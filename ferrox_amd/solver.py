@@ -125,6 +125,35 @@ class FerroX:
         self._check(rc)
         return phi, stats_dict(st)
 
+    # -- differentiation form (extension) ------------------------------------
+    DIFF_PARAMS = ("alpha", "beta", "gamma", "g11", "g44", "epsilonX_fe")
+
+    def diff_set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float64)
+        assert t.shape == (6,)
+        self._check(self._lib.fx_diff_set_params(self._h, _dp(t)))
+
+    def diff_get_params(self) -> np.ndarray:
+        t = np.empty(6)
+        self._check(self._lib.fx_diff_get_params(self._h, _dp(t)))
+        return t
+
+    def diff_forward(self, V, K: int = 20) -> np.ndarray:
+        """Euler steps from the current state with top bias V[n] at step n;
+        returns the mean Pz over the ferroelectric cells after each step."""
+        v = np.ascontiguousarray(V, dtype=np.float64)
+        m = np.empty(v.size)
+        self._check(self._lib.fx_diff_forward(self._h, _dp(v), int(v.size), int(K), _dp(m)))
+        return m
+
+    def diff_backward(self, dL_dm) -> tuple[np.ndarray, np.ndarray]:
+        """(dL/dtheta, dL/dV) through the last diff_forward."""
+        g = np.ascontiguousarray(dL_dm, dtype=np.float64)
+        gt = np.empty(6)
+        gv = np.empty(g.size)
+        self._check(self._lib.fx_diff_backward(self._h, _dp(g), _dp(gt), _dp(gv)))
+        return gt, gv
+
     def bench_kernel(self, which: str, reps: int = 20) -> float:
         """Average ms per launch of a hot-path kernel (HIP events, context stream)."""
         ms = C.c_double()

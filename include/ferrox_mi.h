@@ -258,6 +258,22 @@ int fx_set_bias_waveform(fx_ctx* ctx, const double* v, int n);
  * on the context's stream.  No reference counterpart (instrumentation). */
 int fx_bench_kernel(fx_ctx* ctx, const char* which, int reps, double* avg_ms);
 
+/* ---- Differentiation form (extension; no counterpart in the reference) ----
+ * Discrete adjoint of the forward-Euler TDGL step (main.cpp:293-306) and its
+ * Poisson solve (ElectrostaticSolver.cpp:678-747) for the scalar, unrotated,
+ * semiconductor-free case, with checkpoint-and-recompute.  theta holds
+ * (alpha, beta, gamma, g11, g44, epsilonX_fe) in the inputs' units. */
+int fx_diff_set_params(fx_ctx* ctx, const double theta[6]);
+int fx_diff_get_params(fx_ctx* ctx, double theta[6]);
+/* nsteps Euler steps from the current state, step n with top bias V[n];
+ * P is kept every K steps.  m_out[n] = mean of Pz over the ferroelectric
+ * cells after step n (the P-V curve). */
+int fx_diff_forward(fx_ctx* ctx, const double* V, int nsteps, int K, double* m_out);
+/* Gradients of a loss L(m) through the last fx_diff_forward: given dL/dm
+ * (nsteps values), returns dL/dtheta (6) and dL/dV (nsteps; either output may
+ * be NULL).  The state is left as after the forward. */
+int fx_diff_backward(fx_ctx* ctx, const double* dL_dm, double* grad_theta, double* grad_V);
+
 /* Multigrid hierarchy (diagnostics). */
 int fx_num_levels(fx_ctx* ctx);
 int fx_level_dims(fx_ctx* ctx, int level, int dims[3]);
