@@ -145,3 +145,29 @@ def test_torch_autograd_function():
     assert np.allclose(th.grad.numpy(), gt, rtol=1e-12, atol=0)
     assert np.allclose(Vt.grad.numpy(), gv, rtol=1e-12, atol=1e-14 * np.abs(gv).max())
     assert abs(loss.item() - loss_of(m2, target)) <= 1e-14 * loss.item()
+
+
+def test_gradient_descent_fits_the_pv_curve():
+    """Fitting: a target curve from perturbed parameters; a few Adam steps on
+    log|theta| driven by the adjoint gradient reduce the misfit."""
+    import torch
+    inp, sim, P0, theta, V, _ = setup()
+    rng = np.random.default_rng(7)
+    theta_star = theta * (1 + rng.uniform(-0.1, 0.1, 6))
+    sim.diff_set_params(theta_star)
+    sim.set("Pz", P0)
+    target = torch.tensor(sim.diff_forward(V, 10))
+    sign = torch.tensor(np.sign(theta))
+    logt = torch.tensor(np.log(np.abs(theta)), requires_grad=True)
+    Vt = torch.tensor(V)
+    opt = torch.optim.Adam([logt], lr=0.02)
+    losses = []
+    for it in range(15):
+        opt.zero_grad()
+        m = pv_curve(sim, sign * torch.exp(logt), Vt, P0, K=10)
+        loss = torch.mean((m - target) ** 2) / torch.mean(target ** 2)
+        loss.backward()
+        opt.step()
+        losses.append(loss.item())
+    print("relative misfit", losses[0], "->", losses[-1])
+    assert losses[-1] < 0.2 * losses[0]
