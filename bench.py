@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+FP64-refinement leg")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
+    ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--diff-large", action="store_true", help="also time the adjoint at 128x128x64")
     ap.add_argument("--fused", type=int, default=1, help="one-pass streaming red+black sweep on the large levels (1) or per-colour kernels everywhere (0)")
     return ap.parse_args()
@@ -107,6 +109,41 @@ def diff_leg(large: bool) -> dict:
                         "backward_s": round(t2 - t1, 3),
                         "checkpoint_MB": round((nsteps // K + 1 + 2 * K) * P0.size * 8 / 2**20, 1)}
         sim.close()
+    return out
+
+
+def batch_leg(nsteps: int) -> dict:
+    """Batching form: 48 independent MFIM devices (lateral extents from
+    {32,48,64,96,128}^2 cells, 3-10 nm FE over 4 nm Al2O3 at dz = 0.25 nm,
+    bias U(-1, 1) V, per-device seeds), FP32 solve with one FP64 correction,
+    autotuned launch shapes, captured step graphs; the same batch in FP64 for
+    the error."""
+    import numpy as np
+    import ferrox_amd as fx
+    from ferrox_amd import configs
+    from ferrox_amd.batch import Batch
+
+    texts = configs.batch_devices(n=48, nsteps=nsteps)
+    out = {"workload": "48 MFIM devices, lateral {32..128}^2 x 0.5 nm, FE 3-10 nm / 4 nm Al2O3 at dz 0.25 nm, "
+                       "bias U(-1,1) V", "devices": 48}
+    res = {}
+    for prec, name in ((1, "fp32_refine"), (0, "fp64")):
+        b = Batch([fx.from_text(t, precision=prec) for t in texts])
+        tune = b.autotune()
+        b.step(10)
+        wall = b.step(nsteps)
+        N = b.cells
+        out["cells"] = N
+        out[name] = {"steps": nsteps, "ms_per_step": round(1e3 * wall / nsteps, 4),
+                     "batch_steps_per_s": round(nsteps / wall, 2),
+                     "device_steps_per_s": round(48 * nsteps / wall, 1),
+                     "cell_updates_per_s": round(N * nsteps / wall, 1), "autotune": tune["chosen"]}
+        res[name] = [(b.get(d, "Pz"), b.get(d, "Phi")) for d in range(48)]
+        b.close()
+    eP = max(float(np.abs(p32 - p64).max() / np.abs(p64).max()) for (p32, _), (p64, _) in zip(res["fp32_refine"], res["fp64"]))
+    eF = max(float(np.abs(f32 - f64).max() / np.abs(f64).max()) for (_, f32), (_, f64) in zip(res["fp32_refine"], res["fp64"]))
+    out["max_rel_err_fp32_vs_fp64"] = {"Pz": eP, "Phi": eF}
+    print(f"# batching: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -188,6 +225,7 @@ def main():
         print(f"# fp32+refine: {fp32}", file=sys.stderr, flush=True)
 
     diff = None if a.no_diff else diff_leg(a.diff_large)
+    batch = None if a.no_batch else batch_leg(a.batch_steps)
 
     cpu = None
     if a.cpu_steps > 0:
@@ -227,6 +265,7 @@ def main():
         "roofline": roof,
         "fp32_refine": fp32,
         "differentiation": diff,
+        "batching": batch,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -91,6 +91,17 @@ SIGNATURES = {
     "fx_set_bias": (C.c_int, [_P, C.c_double, C.c_double]),
     "fx_set_bias_waveform": (C.c_int, [_P, _DP, C.c_int]),
     "fx_diff_set_params": (C.c_int, [_P, _DP]),
+    "fx_batch_create": (C.c_int, [C.c_int, C.POINTER(C.POINTER(FxInputs)), C.POINTER(_P)]),
+    "fx_batch_destroy": (C.c_int, [_P]),
+    "fx_batch_last_error": (C.c_char_p, [_P]),
+    "fx_batch_num_devices": (C.c_int, [_P]),
+    "fx_batch_cells": (C.c_longlong, [_P]),
+    "fx_batch_dims": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "fx_batch_step": (C.c_int, [_P, C.c_int, C.c_int, _DP]),
+    "fx_batch_get_field": (C.c_int, [_P, C.c_int, C.c_int, _DP]),
+    "fx_batch_set_field": (C.c_int, [_P, C.c_int, C.c_int, _DP]),
+    "fx_batch_autotune": (C.c_int, [_P, C.c_char_p, C.c_int]),
+    "fx_batch_basis": (C.c_int, [C.c_int, _DP, _DP]),
     "fx_diff_get_params": (C.c_int, [_P, _DP]),
     "fx_diff_forward": (C.c_int, [_P, _DP, C.c_int, C.c_int, _DP]),
     "fx_diff_backward": (C.c_int, [_P, _DP, _DP, _DP]),

@@ -213,3 +213,22 @@ def mfim_poly_256(nsteps=500, seed=1) -> str:
     """The polycrystalline precision-form geometry: mfim_256 with the Euler
     convention of the grain rotations."""
     return mfim_256(nsteps=nsteps, seed=seed) + "\nuse_Euler_angles = 1\n"
+
+
+# ------------------------------------------------------------------ batching
+def batch_devices(n=48, seed=2025, nsteps=1000, sizes=(32, 48, 64, 96, 128)):
+    """The batching workload: n independent MFIM devices, lateral extents
+    drawn from `sizes` (cells of 0.5 nm, x and y independently), a 4 nm Al2O3
+    layer (16 cells of 0.25 nm) under an FE layer of U(3, 10) nm (12-40
+    cells), top bias U(-1, 1) V, per-device seed.  Returns inputs texts."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    texts = []
+    for d in range(n):
+        nx, ny = (int(v) for v in rng.choice(sizes, 2))
+        nfe = int(round(rng.uniform(3e-9, 10e-9) / 0.25e-9))
+        nz = 16 + nfe
+        bias = float(rng.uniform(-1.0, 1.0))
+        texts.append(mfim(nx=nx, ny=ny, nz=nz, lx=nx * 0.5e-9, ly=ny * 0.5e-9, lz=nz * 0.25e-9, de_top=4e-9,
+                          fe_top=nz * 0.25e-9, nsteps=nsteps, seed=int(rng.integers(1, 2**31 - 1)), bias=bias))
+    return texts

@@ -1546,3 +1546,21 @@ int fx_level_dims(fx_ctx* x, int l, int dims[3]) {
 }
 
 }  // extern "C"
+
+/* Library-internal (fx_batch.hip): the material id on the grown box, its
+ * strides, the inputs and the per-plane Poisson coefficients of level 0
+ * (layered levels only). */
+int fxi_export(fx_ctx* x, std::vector<uint8_t>& mat, int& msx, int& msy, fx_inputs& in, double ih2[3],
+               std::vector<double>& epsk, std::vector<double>& bzk, double bv[6]) {
+    if (!x) return FX_EINVAL;
+    mat = x->h_mat; msx = x->msx; msy = x->msy; in = x->in;
+    const DLev& d = x->lev[0].d;
+    for (int q = 0; q < 3; ++q) ih2[q] = d.ih2[q];
+    if (!d.layered) return fail(x, FX_EINVAL, "batched devices must be laterally uniform (layered) stacks");
+    const int nz = x->n[2];
+    epsk.resize(nz); bzk.resize(nz + 1);
+    HIPCHK(x, hipMemcpy(epsk.data(), d.bxk, nz * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(x, hipMemcpy(bzk.data(), d.bzk, (nz + 1) * sizeof(double), hipMemcpyDeviceToHost));
+    bc_values(x, bv);
+    return FX_OK;
+}

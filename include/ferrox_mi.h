@@ -274,6 +274,33 @@ int fx_diff_forward(fx_ctx* ctx, const double* V, int nsteps, int K, double* m_o
  * be NULL).  The state is left as after the forward. */
 int fx_diff_backward(fx_ctx* ctx, const double* dL_dm, double* grad_theta, double* grad_V);
 
+/* ---- Batching form (extension; no counterpart in the reference) ----------
+ * n independent devices, each given by its own inputs (periodic in x and y,
+ * Dirichlet z faces, laterally uniform layers, scalar P, forward Euler,
+ * lateral extents <= 128), advanced together: every stage of a step is one
+ * persistent launch over all devices, a step pair is a captured hipGraph,
+ * and Poisson is solved directly (lateral real-Fourier transforms + a
+ * tridiagonal solve per mode; FP32 with one FP64 residual correction when
+ * inputs[0]->precision is FX_PREC_FP32_REFINE).  Set-up of each device
+ * (materials, permittivity, initial P, bias) is that of fx_create. */
+typedef struct fx_batch fx_batch;
+int fx_batch_create(int n, const fx_inputs* const* inputs, fx_batch** out);
+int fx_batch_destroy(fx_batch* batch);
+const char* fx_batch_last_error(fx_batch* batch);
+int fx_batch_num_devices(fx_batch* batch);
+long long fx_batch_cells(fx_batch* batch);
+int fx_batch_dims(fx_batch* batch, int dev, int dims[3]);
+/* nsteps steps of every device (use_graph: replay the captured step pair) */
+int fx_batch_step(fx_batch* batch, int nsteps, int use_graph, double* wall_seconds);
+/* FX_FIELD_PZ or FX_FIELD_PHI of one device (host array, i fastest) */
+int fx_batch_get_field(fx_batch* batch, int dev, int field, double* host);
+int fx_batch_set_field(fx_batch* batch, int dev, int field, const double* host);
+/* measure and choose tile shapes and grid sizes; JSON summary into report */
+int fx_batch_autotune(fx_batch* batch, char* report, int report_len);
+/* the real Fourier basis of an n-point periodic direction (n x n, column
+ * vectors) and the eigenvalues of the 1-D periodic second difference */
+int fx_batch_basis(int n, double* Q, double* mu);
+
 /* Multigrid hierarchy (diagnostics). */
 int fx_num_levels(fx_ctx* ctx);
 int fx_level_dims(fx_ctx* ctx, int level, int dims[3]);
