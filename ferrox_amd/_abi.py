@@ -53,6 +53,7 @@ class FxInputs(C.Structure):
         ("mg_warm_start", C.c_int), ("outer_max_iter", C.c_int), ("phi_ghost_mode", C.c_int),
         ("precision", C.c_int), ("bias_ramp_steps", C.c_int), ("bias_ramp_start", C.c_double),
         ("bias_ramp_end", C.c_double), ("verbose", C.c_int), ("mg_fused_sweep", C.c_int),
+        ("plot_Phi", C.c_int), ("plot_PoissonRHS", C.c_int), ("plot_E", C.c_int), ("plot_holes", C.c_int), ("plot_electrons", C.c_int), ("plot_charge", C.c_int), ("plot_epsilon", C.c_int), ("plot_mask", C.c_int), ("plot_tphase", C.c_int), ("plot_alpha", C.c_int), ("plot_beta", C.c_int), ("plot_theta", C.c_int), ("plot_PhiDiff", C.c_int),
     ]
 
 
@@ -91,6 +92,8 @@ SIGNATURES = {
     "fx_set_bias": (C.c_int, [_P, C.c_double, C.c_double]),
     "fx_set_bias_waveform": (C.c_int, [_P, _DP, C.c_int]),
     "fx_diff_set_params": (C.c_int, [_P, _DP]),
+    "fx_write_plotfile": (C.c_int, [_P, C.c_char_p, C.c_double, C.c_int]),
+    "fx_set_plot_dir": (C.c_int, [_P, C.c_char_p]),
     "fx_batch_create": (C.c_int, [C.c_int, C.POINTER(C.POINTER(FxInputs)), C.POINTER(_P)]),
     "fx_batch_destroy": (C.c_int, [_P]),
     "fx_batch_last_error": (C.c_char_p, [_P]),

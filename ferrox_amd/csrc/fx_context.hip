@@ -128,6 +128,7 @@ struct fx_ctx {
         double* part = nullptr;      /* 7 x adj_blocks partials */
         size_t cap_ck = 0, cap_segP = 0, cap_segPhi = 0, cap_acc = 0, cap_m = 0;   /* bytes allocated */
     } diff;
+    std::string plot_dir = ".";   /* where plt%08d directories go */
     int step = 0, sign = 1, num_Vapp = 0, steady_state_step = 1000000, inc_step = -1;
     double time = 0.0;
     std::vector<double> waveform;
@@ -1223,6 +1224,13 @@ int fx_step(fx_ctx* x, int nsteps, fx_step_stats* out) {
     auto t0 = std::chrono::steady_clock::now();
     const double tiny = 1.e-6;
     int rc = FX_OK;
+    auto plot = [&](int step) -> int {
+        char name[32];
+        std::snprintf(name, sizeof name, "plt%08d", step);
+        return fx_write_plotfile(x, (x->plot_dir + "/" + name).c_str(), x->time, step);
+    };
+    /* plotfile of the initial data (main.cpp:228-234) */
+    if (in.plot_int > 0 && x->step == 0 && (rc = plot(0))) return fail(x, rc, "writing plt00000000 failed");
     for (int it = 0; it < nsteps; ++it) {
         const int step = ++x->step;
         if (!x->waveform.empty()) x->Phi_Bc_hi = x->waveform[std::min((size_t)step, x->waveform.size()) - 1];
@@ -1238,6 +1246,11 @@ int fx_step(fx_ctx* x, int nsteps, fx_step_stats* out) {
         x->time = x->time + in.dt;
         st.steps_done++;
         if (in.verbose) std::printf("Advanced step %d\n", step);
+        /* main.cpp:563-569 */
+        if (in.plot_int > 0 && (step % in.plot_int == 0 || step == x->steady_state_step) && (rc = plot(step))) {
+            rc = fail(x, rc, "writing a plotfile failed");
+            break;
+        }
         if (in.voltage_sweep == 1 && x->inc_step > 0 && step == x->inc_step) {
             x->Phi_Bc_hi += x->sign * in.Phi_Bc_inc;
             x->num_Vapp += 1;
@@ -1258,6 +1271,12 @@ int fx_step(fx_ctx* x, int nsteps, fx_step_stats* out) {
     st.wall_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     if (out) *out = st;
     return rc;
+}
+
+int fx_set_plot_dir(fx_ctx* x, const char* dir) {
+    if (!x || !dir) return FX_EINVAL;
+    x->plot_dir = dir;
+    return FX_OK;
 }
 
 int fx_poisson_solve(fx_ctx* x, const double* rhs, const double* acoef, double* phi, fx_solve_stats* st) {
@@ -1564,3 +1583,5 @@ int fxi_export(fx_ctx* x, std::vector<uint8_t>& mat, int& msx, int& msy, fx_inpu
     bc_values(x, bv);
     return FX_OK;
 }
+
+const fx_inputs* fxi_inputs(fx_ctx* x) { return x ? &x->in : nullptr; }

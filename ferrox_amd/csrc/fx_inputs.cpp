@@ -240,6 +240,7 @@ struct Applier {
         if (key == "boundary.lo") return bc(v, I.bc_lo, key);
         if (key == "boundary.hi") return bc(v, I.bc_hi, key);
         I1(TimeIntegratorOrder) I1(prob_type) I1(is_polarization_scalar) I1(nsteps) I1(plot_int)
+        I1(plot_Phi) I1(plot_PoissonRHS) I1(plot_E) I1(plot_holes) I1(plot_electrons) I1(plot_charge) I1(plot_epsilon) I1(plot_mask) I1(plot_tphase) I1(plot_alpha) I1(plot_beta) I1(plot_theta) I1(plot_PhiDiff)
         D1(dt) D1(delta) I1(random_seed) I1(voltage_sweep) I1(inc_step) I1(num_Vapp_max)
         D1(Phi_Bc_lo) D1(Phi_Bc_hi) D1(Phi_Bc_inc) D1(Phi_Bc_hi_max) D1(phi_tolerance)
         I1(include_Landau) I1(include_Grad) I1(include_Elec)
@@ -313,6 +314,7 @@ int fx_inputs_default(fx_inputs* in) {
     in->is_polarization_scalar = 1;                    /* :333 */
     in->nsteps = 10;                                   /* :365 */
     in->plot_int = -1;                                 /* :370 */
+    in->plot_Phi = 1; in->plot_PoissonRHS = 1; in->plot_E = 1; in->plot_holes = 1; in->plot_electrons = 1; in->plot_charge = 1; in->plot_epsilon = 1; in->plot_mask = 1; in->plot_tphase = 1; in->plot_alpha = 1; in->plot_beta = 1; in->plot_theta = 1; in->plot_PhiDiff = 1;   /* :300-325 */
     in->dt = UNSET;
     in->delta = 1.e-6;                                 /* :376 */
     in->random_seed = 1;                               /* :409 */

@@ -125,6 +125,15 @@ class FerroX:
         self._check(rc)
         return phi, stats_dict(st)
 
+    # -- plotfiles -----------------------------------------------------------
+    def write_plotfile(self, directory: str, time: float = 0.0, step: int = 0):
+        """AMReX plotfile of the current state (WritePlotfile, Plotfile.cpp:5-177)."""
+        self._check(self._lib.fx_write_plotfile(self._h, str(directory).encode(), float(time), int(step)))
+
+    def set_plot_dir(self, directory: str):
+        """Where fx_step puts its plt%08d directories when plot_int > 0."""
+        self._check(self._lib.fx_set_plot_dir(self._h, str(directory).encode()))
+
     # -- differentiation form (extension) ------------------------------------
     DIFF_PARAMS = ("alpha", "beta", "gamma", "g11", "g44", "epsilonX_fe")
 

@@ -133,6 +133,9 @@ typedef struct fx_inputs {
     double bias_ramp_end;
     int    verbose;
     int    mg_fused_sweep;        /* 1: one-pass streaming red+black sweep on levels of >= 2M cells (default 1); 0: two launches per sweep */
+    /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
+    int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
+    int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;
 } fx_inputs;
 
 /* Field identifiers.  0..17 follow the plotfile variable order of the
@@ -247,6 +250,14 @@ int fx_step(fx_ctx* ctx, int nsteps, fx_step_stats* stats);
  * replaces MLMG::solve, ElectrostaticSolver.cpp:719).  acoef may be NULL (0). */
 int fx_poisson_solve(fx_ctx* ctx, const double* rhs, const double* acoef,
                      double* phi, fx_solve_stats* stats);
+
+/* AMReX single-level plotfile of the current state into directory dir
+ * (WritePlotfile, Plotfile.cpp:5-177; variables gated by the plot_* inputs
+ * flags).  fx_step writes plt%08d itself when plot_int > 0, at step 0 and
+ * every plot_int steps and at the steady-state step (main.cpp:228-234,
+ * 563-569), into the directory set by fx_set_plot_dir (default "."). */
+int fx_write_plotfile(fx_ctx* ctx, const char* dir, double time, int step);
+int fx_set_plot_dir(fx_ctx* ctx, const char* dir);
 
 /* Per-step top bias (extension: the reference changes Phi_Bc_hi only at
  * steady-state / inc_step events, main.cpp:572-604).  Step s (1-based, counted
