@@ -54,6 +54,8 @@ class FxInputs(C.Structure):
         ("precision", C.c_int), ("bias_ramp_steps", C.c_int), ("bias_ramp_start", C.c_double),
         ("bias_ramp_end", C.c_double), ("verbose", C.c_int), ("mg_fused_sweep", C.c_int),
         ("plot_Phi", C.c_int), ("plot_PoissonRHS", C.c_int), ("plot_E", C.c_int), ("plot_holes", C.c_int), ("plot_electrons", C.c_int), ("plot_charge", C.c_int), ("plot_epsilon", C.c_int), ("plot_mask", C.c_int), ("plot_tphase", C.c_int), ("plot_alpha", C.c_int), ("plot_beta", C.c_int), ("plot_theta", C.c_int), ("plot_PhiDiff", C.c_int),
+        ("tphase_geom_function", C.c_char * 4096), ("angle_alpha_function", C.c_char * 2048),
+        ("angle_beta_function", C.c_char * 2048), ("angle_theta_function", C.c_char * 2048),
     ]
 
 
@@ -92,6 +94,7 @@ SIGNATURES = {
     "fx_set_bias": (C.c_int, [_P, C.c_double, C.c_double]),
     "fx_set_bias_waveform": (C.c_int, [_P, _DP, C.c_int]),
     "fx_diff_set_params": (C.c_int, [_P, _DP]),
+    "fx_eval_cell_function": (C.c_int, [C.POINTER(FxInputs), C.c_char_p, _DP, C.c_char_p, C.c_int]),
     "fx_write_plotfile": (C.c_int, [_P, C.c_char_p, C.c_double, C.c_int]),
     "fx_set_plot_dir": (C.c_int, [_P, C.c_char_p]),
     "fx_batch_create": (C.c_int, [C.c_int, C.POINTER(C.POINTER(FxInputs)), C.POINTER(_P)]),

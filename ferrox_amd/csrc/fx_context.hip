@@ -969,8 +969,27 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
     x->h_tphase.assign(x->N, 0.0);
     for (int d = 0; d < 3; ++d) x->h_ang[d].assign(x->N, 0.0);
     x->h_lalpha.assign(x->N, in.alpha);
+    if (in.Coordinate_Transformation == 1) {
+        /* Initialize_tphase_Mask / Initialize_Euler_angles (Initialization.cpp:253-383):
+         * the functions of the cell centre; the reference dereferences an
+         * absent parser, so a missing function is an error here */
+        const char* fns[4] = {in.tphase_geom_function, in.angle_alpha_function, in.angle_beta_function, in.angle_theta_function};
+        const char* keys[4] = {"tphase_geom.tphase_geom_function(x,y,z)", "angle_alpha.alpha_function(x,y,z)",
+                               "angle_beta.beta_function(x,y,z)", "angle_theta.theta_function(x,y,z)"};
+        std::vector<double>* dst[4] = {&x->h_tphase, &x->h_ang[0], &x->h_ang[1], &x->h_ang[2]};
+        for (int q = 0; q < 4; ++q) {
+            char eb[256] = {0};
+            if (!fns[q][0]) rc = fail(x, FX_EINVAL, std::string("Coordinate_Transformation = 1 needs ") + keys[q]);
+            else if ((rc = fx_eval_cell_function(&in, fns[q], dst[q]->data(), eb, sizeof eb))) rc = fail(x, rc, std::string(keys[q]) + ": " + eb);
+            if (rc) { std::fprintf(stderr, "fx_create: %s\n", x->err.c_str()); free_all(x); delete x; return rc; }
+        }
+    }
     mask_from_boxes(x, x->h_mat);
     periodic_fill_u8(x, x->h_mat);
+    for (int k = 0; k < x->n[2]; ++k)
+        for (int j = 0; j < x->n[1]; ++j)
+            for (int i = 0; i < x->n[0]; ++i)
+                if (x->h_tphase[(size_t)i + (size_t)x->n[0] * (j + (size_t)x->n[1] * k)] == 1.0) x->h_mat[gidx(x, i, j, k)] |= 4;
     if ((rc = upload_mat(x)) || (rc = build_coefficients(x)) || (rc = refresh_rotation(x))) {
         std::fprintf(stderr, "fx_create: %s\n", x->err.c_str()); free_all(x); delete x; return rc;
     }

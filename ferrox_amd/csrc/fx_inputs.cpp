@@ -121,6 +121,196 @@ struct Expr {
     }
 };
 
+/* ------------------ functions of the cell centre (x, y, z) ---------------- */
+/* Compiled once, evaluated per cell: the operator subset of the AMReX parser
+ * the reference builds its geometry functions with (makeParser,
+ * Utils/SelectWarpXUtils/WarpXUtil.cpp:101-168).  Precedence, lowest first:
+ * or, and, comparisons, + -, * /, unary +-, ^ ** (right associative). */
+struct FnNode {
+    int op;          /* 0 const, 1 x, 2 y, 3 z, 10+ binary, 40+ unary fn, 60+ binary fn, 70 if */
+    double v;
+    int a, b, c;
+};
+struct Fn {
+    std::vector<FnNode> nodes;
+    int root = -1;
+    double eval(int i, double x, double y, double z) const {
+        const FnNode& n = nodes[i];
+        switch (n.op) {
+        case 0: return n.v;
+        case 1: return x;
+        case 2: return y;
+        case 3: return z;
+        case 10: return eval(n.a, x, y, z) + eval(n.b, x, y, z);
+        case 11: return eval(n.a, x, y, z) - eval(n.b, x, y, z);
+        case 12: return eval(n.a, x, y, z) * eval(n.b, x, y, z);
+        case 13: return eval(n.a, x, y, z) / eval(n.b, x, y, z);
+        case 14: return std::pow(eval(n.a, x, y, z), eval(n.b, x, y, z));
+        case 20: return eval(n.a, x, y, z) < eval(n.b, x, y, z) ? 1.0 : 0.0;
+        case 21: return eval(n.a, x, y, z) > eval(n.b, x, y, z) ? 1.0 : 0.0;
+        case 22: return eval(n.a, x, y, z) <= eval(n.b, x, y, z) ? 1.0 : 0.0;
+        case 23: return eval(n.a, x, y, z) >= eval(n.b, x, y, z) ? 1.0 : 0.0;
+        case 24: return eval(n.a, x, y, z) == eval(n.b, x, y, z) ? 1.0 : 0.0;
+        case 25: return eval(n.a, x, y, z) != eval(n.b, x, y, z) ? 1.0 : 0.0;
+        case 30: return (eval(n.a, x, y, z) != 0.0 && eval(n.b, x, y, z) != 0.0) ? 1.0 : 0.0;
+        case 31: return (eval(n.a, x, y, z) != 0.0 || eval(n.b, x, y, z) != 0.0) ? 1.0 : 0.0;
+        case 39: return -eval(n.a, x, y, z);
+        case 40: return std::sqrt(eval(n.a, x, y, z));
+        case 41: return std::exp(eval(n.a, x, y, z));
+        case 42: return std::log(eval(n.a, x, y, z));
+        case 43: return std::log10(eval(n.a, x, y, z));
+        case 44: return std::sin(eval(n.a, x, y, z));
+        case 45: return std::cos(eval(n.a, x, y, z));
+        case 46: return std::tan(eval(n.a, x, y, z));
+        case 47: return std::asin(eval(n.a, x, y, z));
+        case 48: return std::acos(eval(n.a, x, y, z));
+        case 49: return std::atan(eval(n.a, x, y, z));
+        case 50: return std::sinh(eval(n.a, x, y, z));
+        case 51: return std::cosh(eval(n.a, x, y, z));
+        case 52: return std::tanh(eval(n.a, x, y, z));
+        case 53: return std::fabs(eval(n.a, x, y, z));
+        case 54: return std::floor(eval(n.a, x, y, z));
+        case 55: return std::ceil(eval(n.a, x, y, z));
+        case 60: return std::pow(eval(n.a, x, y, z), eval(n.b, x, y, z));
+        case 61: return std::atan2(eval(n.a, x, y, z), eval(n.b, x, y, z));
+        case 62: return std::fmin(eval(n.a, x, y, z), eval(n.b, x, y, z));
+        case 63: return std::fmax(eval(n.a, x, y, z), eval(n.b, x, y, z));
+        case 64: return std::fmod(eval(n.a, x, y, z), eval(n.b, x, y, z));
+        case 70: return eval(n.a, x, y, z) != 0.0 ? eval(n.b, x, y, z) : eval(n.c, x, y, z);
+        }
+        return 0.0;
+    }
+    double operator()(double x, double y, double z) const { return eval(root, x, y, z); }
+};
+
+struct FnParser {
+    const std::string& s;
+    const std::map<std::string, double>& consts;
+    Fn& f;
+    size_t p = 0;
+    bool ok = true;
+    std::string why;
+    FnParser(const std::string& str, const std::map<std::string, double>& c, Fn& out) : s(str), consts(c), f(out) {}
+    int node(int op, double v = 0, int a = -1, int b = -1, int c = -1) { f.nodes.push_back({op, v, a, b, c}); return (int)f.nodes.size() - 1; }
+    void ws() { while (p < s.size() && std::isspace((unsigned char)s[p])) ++p; }
+    bool lit(const char* t) {
+        ws();
+        const size_t n = std::strlen(t);
+        if (s.compare(p, n, t) == 0) {
+            if (std::isalpha((unsigned char)t[0]) && p + n < s.size() && (std::isalnum((unsigned char)s[p + n]) || s[p + n] == '_')) return false;
+            p += n;
+            return true;
+        }
+        return false;
+    }
+    int fail(const std::string& m) { if (ok) { ok = false; why = m; } return node(0, 0.0); }
+    int parse() { int r = orx(); ws(); if (p != s.size()) fail("trailing characters at '" + s.substr(p, 16) + "'"); f.root = r; return r; }
+    int orx() { int a = andx(); while (ok && (lit("||") || lit("or"))) a = node(31, 0, a, andx()); return a; }
+    int andx() { int a = cmp(); while (ok && (lit("&&") || lit("and"))) a = node(30, 0, a, cmp()); return a; }
+    int cmp() {
+        int a = sum();
+        if (lit("<=")) return node(22, 0, a, sum());
+        if (lit(">=")) return node(23, 0, a, sum());
+        if (lit("==")) return node(24, 0, a, sum());
+        if (lit("!=")) return node(25, 0, a, sum());
+        if (lit("<")) return node(20, 0, a, sum());
+        if (lit(">")) return node(21, 0, a, sum());
+        return a;
+    }
+    int sum() {
+        int a = prod();
+        for (;;) {
+            ws();
+            if (p < s.size() && s[p] == '+') { ++p; a = node(10, 0, a, prod()); }
+            else if (p < s.size() && s[p] == '-') { ++p; a = node(11, 0, a, prod()); }
+            else return a;
+        }
+    }
+    int prod() {
+        int a = unary();
+        for (;;) {
+            ws();
+            if (p + 1 < s.size() && s[p] == '*' && s[p + 1] == '*') return a;
+            if (p < s.size() && s[p] == '*') { ++p; a = node(12, 0, a, unary()); }
+            else if (p < s.size() && s[p] == '/') { ++p; a = node(13, 0, a, unary()); }
+            else return a;
+        }
+    }
+    int unary() {
+        ws();
+        if (p < s.size() && s[p] == '-') { ++p; return node(39, 0, unary()); }
+        if (p < s.size() && s[p] == '+') { ++p; return unary(); }
+        return power();
+    }
+    int power() {
+        int b = atom();
+        ws();
+        if (p < s.size() && s[p] == '^') { ++p; return node(14, 0, b, unary()); }
+        if (p + 1 < s.size() && s[p] == '*' && s[p + 1] == '*') { p += 2; return node(14, 0, b, unary()); }
+        return b;
+    }
+    int atom() {
+        ws();
+        if (p >= s.size()) return fail("unexpected end of expression");
+        if (s[p] == '(') { ++p; int v = orx(); ws(); if (p < s.size() && s[p] == ')') ++p; else fail("missing ')'"); return v; }
+        if (std::isdigit((unsigned char)s[p]) || s[p] == '.') {
+            const char* b = s.c_str() + p;
+            char* e = nullptr;
+            double v = std::strtod(b, &e);
+            if (e == b) return fail("bad number");
+            p += (size_t)(e - b);
+            return node(0, v);
+        }
+        if (std::isalpha((unsigned char)s[p]) || s[p] == '_') {
+            size_t b = p;
+            while (p < s.size() && (std::isalnum((unsigned char)s[p]) || s[p] == '_' || s[p] == '.')) ++p;
+            const std::string name = s.substr(b, p - b);
+            ws();
+            if (p < s.size() && s[p] == '(') {
+                ++p;
+                std::vector<int> args;
+                ws();
+                if (p < s.size() && s[p] == ')') ++p;
+                else {
+                    for (;;) {
+                        args.push_back(orx());
+                        ws();
+                        if (p < s.size() && s[p] == ',') { ++p; continue; }
+                        if (p < s.size() && s[p] == ')') { ++p; break; }
+                        return fail("missing ')' after arguments of " + name);
+                    }
+                }
+                static const char* f1[] = {"sqrt", "exp", "log", "log10", "sin", "cos", "tan", "asin", "acos", "atan",
+                                           "sinh", "cosh", "tanh", "abs", "floor", "ceil"};
+                for (int q = 0; q < 16; ++q)
+                    if (name == f1[q]) return args.size() == 1 ? node(40 + q, 0, args[0]) : fail(name + " takes 1 argument");
+                if (name == "fabs") return args.size() == 1 ? node(53, 0, args[0]) : fail("fabs takes 1 argument");
+                static const char* f2[] = {"pow", "atan2", "min", "max", "fmod"};
+                for (int q = 0; q < 5; ++q)
+                    if (name == f2[q]) return args.size() == 2 ? node(60 + q, 0, args[0], args[1]) : fail(name + " takes 2 arguments");
+                if (name == "if") return args.size() == 3 ? node(70, 0, args[0], args[1], args[2]) : fail("if takes 3 arguments");
+                return fail("unknown function " + name);
+            }
+            if (name == "x") return node(1);
+            if (name == "y") return node(2);
+            if (name == "z") return node(3);
+            if (name == "pi") return node(0, 3.14159265358979323846);
+            auto it = consts.find(name);
+            if (it != consts.end()) return node(0, it->second);
+            return fail("unknown name '" + name + "'");
+        }
+        return fail(std::string("unexpected '") + s[p] + "'");
+    }
+};
+
+bool compile_fn(const std::string& s, const std::map<std::string, double>& consts, Fn& f, std::string& why) {
+    f.nodes.clear();
+    FnParser P(s, consts, f);
+    P.parse();
+    if (!P.ok) { why = P.why; return false; }
+    return true;
+}
+
 /* --------------------------- key/value table ------------------------------ */
 struct Table {
     std::map<std::string, std::vector<std::string>> kv;   /* last definition wins */
@@ -220,6 +410,45 @@ struct Applier {
         return true;
     }
 
+    /* a function of (x,y,z): the value text with my_constants replaced by
+     * their values (checked by compiling it) */
+    bool func(const std::vector<std::string>& vals, char* dst, size_t cap, const std::string& key) {
+        std::string src;
+        for (size_t q = 0; q < vals.size(); ++q) src += (q ? " " : "") + vals[q];
+        std::string out;
+        for (size_t i = 0; i < src.size();) {
+            if (std::isalpha((unsigned char)src[i]) || src[i] == '_') {
+                size_t b = i;
+                while (i < src.size() && (std::isalnum((unsigned char)src[i]) || src[i] == '_' || src[i] == '.')) ++i;
+                const std::string name = src.substr(b, i - b);
+                auto it = consts.find(name);
+                if (it != consts.end()) {
+                    char buf[64];
+                    std::snprintf(buf, sizeof buf, "(%.17g)", it->second);
+                    out += buf;
+                } else {
+                    out += name;
+                }
+            } else if (std::isdigit((unsigned char)src[i]) || src[i] == '.') {
+                /* a number, with its exponent, copied whole */
+                const char* b = src.c_str() + i;
+                char* e = nullptr;
+                (void)std::strtod(b, &e);
+                const size_t n = e > b ? (size_t)(e - b) : 1;
+                out += src.substr(i, n);
+                i += n;
+            } else {
+                out += src[i++];
+            }
+        }
+        Fn f;
+        std::string w;
+        if (!compile_fn(out, {}, f, w)) { why = key + ": " + w; return false; }
+        if (out.size() + 1 > cap) { why = key + ": function text longer than " + std::to_string(cap - 1) + " characters"; return false; }
+        std::memcpy(dst, out.c_str(), out.size() + 1);
+        return true;
+    }
+
     bool apply(const std::string& key, const std::vector<std::string>& v) {
         fx_inputs& I = *in;
 #define D1(name) if (key == #name) return dbl(v, &I.name, 1, key);
@@ -237,6 +466,10 @@ struct Applier {
             if (eb) { why = "domain.embedded_boundary = 1 is not supported (embedded boundaries are out of scope)"; return false; }
             return true;
         }
+        if (key == "tphase_geom.tphase_geom_function(x,y,z)") return func(v, I.tphase_geom_function, sizeof(I.tphase_geom_function), key);
+        if (key == "angle_alpha.alpha_function(x,y,z)") return func(v, I.angle_alpha_function, sizeof(I.angle_alpha_function), key);
+        if (key == "angle_beta.beta_function(x,y,z)") return func(v, I.angle_beta_function, sizeof(I.angle_beta_function), key);
+        if (key == "angle_theta.theta_function(x,y,z)") return func(v, I.angle_theta_function, sizeof(I.angle_theta_function), key);
         if (key == "boundary.lo") return bc(v, I.bc_lo, key);
         if (key == "boundary.hi") return bc(v, I.bc_hi, key);
         I1(TimeIntegratorOrder) I1(prob_type) I1(is_polarization_scalar) I1(nsteps) I1(plot_int)
@@ -419,6 +652,22 @@ int fx_inputs_set(fx_inputs* in, const char* kv, char* err, int errlen) {
     Table t;
     std::string why;
     if (!parse_text(kv, t, why) || !apply_table(t, in, why)) { set_err(err, errlen, why); return FX_EINVAL; }
+    return FX_OK;
+}
+
+int fx_eval_cell_function(const fx_inputs* in, const char* expr, double* out, char* err, int errlen) {
+    if (!in || !expr || !out) return FX_EINVAL;
+    Fn f;
+    std::string why;
+    if (!compile_fn(expr, {}, f, why)) { set_err(err, errlen, why); return FX_EINVAL; }
+    const int nx = in->n_cell[0], ny = in->n_cell[1], nz = in->n_cell[2];
+    double dx[3];
+    for (int d = 0; d < 3; ++d) dx[d] = (in->prob_hi[d] - in->prob_lo[d]) / in->n_cell[d];
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                out[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] =
+                    f(in->prob_lo[0] + (i + 0.5) * dx[0], in->prob_lo[1] + (j + 0.5) * dx[1], in->prob_lo[2] + (k + 0.5) * dx[2]);
     return FX_OK;
 }
 

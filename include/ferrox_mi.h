@@ -136,7 +136,23 @@ typedef struct fx_inputs {
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;
+    /* Coordinate_Transformation = 1: functions of the cell centre (x,y,z)
+     * (Initialization.cpp:253-383), my_constants substituted; keys
+     * tphase_geom.tphase_geom_function(x,y,z), angle_alpha.alpha_function(x,y,z),
+     * angle_beta.beta_function(x,y,z), angle_theta.theta_function(x,y,z) */
+    char   tphase_geom_function[4096];
+    char   angle_alpha_function[2048];
+    char   angle_beta_function[2048];
+    char   angle_theta_function[2048];
 } fx_inputs;
+
+/* Evaluates a function of the cell centre (x,y,z) -- the AMReX parser
+ * subset: + - * / ^ **, comparisons < > <= >= == != (1 or 0), and/or,
+ * if(c,a,b), sqrt exp log log10 sin cos tan asin acos atan sinh cosh tanh abs
+ * floor ceil, pow atan2 min max fmod, pi -- at every cell centre of the
+ * inputs' domain, i fastest (ConvertParserIntoMultiFab_3vars,
+ * eXstaticUtil.cpp). */
+int fx_eval_cell_function(const fx_inputs* in, const char* expr, double* out, char* err, int errlen);
 
 /* Field identifiers.  0..17 follow the plotfile variable order of the
  * reference (Plotfile.cpp:30-106). */

@@ -237,3 +237,41 @@ def test_polycrystal_mixed_precision_vs_fp64(relerr):
         e = relerr(g32.get(name), g64.get(name))
         print(name, e)
         assert e < 1e-6, name
+
+
+def test_coordinate_transformation_functions(relerr):
+    """Coordinate_Transformation = 1: t-phase mask and Euler angles from
+    functions of the cell centre (Initialization.cpp:253-383); the oracle gets
+    the same fields set explicitly from numpy evaluations."""
+    text = configs.mfim(nx=32, ny=24, nz=36, lx=16e-9, ly=12e-9, nsteps=1, bias=0.3) + """
+is_polarization_scalar = 1
+Coordinate_Transformation = 1
+use_Euler_angles = 1
+tphase_geom.tphase_geom_function(x,y,z) = "1.*(x >= -2.e-9)*(x <= 2.e-9)*(z > 5.e-9)*(z < 7.e-9)"
+angle_alpha.alpha_function(x,y,z) = "20*(y > 0)"
+angle_beta.beta_function(x,y,z) = "35*(x > 0) + 10"
+angle_theta.theta_function(x,y,z) = "15"
+"""
+    inp = fx.from_text(text, mg_tol_rel=1e-13)
+    g = fx.FerroX(inp)
+    nx, ny, nz = inp.n_cell
+    lo, hi = np.array(inp.prob_lo[:]), np.array(inp.prob_hi[:])
+    h = (hi - lo) / np.array([nx, ny, nz])
+    z, y, x = np.meshgrid(*[lo[d] + (np.arange(n) + 0.5) * h[d] for d, n in ((2, nz), (1, ny), (0, nx))], indexing="ij")
+    tph = 1.0 * (x >= -2e-9) * (x <= 2e-9) * (z > 5e-9) * (z < 7e-9)
+    assert tph.sum() > 0
+    assert np.array_equal(g.get("tphase"), tph)
+    assert np.array_equal(g.get("alpha"), 20.0 * (y > 0))
+    assert np.array_equal(g.get("beta"), 35.0 * (x > 0) + 10)
+    assert np.array_equal(g.get("theta"), np.full_like(x, 15.0))
+    assert np.all(g.get("Pz")[tph == 1.0] == 0.0)
+    plain = text.replace("Coordinate_Transformation = 1", "Coordinate_Transformation = 0")
+    o = Oracle(fx.from_text(plain, mg_tol_rel=1e-13))
+    o.set("tphase", tph)
+    for nm in ("alpha", "beta", "theta", "Pz"):
+        o.set(nm, g.get(nm))
+    g.step(10); o.step(10)
+    for name in ("Pz", "Phi", "epsilon"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < TOL, name
