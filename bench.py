@@ -48,6 +48,20 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic():
+    """HBM bytes per launch of the finest-level sweep from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-kernel calibrated; the
+    counters cannot be read inside this process)."""
+    path = os.path.join(ROOT, "profiles", "traffic_mg_sweep_poly256.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def diff_leg(large: bool) -> dict:
     """Differentiation form: the P-V curve of a 64x64x36 MFIM under a
     triangular +-3 V gate sweep over 400 steps, loss = mean squared misfit to
@@ -192,10 +206,10 @@ def main():
         fused = bool(inp.mg_fused_sweep)
         alg = (25.0 if fused else 12.5) * N
         ach = alg / (t_s * 1e-3) / 1e9
-        kname = "k_mg_gsrb_fused (finest level, red+black)" if fused else "k_mg_smooth (finest level, one colour)"
+        kname = "k_mg_gsrb_stream (finest level, red+black in one pass)" if fused else "k_mg_smooth (finest level, one colour)"
         roof = {"bound": "hbm", "kernel": kname,
                 "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic() if fused else None,
                 "alg_bytes_per_launch": alg, "avg_launch_ms": t_s,
                 "tdgl_kernel_GBs": round(25.0 * N / (t_t * 1e-3) / 1e9, 1), "tdgl_avg_launch_ms": t_t,
                 "copy_GBs": round(16.0 * N / (t_c * 1e-3) / 1e9, 1)}
