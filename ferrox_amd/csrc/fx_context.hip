@@ -96,6 +96,17 @@ struct fx_ctx {
     std::vector<Level> lev;
     /* levels tail0.. run as one single-workgroup kernel (-1: none) */
     int tail0 = -1;
+    int tail_base = -1;            /* first level small enough for the tail */
+    /* layered hierarchies: every tail level in LDS (k_mg_tail_lay) with the
+     * bottom's LU factors, refactored when the bottom operator changes */
+    int tail_lean = 0;
+    fxd::TailPlan<double> plan64{};
+    fxd::TailPlan<float> plan32{};
+    double* lu64 = nullptr;
+    float* lu32 = nullptr;
+    int* perm64 = nullptr;
+    int* perm32 = nullptr;
+    bool lu_ok64 = false, lu_ok32 = false;
     DLev* d_tail64 = nullptr;
     DLevF* d_tail32 = nullptr;
     int tail_ts64 = 0, tail_ts32 = 0;          /* first tail level staged in LDS */
@@ -272,6 +283,15 @@ int build_levels(fx_ctx* x) {
     x->tail0 = -1;
     for (int l = (int)x->lev.size() - 1; l >= 1 && x->lev[l].N <= TAIL_MAX_CELLS; --l) x->tail0 = l;
     if (x->tail0 >= 0 && (int)x->lev.size() - x->tail0 < 2) x->tail0 = -1;
+    x->tail_base = x->tail0;
+    if (Nb <= (size_t)fxk::BOTTOM_DIRECT_CELLS) {
+        if ((rc = dalloc(x, (void**)&x->lu64, sizeof(double) * Nb * Nb))) return rc;
+        if ((rc = dalloc(x, (void**)&x->perm64, sizeof(int) * (Nb + 1)))) return rc;
+        if (x->in.precision == FX_PREC_FP32_REFINE) {
+            if ((rc = dalloc(x, (void**)&x->lu32, sizeof(float) * Nb * Nb))) return rc;
+            if ((rc = dalloc(x, (void**)&x->perm32, sizeof(int) * (Nb + 1)))) return rc;
+        }
+    }
     if (x->tail0 >= 0) {
         if ((rc = dalloc(x, (void**)&x->d_tail64, sizeof(DLev) * x->lev.size()))) return rc;
         if (x->in.precision == FX_PREC_FP32_REFINE && (rc = dalloc(x, (void**)&x->d_tail32, sizeof(DLevF) * x->lev.size()))) return rc;
@@ -300,6 +320,24 @@ int build_levels(fx_ctx* x) {
         if ((rc = dalloc(x, (void**)&x->phi_alt, sizeof(double) * x->N))) return rc;
     }
     return FX_OK;
+}
+
+/* Plan of the lean tail from level t0 down; lev(l) gives level l's descriptor. */
+template <class T, class F>
+void fill_plan(fxd::TailPlan<T>& P, const fx_ctx* x, int t0, F lev, const T* lu, const int* perm) {
+    std::memset(&P, 0, sizeof(P));
+    P.nl = (int)x->lev.size() - t0;
+    for (int t = 0; t < P.nl; ++t) {
+        const DLevT<T>& d = lev(t0 + t);
+        P.n[t][0] = d.n0; P.n[t][1] = d.n1; P.n[t][2] = d.n2;
+        for (int q = 0; q < 3; ++q) { P.crs[t][q] = d.crs[q]; P.ih2[t][q] = d.ih2[q]; }
+        P.bxk[t] = d.bxk; P.byk[t] = d.byk; P.bzk[t] = d.bzk;
+        P.a[t] = d.a;
+    }
+    P.b0 = lev(t0).b;
+    P.phi0 = lev(t0).phi;
+    P.lu = lu;
+    P.perm = perm;
 }
 
 /* Permittivity with its ghost rule (InitializePermittivity,
@@ -475,8 +513,40 @@ int build_coefficients(fx_ctx* x) {
             f.bxk = pbx; f.byk = pby; f.bzk = pbz;
         }
     }
+    /* the lean tail: the first level from the tail's start on from which
+     * every level is layered and the whole tail fits in LDS (with a
+     * Helmholtz term, so that one plan serves every solve) */
+    x->tail0 = x->tail_base;
+    x->tail_lean = 0;
+    x->lu_ok64 = x->lu_ok32 = false;
+    if (x->tail_base >= 0) {
+        const size_t lmax = fxk::tail_lds_max();
+        const size_t cap = lmax > 4096 ? lmax - 4096 : 0;   /* room for the kernel's static LDS */
+        const int L = (int)x->lev.size();
+        bool lay = true;
+        for (int t = x->tail_base; t < L; ++t) lay = lay && x->lev[t].d.layered;
+        for (int t0 = x->tail_base; lay && t0 + 2 <= L; ++t0) {
+            const int nl = L - t0;
+            if (nl > fxd::TAIL_LEAN_MAXL) continue;
+            int n[fxd::TAIL_LEAN_MAXL][3];
+            for (int t = 0; t < nl; ++t) {
+                const DLev& d = x->lev[t0 + t].d;
+                n[t][0] = d.n0; n[t][1] = d.n1; n[t][2] = d.n2;
+            }
+            if (fxk::tail_lay_lds(n, nl, 1, sizeof(double)) > cap) continue;
+            x->tail0 = t0;
+            x->tail_lean = 1;
+            fill_plan(x->plan64, x, t0, [&](int l) -> const DLevT<double>& { return x->lev[l].d; }, x->lu64, x->perm64);
+            if (!x->lev32.empty())
+                fill_plan(x->plan32, x, t0, [&](int l) -> const DLevT<float>& { return x->lev32[l]; }, x->lu32, x->perm32);
+            if (x->in.verbose)
+                std::printf("multigrid tail: levels %d.. in one workgroup, all in LDS (%zu B FP64 with a Helmholtz term)\n",
+                            t0, fxk::tail_lay_lds(n, nl, 1, sizeof(double)));
+            break;
+        }
+    }
     /* descriptors of the tail levels, read by the single-workgroup kernel */
-    if (x->tail0 >= 0) {
+    if (x->tail0 >= 0 && !x->tail_lean) {
         /* stage as many of the tail's levels (the smallest ones) in LDS as
          * fit, with the bottom solve's work arrays */
         const size_t lmax = fxk::tail_lds_max();
@@ -629,6 +699,13 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
     Level& L = x->lev[li];
     const int inh = li == 0;
     if ((int)li == x->tail0) {
+        if (x->tail_lean) {
+            if (!x->lu_ok64 && x->lu64) fxk::bottom_factor(x->s, x->lev.back().d, bc, x->lu64, x->perm64);
+            x->lu_ok64 = true;
+            fxk::mg_tail_lay(x->s, x->plan64, bc, L.d.read_a ? 1 : 0, x->in.mg_nu1, x->in.mg_nu2, x->cg_work,
+                             BOTTOM_TOL, BOTTOM_MAXIT, x->d_iters);
+            return;
+        }
         fxk::mg_tail(x->s, x->d_tail64, (int)(x->lev.size() - li), bc, L.d.read_a, x->in.mg_nu1, x->in.mg_nu2,
                      x->cg_work, BOTTOM_TOL, BOTTOM_MAXIT, x->d_iters, x->tail_ts64, x->tail_lds64);
         return;
@@ -653,6 +730,9 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
 /* Which levels read the A-coefficient: 0 none, 1 the finest level only in
  * SC cells (it comes from the Newton linearisation), 2 anywhere. */
 void set_read_a(fx_ctx* x, int mode) {
+    /* the bottom operator changes with its Helmholtz term */
+    if (!x->lev.empty() && x->lev.back().d.read_a != (x->lev.size() == 1 ? mode : (mode ? 2 : 0)))
+        x->lu_ok64 = x->lu_ok32 = false;
     for (size_t l = 0; l < x->lev.size(); ++l) x->lev[l].d.read_a = (l == 0) ? mode : (mode ? 2 : 0);
     for (size_t l = 0; l < x->lev32.size(); ++l) x->lev32[l].read_a = (l == 0) ? mode : (mode ? 2 : 0);
 }
@@ -663,6 +743,13 @@ void set_read_a(fx_ctx* x, int mode) {
 void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
     DLevF& L = x->lev32[li];
     if ((int)li == x->tail0) {
+        if (x->tail_lean) {
+            if (!x->lu_ok32 && x->lu32) fxk::bottom_factor(x->s, x->lev32.back(), bc, x->lu32, x->perm32);
+            x->lu_ok32 = true;
+            fxk::mg_tail_lay(x->s, x->plan32, bc, L.read_a ? 1 : 0, x->in.mg_nu1, x->in.mg_nu2, x->cg_work32, 1e-5,
+                             BOTTOM_MAXIT, x->d_iters);
+            return;
+        }
         fxk::mg_tail(x->s, x->d_tail32, (int)(x->lev32.size() - li), bc, L.read_a, x->in.mg_nu1, x->in.mg_nu2,
                      x->cg_work32, 1e-5, BOTTOM_MAXIT, x->d_iters, x->tail_ts32, x->tail_lds32);
         return;
@@ -752,6 +839,7 @@ int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
             fxk::d2f(x->s, x->lev32[0].a, x->alpha_cc, (long long)x->N);
             for (size_t l = 0; l + 1 < x->lev32.size(); ++l)
                 fxk::mg_restrict_cells(x->s, x->lev32[l], x->lev32[l + 1], x->lev32[l].a, x->lev32[l + 1].a);
+            x->lu_ok32 = false;
         }
         return mg_solve_mixed(x, bv, tol, maxit, st);
     }
@@ -759,6 +847,7 @@ int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
 }
 
 void restrict_acoef(fx_ctx* x) {
+    x->lu_ok64 = false;
     for (size_t l = 0; l + 1 < x->lev.size(); ++l)
         fxk::mg_restrict_cells(x->s, x->lev[l].d, x->lev[l + 1].d, x->lev[l].d.a, x->lev[l + 1].d.a);
 }
@@ -893,6 +982,7 @@ void free_all(fx_ctx* x) {
         F(f.phi); F(f.b); F(f.a); F(f.bx); F(f.by); F(f.bz); F((void*)f.bxk); F((void*)f.byk); F((void*)f.bzk); F(f.phi2);
     }
     F(x->cg_work32); F(x->phi_alt); F(x->d_tail64); F(x->d_tail32);
+    F(x->lu64); F(x->lu32); F(x->perm64); F(x->perm32);
     if (x->h_red) hipHostFree(x->h_red);
     for (size_t l = 0; l < x->lev.size(); ++l) { F((void*)x->lev[l].d.bxk); F((void*)x->lev[l].d.byk); F((void*)x->lev[l].d.bzk); F(x->lev[l].d.phi2); }
     if (x->s) hipStreamDestroy(x->s);

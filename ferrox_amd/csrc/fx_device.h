@@ -242,6 +242,28 @@ struct DBC {
     double v[6];
 };
 
+/* The coarse tail of a layered hierarchy as one workgroup (k_mg_tail_lay):
+ * each level's extents, coarsening pattern, 1/h^2, per-plane face
+ * coefficients and Helmholtz term (global), the first level's rhs and
+ * solution, and the bottom level's LU factors (column-major) with its row
+ * permutation; perm[N] is 1 when the factors exist (non-singular bottom). */
+constexpr int TAIL_LEAN_MAXL = 10;
+template <class T>
+struct TailPlan {
+    int nl;
+    int n[TAIL_LEAN_MAXL][3];
+    int crs[TAIL_LEAN_MAXL][3];
+    T ih2[TAIL_LEAN_MAXL][3];
+    const T* bxk[TAIL_LEAN_MAXL];
+    const T* byk[TAIL_LEAN_MAXL];
+    const T* bzk[TAIL_LEAN_MAXL];
+    const T* a[TAIL_LEAN_MAXL];
+    const T* b0;
+    T* phi0;
+    const T* lu;
+    const int* perm;
+};
+
 /* Cell permittivity of a material id: FE (t-phase or not), DE, SC; chosen
  * with selects on kernel-argument scalars rather than an indexed table. */
 template <class T>

@@ -510,8 +510,8 @@ __global__ __launch_bounds__(1024) void k_final1024(const double* __restrict__ p
  * BOTTOM_DIRECT_MAX cells is solved exactly -- the dense matrix assembled
  * column by column from the row operator, Gaussian elimination with partial
  * pivoting (first largest |pivot|; f = M[i][k]/M[k][k]; M[i][j] -= f*M[k][j]
- * for j > k; y[i] -= f*y[k]) and back substitution summed in increasing
- * column order -- and anything larger, or a singular operator (a pivot below
+ * for j > k; y[i] -= f*y[k]) and back substitution by columns (e_k =
+ * y_k/U_kk, then y_i -= U_ik e_k for i < k) -- and anything larger, or a singular operator (a pivot below
  * a small fraction of the largest diagonal entry: no Dirichlet face), by
  * Jacobi-PCG.  work: N*N + N values for the direct solve, 5*N for CG. */
 constexpr int BOTTOM_DIRECT_MAX = 64;
@@ -577,10 +577,10 @@ __device__ bool bottom_direct(const DLevT<T>& l, const DBC& bc, const T* rhs, T*
         __syncthreads();
     }
     if (tid == 0) {
-        for (int i = N - 1; i >= 0; --i) {
-            T sum = y[i];
-            for (int jj = i + 1; jj < N; ++jj) sum -= M[i * N + jj] * e[jj];
-            e[i] = sum / M[i * N + i];
+        for (int jj = N - 1; jj >= 0; --jj) {
+            const T ej = y[jj] / M[jj * N + jj];
+            e[jj] = ej;
+            for (int i = 0; i < jj; ++i) y[i] = y[i] - M[i * N + jj] * ej;
         }
     }
     __syncthreads();
@@ -825,6 +825,345 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const DLevT<T>* __restrict__ l
     }
 }
 
+/* ------------------- bottom LU factors, once per operator ----------------- */
+/* The elimination of bottom_direct split into factors and solve: the dense
+ * matrix of the bottom level (N <= BOTTOM_DIRECT_MAX cells) assembled from
+ * the row operator, Gaussian elimination with partial pivoting (the first
+ * largest |pivot|), the multipliers f = M[i][k]/M[k][k] kept below the
+ * diagonal and whole rows swapped, so that a solve applies the same
+ * operations to any right-hand side: y = rhs permuted, y_i -= f_ik y_k in
+ * order of k, then the column-oriented back substitution.  One workgroup;
+ * the pivot search is one wavefront's reduction.  Output: lu column-major
+ * (lu[col*N + row]); perm[r] = the original row now at r; perm[N] = 1, or 0
+ * when a pivot falls below a small fraction of the largest diagonal entry
+ * (singular operator: the solve then uses Jacobi-PCG). */
+template <class T>
+__global__ __launch_bounds__(1024) void k_bottom_factor(DLevT<T> l, DBC bc, T* __restrict__ lu, int* __restrict__ perm) {
+    constexpr int BS = 1024;
+    constexpr int NM = BOTTOM_DIRECT_MAX;
+    __shared__ T M[NM * NM];
+    __shared__ int sp[NM];
+    __shared__ int piv_sh, ok_sh;
+    __shared__ T dmax_sh;
+    const int n0 = l.n0, n1 = l.n1, n2 = l.n2, s = n0 * n1;
+    const int N = s * n2;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < N * N; idx += BS) {
+        const int r = idx / N, col = idx % N;
+        const int i = r % n0, j = (r / n0) % n1, k = r / s;
+        const int nbi[6] = {i > 0 ? r - 1 : r + (n0 - 1), i < n0 - 1 ? r + 1 : r - (n0 - 1),
+                            j > 0 ? r - n0 : r + n0 * (n1 - 1), j < n1 - 1 ? r + n0 : r - n0 * (n1 - 1),
+                            k > 0 ? r - s : r + s * (n2 - 1), k < n2 - 1 ? r + s : r - s * (n2 - 1)};
+        T a, w[6], nb[6], D, S;
+        mg_coef(l, bc, i, j, k, r, a, w);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) nb[q] = nbi[q] == col ? T(1) : T(0);
+        mg_acc(l, bc, false, i, j, k, a, w, nb, D, S);
+        M[idx] = (r == col ? D : T(0)) - S;
+    }
+    for (int r = tid; r < N; r += BS) sp[r] = r;
+    __syncthreads();
+    if (tid < 64) {
+        T v = tid < N ? fabs(M[tid * N + tid]) : T(0);
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if (tid == 0) dmax_sh = v;
+    }
+    __syncthreads();
+    const T small = sizeof(T) == 8 ? T(1e-14) : T(1e-6);
+    int ok = 1;
+    for (int kk = 0; kk < N; ++kk) {
+        if (tid < 64) {
+            T v = (tid >= kk && tid < N) ? fabs(M[tid * N + kk]) : T(-1);
+            int p = tid;
+            for (int o = 32; o > 0; o >>= 1) {
+                const T v2 = __shfl_xor(v, o, 64);
+                const int p2 = __shfl_xor(p, o, 64);
+                if (v2 > v || (v2 == v && p2 < p)) { v = v2; p = p2; }
+            }
+            if (tid == 0) { piv_sh = p; ok_sh = v > small * dmax_sh; }
+        }
+        __syncthreads();
+        if (!ok_sh) { ok = 0; break; }
+        const int p = piv_sh;
+        if (p != kk) {
+            for (int jj = tid; jj < N; jj += BS) {
+                const T t = M[kk * N + jj];
+                M[kk * N + jj] = M[p * N + jj];
+                M[p * N + jj] = t;
+            }
+            if (tid == 0) { const int t = sp[kk]; sp[kk] = sp[p]; sp[p] = t; }
+        }
+        __syncthreads();
+        for (int i = kk + 1 + tid; i < N; i += BS) M[i * N + kk] = M[i * N + kk] / M[kk * N + kk];
+        __syncthreads();
+        const int cols = N - kk - 1;
+        for (int idx = tid; idx < cols * cols; idx += BS) {
+            const int i = kk + 1 + idx / cols, jj = kk + 1 + idx % cols;
+            M[i * N + jj] = M[i * N + jj] - M[i * N + kk] * M[kk * N + jj];
+        }
+        __syncthreads();
+    }
+    for (int q = tid; q < N * N; q += BS) lu[q] = M[(q % N) * N + q / N];
+    for (int r = tid; r < N; r += BS) perm[r] = sp[r];
+    if (tid == 0) perm[N] = ok;
+}
+
+/* ---------------- coarse tail of a layered hierarchy (lean) ---------------- */
+namespace {
+constexpr int TAIL_NPW = 6;   /* per-plane table: wx wy wzl wzh D 1/D */
+struct TLev {
+    int n0, n1, n2, s, n, half;
+    int cx, cy, cz;
+    int ophi, ob, oa, otab;   /* offsets (in values) into the dynamic LDS */
+    float r0, rs, rh, r1;     /* 1/n0, 1/(n0 n1), 1/half, 1/n1 for index splits */
+};
+/* c / n for 0 <= c < 2^22 from a float reciprocal, corrected by one step */
+__device__ __forceinline__ int qdiv(int c, int n, float r) {
+    const int q = (int)((float)c * r);
+    const int m = c - q * n;
+    return q + (m >= n ? 1 : 0) - (m < 0 ? 1 : 0);
+}
+}  // namespace
+
+/* Values of dynamic LDS the lean tail uses for levels of the given extents. */
+__host__ inline size_t tail_lean_values(const int (*n)[3], int nl, int read_a) {
+    size_t v = 0;
+    for (int t = 0; t < nl; ++t) {
+        const size_t c = (size_t)n[t][0] * n[t][1] * n[t][2];
+        v += (read_a ? 3 : 2) * c + TAIL_NPW * (size_t)n[t][2];
+    }
+    const size_t N = (size_t)n[nl - 1][0] * n[nl - 1][1] * n[nl - 1][2];
+    if (N <= (size_t)BOTTOM_DIRECT_MAX) v += N * N;
+    return v;
+}
+
+/* The sub-V-cycle below a level of a layered hierarchy in one 1024-thread
+ * workgroup, every level held in LDS: phi, b, the Helmholtz term (read_a) and
+ * a per-plane table of the face weights beta/h^2, the interior diagonal
+ * (accumulated in the row operator's order) and its inverse.  Buffers are
+ * addressed as offsets from the dynamic LDS base, so every access is an LDS
+ * instruction.  Per cell: the interior fast path of the streaming sweep
+ * (multiply by the per-plane 1/D when there is no Helmholtz term), the row
+ * operator elsewhere.  Restriction, prolongation and the sweep order are
+ * those of k_mg_tail; the bottom is solved with the LU factors of
+ * k_bottom_factor by one wavefront (lanes own rows), or by Jacobi-PCG when
+ * the bottom is singular or too large. */
+template <class T>
+__global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int read_a, int nu1, int nu2,
+                                                      T* __restrict__ gwork, double tol, int maxit,
+                                                      int* __restrict__ iters) {
+    constexpr int BS = 1024;
+    extern __shared__ __align__(16) unsigned char tail_lds[];
+    T* const V = reinterpret_cast<T*>(tail_lds);
+    __shared__ TLev sl[TAIL_LEAN_MAXL];
+    __shared__ int sperm[BOTTOM_DIRECT_MAX + 1];
+    __shared__ int s_olu;
+    const int tid = threadIdx.x;
+    const int nl = P.nl;
+    if (tid == 0) {
+        int o = 0;
+        for (int t = 0; t < nl; ++t) {
+            TLev L;
+            L.n0 = P.n[t][0]; L.n1 = P.n[t][1]; L.n2 = P.n[t][2];
+            L.s = L.n0 * L.n1; L.n = L.s * L.n2; L.half = (L.n0 + 1) >> 1;
+            L.cx = P.crs[t][0]; L.cy = P.crs[t][1]; L.cz = P.crs[t][2];
+            L.ophi = o; o += L.n;
+            L.ob = o; o += L.n;
+            L.oa = o; if (read_a) o += L.n;
+            L.otab = o; o += TAIL_NPW * L.n2;
+            L.r0 = 1.0f / (float)L.n0; L.rs = 1.0f / (float)L.s;
+            L.rh = 1.0f / (float)L.half; L.r1 = 1.0f / (float)L.n1;
+            sl[t] = L;
+        }
+        s_olu = o;
+    }
+    __syncthreads();
+    /* staging: per-plane tables, Helmholtz terms, the first level's rhs, the bottom factors */
+    for (int t = 0; t < nl; ++t) {
+        const TLev& L = sl[t];
+        for (int e = tid; e < L.n2; e += BS) {
+            const T wx = P.bxk[t][e] * P.ih2[t][0], wy = P.byk[t][e] * P.ih2[t][1];
+            const T wzl = P.bzk[t][e] * P.ih2[t][2], wzh = P.bzk[t][e + 1] * P.ih2[t][2];
+            T D = T(0);
+            D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+            T* tb = V + L.otab + TAIL_NPW * e;
+            tb[0] = wx; tb[1] = wy; tb[2] = wzl; tb[3] = wzh; tb[4] = D; tb[5] = T(1) / D;
+        }
+        if (read_a)
+            for (int c = tid; c < L.n; c += BS) V[L.oa + c] = P.a[t][c];
+    }
+    {
+        const TLev& L = sl[0];
+        for (int c = tid; c < L.n; c += BS) { V[L.ob + c] = P.b0[c]; V[L.ophi + c] = T(0); }
+        const int N = sl[nl - 1].n;
+        if (N <= BOTTOM_DIRECT_MAX) {
+            for (int q = tid; q < N * N; q += BS) V[s_olu + q] = P.lu[q];
+            for (int q = tid; q <= N; q += BS) sperm[q] = P.perm[q];
+        }
+    }
+    __syncthreads();
+    const bool perx = bc.lo[0] == BC_PER, pery = bc.lo[1] == BC_PER, perz = bc.lo[2] == BC_PER;
+    /* (A phi)_c = D phi_c - S of cell (i,j,k) of level L */
+    auto row = [&](const TLev& L, int i, int j, int k, int c, T& D, T& S) -> bool {
+        const T* phi = V + L.ophi;
+        const T* tb = V + L.otab + TAIL_NPW * k;
+        const T wx = tb[0], wy = tb[1], wzl = tb[2], wzh = tb[3];
+        const T v0 = phi[i > 0 ? c - 1 : c + (L.n0 - 1)];
+        const T v1 = phi[i < L.n0 - 1 ? c + 1 : c - (L.n0 - 1)];
+        const T v2 = phi[j > 0 ? c - L.n0 : c + L.n0 * (L.n1 - 1)];
+        const T v3 = phi[j < L.n1 - 1 ? c + L.n0 : c - L.n0 * (L.n1 - 1)];
+        const T v4 = phi[k > 0 ? c - L.s : c + L.s * (L.n2 - 1)];
+        const T v5 = phi[k < L.n2 - 1 ? c + L.s : c - L.s * (L.n2 - 1)];
+        const T av = read_a ? V[L.oa + c] : T(0);
+        const bool in = (perx || (i > 0 && i < L.n0 - 1)) && (pery || (j > 0 && j < L.n1 - 1)) &&
+                        (perz || (k > 0 && k < L.n2 - 1));
+        if (in) {
+            S = T(0);
+            S += wx * v0; S += wx * v1; S += wy * v2; S += wy * v3; S += wzl * v4; S += wzh * v5;
+            if (read_a) { D = -av; D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh; }
+            else D = tb[4];
+            return true;
+        }
+        /* the row operator's accumulation (homogeneous faces), face by face
+         * with selects (adding 0 where the row operator adds nothing) */
+        D = -av;
+        S = T(0);
+        auto face = [&](bool open, int kind, T wv, T nv) {
+            D += open ? wv : (kind == BC_DIR ? T(2) * wv : T(0));
+            S += open ? wv * nv : T(0);
+        };
+        face(i > 0 || perx, bc.lo[0], wx, v0);
+        face(i < L.n0 - 1 || perx, bc.hi[0], wx, v1);
+        face(j > 0 || pery, bc.lo[1], wy, v2);
+        face(j < L.n1 - 1 || pery, bc.hi[1], wy, v3);
+        face(k > 0 || perz, bc.lo[2], wzl, v4);
+        face(k < L.n2 - 1 || perz, bc.hi[2], wzh, v5);
+        return false;
+    };
+    auto half_sweep = [&](const TLev& L, int color) {
+        const int tot = L.half * L.n1 * L.n2;
+        for (int q = tid; q < tot; q += BS) {
+            const int jk = qdiv(q, L.half, L.rh);
+            const int ih = q - jk * L.half;
+            const int k = qdiv(jk, L.n1, L.r1);
+            const int j = jk - k * L.n1;
+            const int i = 2 * ih + ((j + k + color) & 1);
+            if (i >= L.n0) continue;
+            const int c = i + L.n0 * (j + L.n1 * k);
+            T D, S;
+            const bool in = row(L, i, j, k, c, D, S);
+            const T bv = V[L.ob + c];
+            V[L.ophi + c] = (in && !read_a) ? (bv + S) * V[L.otab + TAIL_NPW * k + 5] : (bv + S) / D;
+        }
+        __syncthreads();
+    };
+    auto restrict_to = [&](const TLev& F, const TLev& C) {
+        for (int t = tid; t < C.n; t += BS) {
+            const int K = qdiv(t, C.s, C.rs);
+            const int rr = t - K * C.s;
+            const int J = qdiv(rr, C.n0, C.r0);
+            const int I = rr - J * C.n0;
+            T sum = T(0);
+            for (int kk = 0; kk <= F.cz; ++kk)
+                for (int jj = 0; jj <= F.cy; ++jj)
+                    for (int ii = 0; ii <= F.cx; ++ii) {
+                        const int i = F.cx ? 2 * I + ii : I, j = F.cy ? 2 * J + jj : J, k = F.cz ? 2 * K + kk : K;
+                        const int c = i + F.n0 * (j + F.n1 * k);
+                        T D, S;
+                        row(F, i, j, k, c, D, S);
+                        sum += V[F.ob + c] - (D * V[F.ophi + c] - S);
+                    }
+            V[C.ob + t] = sum / T((1 + F.cx) * (1 + F.cy) * (1 + F.cz));
+            V[C.ophi + t] = T(0);
+        }
+        __syncthreads();
+    };
+    auto prolong_from = [&](const TLev& F, const TLev& C) {
+        const int crs[3] = {F.cx, F.cy, F.cz};
+        const int cn[3] = {C.n0, C.n1, C.n2};
+        for (int t = tid; t < F.n; t += BS) {
+            const int k = qdiv(t, F.s, F.rs);
+            const int rr = t - k * F.s;
+            const int j = qdiv(rr, F.n0, F.r0);
+            const int fi[3] = {rr - j * F.n0, j, k};
+            int id[3][2];
+            T w[3][2];
+            int cnt[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                if (!crs[d]) { id[d][0] = fi[d]; id[d][1] = fi[d]; w[d][0] = T(1); w[d][1] = T(0); cnt[d] = 1; continue; }
+                const int I = fi[d] >> 1;
+                int Jn = (fi[d] & 1) ? I + 1 : I - 1;
+                T sg = T(1);
+                if (Jn < 0 || Jn >= cn[d]) {
+                    const int bct = Jn < 0 ? bc.lo[d] : bc.hi[d];
+                    if (bct == BC_PER) Jn = Jn < 0 ? Jn + cn[d] : Jn - cn[d];
+                    else { sg = (bct == BC_DIR) ? T(-1) : T(1); Jn = I; }
+                }
+                id[d][0] = I; w[d][0] = T(0.75);
+                id[d][1] = Jn; w[d][1] = T(0.25) * sg;
+                cnt[d] = 2;
+            }
+            T e = T(0);
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int q = 0; q < 2; ++q)
+                        if (a < cnt[2] && b < cnt[1] && q < cnt[0])
+                            e += (w[2][a] * w[1][b]) * w[0][q] * V[C.ophi + id[0][q] + C.n0 * (id[1][b] + C.n1 * id[2][a])];
+            V[F.ophi + t] += e;
+        }
+        __syncthreads();
+    };
+    for (int t = 0; t + 1 < nl; ++t) {
+        for (int s = 0; s < nu1; ++s) { half_sweep(sl[t], 0); half_sweep(sl[t], 1); }
+        restrict_to(sl[t], sl[t + 1]);
+    }
+    {
+        const TLev& L = sl[nl - 1];
+        const int N = L.n;
+        if (N <= BOTTOM_DIRECT_MAX && sperm[N]) {
+            if (tid < 64) {
+                const T* lu = V + s_olu;
+                const int lane = tid;
+                T y = lane < N ? V[L.ob + sperm[lane]] : T(0);
+                for (int kk = 0; kk + 1 < N; ++kk) {
+                    const T ykk = __shfl(y, kk, 64);
+                    if (lane > kk && lane < N) y = y - lu[kk * N + lane] * ykk;
+                }
+                T e = T(0);
+                for (int jj = N - 1; jj >= 0; --jj) {
+                    const T ej = __shfl(y, jj, 64) / lu[jj * N + jj];
+                    if (lane == jj) e = ej;
+                    if (lane < jj) y = y - lu[jj * N + lane] * ej;
+                }
+                if (lane < N) V[L.ophi + lane] = e;
+            }
+            __syncthreads();
+        } else {
+            DLevT<T> bl;
+            bl.n0 = L.n0; bl.n1 = L.n1; bl.n2 = L.n2;
+            for (int d = 0; d < 3; ++d) { bl.ih2[d] = P.ih2[nl - 1][d]; bl.crs[d] = 0; }
+            bl.phi = V + L.ophi; bl.b = V + L.ob; bl.a = read_a ? V + L.oa : nullptr;
+            bl.bx = bl.by = bl.bz = nullptr; bl.mat = nullptr; bl.msx = bl.msy = 0; bl.matcoef = 0;
+            bl.read_a = read_a ? 2 : 0;
+            bl.layered = 1; bl.bxk = P.bxk[nl - 1]; bl.byk = P.byk[nl - 1]; bl.bzk = P.bzk[nl - 1];
+            bl.phi2 = nullptr;
+            bottom_cg_dev<T, BS>(bl, bc, bl.b, bl.phi, gwork, tol, maxit, iters);
+        }
+    }
+    for (int t = nl - 2; t >= 0; --t) {
+        prolong_from(sl[t], sl[t + 1]);
+        for (int s = 0; s < nu2; ++s) { half_sweep(sl[t], 0); half_sweep(sl[t], 1); }
+    }
+    {
+        const TLev& L = sl[0];
+        for (int c = tid; c < L.n; c += BS) P.phi0[c] = V[L.ophi + c];
+    }
+}
+
 /* --------------------- FP64 refinement of an FP32 correction -------------- */
 /* phi += e (FP32 correction added into the FP64 solution). */
 __global__ void k_add_f2d(double* __restrict__ phi, const float* __restrict__ e, long long n) {
@@ -1025,6 +1364,33 @@ void mg_tail(hipStream_t s, const DLev* levs, int nl, const DBC& bc, int read_a,
 void mg_tail(hipStream_t s, const DLevF* levs, int nl, const DBC& bc, int read_a, int nu1, int nu2, float* work,
              double tol, int maxit, int* iters, int ts, size_t lds_bytes) {
     tail_impl<float>(s, levs, nl, bc, read_a, nu1, nu2, work, tol, maxit, iters, ts, lds_bytes);
+}
+template <class T>
+static void tail_lay_impl(hipStream_t s, const TailPlan<T>& P, const DBC& bc, int read_a, int nu1, int nu2, T* work,
+                          double tol, int maxit, int* iters) {
+    const size_t lds = tail_lean_values(P.n, P.nl, read_a) * sizeof(T);
+    static size_t attr_set = 0;
+    if (lds > attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail_lay<T>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr_set = lds;
+    }
+    hipLaunchKernelGGL(k_mg_tail_lay<T>, dim3(1), dim3(1024), lds, s, P, bc, read_a, nu1, nu2, work, tol, maxit, iters);
+}
+void mg_tail_lay(hipStream_t s, const TailPlan<double>& P, const DBC& bc, int read_a, int nu1, int nu2, double* work,
+                 double tol, int maxit, int* iters) {
+    tail_lay_impl<double>(s, P, bc, read_a, nu1, nu2, work, tol, maxit, iters);
+}
+void mg_tail_lay(hipStream_t s, const TailPlan<float>& P, const DBC& bc, int read_a, int nu1, int nu2, float* work,
+                 double tol, int maxit, int* iters) {
+    tail_lay_impl<float>(s, P, bc, read_a, nu1, nu2, work, tol, maxit, iters);
+}
+size_t tail_lay_lds(const int (*n)[3], int nl, int read_a, size_t elem) { return tail_lean_values(n, nl, read_a) * elem; }
+void bottom_factor(hipStream_t s, const DLev& l, const DBC& bc, double* lu, int* perm) {
+    hipLaunchKernelGGL(k_bottom_factor<double>, dim3(1), dim3(1024), 0, s, l, bc, lu, perm);
+}
+void bottom_factor(hipStream_t s, const DLevF& l, const DBC& bc, float* lu, int* perm) {
+    hipLaunchKernelGGL(k_bottom_factor<float>, dim3(1), dim3(1024), 0, s, l, bc, lu, perm);
 }
 void d2f(hipStream_t s, float* d, const double* src, long long n) {
     unsigned b = (unsigned)((n + 255) / 256); if (b > 4096) b = 4096; if (b == 0) b = 1;

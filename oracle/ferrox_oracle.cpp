@@ -295,7 +295,7 @@ struct MG {
      * column by column from the row operator, then Gaussian elimination with
      * partial pivoting (first largest |pivot| in row order; factor
      * f = M[i][k]/M[k][k]; M[i][j] -= f*M[k][j] for j > k; y[i] -= f*y[k]) and
-     * back substitution with the sum in increasing column order.  A pivot
+     * back substitution by columns (from the last column down).  A pivot
      * below 1e-14 of the largest diagonal entry (singular operator: no
      * Dirichlet face) falls back to Jacobi-PCG, as do larger bottoms. */
     static constexpr int BOTTOM_DIRECT_MAX = 64;
@@ -337,10 +337,10 @@ struct MG {
                 y[i] = y[i] - f * y[k];
             }
         }
-        for (int i = N - 1; i >= 0; --i) {
-            double s = y[i];
-            for (int j = i + 1; j < N; ++j) s -= M[(size_t)i * N + j] * e[j];
-            e[i] = s / M[(size_t)i * N + i];
+        /* back substitution by columns: e_k = y_k / U_kk, then y_i -= U_ik e_k for i < k */
+        for (int k = N - 1; k >= 0; --k) {
+            e[k] = y[k] / M[(size_t)k * N + k];
+            for (int i = 0; i < k; ++i) y[i] = y[i] - M[(size_t)i * N + k] * e[k];
         }
         return true;
     }
