@@ -1634,6 +1634,11 @@ int fx_bench_kernel(fx_ctx* x, const char* which, int reps, double* avg_ms) {
         else if (w == "resid0") {
             if (x->lev.size() < 2) return false;
             fxk::mg_resid_restrict(x->s, x->lev[0].d, x->lev[1].d, bc, 1);
+        } else if (w == "prolong0") {   /* prolongation from level 1 folded into one sweep of level 0 */
+            if (x->lev.size() < 2) return false;
+            fxk::mg_prolong_sweep(x->s, x->lev[0].d, x->lev[1].d, bc, 1, 1, x->in.mg_fused_sweep);
+        } else if (w == "residmax0") {
+            fxk::mg_resid_norm(x->s, x->lev[0].d, bc, 1, nullptr, x->red, x->red_out);
         } else if (w == "tdgl") {
             const bool fast = scalar_fast(x);
             TdglArgs a = tdgl_args(x, x->P);

@@ -91,7 +91,7 @@ namespace {
 template <class T>
 struct StreamRegs {
     T phi[SLD];      /* next phi plane (raw loads; out-of-domain lanes masked when stored to LDS) */
-    T ck[SLD], cj[SLD];   /* PROLONG: its parent coarse plane and the parent's neighbour on its side */
+    T cn[SLD];       /* PROLONG, odd planes 2K+1: coarse plane K+1, which enters the window */
     T b[3];          /* rhs of red cells u = 0, 1 and of the black cell */
     T a[3];          /* Helmholtz term of the same cells */
     uint8_t m[3];    /* their material id */
@@ -226,22 +226,19 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
     /* the loads of step p (plane parity PAR): phi plane p+1, red cells of
      * plane p, black cell of plane p-1.  Every load is unconditional (clamped
      * to a valid address, masked where used). */
+    auto coarse_plane = [&](T* dst, int K) {
+        const int kc = plane * min(max(K, 0), A.cn2 - 1);
+#pragma unroll
+        for (int s = 0; s < SLD; ++s) dst[s] = A.ec[(goff[s] >= 0 ? goff[s] : 0) + kc];
+    };
     auto issue = [&](StreamRegs<T>& R, int p, auto par_tag) {
         constexpr int PAR = decltype(par_tag)::value;
         const int kf = min(max(p + 1, 0), n2 - 1);
         const int kof = plane * kf;
 #pragma unroll
         for (int s = 0; s < SLD; ++s) R.phi[s] = A.phi_in[(goff[s] >= 0 ? goff[s] : 0) + kof];
-        if (PROLONG) {
-            const int K = kf >> 1;
-            const int J = min(max((kf & 1) ? K + 1 : K - 1, 0), A.cn2 - 1);
-#pragma unroll
-            for (int s = 0; s < SLD; ++s) {
-                const int g = goff[s] >= 0 ? goff[s] : 0;
-                R.ck[s] = A.ec[g + plane * K];
-                R.cj[s] = A.ec[g + plane * J];
-            }
-        }
+        /* plane p+1 odd (= 2K+1) needs coarse planes K and K+1: K+1 is new */
+        if (PROLONG && PAR == 0) coarse_plane(R.cn, ((p + 1) >> 1) + 1);
         const int kr = min(max(p, 0), n2 - 1), kb = min(max(p - 1, 0), n2 - 1);
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
@@ -260,9 +257,19 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
         if (AMODE == 2) return R.a[u];
         return ((R.m[u] & 3) >= 2) ? R.a[u] : T(0);
     };
-    auto put_plane = [&](const StreamRegs<T>& R, int k) {
+    /* PROLONG: the two coarse planes the next fine plane interpolates from,
+     * {K-1, K} before an even plane 2K and {K, K+1} from the odd plane 2K+1 on
+     * (clamped planes stand for the ghost, weighted below); each coarse plane
+     * is loaded once, at the odd plane where it enters. */
+    T cw0[PROLONG ? SLD : 1], cw1[PROLONG ? SLD : 1];
+    auto put_plane = [&](const StreamRegs<T>& R, int k, auto odd_tag) {
+        constexpr int KODD = decltype(odd_tag)::value;
         T* d = sp + ((k - z0 + 2) % SNS) * SPL;
         const bool kin = !A.zero_init && k >= 0 && k < n2;
+        if (PROLONG && KODD) {
+#pragma unroll
+            for (int s = 0; s < SLD; ++s) { cw0[s] = cw1[s]; cw1[s] = R.cn[s]; }
+        }
         /* PROLONG: plane k of phi + P e, with the weights and sum order of
          * k_mg_prolong for a z-only coarsening (parent 3/4, neighbour 1/4;
          * across a z face the neighbour is the homogeneous ghost) */
@@ -280,8 +287,8 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
             T v = R.phi[s];
             if (PROLONG) {
                 T e = T(0);
-                e += T(0.75) * R.ck[s];
-                e += wj * R.cj[s];
+                e += T(0.75) * (KODD ? cw0[s] : cw1[s]);
+                e += wj * (KODD ? cw1[s] : cw0[s]);
                 v += e;
             }
             if (q < SPL) d[q] = (kin && goff[s] >= 0) ? v : T(0);
@@ -323,7 +330,7 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
          * vmcnt [3:0] = 0, expcnt [6:4] = 7 and lgkmcnt [11:8] = 15 leave those
          * counters unconstrained.) */
         __builtin_amdgcn_s_waitcnt(0x0F70);
-        put_plane(cur, p + 1);
+        put_plane(cur, p + 1, std::integral_constant<int, 1 - PAR>{});
         issue(nxt, p + 1, std::integral_constant<int, 1 - PAR>{});
         if (pend) *reinterpret_cast<T2*>(pptr) = pv;
         __syncthreads();
@@ -356,16 +363,229 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
     /* planes z0-2 and z0-1 straight into LDS; step z0-1's loads into R0 */
     using P0 = std::integral_constant<int, 0>;
     using P1 = std::integral_constant<int, 1>;
+    if (PROLONG) {               /* the window before the even plane z0-2 = 2K: {K-1, K} */
+        coarse_plane(cw0, z0 / 2 - 2);
+        coarse_plane(cw1, z0 / 2 - 1);
+    }
     issue(R0, z0 - 3, P1{});
-    put_plane(R0, z0 - 2);
+    put_plane(R0, z0 - 2, P0{});
     issue(R0, z0 - 2, P0{});
-    put_plane(R0, z0 - 1);
+    put_plane(R0, z0 - 1, P1{});
     issue(R0, z0 - 1, P1{});
     for (int p = z0 - 1; p <= z1; p += 2) {    /* z0 even: p odd, p + 1 even */
         step(p, R0, R1, P1{});
         if (p + 1 <= z1) step(p + 1, R1, R0, P0{});
     }
     if (pend) *reinterpret_cast<T2*>(pptr) = pv;
+}
+
+/* ------------------ streaming residual (layered levels) ------------------- */
+namespace {
+constexpr int RTX = 64, RTY = 8;                  /* tile of columns */
+constexpr int RHX = RTX + 2, RHY = RTY + 2;       /* with the 1-cell in-plane halo */
+constexpr int RPL = RHX * RHY;                    /* LDS values per plane */
+constexpr int RLD = (RPL + 255) / 256;            /* plane loads per thread */
+constexpr int RNS = 4;                            /* plane slots */
+}
+
+template <class T>
+struct ResidArgs {
+    int n0, n1, n2, msx, msy;
+    const T* __restrict__ phi;
+    const T* __restrict__ b;
+    const T* __restrict__ a;
+    const uint8_t* __restrict__ mat;
+    const T* __restrict__ bxk;
+    const T* __restrict__ byk;
+    const T* __restrict__ bzk;
+    T ih2[3];
+    DBC bc;
+    int inh;
+    int ntx, nty, cz, remap;
+    T* __restrict__ cb;          /* MODE 1: rhs of the z-coarsened level */
+    T* __restrict__ cphi;        /* MODE 1: its phi (set to 0) */
+    float* __restrict__ r32;     /* MODE 3: the residual in FP32 */
+    double* __restrict__ part;   /* MODE 2, 3: max|r| and max|b| per workgroup */
+};
+
+namespace {
+template <class T>
+struct ResidRegs {
+    T phi[RLD];      /* next phi plane */
+    T b[2];          /* rhs of this thread's two cells of the current plane */
+    T a[2];
+    uint8_t m[2];
+};
+}
+
+/* r = b - A phi over a layered level, streamed upward plane by plane: a
+ * workgroup owns a 64x8 column tile and the planes [z0, z1); phi is staged in
+ * LDS one plane at a time with a 1-cell in-plane halo (four plane slots, one
+ * barrier per plane), each thread takes two cells of the tile per plane, and
+ * the loads of plane p+1 are in flight while plane p is evaluated.  Per cell
+ * the arithmetic of the row operator (interior cells: the per-plane weights
+ * and diagonal accumulated in its order; elsewhere the row operator itself).
+ * MODE 1: the rhs of the z-coarsened level, the mean of the residuals of
+ *         planes 2K and 2K+1 summed in that order, and its phi set to 0
+ *         (the values of k_mg_resid_restrict for that coarsening);
+ * MODE 2: max|r| and max|b| per workgroup into part[blockIdx.x] and
+ *         part[gridDim.x + blockIdx.x];
+ * MODE 3: as MODE 2, with r stored in FP32.
+ * AMODE as in k_mg_gsrb_stream.  Requires: z not periodic, n2 >= 2; MODE 1:
+ * n2 and cz even. */
+template <class T, int AMODE, int MODE>
+__global__ __launch_bounds__(256) void k_mg_resid_stream(ResidArgs<T> A) {
+    __shared__ T sp[RNS * RPL];
+    __shared__ T scw[5 * SCZ_MAX];
+    __shared__ double sh[256 / 64 + 1];
+    const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
+    int w = blockIdx.x;
+    if (A.remap) w = (w & 7) * (gridDim.x >> 3) + (w >> 3);
+    const int tx = w % A.ntx, rest = w / A.ntx, ty = rest % A.nty, tz = rest / A.nty;
+    const int tx0 = tx * RTX, ty0 = ty * RTY, z0 = tz * A.cz, z1 = min(z0 + A.cz, n2);
+    const int tid = threadIdx.x;
+    const bool perx = A.bc.lo[0] == BC_PER, pery = A.bc.lo[1] == BC_PER;
+    const int plane = n0 * n1;
+
+    /* per-plane weights beta/h^2 and the interior diagonal in the row operator's order */
+    for (int e = tid; e < z1 - z0; e += 256) {
+        const int k = z0 + e;
+        const T wx = A.bxk[k] * A.ih2[0], wy = A.byk[k] * A.ih2[1];
+        const T wzl = A.bzk[k] * A.ih2[2], wzh = A.bzk[k + 1] * A.ih2[2];
+        T D = T(0);
+        D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+        T* t = scw + 5 * e;
+        t[0] = wx; t[1] = wy; t[2] = wzl; t[3] = wzh; t[4] = D;
+    }
+    int goff[RLD];
+#pragma unroll
+    for (int s = 0; s < RLD; ++s) {
+        const int q = tid + 256 * s;
+        goff[s] = -1;
+        if (q < RPL) {
+            int i = tx0 + q % RHX - 1, j = ty0 + q / RHX - 1;
+            bool ok = true;
+            if (i < 0) { if (perx) i += n0; else ok = false; }
+            else if (i >= n0) { if (perx) i -= n0; else ok = false; }
+            if (j < 0) { if (pery) j += n1; else ok = false; }
+            else if (j >= n1) { if (pery) j -= n1; else ok = false; }
+            if (ok) goff[s] = i + n0 * j;
+        }
+    }
+    /* this thread's cells: column i, rows j and j + 4 of the tile */
+    const int ci = tx0 + (tid & 63);
+    int cj[2], cq[2], cc[2], cg[2];
+    bool cok[2], clat[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int j = ty0 + (tid >> 6) + 4 * u;
+        cj[u] = j;
+        cok[u] = ci < n0 && j < n1;
+        cq[u] = (tid & 63) + 1 + RHX * ((tid >> 6) + 4 * u + 1);
+        cc[u] = cok[u] ? ci + n0 * j : 0;
+        cg[u] = cok[u] ? (ci + 1) + A.msx * (j + 1) : 0;
+        clat[u] = (perx || (ci > 0 && ci < n0 - 1)) && (pery || (j > 0 && j < n1 - 1));
+    }
+    const T* ap = AMODE ? A.a : A.b;
+    const uint8_t* mp = AMODE == 1 ? A.mat : reinterpret_cast<const uint8_t*>(A.b);
+    auto slot = [&](int k) { return sp + ((k - z0 + 1) & (RNS - 1)) * RPL; };
+    auto load_plane = [&](ResidRegs<T>& R, int k) {
+        const int kof = plane * min(max(k, 0), n2 - 1);
+#pragma unroll
+        for (int s = 0; s < RLD; ++s) R.phi[s] = A.phi[(goff[s] >= 0 ? goff[s] : 0) + kof];
+    };
+    auto load_cells = [&](ResidRegs<T>& R, int k) {
+        const int kk = min(max(k, 0), n2 - 1);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int c = cc[u] + plane * kk;
+            R.b[u] = A.b[c];
+            if (AMODE) R.a[u] = ap[c];
+            if (AMODE == 1) R.m[u] = mp[cg[u] + A.msy * (kk + 1)];
+        }
+    };
+    auto put_plane = [&](const ResidRegs<T>& R, int k) {
+        T* d = slot(k);
+        const bool kin = k >= 0 && k < n2;
+#pragma unroll
+        for (int s = 0; s < RLD; ++s) {
+            const int q = tid + 256 * s;
+            if (q < RPL) d[q] = (kin && goff[s] >= 0) ? R.phi[s] : T(0);
+        }
+    };
+    double mr = 0.0, mb = 0.0;
+    T racc[2] = {T(0), T(0)};
+    auto eval = [&](const ResidRegs<T>& R, int k) {
+        const T* pl = slot(k - 1);
+        const T* pm = slot(k);
+        const T* pu = slot(k + 1);
+        const T* t = scw + 5 * (k - z0);
+        const T wx = t[0], wy = t[1], wzl = t[2], wzh = t[3];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int q = cq[u];
+            const T n0v = pm[q - 1], n1v = pm[q + 1], n2v = pm[q - RHX], n3v = pm[q + RHX], n4v = pl[q], n5v = pu[q];
+            const T pc = pm[q];
+            const T av = AMODE == 0 ? T(0) : (AMODE == 2 ? R.a[u] : (((R.m[u] & 3) >= 2) ? R.a[u] : T(0)));
+            T D, S;
+            if (clat[u] && k > 0 && k < n2 - 1) {
+                S = T(0);
+                S += wx * n0v; S += wx * n1v; S += wy * n2v; S += wy * n3v; S += wzl * n4v; S += wzh * n5v;
+                if (AMODE == 0) D = t[4];
+                else { D = -av; D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh; }
+            } else {
+                DLevT<T> lv;
+                lv.n0 = n0; lv.n1 = n1; lv.n2 = n2;
+                const T wc[6] = {wx, wx, wy, wy, wzl, wzh};
+                const T nb[6] = {n0v, n1v, n2v, n3v, n4v, n5v};
+                mg_acc(lv, A.bc, A.inh != 0, ci, cj[u], k, av, wc, nb, D, S);
+            }
+            const T r = R.b[u] - (D * pc - S);
+            if (!cok[u]) continue;
+            const long long c = cc[u] + (long long)plane * k;
+            if (MODE == 1) {
+                if ((k & 1) == 0) racc[u] = r;
+                else {
+                    T s = T(0);
+                    s += racc[u];
+                    s += r;
+                    const long long cc2 = cc[u] + (long long)plane * (k >> 1);
+                    A.cb[cc2] = s / T(2);
+                    A.cphi[cc2] = T(0);
+                }
+            } else {
+                if (MODE == 3) A.r32[c] = (float)r;
+                mr = nmax(mr, fabs((double)r));
+                mb = nmax(mb, fabs((double)R.b[u]));
+            }
+        }
+    };
+    ResidRegs<T> R0, R1;
+    load_plane(R0, z0 - 1);
+    put_plane(R0, z0 - 1);
+    load_plane(R0, z0);
+    put_plane(R0, z0);
+    load_plane(R0, z0 + 1);
+    load_cells(R0, z0);
+    auto step = [&](int p, ResidRegs<T>& cur, ResidRegs<T>& nxt) {
+        /* cur's loads were issued a step ago (s_waitcnt: vmcnt 0, expcnt and
+         * lgkmcnt unconstrained) */
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        put_plane(cur, p + 1);
+        load_plane(nxt, p + 2);
+        load_cells(nxt, p + 1);
+        __syncthreads();
+        eval(cur, p);
+    };
+    for (int p = z0; p < z1; p += 2) {
+        step(p, R0, R1);
+        if (p + 1 < z1) step(p + 1, R1, R0);
+    }
+    if (MODE >= 2) {
+        mr = block_max<256>(mr, sh);
+        mb = block_max<256>(mb, sh);
+        if (tid == 0) { A.part[blockIdx.x] = mr; A.part[gridDim.x + blockIdx.x] = mb; }
+    }
 }
 
 /* --------------------- residual + restriction ----------------------------- */
@@ -1285,8 +1505,47 @@ void mg_smooth(hipStream_t s, const DLev& l, const DBC& bc, int inh, int color) 
     const dim3 g(((l.n0 + 1) / 2 + 63) / 64, (l.n1 + 3) / 4, l.n2);
     hipLaunchKernelGGL(k_mg_smooth<double>, g, dim3(256), 0, s, l, bc, inh, color, 0);
 }
+/* The streaming residual serves layered levels large enough to fill the
+ * chip with 64x8 tiles; the fixed-grid kernels take the rest. */
+template <class T>
+static bool resid_stream_ok(const DLevT<T>& l, const DBC& bc) {
+    return l.layered && bc.lo[2] != BC_PER && l.n2 >= 2 && l.n0 >= 64 && l.n1 >= 8 &&
+           (l.read_a != 1 || l.mat != nullptr) && (long long)l.n0 * l.n1 * l.n2 >= STREAM_MIN_CELLS;
+}
+template <class T>
+static ResidArgs<T> resid_args(const DLevT<T>& l, const DBC& bc, int inh, int* nblk) {
+    ResidArgs<T> A;
+    A.n0 = l.n0; A.n1 = l.n1; A.n2 = l.n2; A.msx = l.msx; A.msy = l.msy;
+    A.phi = l.phi; A.b = l.b; A.a = l.a; A.mat = l.mat;
+    A.bxk = l.bxk; A.byk = l.byk; A.bzk = l.bzk;
+    for (int d = 0; d < 3; ++d) A.ih2[d] = l.ih2[d];
+    A.bc = bc;
+    A.inh = inh;
+    A.ntx = (l.n0 + RTX - 1) / RTX;
+    A.nty = (l.n1 + RTY - 1) / RTY;
+    A.cz = stream_chunk(A.ntx * A.nty, l.n2);
+    *nblk = A.ntx * A.nty * ((l.n2 + A.cz - 1) / A.cz);
+    A.remap = (*nblk % 8) == 0 ? 1 : 0;
+    A.cb = nullptr; A.cphi = nullptr; A.r32 = nullptr; A.part = nullptr;
+    return A;
+}
+template <class T, int MODE>
+static void resid_stream_launch(hipStream_t s, const DLevT<T>& l, const ResidArgs<T>& A, int nblk) {
+#define RS(am) hipLaunchKernelGGL((k_mg_resid_stream<T, am, MODE>), dim3(nblk), dim3(256), 0, s, A)
+    switch (l.read_a) { case 0: RS(0); break; case 1: RS(1); break; default: RS(2); break; }
+#undef RS
+}
+
 template <class T>
 static void resid_restrict_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>& c, const DBC& bc, int inh) {
+    const int zonly = f.crs[0] == 0 && f.crs[1] == 0 && f.crs[2] == 1 && c.n0 == f.n0 && c.n1 == f.n1 && (f.n2 % 2) == 0;
+    if (zonly && resid_stream_ok(f, bc)) {
+        int nblk;
+        ResidArgs<T> A = resid_args(f, bc, inh, &nblk);
+        A.cb = c.b; A.cphi = c.phi;
+        resid_stream_launch<T, 1>(s, f, A, nblk);
+        return;
+    }
     const dim3 g = grid3(c.n0, c.n1, c.n2);
     const int code = f.crs[0] | (f.crs[1] << 1) | (f.crs[2] << 2);
 #define RR(cx, cy, cz) hipLaunchKernelGGL((k_mg_resid_restrict<T, cx, cy, cz>), g, dim3(256), 0, s, f, c, bc, inh)
@@ -1325,6 +1584,16 @@ static constexpr int RESID_ZC = 4;
 static constexpr int RESID_NB = 1024;   /* workgroups of the residual kernels */
 int resid_norm_blocks(const DLev&) { return RESID_NB; }
 void mg_resid_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, double* r, double* part, double* out) {
+    if (!r && resid_stream_ok(l, bc)) {
+        int nblk;
+        ResidArgs<double> A = resid_args(l, bc, inh, &nblk);
+        if (nblk <= RESID_NB) {
+            A.part = part;
+            resid_stream_launch<double, 2>(s, l, A, nblk);
+            hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, nblk, 2, 1, out);
+            return;
+        }
+    }
     if (r) hipLaunchKernelGGL((k_mg_resid_max<1, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, inh, r, nullptr, part);
     else hipLaunchKernelGGL((k_mg_resid_max<0, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, inh, nullptr, nullptr, part);
     hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, RESID_NB, 2, 1, out);
@@ -1334,6 +1603,17 @@ void add_f2d(hipStream_t s, double* phi, const float* e, long long n) {
     hipLaunchKernelGGL(k_add_f2d, dim3(b), dim3(256), 0, s, phi, e, n);
 }
 void resid_to32(hipStream_t s, const DLev& l, const DBC& bc, float* r32, double* part, double* out) {
+    if (resid_stream_ok(l, bc)) {
+        int nblk;
+        ResidArgs<double> A = resid_args(l, bc, 1, &nblk);
+        if (nblk <= RESID_NB) {
+            A.part = part;
+            A.r32 = r32;
+            resid_stream_launch<double, 3>(s, l, A, nblk);
+            hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, nblk, 2, 1, out);
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_mg_resid_max<2, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, 1, nullptr, r32, part);
     hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, RESID_NB, 2, 1, out);
 }

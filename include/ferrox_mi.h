@@ -281,7 +281,8 @@ int fx_set_plot_dir(fx_ctx* ctx, const char* dir);
 int fx_set_bias_waveform(fx_ctx* ctx, const double* v, int n);
 
 /* Measurement: average duration in ms of one launch of a named hot-path
- * kernel ("smooth0", "resid0", "tdgl", "rhs", "copy"), timed with HIP events
+ * kernel ("smooth0", "prolong0", "resid0", "residmax0", "tdgl", "rhs", "copy"),
+ * timed with HIP events
  * on the context's stream.  No reference counterpart (instrumentation). */
 int fx_bench_kernel(fx_ctx* ctx, const char* which, int reps, double* avg_ms);
 
