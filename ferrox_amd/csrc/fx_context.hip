@@ -37,6 +37,7 @@ constexpr double OUTER_TOL = 1.e-5;  /* ElectrostaticSolver.cpp:697 */
 constexpr double BOTTOM_TOL = 1.e-6;
 constexpr int BOTTOM_MAXIT = 400;
 constexpr size_t TAIL_MAX_CELLS = 8192;   /* levels this small run inside the single-workgroup tail kernel */
+constexpr size_t BLOCK_MAX_CELLS = 1u << 21;   /* below this, a level's legs run as one block launch each */
 
 inline uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ULL;
@@ -720,8 +721,15 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
         }
         return;
     }
-    fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu1, x->in.mg_fused_sweep);
     Level& C = x->lev[li + 1];
+    if (li >= 1 && x->in.mg_block_levels && L.N < BLOCK_MAX_CELLS &&
+        fxk::mg_block_ok(L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2)) {
+        fxk::mg_block_down(x->s, L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2);
+        vcycle(x, li + 1, bc);
+        fxk::mg_block_up(x->s, L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2);
+        return;
+    }
+    fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu1, x->in.mg_fused_sweep);
     fxk::mg_resid_restrict(x->s, L.d, C.d, bc, inh);
     vcycle(x, li + 1, bc);
     fxk::mg_prolong_sweep(x->s, L.d, C.d, bc, inh, x->in.mg_nu2, x->in.mg_fused_sweep);
@@ -758,8 +766,15 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
         fxk::bottom_cg(x->s, L, bc, L.b, L.phi, x->cg_work32, 1e-5, BOTTOM_MAXIT, x->d_iters);
         return;
     }
-    fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
     DLevF& C = x->lev32[li + 1];
+    if (li >= 1 && x->in.mg_block_levels && x->lev[li].N < BLOCK_MAX_CELLS &&
+        fxk::mg_block_ok(L, C, bc, x->in.mg_nu1, x->in.mg_nu2)) {
+        fxk::mg_block_down(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2);
+        vcycle32(x, li + 1, bc, true);
+        fxk::mg_block_up(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2);
+        return;
+    }
+    fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
     fxk::mg_resid_restrict(x->s, L, C, bc);
     vcycle32(x, li + 1, bc, true);
     fxk::mg_prolong_sweep(x->s, L, C, bc, x->in.mg_nu2, x->in.mg_fused_sweep);

@@ -11,7 +11,7 @@
  *                    sweep, with phi and rhs read once and phi written once.
  *   k_mg_smooth      one colour of the sweep (small levels, periodic z)
  *   k_mg_resid_restrict   residual averaged into the coarse rhs; coarse phi = 0
- *   k_mg_prolong     cell-centred linear prolongation added to the fine phi
+ *   k_mg_prolong_blk cell-centred linear prolongation added to the fine phi
  *   k_mg_resid_max   residual with max|r| and max|b| per workgroup (fixed grid)
  *   k_bottom         exact (dense) or Jacobi-PCG bottom solve in one workgroup
  *   k_mg_tail        the coarse end of the V-cycle in one workgroup, staged in LDS
@@ -25,6 +25,15 @@
 #include <type_traits>
 
 using namespace fxd;
+
+namespace {
+/* c / n for 0 <= c < 2^22 from a float reciprocal r = 1/n, corrected by one step */
+__device__ __forceinline__ int qdiv(int c, int n, float r) {
+    const int q = (int)((float)c * r);
+    const int m = c - q * n;
+    return q + (m >= n ? 1 : 0) - (m < 0 ? 1 : 0);
+}
+}  // namespace
 
 /* ------------------------- one colour of RB-GS ---------------------------- */
 /* grid: x covers half-rows in chunks of 64 threads, y = j/4, z = k.
@@ -271,7 +280,7 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
             for (int s = 0; s < SLD; ++s) { cw0[s] = cw1[s]; cw1[s] = R.cn[s]; }
         }
         /* PROLONG: plane k of phi + P e, with the weights and sum order of
-         * k_mg_prolong for a z-only coarsening (parent 3/4, neighbour 1/4;
+         * k_mg_prolong_blk for a z-only coarsening (parent 3/4, neighbour 1/4;
          * across a z face the neighbour is the homogeneous ghost) */
         T wj = T(0);
         if (PROLONG) {
@@ -616,43 +625,171 @@ __global__ __launch_bounds__(256) void k_mg_resid_restrict(DLevT<T> f, DLevT<T> 
     cl.phi[t] = T(0);
 }
 
-/* ------------------------------ prolongation ------------------------------ */
-/* In each coarsened direction the fine cell takes 3/4 of its parent and 1/4 of
- * the parent's neighbour on its side; across a domain face that neighbour is
- * the homogeneous ghost (-parent Dirichlet, +parent Neumann, periodic image).
- * Sum order: z outer, x inner, parent first. */
-template <class T>
-__global__ __launch_bounds__(256) void k_mg_prolong(DLevT<T> f, DLevT<T> cl, DBC bc) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    if (i >= f.n0 || j >= f.n1) return;
-    const int fi[3] = {i, j, k};
-    const int cn[3] = {cl.n0, cl.n1, cl.n2};
-    int id[3][2];
-    T w[3][2];
-    int cnt[3];
+/* The same residual + restriction for a layered level: the children of the
+ * coarse cell and their face neighbours are loaded once (a 4x4x4 block
+ * without its edges and corners when every direction is coarsened), and
+ * each child's row uses the per-plane weights, with the interior diagonal
+ * accumulated in the row operator's order (the row operator itself at
+ * domain faces).  Children summed in the order of k_mg_resid_restrict. */
+template <class T, int CX, int CY, int CZ>
+__global__ __launch_bounds__(256) void k_mg_resid_restrict_lay(DLevT<T> f, DLevT<T> cl, DBC bc, int inh) {
+    const int I = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int K = blockIdx.z;
+    if (I >= cl.n0 || J >= cl.n1) return;
+    const int n0 = f.n0, n1 = f.n1, n2 = f.n2;
+    constexpr int NX = CX ? 2 : 1, NY = CY ? 2 : 1, NZ = CZ ? 2 : 1;
+    const int i0 = CX ? 2 * I : I, j0 = CY ? 2 * J : J, k0 = CZ ? 2 * K : K;
+    /* fine index of offset o (-1 .. N) along a direction, wrapped where periodic, clamped elsewhere */
+    auto wrap = [&](int v, int n, int d) {
+        if (v < 0) return bc.lo[d] == BC_PER ? v + n : 0;
+        if (v >= n) return bc.hi[d] == BC_PER ? v - n : n - 1;
+        return v;
+    };
+    int xs[NX + 2], ys[NY + 2], zs[NZ + 2];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        if (!f.crs[d]) { id[d][0] = fi[d]; id[d][1] = fi[d]; w[d][0] = T(1); w[d][1] = T(0); cnt[d] = 1; continue; }
-        const int I = fi[d] >> 1;
-        int J = (fi[d] & 1) ? I + 1 : I - 1;
-        T sg = T(1);
-        if (J < 0 || J >= cn[d]) {
-            const int bct = J < 0 ? bc.lo[d] : bc.hi[d];
-            if (bct == BC_PER) J = J < 0 ? J + cn[d] : J - cn[d];
-            else { sg = (bct == BC_DIR) ? T(-1) : T(1); J = I; }
-        }
-        id[d][0] = I; w[d][0] = T(0.75);
-        id[d][1] = J; w[d][1] = T(0.25) * sg;
-        cnt[d] = 2;
+    for (int o = 0; o < NX + 2; ++o) xs[o] = wrap(i0 + o - 1, n0, 0);
+#pragma unroll
+    for (int o = 0; o < NY + 2; ++o) ys[o] = wrap(j0 + o - 1, n1, 1);
+#pragma unroll
+    for (int o = 0; o < NZ + 2; ++o) zs[o] = wrap(k0 + o - 1, n2, 2);
+    /* phi on the children and their face neighbours: v[z][y][x], offsets -1 .. N */
+    T v[NZ + 2][NY + 2][NX + 2];
+#pragma unroll
+    for (int z = 0; z < NZ + 2; ++z)
+#pragma unroll
+        for (int y = 0; y < NY + 2; ++y)
+#pragma unroll
+            for (int x = 0; x < NX + 2; ++x) {
+                const int out = (x == 0 || x == NX + 1) + (y == 0 || y == NY + 1) + (z == 0 || z == NZ + 1);
+                v[z][y][x] = out <= 1 ? f.phi[xs[x] + n0 * (ys[y] + (long long)n1 * zs[z])] : T(0);
+            }
+    T s = T(0);
+#pragma unroll
+    for (int kk = 0; kk < NZ; ++kk) {
+        const int k = k0 + kk;
+        const T wx = f.bxk[k] * f.ih2[0], wy = f.byk[k] * f.ih2[1];
+        const T wzl = f.bzk[k] * f.ih2[2], wzh = f.bzk[k + 1] * f.ih2[2];
+#pragma unroll
+        for (int jj = 0; jj < NY; ++jj)
+#pragma unroll
+            for (int ii = 0; ii < NX; ++ii) {
+                const int i = i0 + ii, j = j0 + jj;
+                const long long c = i + n0 * (j + (long long)n1 * k);
+                const T pc = v[kk + 1][jj + 1][ii + 1];
+                const T nb[6] = {v[kk + 1][jj + 1][ii], v[kk + 1][jj + 1][ii + 2], v[kk + 1][jj][ii + 1],
+                                 v[kk + 1][jj + 2][ii + 1], v[kk][jj + 1][ii + 1], v[kk + 2][jj + 1][ii + 1]};
+                T a;
+                if (f.read_a == 0) a = T(0);
+                else if (f.read_a == 2) a = f.a[c];
+                else a = ((f.mat[(i + 1) + f.msx * (j + 1) + f.msy * (k + 1)] & 3) >= 2) ? f.a[c] : T(0);
+                const bool in = (bc.lo[0] == BC_PER || (i > 0 && i < n0 - 1)) &&
+                                (bc.lo[1] == BC_PER || (j > 0 && j < n1 - 1)) &&
+                                (bc.lo[2] == BC_PER || (k > 0 && k < n2 - 1));
+                T D, S;
+                if (in) {
+                    S = T(0);
+                    S += wx * nb[0]; S += wx * nb[1]; S += wy * nb[2]; S += wy * nb[3]; S += wzl * nb[4]; S += wzh * nb[5];
+                    D = -a; D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+                } else {
+                    const T wc[6] = {wx, wx, wy, wy, wzl, wzh};
+                    mg_acc(f, bc, inh != 0, i, j, k, a, wc, nb, D, S);
+                }
+                s += f.b[c] - (D * pc - S);
+            }
     }
-    T e = T(0);
-    for (int a = 0; a < cnt[2]; ++a)
-        for (int b = 0; b < cnt[1]; ++b)
-            for (int q = 0; q < cnt[0]; ++q)
-                e += (w[2][a] * w[1][b]) * w[0][q] * cl.phi[id[0][q] + cl.n0 * (id[1][b] + cl.n1 * id[2][a])];
-    f.phi[i + f.n0 * (j + f.n1 * k)] += e;
+    constexpr int cnt = NX * NY * NZ;
+    const int t = I + cl.n0 * (J + cl.n1 * K);
+    cl.b[t] = s / T(cnt);
+    cl.phi[t] = T(0);
+}
+
+/* ------------------------------ prolongation ------------------------------ */
+/* Cell-centred linear prolongation added to the fine phi.  In each coarsened
+ * direction the fine cell takes 3/4 of its parent and 1/4 of the parent's
+ * neighbour on its side; across a domain face that neighbour is the
+ * homogeneous ghost (-parent Dirichlet, +parent Neumann, periodic image).
+ * Per fine cell: e = sum over (z, then y, then x; parent first) of
+ * (w_z * w_y) * w_x * value.  One thread per coarse cell: the parent's
+ * coarse neighbourhood (-1..1 in each coarsened direction) is loaded once
+ * and every child updated from it; children along x are adjacent, so they
+ * are read and written as pairs when x is coarsened. */
+template <class T, int CX, int CY, int CZ>
+__global__ __launch_bounds__(256) void k_mg_prolong_blk(DLevT<T> f, DLevT<T> cl, DBC bc) {
+    const int I = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int J = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int K = blockIdx.z;
+    if (I >= cl.n0 || J >= cl.n1) return;
+    const int P[3] = {I, J, K};
+    const int cn[3] = {cl.n0, cl.n1, cl.n2};
+    constexpr int CR[3] = {CX, CY, CZ};
+    /* per direction: index of the neighbour below (0) and above (1), and its ghost sign */
+    int nb[3][2];
+    T sg[3][2];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            int Jn = s ? P[d] + 1 : P[d] - 1;
+            T g = T(1);
+            if (Jn < 0 || Jn >= cn[d]) {
+                const int bct = Jn < 0 ? bc.lo[d] : bc.hi[d];
+                if (bct == BC_PER) Jn = Jn < 0 ? Jn + cn[d] : Jn - cn[d];
+                else { g = (bct == BC_DIR) ? T(-1) : T(1); Jn = P[d]; }
+            }
+            nb[d][s] = Jn;
+            sg[d][s] = g;
+        }
+    /* coarse values v[z][y][x]: index 0 parent, 1 neighbour below, 2 neighbour above */
+    T v[CZ ? 3 : 1][CY ? 3 : 1][CX ? 3 : 1];
+#pragma unroll
+    for (int a = 0; a < (CZ ? 3 : 1); ++a)
+#pragma unroll
+        for (int b = 0; b < (CY ? 3 : 1); ++b)
+#pragma unroll
+            for (int q = 0; q < (CX ? 3 : 1); ++q) {
+                const int kz = a == 0 ? K : nb[2][a - 1];
+                const int jy = b == 0 ? J : nb[1][b - 1];
+                const int ix = q == 0 ? I : nb[0][q - 1];
+                v[a][b][q] = cl.phi[ix + cl.n0 * (jy + cl.n1 * kz)];
+            }
+    (void)CR;
+#pragma unroll
+    for (int c = 0; c < (CZ ? 2 : 1); ++c)
+#pragma unroll
+        for (int bb = 0; bb < (CY ? 2 : 1); ++bb) {
+            const int k = CZ ? 2 * K + c : K, j = CY ? 2 * J + bb : J;
+            T e[CX ? 2 : 1];
+#pragma unroll
+            for (int aa = 0; aa < (CX ? 2 : 1); ++aa) {
+                /* child (aa, bb, c): its side's neighbour is 1 + side in v */
+                T wz[2], wy[2], wx[2];
+                int iz[2], iy[2], ix[2];
+                wz[0] = CZ ? T(0.75) : T(1); wz[1] = T(0.25) * sg[2][c]; iz[0] = 0; iz[1] = 1 + c;
+                wy[0] = CY ? T(0.75) : T(1); wy[1] = T(0.25) * sg[1][bb]; iy[0] = 0; iy[1] = 1 + bb;
+                wx[0] = CX ? T(0.75) : T(1); wx[1] = T(0.25) * sg[0][aa]; ix[0] = 0; ix[1] = 1 + aa;
+                T s = T(0);
+#pragma unroll
+                for (int za = 0; za < (CZ ? 2 : 1); ++za)
+#pragma unroll
+                    for (int yb = 0; yb < (CY ? 2 : 1); ++yb)
+#pragma unroll
+                        for (int xq = 0; xq < (CX ? 2 : 1); ++xq)
+                            s += (wz[za] * wy[yb]) * wx[xq] * v[CZ ? iz[za] : 0][CY ? iy[yb] : 0][CX ? ix[xq] : 0];
+                e[aa] = s;
+            }
+            const long long row = (long long)f.n0 * (j + (long long)f.n1 * k);
+            if (CX) {
+                using T2 = typename Vec2<T>::type;
+                T2* p2 = reinterpret_cast<T2*>(f.phi + row + 2 * I);
+                T2 o = *p2;
+                o.x += e[0];
+                o.y += e[CX ? 1 : 0];
+                *p2 = o;
+            } else {
+                f.phi[row + I] += e[0];
+            }
+        }
 }
 
 /* Cell average of a fine array into the coarse level (A-coefficient). */
@@ -1045,6 +1182,323 @@ __global__ __launch_bounds__(1024) void k_mg_tail(const DLevT<T>* __restrict__ l
     }
 }
 
+/* --------- small layered levels: a whole leg per launch, blocks in LDS ------- */
+/* A level too small to fill the chip with streaming planes but too large for
+ * the single-workgroup tail runs each leg of the V-cycle as one launch: every
+ * workgroup owns a bx x by column block over all planes (z not periodic) and
+ * holds it in LDS with an in-plane halo wide enough that its colour phases
+ * need nothing from other workgroups.  A red phase on the block and a ring of
+ * width r, then a black phase on ring r-1, and so on: the values on the
+ * shrinking rings are exactly those the owning workgroups compute, so the
+ * result is the result of separate launches.
+ *   down: phi = 0 (the level's phi after restriction), nu sweeps on rings
+ *         2nu .. 1 (halo 2nu+1), the residual on the block restricted into
+ *         the coarse rhs (coarse phi = 0), and the block's phi stored;
+ *   up:   phi on the block and halo 2nu plus the prolongated correction from
+ *         the coarse level (staged in LDS with its own halo), nu sweeps on
+ *         rings 2nu-1 .. 0, the block stored.
+ * Per cell the arithmetic of the tail: the per-plane weights, the interior
+ * diagonal in the row operator's order (times 1/D without a Helmholtz
+ * term), the row operator's face terms elsewhere. */
+namespace {
+constexpr int BLK_THREADS = 1024;
+constexpr int BLK_MAXZ = 64;
+__device__ __forceinline__ bool blk_wrap(int& v, int n, bool per) {
+    if (v < 0) { if (!per) return false; v += n; }
+    else if (v >= n) { if (!per) return false; v -= n; }
+    return true;
+}
+}  // namespace
+
+template <class T>
+struct BlkArgs {
+    DLevT<T> l;        /* the level (layered) */
+    DLevT<T> c;        /* the next coarser level */
+    DBC bc;
+    int nu, bx, by;
+};
+
+/* (A phi)_c = D phi_c - S of the LDS cell q (global cell (i, j, k)) */
+template <class T>
+__device__ __forceinline__ bool blk_row(const T* V, int q, int sx, int sxy, int i, int j, int k, int n0, int n1,
+                                        int n2, const DBC& bc, bool perx, bool pery, const T* tb, T av, bool ra,
+                                        T& D, T& S) {
+    const T wx = tb[0], wy = tb[1], wzl = tb[2], wzh = tb[3];
+    const T v0 = V[q - 1], v1 = V[q + 1], v2 = V[q - sx], v3 = V[q + sx];
+    const T v4 = V[k > 0 ? q - sxy : q], v5 = V[k < n2 - 1 ? q + sxy : q];
+    const bool in = (perx || (i > 0 && i < n0 - 1)) && (pery || (j > 0 && j < n1 - 1)) && k > 0 && k < n2 - 1;
+    if (in) {
+        S = T(0);
+        S += wx * v0; S += wx * v1; S += wy * v2; S += wy * v3; S += wzl * v4; S += wzh * v5;
+        if (ra) { D = -av; D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh; }
+        else D = tb[4];
+        return true;
+    }
+    D = -av;
+    S = T(0);
+    auto face = [&](bool open, int kind, T wv, T nv) {
+        D += open ? wv : (kind == BC_DIR ? T(2) * wv : T(0));
+        S += open ? wv * nv : T(0);
+    };
+    face(i > 0 || perx, bc.lo[0], wx, v0);
+    face(i < n0 - 1 || perx, bc.hi[0], wx, v1);
+    face(j > 0 || pery, bc.lo[1], wy, v2);
+    face(j < n1 - 1 || pery, bc.hi[1], wy, v3);
+    face(k > 0, bc.lo[2], wzl, v4);
+    face(k < n2 - 1, bc.hi[2], wzh, v5);
+    return false;
+}
+
+/* Shared part of the block kernels: the per-plane table, the rhs (and
+ * Helmholtz term) of the cells the phases update staged in LDS, and the
+ * colour phases on rings of the block held in V (extent (bx+2H) x (by+2H) x
+ * n2; the rhs on rings up to H-1). */
+template <class T>
+struct BlkLevel {
+    const BlkArgs<T>& A;
+    T* V;
+    T* tab;
+    T* Bv;
+    T* Av;
+    int H, LX, LY, BLX, BLY, x0, y0, tid;
+    bool perx, pery, ra;
+    __device__ BlkLevel(const BlkArgs<T>& a, T* v, T* t, int h) : A(a), V(v), tab(t), H(h) {
+        LX = A.bx + 2 * H;
+        LY = A.by + 2 * H;
+        BLX = LX - 2;
+        BLY = LY - 2;
+        Bv = nullptr;
+        Av = nullptr;
+        const int nbx = A.l.n0 / A.bx;
+        x0 = (blockIdx.x % nbx) * A.bx;
+        y0 = (blockIdx.x / nbx) * A.by;
+        tid = threadIdx.x;
+        perx = A.bc.lo[0] == BC_PER;
+        pery = A.bc.lo[1] == BC_PER;
+        ra = A.l.read_a != 0;
+    }
+    __device__ void table() {
+        const DLevT<T>& l = A.l;
+        for (int k = tid; k < l.n2; k += BLK_THREADS) {
+            const T wx = l.bxk[k] * l.ih2[0], wy = l.byk[k] * l.ih2[1];
+            const T wzl = l.bzk[k] * l.ih2[2], wzh = l.bzk[k + 1] * l.ih2[2];
+            T D = T(0);
+            D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+            T* t = tab + 6 * k;
+            t[0] = wx; t[1] = wy; t[2] = wzl; t[3] = wzh; t[4] = D; t[5] = T(1) / D;
+        }
+    }
+    /* the rhs (and Helmholtz term) on rings 0 .. H-1, from b0 on (one pass of loads) */
+    __device__ void load_coef(T* b0) {
+        const DLevT<T>& l = A.l;
+        Bv = b0;
+        const int tot = BLX * BLY * l.n2;
+        Av = ra ? b0 + tot : nullptr;
+        const float rx = 1.0f / (float)BLX, rxy = 1.0f / (float)(BLX * BLY);
+        for (int t = tid; t < tot; t += BLK_THREADS) {
+            const int k = qdiv(t, BLX * BLY, rxy);
+            const int rr = t - k * BLX * BLY;
+            const int yl = qdiv(rr, BLX, rx);
+            const int xl = rr - yl * BLX;
+            int i = x0 - (H - 1) + xl, j = y0 - (H - 1) + yl;
+            const bool ok = blk_wrap(i, l.n0, perx) && blk_wrap(j, l.n1, pery);
+            const long long c = i + (long long)l.n0 * (j + (long long)l.n1 * k);
+            Bv[t] = ok ? l.b[c] : T(0);
+            if (ra) Av[t] = ok ? l.a[c] : T(0);
+        }
+    }
+    __device__ int qb(int xl, int jl, int k) const { return xl + BLX * (jl + BLY * k); }
+    /* one colour (0 red: i+j+k even) on the block and its ring of width r */
+    __device__ void phase(int r, int col) {
+        const DLevT<T>& l = A.l;
+        const int X = A.bx + 2 * r, Y = A.by + 2 * r;
+        const int half = (X + 1) >> 1;
+        const int tot = half * Y * l.n2;
+        const float rh = 1.0f / (float)half, ry = 1.0f / (float)Y;
+        for (int t = tid; t < tot; t += BLK_THREADS) {
+            const int jk = qdiv(t, half, rh);
+            const int ih = t - jk * half;
+            const int k = qdiv(jk, Y, ry);
+            const int jl = jk - k * Y;
+            const int ju = y0 - r + jl;
+            /* x0 is even and periodic extents are even, so the parity of the
+             * unwrapped coordinates is that of the cell */
+            const int xl = 2 * ih + ((ju + k + col + r) & 1);
+            if (xl >= X) continue;
+            int i = x0 - r + xl, j = ju;
+            if (!blk_wrap(i, l.n0, perx) || !blk_wrap(j, l.n1, pery)) continue;
+            const int q = (H - r + xl) + LX * ((H - r + jl) + LY * k);
+            const int qq = qb(H - 1 - r + xl, H - 1 - r + jl, k);
+            const T* tb = tab + 6 * k;
+            T D, S;
+            const bool in = blk_row(V, q, LX, LX * LY, i, j, k, l.n0, l.n1, l.n2, A.bc, perx, pery, tb,
+                                    ra ? Av[qq] : T(0), ra, D, S);
+            const T bv = Bv[qq];
+            V[q] = (in && !ra) ? (bv + S) * tb[5] : (bv + S) / D;
+        }
+    }
+    /* the block (ring 0) back to the level's phi */
+    __device__ void store() {
+        const DLevT<T>& l = A.l;
+        const int tot = A.bx * A.by * l.n2;
+        const float rx = 1.0f / (float)A.bx, rxy = 1.0f / (float)(A.bx * A.by);
+        for (int t = tid; t < tot; t += BLK_THREADS) {
+            const int k = qdiv(t, A.bx * A.by, rxy);
+            const int rr = t - k * A.bx * A.by;
+            const int jl = qdiv(rr, A.bx, rx);
+            const int xl = rr - jl * A.bx;
+            l.phi[(x0 + xl) + (long long)l.n0 * ((y0 + jl) + (long long)l.n1 * k)] = V[(H + xl) + LX * ((H + jl) + LY * k)];
+        }
+    }
+};
+
+template <class T, int CX, int CY, int CZ>
+__global__ __launch_bounds__(BLK_THREADS) void k_mg_block_down(BlkArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char blk_lds[];
+    __shared__ T tab[6 * BLK_MAXZ];
+    BlkLevel<T> B(A, reinterpret_cast<T*>(blk_lds), tab, 2 * A.nu + 1);
+    const DLevT<T>& l = A.l;
+    B.table();
+    const int nv = B.LX * B.LY * l.n2;
+    B.load_coef(B.V + nv);
+    for (int q = B.tid; q < nv; q += BLK_THREADS) B.V[q] = T(0);
+    __syncthreads();
+    for (int s = 0; s < A.nu; ++s)
+        for (int col = 0; col < 2; ++col) {
+            B.phase(B.H - 1 - (2 * s + col), col);
+            __syncthreads();
+        }
+    /* residual on the block, restricted: children summed z outer, x inner */
+    const int cbx = A.bx >> CX, cby = A.by >> CY;
+    const int ctot = cbx * cby * (l.n2 >> CZ);
+    const float rcx = 1.0f / (float)cbx, rcxy = 1.0f / (float)(cbx * cby);
+    for (int t = B.tid; t < ctot; t += BLK_THREADS) {
+        const int K = qdiv(t, cbx * cby, rcxy);
+        const int rr = t - K * cbx * cby;
+        const int Jl = qdiv(rr, cbx, rcx);
+        const int Il = rr - Jl * cbx;
+        T s = T(0);
+#pragma unroll
+        for (int kk = 0; kk <= CZ; ++kk)
+#pragma unroll
+            for (int jj = 0; jj <= CY; ++jj)
+#pragma unroll
+                for (int ii = 0; ii <= CX; ++ii) {
+                    const int xl = (Il << CX) + ii, jl = (Jl << CY) + jj, k = (K << CZ) + kk;
+                    const int i = B.x0 + xl, j = B.y0 + jl;
+                    const int q = (B.H + xl) + B.LX * ((B.H + jl) + B.LY * k);
+                    const int qq = B.qb(B.H - 1 + xl, B.H - 1 + jl, k);
+                    T D, S;
+                    blk_row(B.V, q, B.LX, B.LX * B.LY, i, j, k, l.n0, l.n1, l.n2, A.bc, B.perx, B.pery, tab + 6 * k,
+                            B.ra ? B.Av[qq] : T(0), B.ra, D, S);
+                    s += B.Bv[qq] - (D * B.V[q] - S);
+                }
+        const long long ct = ((B.x0 >> CX) + Il) + (long long)A.c.n0 * (((B.y0 >> CY) + Jl) + (long long)A.c.n1 * K);
+        A.c.b[ct] = s / T((1 + CX) * (1 + CY) * (1 + CZ));
+        A.c.phi[ct] = T(0);
+    }
+    B.store();
+}
+
+template <class T, int CX, int CY, int CZ>
+__global__ __launch_bounds__(BLK_THREADS) void k_mg_block_up(BlkArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char blk_lds[];
+    __shared__ T tab[6 * BLK_MAXZ];
+    BlkLevel<T> B(A, reinterpret_cast<T*>(blk_lds), tab, 2 * A.nu);
+    const DLevT<T>& l = A.l;
+    const DLevT<T>& c = A.c;
+    const int H = B.H, LX = B.LX, LY = B.LY, n2 = l.n2;
+    /* the coarse correction on the parents of the block and halo, with their neighbours */
+    const int HC = H / 2 + 1;
+    const int cx0 = CX ? (B.x0 >> 1) - HC : B.x0 - H, cy0 = CY ? (B.y0 >> 1) - HC : B.y0 - H;
+    const int CLX = CX ? (A.bx >> 1) + 2 * HC : LX, CLY = CY ? (A.by >> 1) + 2 * HC : LY;
+    T* const V = B.V;
+    T* const C = V + LX * LY * n2;
+    B.table();
+    B.load_coef(C + CLX * CLY * c.n2);
+    {
+        const int tot = CLX * CLY * c.n2;
+        const float rx = 1.0f / (float)CLX, rxy = 1.0f / (float)(CLX * CLY);
+        for (int t = B.tid; t < tot; t += BLK_THREADS) {
+            const int K = qdiv(t, CLX * CLY, rxy);
+            const int rr = t - K * CLX * CLY;
+            const int yl = qdiv(rr, CLX, rx);
+            const int xl = rr - yl * CLX;
+            int I = cx0 + xl, J = cy0 + yl;
+            const bool ok = blk_wrap(I, c.n0, B.perx) && blk_wrap(J, c.n1, B.pery);
+            C[t] = ok ? c.phi[I + (long long)c.n0 * (J + (long long)c.n1 * K)] : T(0);
+        }
+        const int nv = LX * LY * n2;
+        const float fx = 1.0f / (float)LX, fxy = 1.0f / (float)(LX * LY);
+        for (int t = B.tid; t < nv; t += BLK_THREADS) {
+            const int k = qdiv(t, LX * LY, fxy);
+            const int rr = t - k * LX * LY;
+            const int yl = qdiv(rr, LX, fx);
+            const int xl = rr - yl * LX;
+            int i = B.x0 - H + xl, j = B.y0 - H + yl;
+            const bool ok = blk_wrap(i, l.n0, B.perx) && blk_wrap(j, l.n1, B.pery);
+            V[t] = ok ? l.phi[i + (long long)l.n0 * (j + (long long)l.n1 * k)] : T(0);
+        }
+    }
+    __syncthreads();
+    /* phi += P e on the block and halo (weights, products and sum order of k_mg_prolong_blk) */
+    {
+        const int nv = LX * LY * n2;
+        const float fx = 1.0f / (float)LX, fxy = 1.0f / (float)(LX * LY);
+        for (int t = B.tid; t < nv; t += BLK_THREADS) {
+            const int k = qdiv(t, LX * LY, fxy);
+            const int rr = t - k * LX * LY;
+            const int yl = qdiv(rr, LX, fx);
+            const int xl = rr - yl * LX;
+            const int iu = B.x0 - H + xl, ju = B.y0 - H + yl;
+            int i = iu, j = ju;
+            if (!blk_wrap(i, l.n0, B.perx) || !blk_wrap(j, l.n1, B.pery)) continue;
+            /* per direction: LDS coordinate of the parent and of its neighbour on the side, weights */
+            int px[2], py[2], pz[2];
+            T wx[2], wy[2], wz[2];
+            if (CX) {
+                const int P = iu >> 1, s = iu & 1;
+                int Nb = s ? P + 1 : P - 1;
+                T g = T(1);
+                const int Nw = s ? (i >> 1) + 1 : (i >> 1) - 1;   /* the neighbour's wrapped index */
+                if (!B.perx && (Nw < 0 || Nw >= c.n0)) { g = ((Nw < 0 ? A.bc.lo[0] : A.bc.hi[0]) == BC_DIR) ? T(-1) : T(1); Nb = P; }
+                px[0] = P - cx0; px[1] = Nb - cx0; wx[0] = T(0.75); wx[1] = T(0.25) * g;
+            } else { px[0] = px[1] = xl; wx[0] = T(1); wx[1] = T(0); }
+            if (CY) {
+                const int P = ju >> 1, s = ju & 1;
+                int Nb = s ? P + 1 : P - 1;
+                T g = T(1);
+                const int Nw = s ? (j >> 1) + 1 : (j >> 1) - 1;
+                if (!B.pery && (Nw < 0 || Nw >= c.n1)) { g = ((Nw < 0 ? A.bc.lo[1] : A.bc.hi[1]) == BC_DIR) ? T(-1) : T(1); Nb = P; }
+                py[0] = P - cy0; py[1] = Nb - cy0; wy[0] = T(0.75); wy[1] = T(0.25) * g;
+            } else { py[0] = py[1] = yl; wy[0] = T(1); wy[1] = T(0); }
+            if (CZ) {
+                const int P = k >> 1, s = k & 1;
+                int Nb = s ? P + 1 : P - 1;
+                T g = T(1);
+                if (Nb < 0 || Nb >= c.n2) { g = ((Nb < 0 ? A.bc.lo[2] : A.bc.hi[2]) == BC_DIR) ? T(-1) : T(1); Nb = P; }
+                pz[0] = P; pz[1] = Nb; wz[0] = T(0.75); wz[1] = T(0.25) * g;
+            } else { pz[0] = pz[1] = k; wz[0] = T(1); wz[1] = T(0); }
+            T e = T(0);
+#pragma unroll
+            for (int za = 0; za < (CZ ? 2 : 1); ++za)
+#pragma unroll
+                for (int yb = 0; yb < (CY ? 2 : 1); ++yb)
+#pragma unroll
+                    for (int xq = 0; xq < (CX ? 2 : 1); ++xq)
+                        e += (wz[za] * wy[yb]) * wx[xq] * C[px[xq] + CLX * (py[yb] + CLY * pz[za])];
+            V[t] += e;
+        }
+    }
+    __syncthreads();
+    for (int s = 0; s < A.nu; ++s)
+        for (int col = 0; col < 2; ++col) {
+            B.phase(H - 1 - (2 * s + col), col);
+            __syncthreads();
+        }
+    B.store();
+}
+
 /* ------------------- bottom LU factors, once per operator ----------------- */
 /* The elimination of bottom_direct split into factors and solve: the dense
  * matrix of the bottom level (N <= BOTTOM_DIRECT_MAX cells) assembled from
@@ -1137,12 +1591,6 @@ struct TLev {
     int ophi, ob, oa, otab;   /* offsets (in values) into the dynamic LDS */
     float r0, rs, rh, r1;     /* 1/n0, 1/(n0 n1), 1/half, 1/n1 for index splits */
 };
-/* c / n for 0 <= c < 2^22 from a float reciprocal, corrected by one step */
-__device__ __forceinline__ int qdiv(int c, int n, float r) {
-    const int q = (int)((float)c * r);
-    const int m = c - q * n;
-    return q + (m >= n ? 1 : 0) - (m < 0 ? 1 : 0);
-}
 }  // namespace
 
 /* Values of dynamic LDS the lean tail uses for levels of the given extents. */
@@ -1432,16 +1880,35 @@ static bool stream_use(const DLevT<T>& l, const DBC& bc, int allow_fused) {
     return allow_fused && stream_ok(l, bc) && (long long)l.n0 * l.n1 * l.n2 >= STREAM_MIN_CELLS;
 }
 
+/* Prolongation of the correction on c into f (one thread per coarse cell). */
+template <class T>
+static void prolong_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>& c, const DBC& bc) {
+    const dim3 g = grid3(c.n0, c.n1, c.n2);
+    const int code = f.crs[0] | (f.crs[1] << 1) | (f.crs[2] << 2);
+#define PB(cx, cy, cz) hipLaunchKernelGGL((k_mg_prolong_blk<T, cx, cy, cz>), g, dim3(256), 0, s, f, c, bc)
+    switch (code) {
+    case 0: PB(0, 0, 0); break;
+    case 1: PB(1, 0, 0); break;
+    case 2: PB(0, 1, 0); break;
+    case 3: PB(1, 1, 0); break;
+    case 4: PB(0, 0, 1); break;
+    case 5: PB(1, 0, 1); break;
+    case 6: PB(0, 1, 1); break;
+    default: PB(1, 1, 1); break;
+    }
+#undef PB
+}
+
 /* nsweeps RB-GS sweeps of level l.  With coarse != null the coarse
  * correction is first prolongated into l: folded into the first streaming
  * sweep's plane loads when l is z-coarsened (the sweep reads phi + P e), else
- * by k_mg_prolong. */
+ * by k_mg_prolong_blk. */
 template <class T>
 static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh, int nsweeps, int allow_fused, int zero_first,
                        const DLevT<T>* coarse = nullptr) {
     const bool zonly = coarse && l.crs[0] == 0 && l.crs[1] == 0 && l.crs[2] == 1 && coarse->n0 == l.n0 && coarse->n1 == l.n1;
     if (coarse && !(zonly && nsweeps > 0 && stream_use(l, bc, allow_fused))) {
-        hipLaunchKernelGGL(k_mg_prolong<T>, grid3(l.n0, l.n1, l.n2), dim3(256), 0, s, l, *coarse, bc);
+        prolong_impl<T>(s, l, *coarse, bc);
         coarse = nullptr;
     }
     if (nsweeps <= 0) return;
@@ -1548,6 +2015,21 @@ static void resid_restrict_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>
     }
     const dim3 g = grid3(c.n0, c.n1, c.n2);
     const int code = f.crs[0] | (f.crs[1] << 1) | (f.crs[2] << 2);
+    if (f.layered) {
+#define RL(cx, cy, cz) hipLaunchKernelGGL((k_mg_resid_restrict_lay<T, cx, cy, cz>), g, dim3(256), 0, s, f, c, bc, inh)
+        switch (code) {
+        case 0: RL(0, 0, 0); break;
+        case 1: RL(1, 0, 0); break;
+        case 2: RL(0, 1, 0); break;
+        case 3: RL(1, 1, 0); break;
+        case 4: RL(0, 0, 1); break;
+        case 5: RL(1, 0, 1); break;
+        case 6: RL(0, 1, 1); break;
+        default: RL(1, 1, 1); break;
+        }
+#undef RL
+        return;
+    }
 #define RR(cx, cy, cz) hipLaunchKernelGGL((k_mg_resid_restrict<T, cx, cy, cz>), g, dim3(256), 0, s, f, c, bc, inh)
     switch (code) {
     case 0: RR(0, 0, 0); break;
@@ -1567,12 +2049,115 @@ void mg_resid_restrict(hipStream_t s, const DLev& f, const DLev& c, const DBC& b
 void mg_resid_restrict(hipStream_t s, const DLevF& f, const DLevF& c, const DBC& bc) {
     resid_restrict_impl<float>(s, f, c, bc, 0);
 }
-void mg_prolong(hipStream_t s, const DLev& f, const DLev& c, const DBC& bc) {
-    hipLaunchKernelGGL(k_mg_prolong<double>, grid3(f.n0, f.n1, f.n2), dim3(256), 0, s, f, c, bc);
+/* Block shape of one leg of level l (coarser level c): among 16x16, 16x8,
+ * 8x8, 8x4 and 4x4 blocks that tile the level and whose LDS fits (down:
+ * phi with halo 2nu+1 and the rhs [and Helmholtz term] on rings up to 2nu;
+ * up: phi with halo 2nu, the coarse parents and the rhs on rings up to
+ * 2nu-1), the one giving the most workgroups without exceeding the compute
+ * units (these levels are latency bound: smaller blocks mean fewer cells per
+ * workgroup and phase), else the fewest workgroups. */
+template <class T>
+static bool block_shape(const DLevT<T>& l, const DLevT<T>& c, const DBC& bc, int nu, bool down, int* bx, int* by,
+                        size_t* lds) {
+    if (!l.layered || l.read_a == 1 || bc.lo[2] == BC_PER || l.n2 < 1 || l.n2 > BLK_MAXZ || nu < 1) return false;
+    const size_t lmax = tail_lds_max();
+    const size_t cap = lmax > 8192 ? lmax - 8192 : 0;
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            ncu = 256;
+    }
+    static const int cand[5][2] = {{16, 16}, {16, 8}, {8, 8}, {8, 4}, {4, 4}};
+    const int ncoef = l.read_a ? 2 : 1;
+    int best = -1;
+    long long best_n = 0;
+    size_t best_lds = 0;
+    for (int q = 0; q < 5; ++q) {
+        const int X = cand[q][0], Y = cand[q][1];
+        if (l.n0 % X || l.n1 % Y) continue;
+        const int H = down ? 2 * nu + 1 : 2 * nu;
+        size_t v = (size_t)(X + 2 * H) * (Y + 2 * H) * l.n2 + (size_t)ncoef * (X + 2 * H - 2) * (Y + 2 * H - 2) * l.n2;
+        if (!down) {
+            const int HC = H / 2 + 1;
+            const size_t clx = l.crs[0] ? X / 2 + 2 * HC : X + 2 * H, cly = l.crs[1] ? Y / 2 + 2 * HC : Y + 2 * H;
+            v += clx * cly * c.n2;
+        }
+        if (v * sizeof(T) > cap) continue;
+        const long long n = (long long)(l.n0 / X) * (l.n1 / Y);
+        const bool better = best < 0 || (n <= ncu && (best_n > ncu || n > best_n)) || (n > ncu && best_n > ncu && n < best_n);
+        if (better) { best = q; best_n = n; best_lds = v * sizeof(T); }
+    }
+    if (best < 0) return false;
+    *bx = cand[best][0]; *by = cand[best][1]; *lds = best_lds;
+    return true;
 }
-void mg_prolong(hipStream_t s, const DLevF& f, const DLevF& c, const DBC& bc) {
-    hipLaunchKernelGGL(k_mg_prolong<float>, grid3(f.n0, f.n1, f.n2), dim3(256), 0, s, f, c, bc);
+template <class T, int CX, int CY, int CZ>
+static void block_launch(hipStream_t s, const BlkArgs<T>& A, bool down, size_t lds) {
+    const int nblk = (A.l.n0 / A.bx) * (A.l.n1 / A.by);
+    if (down) {
+        static size_t set = 0;
+        if (lds > set) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_block_down<T, CX, CY, CZ>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            set = lds;
+        }
+        hipLaunchKernelGGL((k_mg_block_down<T, CX, CY, CZ>), dim3(nblk), dim3(BLK_THREADS), lds, s, A);
+    } else {
+        static size_t set = 0;
+        if (lds > set) {
+            (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_block_up<T, CX, CY, CZ>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            set = lds;
+        }
+        hipLaunchKernelGGL((k_mg_block_up<T, CX, CY, CZ>), dim3(nblk), dim3(BLK_THREADS), lds, s, A);
+    }
 }
+template <class T>
+static bool block_impl(hipStream_t s, const DLevT<T>& l, const DLevT<T>& c, const DBC& bc, int nu1, int nu2, int leg) {
+    BlkArgs<T> A;
+    size_t lds;
+    if (leg < 0) {
+        int bx, by;
+        return block_shape(l, c, bc, nu1, true, &bx, &by, &lds) && block_shape(l, c, bc, nu2, false, &bx, &by, &lds);
+    }
+    const bool down = leg == 0;
+    A.nu = down ? nu1 : nu2;
+    if (!block_shape(l, c, bc, A.nu, down, &A.bx, &A.by, &lds)) return false;
+    A.l = l; A.c = c; A.bc = bc;
+    const int code = l.crs[0] | (l.crs[1] << 1) | (l.crs[2] << 2);
+    switch (code) {
+    case 0: block_launch<T, 0, 0, 0>(s, A, down, lds); break;
+    case 1: block_launch<T, 1, 0, 0>(s, A, down, lds); break;
+    case 2: block_launch<T, 0, 1, 0>(s, A, down, lds); break;
+    case 3: block_launch<T, 1, 1, 0>(s, A, down, lds); break;
+    case 4: block_launch<T, 0, 0, 1>(s, A, down, lds); break;
+    case 5: block_launch<T, 1, 0, 1>(s, A, down, lds); break;
+    case 6: block_launch<T, 0, 1, 1>(s, A, down, lds); break;
+    default: block_launch<T, 1, 1, 1>(s, A, down, lds); break;
+    }
+    return true;
+}
+bool mg_block_ok(const DLev& l, const DLev& c, const DBC& bc, int nu1, int nu2) {
+    return block_impl<double>(nullptr, l, c, bc, nu1, nu2, -1);
+}
+bool mg_block_ok(const DLevF& l, const DLevF& c, const DBC& bc, int nu1, int nu2) {
+    return block_impl<float>(nullptr, l, c, bc, nu1, nu2, -1);
+}
+void mg_block_down(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, int nu1, int nu2) {
+    block_impl<double>(s, l, c, bc, nu1, nu2, 0);
+}
+void mg_block_down(hipStream_t s, const DLevF& l, const DLevF& c, const DBC& bc, int nu1, int nu2) {
+    block_impl<float>(s, l, c, bc, nu1, nu2, 0);
+}
+void mg_block_up(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, int nu1, int nu2) {
+    block_impl<double>(s, l, c, bc, nu1, nu2, 1);
+}
+void mg_block_up(hipStream_t s, const DLevF& l, const DLevF& c, const DBC& bc, int nu1, int nu2) {
+    block_impl<float>(s, l, c, bc, nu1, nu2, 1);
+}
+void mg_prolong(hipStream_t s, const DLev& f, const DLev& c, const DBC& bc) { prolong_impl<double>(s, f, c, bc); }
+void mg_prolong(hipStream_t s, const DLevF& f, const DLevF& c, const DBC& bc) { prolong_impl<float>(s, f, c, bc); }
 void mg_restrict_cells(hipStream_t s, const DLev& f, const DLev& c, const double* src, double* dst) {
     hipLaunchKernelGGL(k_mg_restrict_cells<double>, grid3(c.n0, c.n1, c.n2), dim3(256), 0, s, f, c, src, dst);
 }

@@ -133,6 +133,7 @@ typedef struct fx_inputs {
     double bias_ramp_end;
     int    verbose;
     int    mg_fused_sweep;        /* 1: one-pass streaming red+black sweep on levels of >= 2M cells (default 1); 0: two launches per sweep */
+    int    mg_block_levels;       /* 1: small layered levels run their down and up legs as one launch each, column blocks in LDS (default 1) */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;

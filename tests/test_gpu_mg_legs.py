@@ -1,0 +1,83 @@
+"""The one-launch legs of the small multigrid levels against the per-level
+kernels, V-cycle by V-cycle.
+
+With mg_block_levels = 1 every small layered level below the finest runs its
+down leg (sweeps, residual, restriction) and its up leg (prolongation,
+sweeps) as one launch each, column blocks held in LDS with redundant halos;
+with 0 the same level runs separate colour, restriction and prolongation
+launches.  Both must give the same phi after each of the first V-cycles, to
+rounding (the interior update multiplies by a per-plane 1/D where the
+per-colour kernel divides): bar 1e-12 of max|phi|.  The cases cover the
+coarsening patterns (1,1,1) and (1,1,0), periodic and non-periodic lateral
+faces, and a Helmholtz term.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import ferrox_amd as fx
+from ferrox_amd import configs
+from ferrox_amd._abi import FxSolveStats
+
+pytestmark = pytest.mark.gpu
+
+FX_ENOCONV_MG = 2
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def cycles(text, ncyc, block, rhs, acoef=None, **kw):
+    """phi after exactly ncyc V-cycles from phi = 0 (a zero tolerance stops
+    the solve at mg_max_iter)."""
+    inp = fx.from_text(text, mg_tol_rel=0.0, mg_max_iter=ncyc, mg_block_levels=block, **kw)
+    sim = fx.FerroX(inp)
+    b = np.ascontiguousarray(rhs, dtype=np.float64)
+    a = None if acoef is None else np.ascontiguousarray(acoef, dtype=np.float64)
+    phi = np.zeros(sim.shape)
+    st = FxSolveStats()
+    rc = sim._lib.fx_poisson_solve(sim._h, _dp(b), None if a is None else _dp(a), _dp(phi), C.byref(st))
+    assert rc == FX_ENOCONV_MG and st.vcycles == ncyc, (rc, st.vcycles)
+    levels = sim.levels()
+    sim.close()
+    return phi, levels
+
+
+def nonperiodic_mfim():
+    text = configs.mfim(nx=64, ny=48, nz=36, lx=32e-9, ly=24e-9)
+    text = text.replace("domain.is_periodic = 1 1 0", "domain.is_periodic = 0 0 0")
+    text = text.replace("boundary.lo = per per dir(0.0)", "boundary.lo = neu dir(0.0) dir(0.0)")
+    return text.replace("boundary.hi = per per dir(0.0)", "boundary.hi = dir(0.0) neu dir(0.0)")
+
+
+CASES = {
+    "periodic_111_110": lambda: configs.mfim_64(nsteps=1),
+    "nonperiodic": nonperiodic_mfim,
+}
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+@pytest.mark.parametrize("helmholtz", [False, True])
+def test_block_legs_match_per_level_kernels(case, helmholtz):
+    text = CASES[case]()
+    probe = fx.FerroX(fx.from_text(text))
+    shape = probe.shape
+    probe.close()
+    rng = np.random.default_rng(17)
+    rhs = rng.standard_normal(shape) * 1e6
+    acoef = None
+    if helmholtz:
+        # negative, as the Newton term d(rho)/d(phi) of the semiconductor
+        # cells is: -a adds to the diagonal
+        acoef = np.zeros(shape)
+        acoef[: shape[0] // 3] = -1e2 * (1.0 + rng.random((shape[0] // 3,) + shape[1:]))
+        acoef *= 8.854e-12 / (0.5e-9) ** 2
+    for ncyc in (1, 3):
+        p1, lv = cycles(text, ncyc, 1, rhs, acoef)
+        p0, _ = cycles(text, ncyc, 0, rhs, acoef)
+        scale = np.abs(p0).max()
+        err = np.abs(p1 - p0).max() / scale
+        print(case, helmholtz, ncyc, lv, err)
+        assert scale > 0 and err < 1e-12, (ncyc, err)
