@@ -581,8 +581,34 @@ int build_coefficients(fx_ctx* x) {
     return FX_OK;
 }
 
+/* The device copy of the material ids carries bit 3 (MAT_INTERIOR_FE) on FE
+ * cells whose polarization stencils and grad(phi) are all the centred ones:
+ * the 18 face and edge neighbours are FE cells inside the domain or across a
+ * periodic face, and k is neither the first nor the last plane (where the
+ * z-derivative of phi is one-sided).  The host shadow keeps bits 0-2. */
 int upload_mat(fx_ctx* x) {
-    HIPCHK(x, hipMemcpyAsync(x->mat, x->h_mat.data(), x->h_mat.size(), hipMemcpyHostToDevice, x->s));
+    std::vector<uint8_t> dm(x->h_mat);
+    const int nx = x->n[0], ny = x->n[1], nz = x->n[2];
+    const bool px = x->in.is_periodic[0], py = x->in.is_periodic[1];
+    for (int k = 1; k + 1 < nz; ++k)
+        for (int j = 0; j < ny; ++j) {
+            if (!py && (j < 1 || j > ny - 2)) continue;
+            for (int i = 0; i < nx; ++i) {
+                if (!px && (i < 1 || i > nx - 2)) continue;
+                const size_t g = gidx(x, i, j, k);
+                if (x->h_mat[g] & 3) continue;
+                bool ok = true;
+                for (int dk = -1; dk <= 1 && ok; ++dk)
+                    for (int dj = -1; dj <= 1 && ok; ++dj)
+                        for (int di = -1; di <= 1 && ok; ++di) {
+                            const int nn = (di != 0) + (dj != 0) + (dk != 0);
+                            if (nn == 0 || nn == 3) continue;
+                            ok = (x->h_mat[gidx(x, i + di, j + dj, k + dk)] & 3) == 0;
+                        }
+                if (ok) dm[g] |= fxd::MAT_INTERIOR_FE;
+            }
+        }
+    HIPCHK(x, hipMemcpyAsync(x->mat, dm.data(), dm.size(), hipMemcpyHostToDevice, x->s));
     x->has_sc = false;
     for (int k = 0; k < x->n[2]; ++k)
         for (int j = 0; j < x->n[1]; ++j)

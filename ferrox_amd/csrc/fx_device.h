@@ -11,7 +11,8 @@
  * The material id is one byte per cell on a box grown by one cell, so the
  * stencils can test the neighbour's material exactly as the reference tests
  * MaterialMask's ghost layer (Initialization.cpp:177-205): bits 0-1 hold the
- * mask code (FE 0, DE 1, SC 2, Channel 3), bit 2 the t-phase flag.
+ * mask code (FE 0, DE 1, SC 2, Channel 3), bit 2 the t-phase flag, and in the
+ * device copy bit 3 marks FE cells whose stencils are all centred.
  */
 #pragma once
 #include <hip/hip_runtime.h>
@@ -43,6 +44,10 @@ struct PView {
     }
 };
 
+/* Material id bit 3 (device copy only): an FE cell whose P stencils and
+ * grad(phi) are all centred (see upload_mat). */
+constexpr int MAT_INTERIOR_FE = 8;
+
 /* Material id on the grown box. */
 struct MView {
     const uint8_t* __restrict__ m;
@@ -54,6 +59,20 @@ struct MView {
 struct PBC {
     int lo[3], hi[3];
     double lambda;
+};
+
+/* Neighbour offsets of an interior FE cell (periodic wrap laterally; such
+ * cells never sit on a non-periodic face or the first or last plane). */
+struct NbrOff {
+    int xm, xp, ym, yp, zm, zp;
+    __device__ __forceinline__ NbrOff(int i, int j, int nx, int ny) {
+        xm = i > 0 ? -1 : nx - 1;
+        xp = i < nx - 1 ? 1 : -(nx - 1);
+        ym = j > 0 ? -nx : nx * (ny - 1);
+        yp = j < ny - 1 ? nx : -nx * (ny - 1);
+        zp = nx * ny;
+        zm = -zp;
+    }
 };
 
 /* DPDx/DPDy/DPDz, DerivativeAlgorithm.H:45-252.  Flags outside 0..4 are

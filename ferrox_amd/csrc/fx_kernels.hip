@@ -128,7 +128,14 @@ __global__ __launch_bounds__(256) void k_rhs_newton(RhsArgs a) {
         const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
         const double* dx = a.g.dx;
         double x = 0.0;
-        if (a.pxy_zero) {
+        if (a.pxy_zero && (a.M.raw(i, j, k) & MAT_INTERIOR_FE)) {   /* centred DPDx/y/z */
+            const NbrOff o(i, j, nx, ny);
+            const double* P = a.P[2];
+            if (R[6] != 0.0) x += R[6] * ((P[c + o.xp] - P[c + o.xm]) / (2. * dx[0]));
+            if (R[7] != 0.0) x += R[7] * ((P[c + o.yp] - P[c + o.ym]) / (2. * dx[1]));
+            x += R[8] * ((P[c + o.zp] - P[c + o.zm]) / (2. * dx[2]));
+            rhs = -x;
+        } else if (a.pxy_zero) {
             if (R[6] != 0.0) x += R[6] * dpd<0>(Fr, a.M, i, j, k, dx, a.pbc);
             if (R[7] != 0.0) x += R[7] * dpd<1>(Fr, a.M, i, j, k, dx, a.pbc);
             x += R[8] * dpd<2>(Fr, a.M, i, j, k, dx, a.pbc);
@@ -182,22 +189,53 @@ __global__ __launch_bounds__(256) void k_rho_update(RhoArgs a) {
 }
 
 /* ------------------------------ TDGL right-hand side ---------------------- */
-/* dF_grad/dP_r with the rotated tensor, TotalEnergyDensity.cpp:209-231. */
-__device__ __forceinline__ double grad_r(const PView& Fr, const MView& M, int i, int j, int k, const double* dx,
-                                         const PBC& bc, const double R[9], double g11, double g44m) {
+/* dF_grad/dP_r with the rotated tensor, TotalEnergyDensity.cpp:209-231, from
+ * the Hessian of P_r; the mixed entries are needed only where R couples them. */
+__device__ __forceinline__ void mixed_needed(const double R[9], bool& nxy, bool& nyz, bool& nxz) {
     const double R11 = R[0], R12 = R[1], R13 = R[2], R21 = R[3], R22 = R[4], R23 = R[5], R31 = R[6], R32 = R[7], R33 = R[8];
-    const double Hxx = ddpd<0>(Fr, M, i, j, k, dx, bc);
-    const double Hyy = ddpd<1>(Fr, M, i, j, k, dx, bc);
-    const double Hzz = ddpd<2>(Fr, M, i, j, k, dx, bc);
-    const bool nxy = (R31 * R32 != 0.0) || (R11 * R12 != 0.0) || (R21 * R22 != 0.0);
-    const bool nyz = (R32 * R33 != 0.0) || (R12 * R13 != 0.0) || (R22 * R23 != 0.0);
-    const bool nxz = (R33 * R31 != 0.0) || (R13 * R11 != 0.0) || (R23 * R21 != 0.0);
-    const double Hxy = nxy ? ddpd_mixed<0, 1>(Fr, M, i, j, k, dx, bc) : 0.0;
-    const double Hyz = nyz ? ddpd_mixed<1, 2>(Fr, M, i, j, k, dx, bc) : 0.0;
-    const double Hxz = nxz ? ddpd_mixed<0, 2>(Fr, M, i, j, k, dx, bc) : 0.0;
+    nxy = (R31 * R32 != 0.0) || (R11 * R12 != 0.0) || (R21 * R22 != 0.0);
+    nyz = (R32 * R33 != 0.0) || (R12 * R13 != 0.0) || (R22 * R23 != 0.0);
+    nxz = (R33 * R31 != 0.0) || (R13 * R11 != 0.0) || (R23 * R21 != 0.0);
+}
+__device__ __forceinline__ double grad_r_of(const double R[9], double Hxx, double Hyy, double Hzz, double Hxy,
+                                            double Hyz, double Hxz, double g11, double g44m) {
+    const double R11 = R[0], R12 = R[1], R13 = R[2], R21 = R[3], R22 = R[4], R23 = R[5], R31 = R[6], R32 = R[7], R33 = R[8];
     return -g11 * (R31 * R31 * Hxx + R32 * R32 * Hyy + R33 * R33 * Hzz + 2. * R31 * R32 * Hxy + 2. * R32 * R33 * Hyz + 2. * R33 * R31 * Hxz)
            - g44m * (R11 * R11 * Hxx + R12 * R12 * Hyy + R13 * R13 * Hzz + 2. * R11 * R12 * Hxy + 2. * R12 * R13 * Hyz + 2. * R13 * R11 * Hxz)
            - g44m * (R21 * R21 * Hxx + R22 * R22 * Hyy + R23 * R23 * Hzz + 2. * R21 * R22 * Hxy + 2. * R22 * R23 * Hyz + 2. * R23 * R21 * Hxz);
+}
+__device__ __forceinline__ double grad_r(const PView& Fr, const MView& M, int i, int j, int k, const double* dx,
+                                         const PBC& bc, const double R[9], double g11, double g44m) {
+    const double Hxx = ddpd<0>(Fr, M, i, j, k, dx, bc);
+    const double Hyy = ddpd<1>(Fr, M, i, j, k, dx, bc);
+    const double Hzz = ddpd<2>(Fr, M, i, j, k, dx, bc);
+    bool nxy, nyz, nxz;
+    mixed_needed(R, nxy, nyz, nxz);
+    const double Hxy = nxy ? ddpd_mixed<0, 1>(Fr, M, i, j, k, dx, bc) : 0.0;
+    const double Hyz = nyz ? ddpd_mixed<1, 2>(Fr, M, i, j, k, dx, bc) : 0.0;
+    const double Hxz = nxz ? ddpd_mixed<0, 2>(Fr, M, i, j, k, dx, bc) : 0.0;
+    return grad_r_of(R, Hxx, Hyy, Hzz, Hxy, Hyz, Hxz, g11, g44m);
+}
+
+/* The same Hessian for an interior FE cell: every masked stencil is the
+ * centred one (DoubleDPDx/y/z and DoubleDPDxDy/DxDz/DyDz interior branches,
+ * with their operation order), so the neighbours are read directly. */
+__device__ __forceinline__ double grad_r_interior(const double* __restrict__ P, int c, const NbrOff& o,
+                                                  const double* dx, const double R[9], double g11, double g44m) {
+    const double pc = P[c];
+    const double h0 = dx[0], h1 = dx[1], h2 = dx[2];
+    const double Hxx = (P[c + o.xp] - 2. * pc + P[c + o.xm]) / (h0 * h0);
+    const double Hyy = (P[c + o.yp] - 2. * pc + P[c + o.ym]) / (h1 * h1);
+    const double Hzz = (P[c + o.zp] - 2. * pc + P[c + o.zm]) / (h2 * h2);
+    bool nxy, nyz, nxz;
+    mixed_needed(R, nxy, nyz, nxz);
+    const double Hxy = nxy ? ((P[c + o.xp + o.yp] - P[c + o.xp + o.ym]) / (2. * h1) -
+                              (P[c + o.xm + o.yp] - P[c + o.xm + o.ym]) / (2. * h1)) / 2. / h0 : 0.0;
+    const double Hyz = nyz ? ((P[c + o.yp + o.zp] - P[c + o.yp + o.zm]) / (2. * h2) -
+                              (P[c + o.ym + o.zp] - P[c + o.ym + o.zm]) / (2. * h2)) / 2. / h1 : 0.0;
+    const double Hxz = nxz ? ((P[c + o.xp + o.zp] - P[c + o.xp + o.zm]) / (2. * h2) -
+                              (P[c + o.xm + o.zp] - P[c + o.xm + o.zm]) / (2. * h2)) / 2. / h0 : 0.0;
+    return grad_r_of(R, Hxx, Hyy, Hzz, Hxy, Hyz, Hxz, g11, g44m);
 }
 
 /* SCALAR_FAST: is_polarization_scalar with P_p = P_q = 0 everywhere (only P_r
@@ -227,11 +265,20 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
     const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
     double GL[3] = {0.0, 0.0, 0.0};
 
+    const bool inner = SCALAR_FAST && (raw & MAT_INTERIOR_FE);
+    const NbrOff o(i, j, nx, ny);
     /* electric field E = -R grad(phi) */
     double E[3] = {0.0, 0.0, 0.0};
     if (a.incE == 1) {
         double fx, fy, fz;
-        grad_phi(a.phv, a.g, i, j, k, fx, fy, fz);
+        if (inner) {   /* centred in every direction (DFDx, DFDy, DphiDz interior) */
+            const double* ph = a.phv.p;
+            fx = (ph[c + o.xp] - ph[c + o.xm]) / (2. * dx[0]);
+            fy = (ph[c + o.yp] - ph[c + o.ym]) / (2. * dx[1]);
+            fz = (ph[c + o.zp] - ph[c + o.zm]) / (2. * dx[2]);
+        } else {
+            grad_phi(a.phv, a.g, i, j, k, fx, fy, fz);
+        }
         if (SCALAR_FAST) {
             double s = 0.0;
             if (R[6] != 0.0) s += R[6] * fx;
@@ -254,7 +301,9 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
             const double dF = alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
             L = -1.0 * Gam * dF;
         }
-        if (a.incG == 1) G = -1.0 * Gam * grad_r(Fr, a.M, i, j, k, dx, a.pbc, R, a.g11, a.g44 - a.g44_p);
+        if (a.incG == 1)
+            G = -1.0 * Gam * (inner ? grad_r_interior(a.P[2], c, o, dx, R, a.g11, a.g44 - a.g44_p)
+                                    : grad_r(Fr, a.M, i, j, k, dx, a.pbc, R, a.g11, a.g44 - a.g44_p));
         if (a.incE == 1) El = -1.0 * Gam * (-E[2]);
         if (tph) { L = 0.0; G = 0.0; El = 0.0; }
         double g = 0.0;
