@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--diff-large", action="store_true", help="also time the adjoint at 128x128x64")
-    ap.add_argument("--fused", type=int, default=1, help="one-pass streaming red+black sweep on the large levels (1) or per-colour kernels everywhere (0)")
+    ap.add_argument("--fused", type=int, default=1, help="streamed levels: one red+black sweep per pass (1), two sweeps per pass (2), or per-colour kernels everywhere (0)")
     return ap.parse_args()
 
 

@@ -37,7 +37,6 @@ constexpr double OUTER_TOL = 1.e-5;  /* ElectrostaticSolver.cpp:697 */
 constexpr double BOTTOM_TOL = 1.e-6;
 constexpr int BOTTOM_MAXIT = 400;
 constexpr size_t TAIL_MAX_CELLS = 8192;   /* levels this small run inside the single-workgroup tail kernel */
-constexpr size_t BLOCK_MAX_CELLS = 1u << 21;   /* below this, a level's legs run as one block launch each */
 
 inline uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ULL;
@@ -261,6 +260,7 @@ int build_levels(fx_ctx* x) {
         if ((rc = dalloc(x, (void**)&d.bz, sizeof(double) * (size_t)d.n0 * d.n1 * (d.n2 + 1)))) return rc;
         d.mat = nullptr; d.msx = 0; d.msy = 0; d.matcoef = 0; d.read_a = 0;
         d.layered = 0; d.bxk = d.byk = d.bzk = nullptr; d.phi2 = nullptr;
+        d.stream = L.N >= (size_t)std::max(0, x->in.mg_stream_min_cells) ? 1 : 0;
         for (int q = 0; q < 8; ++q) d.eps[q] = 0.0;
         if (l == 0) {
             d.phi = x->phi; d.b = x->rhs; d.a = x->alpha_cc;
@@ -307,6 +307,7 @@ int build_levels(fx_ctx* x) {
             DLevF& f = x->lev32[l];
             std::memset(&f, 0, sizeof(f));
             f.n0 = d.n0; f.n1 = d.n1; f.n2 = d.n2;
+            f.stream = d.stream;
             for (int q = 0; q < 3; ++q) { f.ih2[q] = (float)d.ih2[q]; f.crs[q] = d.crs[q]; }
             const size_t Nl = x->lev[l].N;
             if ((rc = dalloc(x, (void**)&f.phi, sizeof(float) * Nl))) return rc;
@@ -748,7 +749,7 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
         return;
     }
     Level& C = x->lev[li + 1];
-    if (li >= 1 && x->in.mg_block_levels && L.N < BLOCK_MAX_CELLS &&
+    if (li >= 1 && x->in.mg_block_levels && !(L.d.stream && x->in.mg_fused_sweep && fxk::fused_ok(L.d, bc)) &&
         fxk::mg_block_ok(L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2)) {
         fxk::mg_block_down(x->s, L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2);
         vcycle(x, li + 1, bc);
@@ -793,7 +794,7 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
         return;
     }
     DLevF& C = x->lev32[li + 1];
-    if (li >= 1 && x->in.mg_block_levels && x->lev[li].N < BLOCK_MAX_CELLS &&
+    if (li >= 1 && x->in.mg_block_levels && !(L.stream && x->in.mg_fused_sweep && fxk::fused_ok32(L, bc)) &&
         fxk::mg_block_ok(L, C, bc, x->in.mg_nu1, x->in.mg_nu2)) {
         fxk::mg_block_down(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2);
         vcycle32(x, li + 1, bc, true);
@@ -1675,6 +1676,9 @@ int fx_bench_kernel(fx_ctx* x, const char* which, int reps, double* avg_ms) {
         else if (w == "resid0") {
             if (x->lev.size() < 2) return false;
             fxk::mg_resid_restrict(x->s, x->lev[0].d, x->lev[1].d, bc, 1);
+        } else if (w == "smooth0x2") {   /* two sweeps in one streaming pass */
+            if (!(x->in.mg_fused_sweep >= 2 && fxk::fused_ok(x->lev[0].d, bc))) return false;
+            fxk::mg_sweep(x->s, x->lev[0].d, bc, 1, 2, 2);
         } else if (w == "prolong0") {   /* prolongation from level 1 folded into one sweep of level 0 */
             if (x->lev.size() < 2) return false;
             fxk::mg_prolong_sweep(x->s, x->lev[0].d, x->lev[1].d, bc, 1, 1, x->in.mg_fused_sweep);

@@ -252,6 +252,7 @@ struct DLevT {
     const T* byk;
     const T* bzk;
     T* phi2;   /* second solution buffer of the one-pass sweep (null if not allocated) */
+    int stream;   /* large enough for the streaming kernels (the mg_stream_min_cells input) */
 };
 using DLev = DLevT<double>;
 using DLevF = DLevT<float>;

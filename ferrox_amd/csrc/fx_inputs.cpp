@@ -492,7 +492,7 @@ struct Applier {
         /* solver controls added by this build */
         D1(mg_tol_rel) I1(mg_max_iter) I1(mg_nu1) I1(mg_nu2) I1(mg_warm_start) I1(outer_max_iter)
         I1(phi_ghost_mode) I1(precision) I1(bias_ramp_steps) D1(bias_ramp_start) D1(bias_ramp_end) I1(verbose)
-        I1(mg_fused_sweep) I1(mg_block_levels)
+        I1(mg_fused_sweep) I1(mg_block_levels) I1(mg_stream_min_cells)
 #undef D1
 #undef D3
 #undef I1
@@ -581,6 +581,7 @@ int fx_inputs_default(fx_inputs* in) {
     in->verbose = 0;
     in->mg_fused_sweep = 1;
     in->mg_block_levels = 1;
+    in->mg_stream_min_cells = 1 << 21;
     return FX_OK;
 }
 

@@ -100,6 +100,7 @@ void mg_prolong_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, con
 void mg_prolong_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nsweeps,
                       int allow_fused);
 bool fused_ok(const fxd::DLev& l, const fxd::DBC& bc);
+bool fused_ok32(const fxd::DLevF& l, const fxd::DBC& bc);
 void mg_resid_restrict(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc, int inh);
 void mg_prolong(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc);
 void mg_restrict_cells(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const double* src, double* dst);

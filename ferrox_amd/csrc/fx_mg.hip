@@ -61,13 +61,15 @@ __global__ __launch_bounds__(256) void k_mg_smooth(DLevT<T> l, DBC bc, int inh, 
 
 /* ------------------- one-pass red + black sweep (streaming) --------------- */
 namespace {
-constexpr int STX = 64, STY = 8;                  /* output tile of columns */
+constexpr int STX = 64, STY = 16;                 /* output tile of columns */
+constexpr int STHR = STX * STY / 2;               /* threads: one black cell each */
 constexpr int SHX = STX + 4, SHY = STY + 4;       /* with the 2-cell in-plane halo */
 constexpr int SPL = SHX * SHY;                    /* LDS values per plane */
 constexpr int SNS = 5;                            /* plane slots in LDS */
-constexpr int SLD = (SPL + 255) / 256;            /* plane loads per thread */
+constexpr int SLD = (SPL + STHR - 1) / STHR;      /* plane loads per thread */
 constexpr int SCZ_MAX = 64;                       /* largest z-chunk */
 constexpr int NRED = (STY + 2) * (STX / 2 + 1);   /* red cells of the tile and its ring */
+static_assert(NRED <= 2 * STHR, "two red cells per thread");
 constexpr int NPW = 6;                            /* per-plane table: wx wy wzl wzh D 1/D */
 }
 
@@ -143,7 +145,7 @@ struct StreamCell {
  * in-plane halos are shared in that XCD's L2.  Requires: z not periodic, n0
  * even, n0 >= 64, n1 >= 8, periodic extents even, cz even. */
 template <class T, int AMODE, int PROLONG>
-__global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
+__global__ __launch_bounds__(STHR) void k_mg_gsrb_stream(StreamArgs<T> A) {
     __shared__ T sp[SNS * SPL];
     __shared__ T scw[NPW * (SCZ_MAX + 2)];
     const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
@@ -158,7 +160,7 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
 
     /* per-plane weights beta/h^2 (the products of mg_coef's layered branch),
      * the interior diagonal accumulated in mg_acc's order, and its inverse */
-    for (int e = tid; e < z1 - z0 + 2; e += 256) {
+    for (int e = tid; e < z1 - z0 + 2; e += STHR) {
         const int k = z0 - 1 + e;
         T wx = T(0), wy = T(0), wzl = T(0), wzh = T(0);
         if (k >= 0 && k < n2) {
@@ -174,11 +176,11 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
         t[5] = D != T(0) ? T(1) / D : T(0);
     }
 
-    /* plane loads: LDS index tid + 256 s */
+    /* plane loads: LDS index tid + STHR s */
     int goff[SLD];
 #pragma unroll
     for (int s = 0; s < SLD; ++s) {
-        const int q = tid + 256 * s;
+        const int q = tid + STHR * s;
         goff[s] = -1;
         if (q < SPL) {
             int i = tx0 + q % SHX - 2, j = ty0 + q / SHX - 2;
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
         c.lat = (perx || (i > 0 && i < n0 - 1)) && (pery || (j > 0 && j < n1 - 1));
         return c;
     };
-    /* red cells u = 0, 1 (positions tid and tid + 256 of the 10 rows x 33 red
+    /* red cells u = 0, 1 (positions tid and tid + STHR of the rows x 33 red
      * cells of the tile and its ring) for planes of parity 0 and 1; the black
      * cell (one of the pair 2*x2, 2*x2+1 of tile row tid/32) for planes of
      * parity 0 and 1 */
@@ -215,7 +217,7 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
     for (int par = 0; par < 2; ++par) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
-            const int qn = tid + 256 * u;
+            const int qn = tid + STHR * u;
             const bool valid = qn < NRED;
             const int qq = valid ? qn : 0;
             const int hy = 1 + qq / (STX / 2 + 1), x2 = qq % (STX / 2 + 1);
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
         }
 #pragma unroll
         for (int s = 0; s < SLD; ++s) {
-            const int q = tid + 256 * s;
+            const int q = tid + STHR * s;
             T v = R.phi[s];
             if (PROLONG) {
                 T e = T(0);
@@ -384,6 +386,274 @@ __global__ __launch_bounds__(256) void k_mg_gsrb_stream(StreamArgs<T> A) {
     for (int p = z0 - 1; p <= z1; p += 2) {    /* z0 even: p odd, p + 1 even */
         step(p, R0, R1, P1{});
         if (p + 1 <= z1) step(p + 1, R1, R0, P0{});
+    }
+    if (pend) *reinterpret_cast<T2*>(pptr) = pv;
+}
+
+/* -------------- two red-black sweeps in one streaming pass ---------------- */
+namespace {
+constexpr int W2X = 64, W2Y = 8, W2H = 4;                  /* output tile; halo of two sweeps */
+constexpr int W2SX = W2X + 2 * W2H, W2SY = W2Y + 2 * W2H;  /* 72 x 16 */
+constexpr int W2P = W2SX * W2SY;                           /* LDS values per plane */
+constexpr int W2LD = (W2P + 255) / 256;                    /* plane loads per thread */
+constexpr int W2NS = 7;                                    /* plane slots */
+constexpr int W2C = 7;                                     /* cells per thread per step */
+constexpr int W2ZX = 8;                                    /* planes beyond the chunk in the table */
+}
+/* dynamic LDS bytes of k_mg_gsrb_stream2 */
+template <class T>
+constexpr size_t stream2_lds() { return sizeof(T) * (size_t)(W2NS * W2P + NPW * (SCZ_MAX + W2ZX)); }
+
+namespace {
+template <class T>
+struct Stream2Regs {
+    T phi[W2LD];     /* the plane two steps ahead of the red phase */
+    T cn[W2LD];      /* PROLONG: coarse plane entering the window */
+    T b[W2C];        /* rhs of this thread's cells of the next step */
+    T a[W2C];
+    uint8_t m[W2C];
+    int q[W2C];      /* LDS index of the cell in its plane, -1: none */
+    int ij[W2C];     /* its wrapped coordinates, i | j << 16 */
+};
+}
+
+/* Two full red-black Gauss-Seidel sweeps, phi_in -> phi_out, in one pass.
+ * A workgroup owns a 64x8 column tile and the planes [z0, z1) and holds old
+ * phi with a 4-cell in-plane halo in LDS, one plane per slot.  At step p it
+ * updates red of the first sweep on plane p (tile and a ring of 3), black of
+ * the first sweep on plane p-1 (ring 2), red of the second sweep on plane
+ * p-2 (ring 1) and black of the second sweep on plane p-3 (the tile), and
+ * stores plane p-3.  Each phase reads its neighbours after the phases it
+ * depends on and before the phases that overwrite them, so every value equals
+ * that of four separate colour passes, and the ring values are the ones the
+ * neighbouring workgroups compute; the chunk's z-halo (planes z0-4 .. z0-1
+ * and z1 .. z1+3) is recomputed the same way.  Cell geometry and the loads of
+ * rhs and Helmholtz term are prepared one step ahead, the next phi plane two
+ * steps ahead of its red phase.  Arithmetic per cell as in k_mg_gsrb_stream.
+ * Requires what k_mg_gsrb_stream requires; the chunk cz is even. */
+template <class T, int AMODE, int PROLONG>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void k_mg_gsrb_stream2(StreamArgs<T> A) {
+    extern __shared__ __align__(16) unsigned char s2_lds[];
+    T* const sp = reinterpret_cast<T*>(s2_lds);
+    T* const scw = sp + W2NS * W2P;
+    const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
+    int w = blockIdx.x;
+    if (A.remap) w = (w & 7) * (gridDim.x >> 3) + (w >> 3);
+    const int tx = w % A.ntx, rest = w / A.ntx, ty = rest % A.nty, tz = rest / A.nty;
+    const int tx0 = tx * W2X, ty0 = ty * W2Y, z0 = tz * A.cz, z1 = min(z0 + A.cz, n2);
+    const int tid = threadIdx.x;
+    const bool perx = A.bc.lo[0] == BC_PER, pery = A.bc.lo[1] == BC_PER;
+    const int plane = n0 * n1;
+    const int kt0 = z0 - W2H;                         /* first plane of the table */
+
+    for (int e = tid; e < z1 - z0 + W2ZX; e += 256) {
+        const int k = kt0 + e;
+        T wx = T(0), wy = T(0), wzl = T(0), wzh = T(0);
+        if (k >= 0 && k < n2) {
+            wx = A.bxk[k] * A.ih2[0];
+            wy = A.byk[k] * A.ih2[1];
+            wzl = A.bzk[k] * A.ih2[2];
+            wzh = A.bzk[k + 1] * A.ih2[2];
+        }
+        T D = T(0);
+        D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+        T* t = scw + NPW * e;
+        t[0] = wx; t[1] = wy; t[2] = wzl; t[3] = wzh; t[4] = D;
+        t[5] = D != T(0) ? T(1) / D : T(0);
+    }
+    int goff[W2LD];
+#pragma unroll
+    for (int s = 0; s < W2LD; ++s) {
+        const int q = tid + 256 * s;
+        goff[s] = -1;
+        if (q < W2P) {
+            int i = tx0 + q % W2SX - W2H, j = ty0 + q / W2SX - W2H;
+            bool ok = true;
+            if (i < 0) { if (perx) i += n0; else ok = false; }
+            else if (i >= n0) { if (perx) i -= n0; else ok = false; }
+            if (j < 0) { if (pery) j += n1; else ok = false; }
+            else if (j >= n1) { if (pery) j -= n1; else ok = false; }
+            if (ok) goff[s] = i + n0 * j;
+        }
+    }
+    /* this thread's slots: phases 0..3 (red ring 3, black ring 2, red ring 1,
+     * black ring 0) take cells 0-1, 2-3, 4-5 and 6; slot = row << 8 | x2 */
+    int slot2[W2C];
+#pragma unroll
+    for (int cidx = 0; cidx < W2C; ++cidx) {
+        const int ph = cidx >> 1, u = cidx & 1, r = 3 - ph;
+        const int per_row = W2X / 2 + r, rows = W2Y + 2 * r;
+        const int s = tid + 256 * u;
+        slot2[cidx] = (s < per_row * rows) ? (((W2H - r + s / per_row) << 8) | (s % per_row)) : -1;
+    }
+    const T* ap = AMODE ? A.a : A.b;
+    const uint8_t* mp = AMODE == 1 ? A.mat : reinterpret_cast<const uint8_t*>(A.b);
+    auto slot = [&](int k) { return sp + ((k - kt0) % W2NS) * W2P; };
+    auto coarse_plane = [&](T* dst, int K) {
+        const int kc = plane * min(max(K, 0), A.cn2 - 1);
+#pragma unroll
+        for (int s = 0; s < W2LD; ++s) dst[s] = A.ec[(goff[s] >= 0 ? goff[s] : 0) + kc];
+    };
+    /* geometry and loads of the cells of step pn (phase ph on plane pn - ph) and
+     * the phi plane pn + 1 */
+    auto issue = [&](Stream2Regs<T>& R, int pn, auto par_tag) {
+        constexpr int PPAR = decltype(par_tag)::value;   /* parity of pn */
+        const int kf = min(max(pn + 1, 0), n2 - 1);
+#pragma unroll
+        for (int s = 0; s < W2LD; ++s) R.phi[s] = A.phi_in[(goff[s] >= 0 ? goff[s] : 0) + plane * kf];
+        /* plane pn+1 odd (= 2K+1) brings coarse plane K+1 into the window */
+        if (PROLONG && PPAR == 0) coarse_plane(R.cn, ((pn + 1) >> 1) + 1);
+#pragma unroll
+        for (int cidx = 0; cidx < W2C; ++cidx) {
+            const int ph = cidx >> 1, r = 3 - ph, col = ph & 1;
+            const int k = pn - ph;
+            const int sl = slot2[cidx];
+            int q = -1, ij = 0, c = 0, g = 0;
+            if (sl >= 0 && k >= 0 && k < n2 && k >= z0 - 3 + ph && k <= z1 + 2 - ph) {
+                const int hy = sl >> 8, x2 = sl & 255;
+                int jg = ty0 - W2H + hy;
+                const int hx = W2H - r + 2 * x2 + ((col + r + jg + k) & 1);
+                int ig = tx0 - W2H + hx;
+                bool ok = true;
+                if (ig < 0) { if (perx) ig += n0; else ok = false; }
+                else if (ig >= n0) { if (perx) ig -= n0; else ok = false; }
+                if (jg < 0) { if (pery) jg += n1; else ok = false; }
+                else if (jg >= n1) { if (pery) jg -= n1; else ok = false; }
+                if (ok) {
+                    q = hx + W2SX * hy;
+                    ij = ig | (jg << 16);
+                    c = ig + n0 * jg + plane * k;
+                    g = (ig + 1) + A.msx * (jg + 1) + A.msy * (k + 1);
+                }
+            }
+            R.q[cidx] = q;
+            R.ij[cidx] = ij;
+            R.b[cidx] = A.b[c];
+            if (AMODE) R.a[cidx] = ap[c];
+            if (AMODE == 1) R.m[cidx] = mp[g];
+        }
+    };
+    T cw0[PROLONG ? W2LD : 1], cw1[PROLONG ? W2LD : 1];
+    auto put_plane = [&](const Stream2Regs<T>& R, int k, auto odd_tag) {
+        constexpr int KODD = decltype(odd_tag)::value;
+        T* d = slot(k);
+        const bool kin = !A.zero_init && k >= 0 && k < n2;
+        if (PROLONG && KODD) {
+#pragma unroll
+            for (int s = 0; s < W2LD; ++s) { cw0[s] = cw1[s]; cw1[s] = R.cn[s]; }
+        }
+        T wj = T(0);
+        if (PROLONG) {
+            const int K = k >> 1;
+            const int J = (k & 1) ? K + 1 : K - 1;
+            T sg = T(1);
+            if (J < 0 || J >= A.cn2) sg = ((J < 0 ? A.bc.lo[2] : A.bc.hi[2]) == BC_DIR) ? T(-1) : T(1);
+            wj = T(0.25) * sg;
+        }
+#pragma unroll
+        for (int s = 0; s < W2LD; ++s) {
+            const int q = tid + 256 * s;
+            T v = R.phi[s];
+            if (PROLONG) {
+                T e = T(0);
+                e += T(0.75) * (KODD ? cw0[s] : cw1[s]);
+                e += wj * (KODD ? cw1[s] : cw0[s]);
+                v += e;
+            }
+            if (q < W2P) d[q] = (kin && goff[s] >= 0) ? v : T(0);
+        }
+    };
+    auto a_use = [&](const Stream2Regs<T>& R, int cidx) -> T {
+        if (AMODE == 0) return T(0);
+        if (AMODE == 2) return R.a[cidx];
+        return ((R.m[cidx] & 3) >= 2) ? R.a[cidx] : T(0);
+    };
+    auto relax = [&](const Stream2Regs<T>& R, int cidx, int k) -> T {
+        const int q = R.q[cidx];
+        const T* pl = slot(k - 1);
+        const T* pm = slot(k);
+        const T* pu = slot(k + 1);
+        const T* t = scw + NPW * (k - kt0);
+        const T wx = t[0], wy = t[1], wzl = t[2], wzh = t[3];
+        const T n0v = pm[q - 1], n1v = pm[q + 1], n2v = pm[q - W2SX], n3v = pm[q + W2SX], n4v = pl[q], n5v = pu[q];
+        const int i = R.ij[cidx] & 0xFFFF, j = R.ij[cidx] >> 16;
+        const T bv = R.b[cidx], av = a_use(R, cidx);
+        const bool lat = (perx || (i > 0 && i < n0 - 1)) && (pery || (j > 0 && j < n1 - 1));
+        if (lat && k > 0 && k < n2 - 1) {
+            T S = T(0);
+            S += wx * n0v; S += wx * n1v; S += wy * n2v; S += wy * n3v; S += wzl * n4v; S += wzh * n5v;
+            if (AMODE == 0) return (bv + S) * t[5];
+            T D = -av;
+            D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
+            return (bv + S) / D;
+        }
+        DLevT<T> lv;
+        lv.n0 = n0; lv.n1 = n1; lv.n2 = n2;
+        const T wc[6] = {wx, wx, wy, wy, wzl, wzh};
+        const T nb[6] = {n0v, n1v, n2v, n3v, n4v, n5v};
+        T D, S;
+        mg_acc(lv, A.bc, A.inh != 0, i, j, k, av, wc, nb, D, S);
+        return (bv + S) / D;
+    };
+    /* the output pair of this thread (black of ring 0 with its red neighbour) */
+    const int orow = tid >> 5, ox2 = tid & 31;
+    const int oj = ty0 + orow, oi0 = tx0 + 2 * ox2;
+    const bool ostore = oj < n1 && oi0 < n0;
+    using T2 = typename Vec2<T>::type;
+    T2 pv;
+    T* pptr = nullptr;
+    bool pend = false;
+    auto step = [&](int p, Stream2Regs<T>& cur, Stream2Regs<T>& nxt, auto par_tag) {
+        constexpr int PAR = decltype(par_tag)::value;    /* parity of p */
+        /* cur's loads and the store were issued a step ago (s_waitcnt: vmcnt 0,
+         * expcnt and lgkmcnt unconstrained) */
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        put_plane(cur, p + 1, std::integral_constant<int, 1 - PAR>{});
+        issue(nxt, p + 1, std::integral_constant<int, 1 - PAR>{});
+        if (pend) *reinterpret_cast<T2*>(pptr) = pv;
+        pend = false;
+#pragma unroll
+        for (int ph = 0; ph < 4; ++ph) {
+            __syncthreads();
+            const int k = p - ph;
+#pragma unroll
+            for (int u = 0; u < (ph == 3 ? 1 : 2); ++u) {
+                const int cidx = 2 * ph + u;
+                if (cur.q[cidx] < 0) continue;
+                const T v = relax(cur, cidx, k);
+                if (ph < 3) {
+                    slot(k)[cur.q[cidx]] = v;
+                } else {
+                    /* black of the second sweep on plane p-3: store it with its red neighbour */
+                    const int q = cur.q[cidx];
+                    const T* pm = slot(k);
+                    const int ob = (oj + k + 1) & 1;      /* black sits at the odd position of the pair */
+                    const T vr = pm[q + (ob ? -1 : 1)];
+                    pv.x = ob ? vr : v;
+                    pv.y = ob ? v : vr;
+                    pptr = A.phi_out + oi0 + (long long)n0 * oj + (long long)plane * k;
+                    pend = ostore;
+                }
+            }
+        }
+    };
+    Stream2Regs<T> R0, R1;
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    if (PROLONG) {                 /* the window before the even plane z0-4 = 2K: {K-1, K} */
+        coarse_plane(cw0, z0 / 2 - 3);
+        coarse_plane(cw1, z0 / 2 - 2);
+    }
+    /* planes z0-4 and z0-3 into LDS, then step z0-3's loads */
+    issue(R0, z0 - 5, P1{});
+    put_plane(R0, z0 - 4, P0{});
+    issue(R0, z0 - 4, P0{});
+    put_plane(R0, z0 - 3, P1{});
+    issue(R0, z0 - 3, P1{});
+    for (int p = z0 - 3; p <= z1 + 2; p += 2) {   /* z0 even: p odd, p + 1 even */
+        step(p, R0, R1, P1{});
+        if (p + 1 <= z1 + 2) step(p + 1, R1, R0, P0{});
     }
     if (pend) *reinterpret_cast<T2*>(pptr) = pv;
 }
@@ -1856,28 +2126,48 @@ static bool stream_ok(const DLevT<T>& l, const DBC& bc) {
            (bc.lo[1] != BC_PER || (l.n1 % 2) == 0) && l.n2 >= 2 && (l.read_a != 1 || l.mat != nullptr);
 }
 bool fused_ok(const DLev& l, const DBC& bc) { return stream_ok(l, bc); }
+bool fused_ok32(const DLevF& l, const DBC& bc) { return stream_ok(l, bc); }
 
 template <class T>
 __global__ void k_copyT(T* __restrict__ d, const T* __restrict__ s, long long n) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) d[c] = s[c];
 }
 
-/* Below this size the two per-colour launches are faster: a streaming
- * workgroup's planes are sequential, so a small level has too few of them
- * in flight to cover the memory latency of each plane step. */
-static constexpr long long STREAM_MIN_CELLS = 1LL << 21;
-
-/* z-chunk of the streaming sweep: even, about 1024 workgroups, 8..64 planes */
-static int stream_chunk(int tiles, int n2) {
-    int cz = (int)(((long long)n2 * tiles + 1023) / 1024);
+/* z-chunk of a streaming kernel: even, about nwg workgroups, 8..64 planes */
+static int stream_chunk(int tiles, int n2, int nwg = 1024) {
+    int cz = (int)(((long long)n2 * tiles + nwg - 1) / nwg);
     cz = std::max(cz, 8);
     cz = std::min(std::min(cz, n2), SCZ_MAX);
     return (cz + 1) & ~1;
 }
 
+/* Levels are streamed when large enough (DLevT::stream, from the
+ * mg_stream_min_cells input, default 2M cells): below that a streaming
+ * workgroup's planes are too few to cover the memory latency of each plane
+ * step, and the per-level kernels or the block legs are faster. */
 template <class T>
 static bool stream_use(const DLevT<T>& l, const DBC& bc, int allow_fused) {
-    return allow_fused && stream_ok(l, bc) && (long long)l.n0 * l.n1 * l.n2 >= STREAM_MIN_CELLS;
+    return allow_fused && stream_ok(l, bc) && l.stream;
+}
+
+/* z-chunk of the two-sweep pass: even, about two workgroups per compute unit
+ * (512 on this chip), at least 8 planes (the chunk recomputes 6 planes of
+ * z-halo) and at most 64 */
+static int stream2_chunk(int tiles, int n2) {
+    int cz = (int)(((long long)n2 * tiles + 511) / 512);
+    cz = std::max(cz, 8);
+    cz = std::min(std::min(cz, n2), SCZ_MAX);
+    return (cz + 1) & ~1;
+}
+template <class T, int AMODE, int PRO>
+static void stream2_launch(hipStream_t s, const StreamArgs<T>& A, int nblk) {
+    static bool set = false;
+    if (!set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_gsrb_stream2<T, AMODE, PRO>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)stream2_lds<T>());
+        set = true;
+    }
+    hipLaunchKernelGGL((k_mg_gsrb_stream2<T, AMODE, PRO>), dim3(nblk), dim3(256), stream2_lds<T>(), s, A);
 }
 
 /* Prolongation of the correction on c into f (one thread per coarse cell). */
@@ -1922,20 +2212,46 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
         A.inh = inh;
         A.ntx = (l.n0 + STX - 1) / STX;
         A.nty = (l.n1 + STY - 1) / STY;
-        A.cz = stream_chunk(A.ntx * A.nty, l.n2);
+        A.cz = stream_chunk(A.ntx * A.nty, l.n2, 1024 * 256 / STHR);   /* about four waves per SIMD */
         const int nz = (l.n2 + A.cz - 1) / A.cz;
         const int nblk = A.ntx * A.nty * nz;
         A.remap = (nblk % 8) == 0 ? 1 : 0;
         T* cur = l.phi;
         T* nxt = l.phi2;
-        for (int it = 0; it < nsweeps; ++it) {
+        int it = 0;
+        if (allow_fused >= 2 && nsweeps >= 2) {
+            /* pairs of sweeps in one pass each */
+            StreamArgs<T> B = A;
+            B.ntx = (l.n0 + W2X - 1) / W2X;
+            B.nty = (l.n1 + W2Y - 1) / W2Y;
+            B.cz = stream2_chunk(B.ntx * B.nty, l.n2);
+            const int nb2 = B.ntx * B.nty * ((l.n2 + B.cz - 1) / B.cz);
+            B.remap = (nb2 % 8) == 0 ? 1 : 0;
+            for (; it + 1 < nsweeps; it += 2) {
+                const bool pro = coarse && it == 0;
+                B.zero_init = (zero_first && it == 0 && !pro) ? 1 : 0;
+                B.phi_in = cur;
+                B.phi_out = nxt;
+                B.ec = pro ? coarse->phi : nullptr;
+                B.cn2 = pro ? coarse->n2 : 0;
+#define SWEEP2(am, pr) stream2_launch<T, am, pr>(s, B, nb2)
+                if (pro) {
+                    switch (l.read_a) { case 0: SWEEP2(0, 1); break; case 1: SWEEP2(1, 1); break; default: SWEEP2(2, 1); break; }
+                } else {
+                    switch (l.read_a) { case 0: SWEEP2(0, 0); break; case 1: SWEEP2(1, 0); break; default: SWEEP2(2, 0); break; }
+                }
+#undef SWEEP2
+                std::swap(cur, nxt);
+            }
+        }
+        for (; it < nsweeps; ++it) {
             const bool pro = coarse && it == 0;
             A.zero_init = (zero_first && it == 0 && !pro) ? 1 : 0;
             A.phi_in = cur;
             A.phi_out = nxt;
             A.ec = pro ? coarse->phi : nullptr;
             A.cn2 = pro ? coarse->n2 : 0;
-#define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr>), dim3(nblk), dim3(256), 0, s, A)
+#define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr>), dim3(nblk), dim3(STHR), 0, s, A)
             if (pro) {
                 switch (l.read_a) { case 0: SWEEP(0, 1); break; case 1: SWEEP(1, 1); break; default: SWEEP(2, 1); break; }
             } else {
@@ -1977,7 +2293,7 @@ void mg_smooth(hipStream_t s, const DLev& l, const DBC& bc, int inh, int color) 
 template <class T>
 static bool resid_stream_ok(const DLevT<T>& l, const DBC& bc) {
     return l.layered && bc.lo[2] != BC_PER && l.n2 >= 2 && l.n0 >= 64 && l.n1 >= 8 &&
-           (l.read_a != 1 || l.mat != nullptr) && (long long)l.n0 * l.n1 * l.n2 >= STREAM_MIN_CELLS;
+           (l.read_a != 1 || l.mat != nullptr) && l.stream;
 }
 template <class T>
 static ResidArgs<T> resid_args(const DLevT<T>& l, const DBC& bc, int inh, int* nblk) {

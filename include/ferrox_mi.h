@@ -132,8 +132,9 @@ typedef struct fx_inputs {
     double bias_ramp_start;
     double bias_ramp_end;
     int    verbose;
-    int    mg_fused_sweep;        /* 1: one-pass streaming red+black sweep on levels of >= 2M cells (default 1); 0: two launches per sweep */
+    int    mg_fused_sweep;        /* streamed levels: 1 (default) one red+black sweep per pass, 2 two sweeps per pass, 0 two launches per sweep */
     int    mg_block_levels;       /* 1: small layered levels run their down and up legs as one launch each, column blocks in LDS (default 1) */
+    int    mg_stream_min_cells;   /* levels of at least this many cells use the streaming kernels (default 2097152) */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;
@@ -282,7 +283,8 @@ int fx_set_plot_dir(fx_ctx* ctx, const char* dir);
 int fx_set_bias_waveform(fx_ctx* ctx, const double* v, int n);
 
 /* Measurement: average duration in ms of one launch of a named hot-path
- * kernel ("smooth0", "prolong0", "resid0", "residmax0", "tdgl", "rhs", "copy"),
+ * kernel ("smooth0", "smooth0x2", "prolong0", "resid0", "residmax0", "tdgl",
+ * "rhs", "copy"),
  * timed with HIP events
  * on the context's stream.  No reference counterpart (instrumentation). */
 int fx_bench_kernel(fx_ctx* ctx, const char* which, int reps, double* avg_ms);

@@ -1,5 +1,4 @@
-"""The one-launch legs of the small multigrid levels against the per-level
-kernels, V-cycle by V-cycle.
+"""Fused multigrid kernels against the per-level kernels, V-cycle by V-cycle.
 
 With mg_block_levels = 1 every small layered level below the finest runs its
 down leg (sweeps, residual, restriction) and its up leg (prolongation,
@@ -10,6 +9,11 @@ rounding (the interior update multiplies by a per-plane 1/D where the
 per-colour kernel divides): bar 1e-12 of max|phi|.  The cases cover the
 coarsening patterns (1,1,1) and (1,1,0), periodic and non-periodic lateral
 faces, and a Helmholtz term.
+
+The same holds for the streaming sweeps of the large levels: two sweeps in
+one pass (mg_fused_sweep = 2) against one sweep per pass (1) and against the
+per-colour kernels (0), with the streaming kernels forced onto every level
+they can serve (mg_stream_min_cells = 0), including tiles cut by the domain.
 """
 import ctypes as C
 
@@ -81,3 +85,39 @@ def test_block_legs_match_per_level_kernels(case, helmholtz):
         err = np.abs(p1 - p0).max() / scale
         print(case, helmholtz, ncyc, lv, err)
         assert scale > 0 and err < 1e-12, (ncyc, err)
+
+
+def partial_tiles():
+    # 96 x 20 columns: the last tile of 64 x 8 is cut in x and in y
+    return configs.mfim(nx=96, ny=20, nz=36, lx=48e-9, ly=10e-9)
+
+
+STREAM_CASES = {
+    "periodic": lambda: configs.mfim_64(nsteps=1),
+    "nonperiodic": nonperiodic_mfim,
+    "partial_tiles": partial_tiles,
+}
+
+
+@pytest.mark.parametrize("case", sorted(STREAM_CASES))
+@pytest.mark.parametrize("helmholtz", [False, True])
+def test_two_sweep_pass_matches_single_sweeps(case, helmholtz):
+    text = STREAM_CASES[case]()
+    probe = fx.FerroX(fx.from_text(text))
+    shape = probe.shape
+    probe.close()
+    rng = np.random.default_rng(23)
+    rhs = rng.standard_normal(shape) * 1e6
+    acoef = None
+    if helmholtz:
+        acoef = np.zeros(shape)
+        acoef[: shape[0] // 3] = -1e2 * (1.0 + rng.random((shape[0] // 3,) + shape[1:]))
+        acoef *= 8.854e-12 / (0.5e-9) ** 2
+    for ncyc in (1, 3):
+        ref, lv = cycles(text, ncyc, 1, rhs, acoef, mg_fused_sweep=0)
+        scale = np.abs(ref).max()
+        for fused in (1, 2):
+            p, _ = cycles(text, ncyc, 1, rhs, acoef, mg_fused_sweep=fused, mg_stream_min_cells=0)
+            err = np.abs(p - ref).max() / scale
+            print(case, helmholtz, ncyc, fused, lv, err)
+            assert scale > 0 and err < 1e-12, (ncyc, fused, err)

@@ -18,6 +18,10 @@ TOL = 1e-10
 
 def pair(text, **kw):
     kw.setdefault("mg_tol_rel", 1e-13)
+    if kw.get("mg_fused_sweep", 0):
+        # the streaming sweeps on every level they can serve, not only on
+        # levels of 2M cells and more
+        kw.setdefault("mg_stream_min_cells", 0)
     inp = fx.from_text(text, **kw)
     return fx.FerroX(inp), Oracle(inp)
 
@@ -39,7 +43,7 @@ def test_setup_fields_match(relerr):
         assert np.array_equal(a, b), name
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2])
 def test_poisson_solve_matches_oracle(relerr, fused):
     g, o = pair(configs.mfim_64(nsteps=1), mg_tol_rel=1e-13, mg_fused_sweep=fused)
     rng = np.random.default_rng(3)
@@ -51,7 +55,7 @@ def test_poisson_solve_matches_oracle(relerr, fused):
     assert relerr(pg, po) < 1e-12
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2])
 def test_poisson_solve_reference_tolerance(relerr, fused):
     """The reference's stopping rule (1e-10): both sides stop at the same
     V-cycle and agree to the solver's own accuracy at that tolerance (the
@@ -89,7 +93,7 @@ def test_mfim_steps(relerr, ghost, fused):
         assert e < TOL, (name, e)
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2])
 def test_mfim_reference_stopping_rule(relerr, fused):
     g, o = pair(small_mfim(nsteps=20), mg_tol_rel=1e-10, mg_fused_sweep=fused)
     sg, so = g.step(20), o.step(20)
@@ -105,6 +109,21 @@ def test_mfism_semiconductor_steps(relerr):
     print(sg, so)
     assert sg["total_outer_iters"] == so["total_outer_iters"]
     for name in ("Pz", "Phi", "electrons", "holes", "charge"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < TOL, (name, e)
+
+
+@pytest.mark.parametrize("fused", [1, 2])
+def test_mfism_streaming_sweeps(relerr, fused):
+    """The streaming sweeps with the Newton term of the semiconductor cells on
+    the finest level (Helmholtz term read through the material id)."""
+    text = configs.mfism(nx=64, ny=16, nz=64, lx=32e-9, ly=8e-9, nsteps=3, ramp_steps=3, v_end=1.0)
+    g, o = pair(text, mg_fused_sweep=fused)
+    sg, so = g.step(3), o.step(3)
+    print(sg, so)
+    assert sg["total_outer_iters"] == so["total_outer_iters"]
+    for name in ("Pz", "Phi", "charge"):
         e = relerr(g.get(name), o.get(name))
         print(name, e)
         assert e < TOL, (name, e)
@@ -160,7 +179,7 @@ def test_large_layered_solve_matches_oracle(relerr):
     """256x256x32 slab of the throughput workload: semi-coarsened levels in the
     layered-coefficient mode, fused and two-kernel sweeps."""
     text = configs.mfim(nx=128, ny=128, nz=64, lx=64e-9, ly=64e-9, lz=6.4e-9, de_top=2e-9, fe_top=6e-9, bias=0.5)
-    for fused in (0, 1):
+    for fused in (0, 1, 2):
         g, o = pair(text, mg_tol_rel=1e-12, mg_fused_sweep=fused)
         sg, so = g.ComputePhi_Rho(), o.ComputePhi_Rho()
         print(fused, sg["vcycles"], so["vcycles"])
@@ -168,7 +187,7 @@ def test_large_layered_solve_matches_oracle(relerr):
 
 
 # ----------------------------------------------------------- precision form
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2])
 def test_mixed_precision_solve_reaches_fp64_accuracy(relerr, fused):
     """FP32 V-cycles inside FP64 iterative refinement converge to the same
     FP64 tolerance as the plain solve."""
@@ -182,7 +201,7 @@ def test_mixed_precision_solve_reaches_fp64_accuracy(relerr, fused):
     assert relerr(pg, po) < 1e-10
 
 
-@pytest.mark.parametrize("fused", [0, 1])
+@pytest.mark.parametrize("fused", [0, 1, 2])
 def test_mixed_precision_steps_mfim(relerr, fused):
     g, o = pair(small_mfim(nsteps=20), precision=1, mg_tol_rel=1e-10, mg_fused_sweep=fused)
     sg, so = g.step(20), o.step(20)
