@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -89,7 +90,9 @@ struct fx_ctx {
     double* red = nullptr;
     int nred = 0;                  /* capacity of red, in partials per quantity */
     double* red_out = nullptr;
-    double* h_red = nullptr;       /* pinned */
+    double* h_red = nullptr;       /* pinned: 8 reduction values, then the publish sequence number */
+    double* d_red_host = nullptr;  /* device view of h_red */
+    unsigned pub_seq = 0;
     int* d_iters = nullptr;
     double* cg_work = nullptr;
     double* tmp = nullptr;         /* N doubles: residual of a one-level solve */
@@ -715,15 +718,38 @@ DBC dbc_of(const fx_ctx* x, const double v[6]) {
     return b;
 }
 
+/* The first nq values of red_out, once the work queued before them is done.
+ * A one-lane kernel writes them into coherent pinned host memory followed by
+ * a sequence number, and the host spins on that number (a stream
+ * synchronisation wakes the host much later than the GPU finishes); the
+ * stream is polled now and then so that an error ends the wait, and a wait
+ * of several seconds falls back to the synchronisation. */
 int read_red(fx_ctx* x, int nq, double* out) {
-    HIPCHK(x, hipMemcpyAsync(x->h_red, x->red_out, nq * sizeof(double), hipMemcpyDeviceToHost, x->s));
-    HIPCHK(x, hipStreamSynchronize(x->s));
+    const unsigned v = ++x->pub_seq;
+    fxk::publish(x->s, x->red_out, nq, x->d_red_host, reinterpret_cast<unsigned*>(x->d_red_host + 8), v);
+    HIPCHK(x, hipGetLastError());
+    volatile unsigned* seq = reinterpret_cast<volatile unsigned*>(x->h_red + 8);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spins = 1; *seq != v; ++spins) {
+        if ((spins & 1023) == 0) {
+            const hipError_t q = hipStreamQuery(x->s);
+            if (q != hipSuccess && q != hipErrorNotReady) return fail(x, FX_EHIP, hipGetErrorString(q));
+            if (q == hipSuccess || std::chrono::steady_clock::now() - t0 > std::chrono::seconds(5)) break;
+        }
+    }
+    if (*seq != v) {
+        HIPCHK(x, hipStreamSynchronize(x->s));
+        if (*seq != v) return fail(x, FX_EHIP, "reduction results did not reach the host");
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
     for (int q = 0; q < nq; ++q) out[q] = x->h_red[q];
     return FX_OK;
 }
 
 /* --------------------------------- multigrid ------------------------------ */
-void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
+/* pre_done: the first pre-smoothing sweep of level 0 has already run
+ * (phi -> phi2, queued speculatively behind the previous convergence check) */
+void vcycle(fx_ctx* x, size_t li, const DBC& bc, bool pre_done = false) {
     Level& L = x->lev[li];
     const int inh = li == 0;
     if ((int)li == x->tail0) {
@@ -756,7 +782,8 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc) {
         fxk::mg_block_up(x->s, L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2);
         return;
     }
-    fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu1, x->in.mg_fused_sweep);
+    if (pre_done) fxk::mg_sweep_rest(x->s, L.d, bc, inh, x->in.mg_nu1 - 1, x->in.mg_fused_sweep);
+    else fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu1, x->in.mg_fused_sweep);
     fxk::mg_resid_restrict(x->s, L.d, C.d, bc, inh);
     vcycle(x, li + 1, bc);
     fxk::mg_prolong_sweep(x->s, L.d, C.d, bc, inh, x->in.mg_nu2, x->in.mg_fused_sweep);
@@ -814,7 +841,13 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     const DBC bc = dbc_of(x, bv);
     Level& L0 = x->lev[0];
     double nr[2];
+    /* The first pre-smoothing sweep of the next V-cycle only reads phi (it
+     * writes phi2), so it is queued behind each convergence check and runs
+     * while the host waits for the norms; when the check stops the solve,
+     * phi is the answer and phi2 is simply not used. */
+    const bool spec = x->lev.size() > 1 && x->tail0 != 0 && x->in.mg_nu1 >= 1;
     fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
+    bool pre = spec && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
     int rc = read_red(x, 2, nr);
     if (rc) return rc;
     const double r0 = nr[0], bn = nr[1];
@@ -823,8 +856,9 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     if (!(r0 == r0) || !(bn == bn)) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
     if (r0 <= target) return FX_OK;
     for (int it = 1; it <= maxit; ++it) {
-        vcycle(x, 0, bc);
+        vcycle(x, 0, bc, pre);
         fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
+        pre = spec && it < maxit && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
         if ((rc = read_red(x, 2, nr))) return rc;
         st->mg_resid = nr[0]; st->vcycles = it;
         if (!(nr[0] == nr[0])) return fail(x, FX_ENAN, "multigrid produced a non-finite residual");
@@ -1096,7 +1130,11 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
     ALLOC(x->red_out, sizeof(double) * 8);
     ALLOC(x->d_iters, sizeof(int) * 4);
 #undef ALLOC
-    if (hipHostMalloc((void**)&x->h_red, 8 * sizeof(double)) != hipSuccess) { free_all(x); delete x; return FX_EHIP; }
+    if (hipHostMalloc((void**)&x->h_red, 8 * sizeof(double) + 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+        hipHostGetDevicePointer((void**)&x->d_red_host, x->h_red, 0) != hipSuccess) {
+        free_all(x); delete x; return FX_EHIP;
+    }
+    *reinterpret_cast<volatile unsigned*>(x->h_red + 8) = 0;
     if ((rc = build_levels(x))) { std::fprintf(stderr, "fx_create: %s\n", x->err.c_str()); free_all(x); delete x; return rc; }
     x->h_tphase.assign(x->N, 0.0);
     for (int d = 0; d < 3; ++d) x->h_ang[d].assign(x->N, 0.0);

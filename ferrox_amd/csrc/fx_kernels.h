@@ -78,6 +78,9 @@ struct EArgs {
 
 namespace fxk {
 void fill(hipStream_t s, double* p, long long n, double v);
+/* host[0..nq) = src[0..nq), then *seq = v (host and seq: device views of
+ * coherent pinned host memory) */
+void publish(hipStream_t s, const double* src, int nq, double* host, unsigned* seq, unsigned v);
 void copy(hipStream_t s, double* d, const double* src, long long n);
 void add(hipStream_t s, double* d, const double* src, long long n);
 void final_reduce(hipStream_t s, const double* part, int nb, int nq, int is_max, double* out);
@@ -100,6 +103,11 @@ void mg_prolong_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, con
 void mg_prolong_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nsweeps,
                       int allow_fused);
 bool fused_ok(const fxd::DLev& l, const fxd::DBC& bc);
+/* the first streaming sweep of a level on its own, phi -> phi2 (phi is left
+ * as it was; false when the level is not streamed one sweep per pass), and
+ * the remaining sweeps from phi2, the result in phi */
+bool mg_sweep_first(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int allow_fused);
+void mg_sweep_rest(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused);
 bool fused_ok32(const fxd::DLevF& l, const fxd::DBC& bc);
 void mg_resid_restrict(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc, int inh);
 void mg_prolong(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc);

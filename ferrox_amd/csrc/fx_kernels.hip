@@ -30,6 +30,17 @@ __global__ void k_add(double* __restrict__ d, const double* __restrict__ s, long
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) d[c] += s[c];
 }
 
+/* The host reads reductions without a stream synchronisation: one lane
+ * copies them into coherent pinned host memory, fences at system scope and
+ * then publishes the sequence number the host is waiting for. */
+__global__ void k_publish(const double* __restrict__ src, int nq, double* host, unsigned* seq, unsigned v) {
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < nq; ++q) host[q] = src[q];
+        __threadfence_system();
+        __hip_atomic_store(seq, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 /* Final reduction of per-block partials: out[q] = reduce_b partial[q*nb + b]. */
 __global__ __launch_bounds__(256) void k_final(const double* __restrict__ part, int nb, int nq, int is_max, double* __restrict__ out) {
     __shared__ double sh[256 / 64 + 1];
@@ -420,6 +431,9 @@ namespace fxk {
 static inline unsigned nblk(long long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 static inline dim3 grid_cells(const int n[3]) { return dim3((n[0] + 63) / 64, (n[1] + 3) / 4, n[2]); }
 
+void publish(hipStream_t s, const double* src, int nq, double* host, unsigned* seq, unsigned v) {
+    hipLaunchKernelGGL(k_publish, dim3(1), dim3(64), 0, s, src, nq, host, seq, v);
+}
 void fill(hipStream_t s, double* p, long long n, double v) {
     unsigned b = nblk(n, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
     hipLaunchKernelGGL(k_fill, dim3(b), dim3(256), 0, s, p, n, v);

@@ -2195,7 +2195,7 @@ static void prolong_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>& c, co
  * by k_mg_prolong_blk. */
 template <class T>
 static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh, int nsweeps, int allow_fused, int zero_first,
-                       const DLevT<T>* coarse = nullptr) {
+                       const DLevT<T>* coarse = nullptr, bool start_alt = false, bool keep_alt = false) {
     const bool zonly = coarse && l.crs[0] == 0 && l.crs[1] == 0 && l.crs[2] == 1 && coarse->n0 == l.n0 && coarse->n1 == l.n1;
     if (coarse && !(zonly && nsweeps > 0 && stream_use(l, bc, allow_fused))) {
         prolong_impl<T>(s, l, *coarse, bc);
@@ -2216,8 +2216,9 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
         const int nz = (l.n2 + A.cz - 1) / A.cz;
         const int nblk = A.ntx * A.nty * nz;
         A.remap = (nblk % 8) == 0 ? 1 : 0;
-        T* cur = l.phi;
-        T* nxt = l.phi2;
+        /* start_alt: the input is in phi2; keep_alt: leave an odd result in phi2 */
+        T* cur = start_alt ? l.phi2 : l.phi;
+        T* nxt = start_alt ? l.phi : l.phi2;
         int it = 0;
         if (allow_fused >= 2 && nsweeps >= 2) {
             /* pairs of sweeps in one pass each */
@@ -2260,7 +2261,7 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
 #undef SWEEP
             std::swap(cur, nxt);
         }
-        if (cur != l.phi) {   /* odd number of sweeps: the result goes back to phi */
+        if (cur != l.phi && !keep_alt) {   /* the result goes back to phi */
             const long long n = (long long)l.n0 * l.n1 * l.n2;
             hipLaunchKernelGGL(k_copyT<T>, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, s, l.phi, cur, n);
         }
@@ -2274,6 +2275,19 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
 }
 void mg_sweep(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps, int allow_fused) {
     sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, 0);
+}
+bool mg_sweep_first(hipStream_t s, const DLev& l, const DBC& bc, int inh, int allow_fused) {
+    if (allow_fused != 1 || !stream_use(l, bc, allow_fused)) return false;
+    sweep_impl<double>(s, l, bc, inh, 1, allow_fused, 0, nullptr, false, true);
+    return true;
+}
+void mg_sweep_rest(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps, int allow_fused) {
+    if (nsweeps <= 0) {
+        const long long n = (long long)l.n0 * l.n1 * l.n2;
+        hipLaunchKernelGGL(k_copyT<double>, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, s, l.phi, l.phi2, n);
+        return;
+    }
+    sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, 0, nullptr, true, false);
 }
 void mg_sweep(hipStream_t s, const DLevF& l, const DBC& bc, int nsweeps, int zero_first, int allow_fused) {
     sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, zero_first);
