@@ -61,17 +61,23 @@ __global__ __launch_bounds__(256) void k_mg_smooth(DLevT<T> l, DBC bc, int inh, 
 
 /* ------------------- one-pass red + black sweep (streaming) --------------- */
 namespace {
-constexpr int STX = 64, STY = 16;                 /* output tile of columns */
-constexpr int STHR = STX * STY / 2;               /* threads: one black cell each */
-constexpr int SHX = STX + 4, SHY = STY + 4;       /* with the 2-cell in-plane halo */
-constexpr int SPL = SHX * SHY;                    /* LDS values per plane */
 constexpr int SNS = 5;                            /* plane slots in LDS */
-constexpr int SLD = (SPL + STHR - 1) / STHR;      /* plane loads per thread */
 constexpr int SCZ_MAX = 64;                       /* largest z-chunk */
-constexpr int NRED = (STY + 2) * (STX / 2 + 1);   /* red cells of the tile and its ring */
-static_assert(NRED <= 2 * STHR, "two red cells per thread");
 constexpr int NPW = 6;                            /* per-plane table: wx wy wzl wzh D 1/D */
 }
+/* Tile of the streaming sweep: 64x16 columns, one thread per black cell of
+ * the tile (64x32 tiles for FP32, twice the cells per plane step in the
+ * same LDS, measured slower: 165 against 169 refined steps/s). */
+template <class T>
+struct SGeo {
+    static constexpr int TX = 64, TY = 16;
+    static constexpr int THR = TX * TY / 2;
+    static constexpr int HX = TX + 4, HY = TY + 4;     /* with the 2-cell in-plane halo */
+    static constexpr int PL = HX * HY;                 /* LDS values per plane */
+    static constexpr int LD = (PL + THR - 1) / THR;    /* plane loads per thread */
+    static constexpr int NRED = (TY + 2) * (TX / 2 + 1);   /* red cells of the tile and its ring */
+    static_assert(NRED <= 2 * THR, "two red cells per thread");
+};
 
 template <class T> struct Vec2;
 template <> struct Vec2<double> { using type = double2; };
@@ -101,8 +107,8 @@ struct StreamArgs {
 namespace {
 template <class T>
 struct StreamRegs {
-    T phi[SLD];      /* next phi plane (raw loads; out-of-domain lanes masked when stored to LDS) */
-    T cn[SLD];       /* PROLONG, odd planes 2K+1: coarse plane K+1, which enters the window */
+    T phi[SGeo<T>::LD];   /* next phi plane (raw loads; out-of-domain lanes masked when stored to LDS) */
+    T cn[SGeo<T>::LD];    /* PROLONG, odd planes 2K+1: coarse plane K+1, which enters the window */
     T b[3];          /* rhs of red cells u = 0, 1 and of the black cell */
     T a[3];          /* Helmholtz term of the same cells */
     uint8_t m[3];    /* their material id */
@@ -145,7 +151,9 @@ struct StreamCell {
  * in-plane halos are shared in that XCD's L2.  Requires: z not periodic, n0
  * even, n0 >= 64, n1 >= 8, periodic extents even, cz even. */
 template <class T, int AMODE, int PROLONG>
-__global__ __launch_bounds__(STHR) void k_mg_gsrb_stream(StreamArgs<T> A) {
+__global__ __launch_bounds__(SGeo<T>::THR) void k_mg_gsrb_stream(StreamArgs<T> A) {
+    using G = SGeo<T>;
+    constexpr int STX = G::TX, STY = G::TY, STHR = G::THR, SHX = G::HX, SPL = G::PL, SLD = G::LD, NRED = G::NRED;
     __shared__ T sp[SNS * SPL];
     __shared__ T scw[NPW * (SCZ_MAX + 2)];
     const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
@@ -2210,9 +2218,9 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
         for (int d = 0; d < 3; ++d) A.ih2[d] = l.ih2[d];
         A.bc = bc;
         A.inh = inh;
-        A.ntx = (l.n0 + STX - 1) / STX;
-        A.nty = (l.n1 + STY - 1) / STY;
-        A.cz = stream_chunk(A.ntx * A.nty, l.n2, 1024 * 256 / STHR);   /* about four waves per SIMD */
+        A.ntx = (l.n0 + SGeo<T>::TX - 1) / SGeo<T>::TX;
+        A.nty = (l.n1 + SGeo<T>::TY - 1) / SGeo<T>::TY;
+        A.cz = stream_chunk(A.ntx * A.nty, l.n2, 1024 * 256 / SGeo<T>::THR);   /* about four waves per SIMD */
         const int nz = (l.n2 + A.cz - 1) / A.cz;
         const int nblk = A.ntx * A.nty * nz;
         A.remap = (nblk % 8) == 0 ? 1 : 0;
@@ -2252,7 +2260,7 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
             A.phi_out = nxt;
             A.ec = pro ? coarse->phi : nullptr;
             A.cn2 = pro ? coarse->n2 : 0;
-#define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr>), dim3(nblk), dim3(STHR), 0, s, A)
+#define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr>), dim3(nblk), dim3(SGeo<T>::THR), 0, s, A)
             if (pro) {
                 switch (l.read_a) { case 0: SWEEP(0, 1); break; case 1: SWEEP(1, 1); break; default: SWEEP(2, 1); break; }
             } else {
