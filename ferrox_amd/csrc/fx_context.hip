@@ -54,6 +54,7 @@ inline double uniform01(uint64_t seed, uint64_t idx) {
 struct Level {
     DLev d;
     size_t N;
+    double* up = nullptr;   /* block levels: the up leg's output, swapped with d.phi after it */
 };
 
 }  // namespace
@@ -112,6 +113,7 @@ struct fx_ctx {
     bool lu_ok64 = false, lu_ok32 = false;
     DLev* d_tail64 = nullptr;
     DLevF* d_tail32 = nullptr;
+    std::vector<float*> up32;     /* FP32 block levels: the up leg's output (Level::up) */
     int tail_ts64 = 0, tail_ts32 = 0;          /* first tail level staged in LDS */
     size_t tail_lds64 = 0, tail_lds32 = 0;     /* dynamic LDS of the tail kernel */
     /* FP32 correction hierarchy of the mixed-precision solve (precision = FP32_REFINE) */
@@ -222,7 +224,8 @@ inline size_t gidx(const fx_ctx* x, int i, int j, int k) {
 }
 
 /* Coarsening rule shared with the oracle's specification: direction d is
- * coarsened when n_d is even and h_d < 1.9*min(h); a level with an odd
+ * coarsened when n_d is even and h_d < mg_coarsen_ratio*min(h) (default
+ * 1.9); a level with an odd
  * periodic extent > 1, or with nothing left to coarsen, is the bottom. */
 void plan_levels(fx_ctx* x, std::vector<std::array<int, 3>>& dims, std::vector<std::array<double, 3>>& hs,
                  std::vector<std::array<int, 3>>& crs) {
@@ -235,7 +238,7 @@ void plan_levels(fx_ctx* x, std::vector<std::array<int, 3>>& dims, std::vector<s
         std::array<int, 3> c = {0, 0, 0};
         if (!oddp) {
             const double hmin = std::min(h[0], std::min(h[1], h[2]));
-            for (int d = 0; d < 3; ++d) c[d] = (n[d] % 2 == 0 && n[d] >= 2 && h[d] < 1.9 * hmin) ? 1 : 0;
+            for (int d = 0; d < 3; ++d) c[d] = (n[d] % 2 == 0 && n[d] >= 2 && h[d] < x->in.mg_coarsen_ratio * hmin) ? 1 : 0;
         }
         const size_t Nl = (size_t)n[0] * n[1] * n[2];
         if (oddp || !(c[0] | c[1] | c[2]) || Nl <= 1) { crs.push_back({0, 0, 0}); break; }
@@ -300,11 +303,16 @@ int build_levels(fx_ctx* x) {
         if ((rc = dalloc(x, (void**)&x->d_tail64, sizeof(DLev) * x->lev.size()))) return rc;
         if (x->in.precision == FX_PREC_FP32_REFINE && (rc = dalloc(x, (void**)&x->d_tail32, sizeof(DLevF) * x->lev.size()))) return rc;
     }
+    /* levels that may run block legs (below the finest, above the tail) */
+    auto block_level = [&](size_t l) { return x->in.mg_block_levels && l >= 1 && (x->tail0 < 0 || (int)l < x->tail0); };
+    for (size_t l = 0; l < x->lev.size(); ++l)
+        if (block_level(l) && (rc = dalloc(x, (void**)&x->lev[l].up, sizeof(double) * x->lev[l].N))) return rc;
     if (x->in.mg_fused_sweep)
         for (size_t l = 0; l < x->lev.size(); ++l)
             if (x->lev[l].d.n0 >= 64 && x->lev[l].d.n1 >= 8 && (rc = dalloc(x, (void**)&x->lev[l].d.phi2, sizeof(double) * x->lev[l].N))) return rc;
     if (x->in.precision == FX_PREC_FP32_REFINE) {
         x->lev32.resize(x->lev.size());
+        x->up32.assign(x->lev.size(), nullptr);
         for (size_t l = 0; l < x->lev.size(); ++l) {
             const DLev& d = x->lev[l].d;
             DLevF& f = x->lev32[l];
@@ -320,6 +328,7 @@ int build_levels(fx_ctx* x) {
             if ((rc = dalloc(x, (void**)&f.by, sizeof(float) * (size_t)d.n0 * (d.n1 + 1) * d.n2))) return rc;
             if ((rc = dalloc(x, (void**)&f.bz, sizeof(float) * (size_t)d.n0 * d.n1 * (d.n2 + 1)))) return rc;
             if (x->in.mg_fused_sweep && d.n0 >= 64 && d.n1 >= 8 && (rc = dalloc(x, (void**)&f.phi2, sizeof(float) * Nl))) return rc;
+            if (block_level(l) && (rc = dalloc(x, (void**)&x->up32[l], sizeof(float) * Nl))) return rc;
         }
         if ((rc = dalloc(x, (void**)&x->cg_work32, sizeof(float) * bottom_work))) return rc;
         if ((rc = dalloc(x, (void**)&x->phi_alt, sizeof(double) * x->N))) return rc;
@@ -775,11 +784,12 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc, bool pre_done = false) {
         return;
     }
     Level& C = x->lev[li + 1];
-    if (li >= 1 && x->in.mg_block_levels && !(L.d.stream && x->in.mg_fused_sweep && fxk::fused_ok(L.d, bc)) &&
+    if (li >= 1 && x->in.mg_block_levels && L.up && !(L.d.stream && x->in.mg_fused_sweep && fxk::fused_ok(L.d, bc)) &&
         fxk::mg_block_ok(L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2)) {
         fxk::mg_block_down(x->s, L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2);
         vcycle(x, li + 1, bc);
-        fxk::mg_block_up(x->s, L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2);
+        fxk::mg_block_up(x->s, L.d, C.d, bc, x->in.mg_nu1, x->in.mg_nu2, L.up);
+        std::swap(L.d.phi, L.up);
         return;
     }
     if (pre_done) fxk::mg_sweep_rest(x->s, L.d, bc, inh, x->in.mg_nu1 - 1, x->in.mg_fused_sweep);
@@ -821,11 +831,12 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
         return;
     }
     DLevF& C = x->lev32[li + 1];
-    if (li >= 1 && x->in.mg_block_levels && !(L.stream && x->in.mg_fused_sweep && fxk::fused_ok32(L, bc)) &&
+    if (li >= 1 && x->in.mg_block_levels && x->up32[li] && !(L.stream && x->in.mg_fused_sweep && fxk::fused_ok32(L, bc)) &&
         fxk::mg_block_ok(L, C, bc, x->in.mg_nu1, x->in.mg_nu2)) {
         fxk::mg_block_down(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2);
         vcycle32(x, li + 1, bc, true);
-        fxk::mg_block_up(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2);
+        fxk::mg_block_up(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2, x->up32[li]);
+        std::swap(L.phi, x->up32[li]);
         return;
     }
     fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
@@ -1050,13 +1061,14 @@ void free_all(fx_ctx* x) {
     F(x->red); F(x->red_out); F(x->d_iters); F(x->cg_work); F(x->tmp);
     for (size_t l = 0; l < x->lev.size(); ++l) {
         DLev& d = x->lev[l].d;
-        F(d.bx); F(d.by); F(d.bz);
+        F(d.bx); F(d.by); F(d.bz); F(x->lev[l].up);
         if (l > 0) { F(d.phi); F(d.b); F(d.a); }
     }
     for (size_t l = 0; l < x->lev32.size(); ++l) {
         DLevF& f = x->lev32[l];
         F(f.phi); F(f.b); F(f.a); F(f.bx); F(f.by); F(f.bz); F((void*)f.bxk); F((void*)f.byk); F((void*)f.bzk); F(f.phi2);
     }
+    for (float* p : x->up32) F(p);
     F(x->cg_work32); F(x->phi_alt); F(x->d_tail64); F(x->d_tail32);
     F(x->lu64); F(x->lu32); F(x->perm64); F(x->perm32);
     if (x->h_red) hipHostFree(x->h_red);

@@ -147,14 +147,18 @@ void mg_tail_lay(hipStream_t s, const fxd::TailPlan<double>& P, const fxd::DBC& 
 void mg_tail_lay(hipStream_t s, const fxd::TailPlan<float>& P, const fxd::DBC& bc, int read_a, int nu1, int nu2,
                  float* work, double tol, int maxit, int* iters);
 /* one launch per leg for a small layered level l above c: down = nu1 sweeps
- * from phi = 0 + residual restricted into c; up = prolongation from c + nu2
- * sweeps (mg_block_ok: the level qualifies and the blocks fit in LDS) */
+ * from phi = 0 + residual restricted into c, written to l.phi; up =
+ * prolongation from c + nu2 sweeps, read from l.phi and written to out, a
+ * second buffer of the level's size (each workgroup reads the halo of its
+ * neighbours' blocks, so the leg cannot update l.phi in place); the caller
+ * then takes out as the level's phi (mg_block_ok: the level qualifies and
+ * the blocks fit in LDS) */
 bool mg_block_ok(const fxd::DLev& l, const fxd::DLev& c, const fxd::DBC& bc, int nu1, int nu2);
 bool mg_block_ok(const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nu1, int nu2);
 void mg_block_down(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, const fxd::DBC& bc, int nu1, int nu2);
 void mg_block_down(hipStream_t s, const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nu1, int nu2);
-void mg_block_up(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, const fxd::DBC& bc, int nu1, int nu2);
-void mg_block_up(hipStream_t s, const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nu1, int nu2);
+void mg_block_up(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, const fxd::DBC& bc, int nu1, int nu2, double* out);
+void mg_block_up(hipStream_t s, const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nu1, int nu2, float* out);
 /* dynamic LDS bytes of mg_tail_lay for levels of extents n[0..nl-1] */
 size_t tail_lay_lds(const int (*n)[3], int nl, int read_a, size_t elem);
 /* LU factors of the bottom operator (<= BOTTOM_DIRECT_CELLS cells): lu N*N

@@ -1494,6 +1494,7 @@ struct BlkArgs {
     DLevT<T> c;        /* the next coarser level */
     DBC bc;
     int nu, bx, by;
+    T* out;            /* where the leg writes the block's phi */
 };
 
 /* (A phi)_c = D phi_c - S of the LDS cell q (global cell (i, j, k)) */
@@ -1615,7 +1616,7 @@ struct BlkLevel {
             V[q] = (in && !ra) ? (bv + S) * tb[5] : (bv + S) / D;
         }
     }
-    /* the block (ring 0) back to the level's phi */
+    /* the block (ring 0) to the leg's output */
     __device__ void store() {
         const DLevT<T>& l = A.l;
         const int tot = A.bx * A.by * l.n2;
@@ -1625,7 +1626,7 @@ struct BlkLevel {
             const int rr = t - k * A.bx * A.by;
             const int jl = qdiv(rr, A.bx, rx);
             const int xl = rr - jl * A.bx;
-            l.phi[(x0 + xl) + (long long)l.n0 * ((y0 + jl) + (long long)l.n1 * k)] = V[(H + xl) + LX * ((H + jl) + LY * k)];
+            A.out[(x0 + xl) + (long long)l.n0 * ((y0 + jl) + (long long)l.n1 * k)] = V[(H + xl) + LX * ((H + jl) + LY * k)];
         }
     }
 };
@@ -2452,8 +2453,10 @@ static void block_launch(hipStream_t s, const BlkArgs<T>& A, bool down, size_t l
     }
 }
 template <class T>
-static bool block_impl(hipStream_t s, const DLevT<T>& l, const DLevT<T>& c, const DBC& bc, int nu1, int nu2, int leg) {
+static bool block_impl(hipStream_t s, const DLevT<T>& l, const DLevT<T>& c, const DBC& bc, int nu1, int nu2, int leg,
+                       T* out = nullptr) {
     BlkArgs<T> A;
+    A.out = leg == 0 ? l.phi : out;
     size_t lds;
     if (leg < 0) {
         int bx, by;
@@ -2488,11 +2491,11 @@ void mg_block_down(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, i
 void mg_block_down(hipStream_t s, const DLevF& l, const DLevF& c, const DBC& bc, int nu1, int nu2) {
     block_impl<float>(s, l, c, bc, nu1, nu2, 0);
 }
-void mg_block_up(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, int nu1, int nu2) {
-    block_impl<double>(s, l, c, bc, nu1, nu2, 1);
+void mg_block_up(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, int nu1, int nu2, double* out) {
+    block_impl<double>(s, l, c, bc, nu1, nu2, 1, out);
 }
-void mg_block_up(hipStream_t s, const DLevF& l, const DLevF& c, const DBC& bc, int nu1, int nu2) {
-    block_impl<float>(s, l, c, bc, nu1, nu2, 1);
+void mg_block_up(hipStream_t s, const DLevF& l, const DLevF& c, const DBC& bc, int nu1, int nu2, float* out) {
+    block_impl<float>(s, l, c, bc, nu1, nu2, 1, out);
 }
 void mg_prolong(hipStream_t s, const DLev& f, const DLev& c, const DBC& bc) { prolong_impl<double>(s, f, c, bc); }
 void mg_prolong(hipStream_t s, const DLevF& f, const DLevF& c, const DBC& bc) { prolong_impl<float>(s, f, c, bc); }

@@ -135,6 +135,7 @@ typedef struct fx_inputs {
     int    mg_fused_sweep;        /* streamed levels: 1 (default) one red+black sweep per pass, 2 two sweeps per pass, 0 two launches per sweep */
     int    mg_block_levels;       /* 1: small layered levels run their down and up legs as one launch each, column blocks in LDS (default 1) */
     int    mg_stream_min_cells;   /* levels of at least this many cells use the streaming kernels (default 2097152) */
+    double mg_coarsen_ratio;      /* direction d is coarsened when h_d < ratio * min(h) (default 1.9) */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;

@@ -118,6 +118,7 @@ struct Lev {
 struct MG {
     std::vector<Lev> L;
     int bclo[3], bchi[3];
+    double ratio = 1.9;
     int nu1 = 2, nu2 = 2;
     double bottom_tol = 1e-6;
     int bottom_maxit = 400;
@@ -159,7 +160,8 @@ struct MG {
 
     void build(int depth_cap) {
         /* Levels after the finest are made by the coarsening rule: direction d
-         * is coarsened when n_d is even and h_d < 1.9*min(h).  A level with an
+         * is coarsened when n_d is even and h_d < ratio*min(h) (the
+         * mg_coarsen_ratio input, default 1.9).  A level with an
          * odd periodic extent > 1 (red-black colouring would break across the
          * wrap) or with no coarsenable direction is the bottom. */
         while ((int)L.size() < depth_cap) {
@@ -171,7 +173,7 @@ struct MG {
             double hmin = std::min(f.h[0], std::min(f.h[1], f.h[2]));
             int any = 0;
             for (int d = 0; d < 3; ++d) {
-                f.crs[d] = (f.n[d] % 2 == 0 && f.n[d] >= 2 && f.h[d] < 1.9 * hmin) ? 1 : 0;
+                f.crs[d] = (f.n[d] % 2 == 0 && f.n[d] >= 2 && f.h[d] < ratio * hmin) ? 1 : 0;
                 any |= f.crs[d];
             }
             if (!any || f.N <= 1) { f.crs[0] = f.crs[1] = f.crs[2] = 0; break; }
@@ -980,7 +982,7 @@ fxo_state* fxo_create(const fx_inputs* in) {
     refresh_rotation(s);
     /* multigrid finest level */
     for (int d = 0; d < 3; ++d) { s->mg.bclo[d] = in->bc_lo[d]; s->mg.bchi[d] = in->bc_hi[d]; }
-    s->mg.nu1 = in->mg_nu1; s->mg.nu2 = in->mg_nu2;
+    s->mg.nu1 = in->mg_nu1; s->mg.nu2 = in->mg_nu2; s->mg.ratio = in->mg_coarsen_ratio;
     Lev l0;
     for (int d = 0; d < 3; ++d) { l0.n[d] = s->n[d]; l0.h[d] = s->dx[d]; l0.ih2[d] = 1.0 / (s->dx[d] * s->dx[d]); l0.crs[d] = 0; }
     l0.N = N;

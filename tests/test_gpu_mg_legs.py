@@ -56,16 +56,32 @@ def nonperiodic_mfim():
     return text.replace("boundary.hi = per per dir(0.0)", "boundary.hi = dir(0.0) neu dir(0.0)")
 
 
+def thin_dz():
+    # dz = 0.1 nm: two z-only coarsenings (64 and 32 planes) before the
+    # first full one, so the block legs serve levels of 64 and 32 planes
+    return configs.mfim(nx=64, ny=64, nz=128, lx=32e-9, ly=32e-9, lz=12.8e-9, de_top=4e-9, fe_top=12e-9)
+
+
+def thin_dz_256():
+    # the same stack at 256 x 256: with the streaming kernels kept to levels
+    # of 4M cells, the 256 x 256 x 32 level runs block legs of 8 x 8 columns
+    # (the largest LDS footprint the legs take)
+    return configs.mfim(nx=256, ny=256, nz=128, lx=128e-9, ly=128e-9, lz=12.8e-9, de_top=4e-9, fe_top=12e-9)
+
+
 CASES = {
-    "periodic_111_110": lambda: configs.mfim_64(nsteps=1),
-    "nonperiodic": nonperiodic_mfim,
+    "periodic_111_110": (lambda: configs.mfim_64(nsteps=1), {}),
+    "nonperiodic": (nonperiodic_mfim, {}),
+    "thin_dz": (thin_dz, {}),
+    "thin_dz_256": (thin_dz_256, {"mg_stream_min_cells": 1 << 22}),
 }
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
 @pytest.mark.parametrize("helmholtz", [False, True])
 def test_block_legs_match_per_level_kernels(case, helmholtz):
-    text = CASES[case]()
+    make, kw = CASES[case]
+    text = make()
     probe = fx.FerroX(fx.from_text(text))
     shape = probe.shape
     probe.close()
@@ -79,8 +95,8 @@ def test_block_legs_match_per_level_kernels(case, helmholtz):
         acoef[: shape[0] // 3] = -1e2 * (1.0 + rng.random((shape[0] // 3,) + shape[1:]))
         acoef *= 8.854e-12 / (0.5e-9) ** 2
     for ncyc in (1, 3):
-        p1, lv = cycles(text, ncyc, 1, rhs, acoef)
-        p0, _ = cycles(text, ncyc, 0, rhs, acoef)
+        p1, lv = cycles(text, ncyc, 1, rhs, acoef, **kw)
+        p0, _ = cycles(text, ncyc, 0, rhs, acoef, **kw)
         scale = np.abs(p0).max()
         err = np.abs(p1 - p0).max() / scale
         print(case, helmholtz, ncyc, lv, err)
