@@ -733,9 +733,14 @@ DBC dbc_of(const fx_ctx* x, const double v[6]) {
  * synchronisation wakes the host much later than the GPU finishes); the
  * stream is polled now and then so that an error ends the wait, and a wait
  * of several seconds falls back to the synchronisation. */
-int read_red(fx_ctx* x, int nq, double* out) {
+unsigned post_red(fx_ctx* x, int nq) {
     const unsigned v = ++x->pub_seq;
     fxk::publish(x->s, x->red_out, nq, x->d_red_host, reinterpret_cast<unsigned*>(x->d_red_host + 8), v);
+    return v;
+}
+/* Waits for the values published under sequence number v (work queued after
+ * the publishing kernel does not delay them). */
+int wait_red(fx_ctx* x, unsigned v, int nq, double* out) {
     HIPCHK(x, hipGetLastError());
     volatile unsigned* seq = reinterpret_cast<volatile unsigned*>(x->h_red + 8);
     const auto t0 = std::chrono::steady_clock::now();
@@ -754,6 +759,7 @@ int read_red(fx_ctx* x, int nq, double* out) {
     for (int q = 0; q < nq; ++q) out[q] = x->h_red[q];
     return FX_OK;
 }
+int read_red(fx_ctx* x, int nq, double* out) { return wait_red(x, post_red(x, nq), nq, out); }
 
 /* --------------------------------- multigrid ------------------------------ */
 /* pre_done: the first pre-smoothing sweep of level 0 has already run
@@ -812,7 +818,7 @@ void set_read_a(fx_ctx* x, int mode) {
 /* FP32 V-cycle on the correction equation A e = r (homogeneous BCs).  zero0:
  * level 0 starts from e = 0 (the zero-initial first red pass); coarse levels
  * always do, their phi being cleared by the restriction. */
-void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
+void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true, bool pre_done = false) {
     DLevF& L = x->lev32[li];
     if ((int)li == x->tail0) {
         if (x->tail_lean) {
@@ -839,7 +845,8 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true) {
         std::swap(L.phi, x->up32[li]);
         return;
     }
-    fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
+    if (pre_done) fxk::mg_sweep_rest(x->s, L, bc, x->in.mg_nu1 - 1, x->in.mg_fused_sweep);
+    else fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
     fxk::mg_resid_restrict(x->s, L, C, bc);
     vcycle32(x, li + 1, bc, true);
     fxk::mg_prolong_sweep(x->s, L, C, bc, x->in.mg_nu2, x->in.mg_fused_sweep);
@@ -853,13 +860,15 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     Level& L0 = x->lev[0];
     double nr[2];
     /* The first pre-smoothing sweep of the next V-cycle only reads phi (it
-     * writes phi2), so it is queued behind each convergence check and runs
-     * while the host waits for the norms; when the check stops the solve,
-     * phi is the answer and phi2 is simply not used. */
+     * writes phi2), so it is queued right behind the kernel that publishes
+     * each convergence check: it runs while the host reads the norms and
+     * queues the rest of the next V-cycle behind it.  When the check stops
+     * the solve, phi is the answer and phi2 is simply not used. */
     const bool spec = x->lev.size() > 1 && x->tail0 != 0 && x->in.mg_nu1 >= 1;
     fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
+    unsigned v = post_red(x, 2);
     bool pre = spec && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
-    int rc = read_red(x, 2, nr);
+    int rc = wait_red(x, v, 2, nr);
     if (rc) return rc;
     const double r0 = nr[0], bn = nr[1];
     const double target = tol * std::max(bn, r0);
@@ -869,8 +878,9 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     for (int it = 1; it <= maxit; ++it) {
         vcycle(x, 0, bc, pre);
         fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
+        v = post_red(x, 2);
         pre = spec && it < maxit && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
-        if ((rc = read_red(x, 2, nr))) return rc;
+        if ((rc = wait_red(x, v, 2, nr))) return rc;
         st->mg_resid = nr[0]; st->vcycles = it;
         if (!(nr[0] == nr[0])) return fail(x, FX_ENAN, "multigrid produced a non-finite residual");
         if (nr[0] <= target) return FX_OK;
@@ -879,18 +889,24 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
 }
 
 /* Mixed precision: FP64 iterative refinement around FP32 V-cycles.
- * r0 = b - A phi is formed in FP64 and stored in FP32.  While the residual is
- * still far above the target, several FP32 V-cycles run back to back on the
- * same correction equation (an FP32 V-cycle cuts the error by ~0.1-0.15, so
- * the conservative count n_A = floor(log50(r/target)) - 1 cannot reach the
- * target); then phi += e in FP64, the FP64 residual is formed again and the
- * loop continues with one FP32 V-cycle per refinement, checking the same
- * stopping rule as mg_solve after every refinement. */
+ * r = b - A phi is formed in FP64 and stored in FP32; n_A FP32 V-cycles run
+ * back to back on the correction equation A e = r; then phi += e in FP64, the
+ * FP64 residual is formed again and the same stopping rule as mg_solve is
+ * checked after every refinement.  n_A is the number of cycles that cannot
+ * overshoot the target: the first refinement assumes a cycle cuts the
+ * residual by at most 50x (floor(log50(r/target)) - 1); later ones use the
+ * contraction per cycle measured on the refinement before, rounded down, and
+ * stop at the FP32 limit (a correction solved to 1e-6 of its right-hand
+ * side).  As in mg_solve, the first sweep of the next correction solve is
+ * queued right behind the kernel that publishes each check. */
 int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
     const DBC bc = dbc_of(x, bv);
     double nr[2];
+    const bool spec = x->lev32.size() > 1 && x->tail0 != 0 && x->in.mg_nu1 >= 1;
     fxk::resid_to32(x->s, x->lev[0].d, bc, x->lev32[0].b, x->red, x->red_out);
-    int rc = read_red(x, 2, nr);
+    unsigned v = post_red(x, 2);
+    bool pre = spec && fxk::mg_sweep_first(x->s, x->lev32[0], bc, x->in.mg_fused_sweep);
+    int rc = wait_red(x, v, 2, nr);
     if (rc) return rc;
     const double r0 = nr[0], bn = nr[1];
     const double target = tol * std::max(bn, r0);
@@ -898,19 +914,28 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
     if (!(r0 == r0) || !(bn == bn)) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
     if (r0 <= target) return FX_OK;
     int cycles = 0;
-    double r = r0;
+    double r = r0, rho = 0.0;   /* measured contraction per FP32 cycle (0: none yet) */
     while (cycles < maxit) {
         int nA = 1;
         if (r > 0.0 && target > 0.0) {
-            const double q = std::log(r / target) / std::log(50.0);
-            nA = std::max(1, (int)std::floor(q) - 1);
+            if (rho > 0.0 && rho < 1.0) {
+                const double need = std::log(r / target) / -std::log(rho);
+                const double cap = std::log(1e6) / -std::log(rho);
+                nA = std::max(1, (int)std::floor(std::min(need, cap)));
+            } else {
+                const double q = std::log(r / target) / std::log(50.0);
+                nA = std::max(1, (int)std::floor(q) - 1);
+            }
         }
         nA = std::min(nA, maxit - cycles);
-        for (int c = 0; c < nA; ++c) vcycle32(x, 0, bc, c == 0);
+        for (int c = 0; c < nA; ++c) vcycle32(x, 0, bc, c == 0, pre && c == 0);
         cycles += nA;
         fxk::add_f2d(x->s, x->phi, x->lev32[0].phi, (long long)x->N);
         fxk::resid_to32(x->s, x->lev[0].d, bc, x->lev32[0].b, x->red, x->red_out);
-        if ((rc = read_red(x, 2, nr))) return rc;
+        v = post_red(x, 2);
+        pre = spec && cycles < maxit && fxk::mg_sweep_first(x->s, x->lev32[0], bc, x->in.mg_fused_sweep);
+        if ((rc = wait_red(x, v, 2, nr))) return rc;
+        if (r > 0.0 && nr[0] > 0.0) rho = std::pow(nr[0] / r, 1.0 / nA);
         r = nr[0];
         st->mg_resid = r; st->vcycles = cycles;
         if (!(r == r)) return fail(x, FX_ENAN, "mixed-precision multigrid produced a non-finite residual");

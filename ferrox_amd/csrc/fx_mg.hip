@@ -2301,6 +2301,19 @@ void mg_sweep_rest(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsw
 void mg_sweep(hipStream_t s, const DLevF& l, const DBC& bc, int nsweeps, int zero_first, int allow_fused) {
     sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, zero_first);
 }
+bool mg_sweep_first(hipStream_t s, const DLevF& l, const DBC& bc, int allow_fused) {
+    if (allow_fused != 1 || !stream_use(l, bc, allow_fused)) return false;
+    sweep_impl<float>(s, l, bc, 0, 1, allow_fused, 1, nullptr, false, true);
+    return true;
+}
+void mg_sweep_rest(hipStream_t s, const DLevF& l, const DBC& bc, int nsweeps, int allow_fused) {
+    if (nsweeps <= 0) {
+        const long long n = (long long)l.n0 * l.n1 * l.n2;
+        hipLaunchKernelGGL(k_copyT<float>, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, s, l.phi, l.phi2, n);
+        return;
+    }
+    sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, 0, nullptr, true, false);
+}
 void mg_prolong_sweep(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, int inh, int nsweeps, int allow_fused) {
     sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, 0, &c);
 }
