@@ -350,7 +350,7 @@ void fill_plan(fxd::TailPlan<T>& P, const fx_ctx* x, int t0, F lev, const T* lu,
     }
     P.b0 = lev(t0).b;
     P.phi0 = lev(t0).phi;
-    P.lu = lu;
+    P.binv = lu;
     P.perm = perm;
 }
 

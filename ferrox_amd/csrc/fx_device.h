@@ -280,7 +280,7 @@ struct TailPlan {
     const T* a[TAIL_LEAN_MAXL];
     const T* b0;
     T* phi0;
-    const T* lu;
+    const T* binv;   /* the bottom operator's inverse, column-major (bottom_factor) */
     const int* perm;
 };
 

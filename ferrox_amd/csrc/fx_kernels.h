@@ -145,7 +145,7 @@ size_t tail_level_values(const fxd::DLev& l);
 /* bottom solves of more cells than this use Jacobi-PCG, below it the exact dense solve */
 constexpr int BOTTOM_DIRECT_CELLS = 64;
 /* the sub-V-cycle of a layered hierarchy's coarse levels in one workgroup,
- * every level in LDS; lu/perm from bottom_factor */
+ * every level in LDS; the bottom's inverse and perm from bottom_factor */
 void mg_tail_lay(hipStream_t s, const fxd::TailPlan<double>& P, const fxd::DBC& bc, int read_a, int nu1, int nu2,
                  double* work, double tol, int maxit, int* iters);
 void mg_tail_lay(hipStream_t s, const fxd::TailPlan<float>& P, const fxd::DBC& bc, int read_a, int nu1, int nu2,
@@ -165,10 +165,11 @@ void mg_block_up(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, const fx
 void mg_block_up(hipStream_t s, const fxd::DLevF& l, const fxd::DLevF& c, const fxd::DBC& bc, int nu1, int nu2, float* out);
 /* dynamic LDS bytes of mg_tail_lay for levels of extents n[0..nl-1] */
 size_t tail_lay_lds(const int (*n)[3], int nl, int read_a, size_t elem);
-/* LU factors of the bottom operator (<= BOTTOM_DIRECT_CELLS cells): lu N*N
- * values column-major, perm N+1 ints (perm[N] = 1 when non-singular) */
-void bottom_factor(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, double* lu, int* perm);
-void bottom_factor(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, float* lu, int* perm);
+/* Inverse of the bottom operator (<= BOTTOM_DIRECT_CELLS cells) through its
+ * LU factors with partial pivoting: inv N*N values column-major, perm N+1
+ * ints (the pivot order; perm[N] = 1 when non-singular) */
+void bottom_factor(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, double* inv, int* perm);
+void bottom_factor(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, float* inv, int* perm);
 /* r32 = (float)(b - A phi) in FP64 (inhomogeneous BCs); out[0] = max|r|, out[1] = max|b| */
 void resid_to32(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, float* r32, double* part, double* out);
 /* ------------------------------- adjoint ---------------------------------- */

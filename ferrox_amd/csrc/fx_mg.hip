@@ -1785,16 +1785,19 @@ __global__ __launch_bounds__(BLK_THREADS) void k_mg_block_up(BlkArgs<T> A) {
  * largest |pivot|), the multipliers f = M[i][k]/M[k][k] kept below the
  * diagonal and whole rows swapped, so that a solve applies the same
  * operations to any right-hand side: y = rhs permuted, y_i -= f_ik y_k in
- * order of k, then the column-oriented back substitution.  One workgroup;
- * the pivot search is one wavefront's reduction.  Output: lu column-major
- * (lu[col*N + row]); perm[r] = the original row now at r; perm[N] = 1, or 0
+ * order of k, then the column-oriented back substitution.  That solve is
+ * applied to every unit vector at once, which gives the explicit inverse the
+ * tail applies as one matrix-vector product.  One workgroup; the pivot search
+ * is one wavefront's reduction.  Output: inv = A^-1 column-major
+ * (inv[col*N + row]); perm[r] = the original row now at r; perm[N] = 1, or 0
  * when a pivot falls below a small fraction of the largest diagonal entry
  * (singular operator: the solve then uses Jacobi-PCG). */
 template <class T>
-__global__ __launch_bounds__(1024) void k_bottom_factor(DLevT<T> l, DBC bc, T* __restrict__ lu, int* __restrict__ perm) {
+__global__ __launch_bounds__(1024) void k_bottom_factor(DLevT<T> l, DBC bc, T* __restrict__ inv, int* __restrict__ perm) {
     constexpr int BS = 1024;
     constexpr int NM = BOTTOM_DIRECT_MAX;
     __shared__ T M[NM * NM];
+    __shared__ T X[NM * NM];
     __shared__ int sp[NM];
     __shared__ int piv_sh, ok_sh;
     __shared__ T dmax_sh;
@@ -1856,20 +1859,60 @@ __global__ __launch_bounds__(1024) void k_bottom_factor(DLevT<T> l, DBC bc, T* _
         }
         __syncthreads();
     }
-    for (int q = tid; q < N * N; q += BS) lu[q] = M[(q % N) * N + q / N];
+    if (ok) {
+        /* the columns of A^-1: every unit vector through the solve above at once */
+        for (int idx = tid; idx < N * N; idx += BS) X[idx] = sp[idx / N] == idx % N ? T(1) : T(0);
+        __syncthreads();
+        for (int kk = 0; kk + 1 < N; ++kk) {
+            for (int idx = tid; idx < (N - kk - 1) * N; idx += BS) {
+                const int i = kk + 1 + idx / N, c = idx % N;
+                X[i * N + c] = X[i * N + c] - M[i * N + kk] * X[kk * N + c];
+            }
+            __syncthreads();
+        }
+        for (int jj = N - 1; jj >= 0; --jj) {
+            for (int c = tid; c < N; c += BS) X[jj * N + c] = X[jj * N + c] / M[jj * N + jj];
+            __syncthreads();
+            for (int idx = tid; idx < jj * N; idx += BS) {
+                const int i = idx / N, c = idx % N;
+                X[i * N + c] = X[i * N + c] - M[i * N + jj] * X[jj * N + c];
+            }
+            __syncthreads();
+        }
+    }
+    for (int q = tid; q < N * N; q += BS) inv[q] = X[(q % N) * N + q / N];
     for (int r = tid; r < N; r += BS) perm[r] = sp[r];
     if (tid == 0) perm[N] = ok;
 }
 
 /* ---------------- coarse tail of a layered hierarchy (lean) ---------------- */
 namespace {
-constexpr int TAIL_NPW = 6;   /* per-plane table: wx wy wzl wzh D 1/D */
+/* per-plane table: face weights beta/h^2 (wx, wy), the z weights applied to
+ * the neighbours (0 across a closed z face) and to the diagonal (2w across a
+ * Dirichlet face, 0 across a Neumann one), the diagonal D accumulated in the
+ * row operator's order, and 1/D */
+constexpr int TAIL_NPW = 8;
+enum { TW_X = 0, TW_Y, TW_ZLN, TW_ZHN, TW_ZLD, TW_ZHD, TW_D, TW_ID };
+constexpr int TAIL_CPT = 4;   /* cells per thread and pass of a phase (loads of all of them first) */
 struct TLev {
     int n0, n1, n2, s, n, half;
     int cx, cy, cz;
     int ophi, ob, oa, otab;   /* offsets (in values) into the dynamic LDS */
     float r0, rs, rh, r1;     /* 1/n0, 1/(n0 n1), 1/half, 1/n1 for index splits */
 };
+__device__ __forceinline__ TLev tail_level(const int* n, const int* crs, int read_a, int& o) {
+    TLev L;
+    L.n0 = n[0]; L.n1 = n[1]; L.n2 = n[2];
+    L.s = L.n0 * L.n1; L.n = L.s * L.n2; L.half = (L.n0 + 1) >> 1;
+    L.cx = crs[0]; L.cy = crs[1]; L.cz = crs[2];
+    L.ophi = o; o += L.n;
+    L.ob = o; o += L.n;
+    L.oa = o; if (read_a) o += L.n;
+    L.otab = o; o += TAIL_NPW * L.n2;
+    L.r0 = 1.0f / (float)L.n0; L.rs = 1.0f / (float)L.s;
+    L.rh = 1.0f / (float)L.half; L.r1 = 1.0f / (float)L.n1;
+    return L;
+}
 }  // namespace
 
 /* Values of dynamic LDS the lean tail uses for levels of the given extents. */
@@ -1886,15 +1929,20 @@ __host__ inline size_t tail_lean_values(const int (*n)[3], int nl, int read_a) {
 
 /* The sub-V-cycle below a level of a layered hierarchy in one 1024-thread
  * workgroup, every level held in LDS: phi, b, the Helmholtz term (read_a) and
- * a per-plane table of the face weights beta/h^2, the interior diagonal
- * (accumulated in the row operator's order) and its inverse.  Buffers are
- * addressed as offsets from the dynamic LDS base, so every access is an LDS
- * instruction.  Per cell: the interior fast path of the streaming sweep
- * (multiply by the per-plane 1/D when there is no Helmholtz term), the row
- * operator elsewhere.  Restriction, prolongation and the sweep order are
- * those of k_mg_tail; the bottom is solved with the LU factors of
- * k_bottom_factor by one wavefront (lanes own rows), or by Jacobi-PCG when
- * the bottom is singular or too large. */
+ * the per-plane table.  Buffers are addressed as offsets from the dynamic LDS
+ * base, so every access is an LDS instruction.
+ *  - Staging is one round of global loads (every thread issues all of its
+ *    loads before its first LDS store).
+ *  - A cell away from every non-periodic lateral face takes the per-plane
+ *    table (z faces included: the weights there are the row operator's) and,
+ *    without a Helmholtz term, multiplies by the per-plane 1/D; other cells
+ *    run the row operator.
+ *  - Each phase loads the neighbours of up to TAIL_CPT cells per thread
+ *    before it updates any of them (the cells a phase writes are never read
+ *    in that phase).
+ *  - Bottom: the explicit inverse from k_bottom_factor, one lane per row, or
+ *    Jacobi-PCG when the bottom is singular or too large.
+ * Restriction, prolongation and the sweep order are those of k_mg_tail. */
 template <class T>
 __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int read_a, int nu1, int nu2,
                                                       T* __restrict__ gwork, double tol, int maxit,
@@ -1903,58 +1951,83 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
     extern __shared__ __align__(16) unsigned char tail_lds[];
     T* const V = reinterpret_cast<T*>(tail_lds);
     __shared__ TLev sl[TAIL_LEAN_MAXL];
-    __shared__ int sperm[BOTTOM_DIRECT_MAX + 1];
-    __shared__ int s_olu;
     const int tid = threadIdx.x;
     const int nl = P.nl;
-    if (tid == 0) {
-        int o = 0;
-        for (int t = 0; t < nl; ++t) {
-            TLev L;
-            L.n0 = P.n[t][0]; L.n1 = P.n[t][1]; L.n2 = P.n[t][2];
-            L.s = L.n0 * L.n1; L.n = L.s * L.n2; L.half = (L.n0 + 1) >> 1;
-            L.cx = P.crs[t][0]; L.cy = P.crs[t][1]; L.cz = P.crs[t][2];
-            L.ophi = o; o += L.n;
-            L.ob = o; o += L.n;
-            L.oa = o; if (read_a) o += L.n;
-            L.otab = o; o += TAIL_NPW * L.n2;
-            L.r0 = 1.0f / (float)L.n0; L.rs = 1.0f / (float)L.s;
-            L.rh = 1.0f / (float)L.half; L.r1 = 1.0f / (float)L.n1;
-            sl[t] = L;
-        }
-        s_olu = o;
-    }
-    __syncthreads();
-    /* staging: per-plane tables, Helmholtz terms, the first level's rhs, the bottom factors */
-    for (int t = 0; t < nl; ++t) {
-        const TLev& L = sl[t];
-        for (int e = tid; e < L.n2; e += BS) {
-            const T wx = P.bxk[t][e] * P.ih2[t][0], wy = P.byk[t][e] * P.ih2[t][1];
-            const T wzl = P.bzk[t][e] * P.ih2[t][2], wzh = P.bzk[t][e + 1] * P.ih2[t][2];
-            T D = T(0);
-            D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh;
-            T* tb = V + L.otab + TAIL_NPW * e;
-            tb[0] = wx; tb[1] = wy; tb[2] = wzl; tb[3] = wzh; tb[4] = D; tb[5] = T(1) / D;
-        }
-        if (read_a)
-            for (int c = tid; c < L.n; c += BS) V[L.oa + c] = P.a[t][c];
-    }
-    {
-        const TLev& L = sl[0];
-        for (int c = tid; c < L.n; c += BS) { V[L.ob + c] = P.b0[c]; V[L.ophi + c] = T(0); }
-        const int N = sl[nl - 1].n;
-        if (N <= BOTTOM_DIRECT_MAX) {
-            for (int q = tid; q < N * N; q += BS) V[s_olu + q] = P.lu[q];
-            for (int q = tid; q <= N; q += BS) sperm[q] = P.perm[q];
-        }
-    }
-    __syncthreads();
     const bool perx = bc.lo[0] == BC_PER, pery = bc.lo[1] == BC_PER, perz = bc.lo[2] == BC_PER;
-    /* (A phi)_c = D phi_c - S of cell (i,j,k) of level L */
+
+    /* ---- staging: the layout is computed by every thread, then one round of loads ---- */
+    int o = 0, ntab = 0;
+    TLev L0, Lb;
+    for (int t = 0; t < nl; ++t) {
+        const TLev L = tail_level(P.n[t], P.crs[t], read_a, o);
+        if (tid == 0) sl[t] = L;
+        if (t == 0) L0 = L;
+        Lb = L;
+        ntab += L.n2;
+    }
+    const int oinv = o;
+    const int NB = Lb.n;
+    const bool direct = NB <= BOTTOM_DIRECT_MAX;
+    /* the table entry of this thread (level tt, plane te) */
+    int tt = -1, te = 0;
+    for (int t = 0, acc = 0; t < nl; ++t) {
+        if (tid >= acc && tid < acc + P.n[t][2]) { tt = t; te = tid - acc; }
+        acc += P.n[t][2];
+    }
+    T gx = T(0), gy = T(0), gzl = T(0), gzh = T(0);
+    if (tt >= 0) {
+        gx = P.bxk[tt][te]; gy = P.byk[tt][te]; gzl = P.bzk[tt][te]; gzh = P.bzk[tt][te + 1];
+    }
+    T rb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) { const int c = tid + u * BS; rb[u] = c < L0.n ? P.b0[c] : T(0); }
+    T ri[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { const int q = tid + u * BS; ri[u] = (direct && q < NB * NB) ? P.binv[q] : T(0); }
+    const int okb = direct ? P.perm[NB] : 0;
+    /* stores */
+    auto put_table = [&](int tl, int e, T bx, T by, T bzl, T bzh) {
+        const T wx = bx * P.ih2[tl][0], wy = by * P.ih2[tl][1];
+        const T wzl = bzl * P.ih2[tl][2], wzh = bzh * P.ih2[tl][2];
+        const int n2 = P.n[tl][2];
+        const bool lo_open = e > 0 || perz, hi_open = e < n2 - 1 || perz;
+        const T zln = lo_open ? wzl : T(0), zhn = hi_open ? wzh : T(0);
+        const T zld = lo_open ? wzl : (bc.lo[2] == BC_DIR ? T(2) * wzl : T(0));
+        const T zhd = hi_open ? wzh : (bc.hi[2] == BC_DIR ? T(2) * wzh : T(0));
+        T D = T(0);
+        D += wx; D += wx; D += wy; D += wy; D += zld; D += zhd;
+        int ot = 0;
+        for (int q = 0, oo = 0; q <= tl; ++q) ot = tail_level(P.n[q], P.crs[q], read_a, oo).otab;
+        T* tb = V + ot + TAIL_NPW * e;
+        tb[TW_X] = wx; tb[TW_Y] = wy; tb[TW_ZLN] = zln; tb[TW_ZHN] = zhn; tb[TW_ZLD] = zld; tb[TW_ZHD] = zhd;
+        tb[TW_D] = D; tb[TW_ID] = D != T(0) ? T(1) / D : T(0);
+    };
+    if (tt >= 0) put_table(tt, te, gx, gy, gzl, gzh);
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        const int c = tid + u * BS;
+        if (c < L0.n) { V[L0.ob + c] = rb[u]; V[L0.ophi + c] = T(0); }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { const int q = tid + u * BS; if (direct && q < NB * NB) V[oinv + q] = ri[u]; }
+    for (int c = tid + 8 * BS; c < L0.n; c += BS) { V[L0.ob + c] = P.b0[c]; V[L0.ophi + c] = T(0); }
+    for (int q = tid + BS; q < ntab; q += BS) {   /* more planes than threads (never in practice) */
+        int t2 = 0, e2 = q;
+        while (e2 >= P.n[t2][2]) { e2 -= P.n[t2][2]; ++t2; }
+        put_table(t2, e2, P.bxk[t2][e2], P.byk[t2][e2], P.bzk[t2][e2], P.bzk[t2][e2 + 1]);
+    }
+    if (read_a) {
+        for (int t = 0, oo = 0; t < nl; ++t) {
+            const TLev L = tail_level(P.n[t], P.crs[t], read_a, oo);
+            for (int c = tid; c < L.n; c += BS) V[L.oa + c] = P.a[t][c];
+        }
+    }
+    __syncthreads();
+
+    /* (A phi)_c = D phi_c - S of cell (i,j,k) of level L; true on the table path */
     auto row = [&](const TLev& L, int i, int j, int k, int c, T& D, T& S) -> bool {
         const T* phi = V + L.ophi;
         const T* tb = V + L.otab + TAIL_NPW * k;
-        const T wx = tb[0], wy = tb[1], wzl = tb[2], wzh = tb[3];
         const T v0 = phi[i > 0 ? c - 1 : c + (L.n0 - 1)];
         const T v1 = phi[i < L.n0 - 1 ? c + 1 : c - (L.n0 - 1)];
         const T v2 = phi[j > 0 ? c - L.n0 : c + L.n0 * (L.n1 - 1)];
@@ -1962,13 +2035,13 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
         const T v4 = phi[k > 0 ? c - L.s : c + L.s * (L.n2 - 1)];
         const T v5 = phi[k < L.n2 - 1 ? c + L.s : c - L.s * (L.n2 - 1)];
         const T av = read_a ? V[L.oa + c] : T(0);
-        const bool in = (perx || (i > 0 && i < L.n0 - 1)) && (pery || (j > 0 && j < L.n1 - 1)) &&
-                        (perz || (k > 0 && k < L.n2 - 1));
-        if (in) {
+        const T wx = tb[TW_X], wy = tb[TW_Y];
+        const bool lat = (perx || (i > 0 && i < L.n0 - 1)) && (pery || (j > 0 && j < L.n1 - 1));
+        if (lat) {
             S = T(0);
-            S += wx * v0; S += wx * v1; S += wy * v2; S += wy * v3; S += wzl * v4; S += wzh * v5;
-            if (read_a) { D = -av; D += wx; D += wx; D += wy; D += wy; D += wzl; D += wzh; }
-            else D = tb[4];
+            S += wx * v0; S += wx * v1; S += wy * v2; S += wy * v3; S += tb[TW_ZLN] * v4; S += tb[TW_ZHN] * v5;
+            if (read_a) { D = -av; D += wx; D += wx; D += wy; D += wy; D += tb[TW_ZLD]; D += tb[TW_ZHD]; }
+            else D = tb[TW_D];
             return true;
         }
         /* the row operator's accumulation (homogeneous faces), face by face
@@ -1983,27 +2056,42 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
         face(i < L.n0 - 1 || perx, bc.hi[0], wx, v1);
         face(j > 0 || pery, bc.lo[1], wy, v2);
         face(j < L.n1 - 1 || pery, bc.hi[1], wy, v3);
-        face(k > 0 || perz, bc.lo[2], wzl, v4);
-        face(k < L.n2 - 1 || perz, bc.hi[2], wzh, v5);
+        /* z faces: the table holds the row operator's terms */
+        D += tb[TW_ZLD]; S += tb[TW_ZLN] * v4;
+        D += tb[TW_ZHD]; S += tb[TW_ZHN] * v5;
         return false;
     };
     auto half_sweep = [&](const TLev& L, int color) {
         const int tot = L.half * L.n1 * L.n2;
-        for (int q = tid; q < tot; q += BS) {
-            const int jk = qdiv(q, L.half, L.rh);
-            const int ih = q - jk * L.half;
-            const int k = qdiv(jk, L.n1, L.r1);
-            const int j = jk - k * L.n1;
-            const int i = 2 * ih + ((j + k + color) & 1);
-            if (i >= L.n0) continue;
-            const int c = i + L.n0 * (j + L.n1 * k);
-            T D, S;
-            const bool in = row(L, i, j, k, c, D, S);
-            const T bv = V[L.ob + c];
-            V[L.ophi + c] = (in && !read_a) ? (bv + S) * V[L.otab + TAIL_NPW * k + 5] : (bv + S) / D;
+        for (int q0 = tid; q0 < tot; q0 += TAIL_CPT * BS) {
+            T nv[TAIL_CPT];
+            int cc[TAIL_CPT];
+#pragma unroll
+            for (int u = 0; u < TAIL_CPT; ++u) {
+                const int q = q0 + u * BS;
+                cc[u] = -1;
+                nv[u] = T(0);
+                if (q >= tot) continue;
+                const int jk = qdiv(q, L.half, L.rh);
+                const int ih = q - jk * L.half;
+                const int k = qdiv(jk, L.n1, L.r1);
+                const int j = jk - k * L.n1;
+                const int i = 2 * ih + ((j + k + color) & 1);
+                if (i >= L.n0) continue;
+                const int c = i + L.n0 * (j + L.n1 * k);
+                T D, S;
+                const bool in = row(L, i, j, k, c, D, S);
+                const T bv = V[L.ob + c];
+                nv[u] = (in && !read_a) ? (bv + S) * V[L.otab + TAIL_NPW * k + TW_ID] : (bv + S) / D;
+                cc[u] = c;
+            }
+#pragma unroll
+            for (int u = 0; u < TAIL_CPT; ++u)
+                if (cc[u] >= 0) V[L.ophi + cc[u]] = nv[u];
         }
         __syncthreads();
     };
+    /* residual restricted: one thread per coarse cell, children summed z outer, x inner */
     auto restrict_to = [&](const TLev& F, const TLev& C) {
         for (int t = tid; t < C.n; t += BS) {
             const int K = qdiv(t, C.s, C.rs);
@@ -2028,39 +2116,51 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
     auto prolong_from = [&](const TLev& F, const TLev& C) {
         const int crs[3] = {F.cx, F.cy, F.cz};
         const int cn[3] = {C.n0, C.n1, C.n2};
-        for (int t = tid; t < F.n; t += BS) {
-            const int k = qdiv(t, F.s, F.rs);
-            const int rr = t - k * F.s;
-            const int j = qdiv(rr, F.n0, F.r0);
-            const int fi[3] = {rr - j * F.n0, j, k};
-            int id[3][2];
-            T w[3][2];
-            int cnt[3];
+        for (int t0 = tid; t0 < F.n; t0 += TAIL_CPT * BS) {
+            T ev[TAIL_CPT];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                if (!crs[d]) { id[d][0] = fi[d]; id[d][1] = fi[d]; w[d][0] = T(1); w[d][1] = T(0); cnt[d] = 1; continue; }
-                const int I = fi[d] >> 1;
-                int Jn = (fi[d] & 1) ? I + 1 : I - 1;
-                T sg = T(1);
-                if (Jn < 0 || Jn >= cn[d]) {
-                    const int bct = Jn < 0 ? bc.lo[d] : bc.hi[d];
-                    if (bct == BC_PER) Jn = Jn < 0 ? Jn + cn[d] : Jn - cn[d];
-                    else { sg = (bct == BC_DIR) ? T(-1) : T(1); Jn = I; }
+            for (int u = 0; u < TAIL_CPT; ++u) {
+                const int t = t0 + u * BS;
+                ev[u] = T(0);
+                if (t >= F.n) continue;
+                const int k = qdiv(t, F.s, F.rs);
+                const int rr = t - k * F.s;
+                const int j = qdiv(rr, F.n0, F.r0);
+                const int fi[3] = {rr - j * F.n0, j, k};
+                int id[3][2];
+                T w[3][2];
+                int cnt[3];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    if (!crs[d]) { id[d][0] = fi[d]; id[d][1] = fi[d]; w[d][0] = T(1); w[d][1] = T(0); cnt[d] = 1; continue; }
+                    const int I = fi[d] >> 1;
+                    int Jn = (fi[d] & 1) ? I + 1 : I - 1;
+                    T sg = T(1);
+                    if (Jn < 0 || Jn >= cn[d]) {
+                        const int bct = Jn < 0 ? bc.lo[d] : bc.hi[d];
+                        if (bct == BC_PER) Jn = Jn < 0 ? Jn + cn[d] : Jn - cn[d];
+                        else { sg = (bct == BC_DIR) ? T(-1) : T(1); Jn = I; }
+                    }
+                    id[d][0] = I; w[d][0] = T(0.75);
+                    id[d][1] = Jn; w[d][1] = T(0.25) * sg;
+                    cnt[d] = 2;
                 }
-                id[d][0] = I; w[d][0] = T(0.75);
-                id[d][1] = Jn; w[d][1] = T(0.25) * sg;
-                cnt[d] = 2;
+                T e = T(0);
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+#pragma unroll
+                        for (int q = 0; q < 2; ++q)
+                            if (a < cnt[2] && b < cnt[1] && q < cnt[0])
+                                e += (w[2][a] * w[1][b]) * w[0][q] * V[C.ophi + id[0][q] + C.n0 * (id[1][b] + C.n1 * id[2][a])];
+                ev[u] = e;
             }
-            T e = T(0);
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-                for (int b = 0; b < 2; ++b)
-#pragma unroll
-                    for (int q = 0; q < 2; ++q)
-                        if (a < cnt[2] && b < cnt[1] && q < cnt[0])
-                            e += (w[2][a] * w[1][b]) * w[0][q] * V[C.ophi + id[0][q] + C.n0 * (id[1][b] + C.n1 * id[2][a])];
-            V[F.ophi + t] += e;
+            for (int u = 0; u < TAIL_CPT; ++u) {
+                const int t = t0 + u * BS;
+                if (t < F.n) V[F.ophi + t] += ev[u];
+            }
         }
         __syncthreads();
     };
@@ -2071,22 +2171,14 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
     {
         const TLev& L = sl[nl - 1];
         const int N = L.n;
-        if (N <= BOTTOM_DIRECT_MAX && sperm[N]) {
-            if (tid < 64) {
-                const T* lu = V + s_olu;
-                const int lane = tid;
-                T y = lane < N ? V[L.ob + sperm[lane]] : T(0);
-                for (int kk = 0; kk + 1 < N; ++kk) {
-                    const T ykk = __shfl(y, kk, 64);
-                    if (lane > kk && lane < N) y = y - lu[kk * N + lane] * ykk;
-                }
+        if (direct && okb) {
+            /* e = A^-1 b: lane i sums row i of the inverse (stored column-major) in column order */
+            if (tid < N) {
+                const T* inv = V + oinv;
+                const T* b = V + L.ob;
                 T e = T(0);
-                for (int jj = N - 1; jj >= 0; --jj) {
-                    const T ej = __shfl(y, jj, 64) / lu[jj * N + jj];
-                    if (lane == jj) e = ej;
-                    if (lane < jj) y = y - lu[jj * N + lane] * ej;
-                }
-                if (lane < N) V[L.ophi + lane] = e;
+                for (int jj = 0; jj < N; ++jj) e += inv[jj * N + tid] * b[jj];
+                V[L.ophi + tid] = e;
             }
             __syncthreads();
         } else {
