@@ -798,9 +798,13 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc, bool pre_done = false) {
         std::swap(L.d.phi, L.up);
         return;
     }
+    /* below the finest level phi starts at 0 (the restriction's), so the first
+     * sweep takes it as 0 without reading it; the restriction then need not
+     * clear the next level's phi unless the tail or the bottom solve reads it */
     if (pre_done) fxk::mg_sweep_rest(x->s, L.d, bc, inh, x->in.mg_nu1 - 1, x->in.mg_fused_sweep);
-    else fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu1, x->in.mg_fused_sweep);
-    fxk::mg_resid_restrict(x->s, L.d, C.d, bc, inh);
+    else fxk::mg_sweep(x->s, L.d, bc, inh, x->in.mg_nu1, x->in.mg_fused_sweep, li > 0 ? 1 : 0);
+    const bool plain_next = (int)(li + 1) != x->tail0 && li + 2 < x->lev.size();
+    fxk::mg_resid_restrict(x->s, L.d, C.d, bc, inh, !plain_next);
     vcycle(x, li + 1, bc);
     fxk::mg_prolong_sweep(x->s, L.d, C.d, bc, inh, x->in.mg_nu2, x->in.mg_fused_sweep);
 }
@@ -847,7 +851,8 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true, bool pre_d
     }
     if (pre_done) fxk::mg_sweep_rest(x->s, L, bc, x->in.mg_nu1 - 1, x->in.mg_fused_sweep);
     else fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
-    fxk::mg_resid_restrict(x->s, L, C, bc);
+    const bool plain_next = (int)(li + 1) != x->tail0 && li + 2 < x->lev32.size();
+    fxk::mg_resid_restrict(x->s, L, C, bc, !plain_next);
     vcycle32(x, li + 1, bc, true);
     fxk::mg_prolong_sweep(x->s, L, C, bc, x->in.mg_nu2, x->in.mg_fused_sweep);
 }

@@ -95,7 +95,8 @@ void heun(hipStream_t s, const HeunArgs& a);
 void compute_E(hipStream_t s, const EArgs& a);
 void mg_smooth(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int color);
 /* nsweeps full red-black sweeps (fused kernel where the level allows it) */
-void mg_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused);
+void mg_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused,
+              int zero_first = 0);
 /* prolongation of the coarse correction c into l followed by nsweeps sweeps
  * (the prolongation folded into the first one-pass sweep where it applies) */
 void mg_prolong_sweep(hipStream_t s, const fxd::DLev& l, const fxd::DLev& c, const fxd::DBC& bc, int inh, int nsweeps,
@@ -109,7 +110,10 @@ bool fused_ok(const fxd::DLev& l, const fxd::DBC& bc);
 bool mg_sweep_first(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int allow_fused);
 void mg_sweep_rest(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused);
 bool fused_ok32(const fxd::DLevF& l, const fxd::DBC& bc);
-void mg_resid_restrict(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc, int inh);
+/* residual of f restricted into c's rhs; c's phi is set to 0 unless zero_cphi
+ * is false (the caller's next use of c takes its phi as 0 without reading it) */
+void mg_resid_restrict(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc, int inh,
+                       bool zero_cphi = true);
 void mg_prolong(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const fxd::DBC& bc);
 void mg_restrict_cells(hipStream_t s, const fxd::DLev& f, const fxd::DLev& c, const double* src, double* dst);
 /* r = b - A phi (r may be null); out[0] = max|r|, out[1] = max|b|; part holds
@@ -126,7 +130,7 @@ void mg_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int nsweep
  * and the remaining sweeps from phi2 (as the FP64 pair above) */
 bool mg_sweep_first(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int allow_fused);
 void mg_sweep_rest(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int nsweeps, int allow_fused);
-void mg_resid_restrict(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc);
+void mg_resid_restrict(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc, bool zero_cphi = true);
 void mg_prolong(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc);
 void mg_restrict_cells(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const float* src, float* dst);
 void bottom_cg(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, const float* rhs, float* e, float* work,

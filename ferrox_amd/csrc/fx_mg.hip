@@ -254,8 +254,10 @@ __global__ __launch_bounds__(SGeo<T>::THR) void k_mg_gsrb_stream(StreamArgs<T> A
         constexpr int PAR = decltype(par_tag)::value;
         const int kf = min(max(p + 1, 0), n2 - 1);
         const int kof = plane * kf;
+        if (!A.zero_init) {   /* zero_init: the old phi is taken as 0 and never read */
 #pragma unroll
-        for (int s = 0; s < SLD; ++s) R.phi[s] = A.phi_in[(goff[s] >= 0 ? goff[s] : 0) + kof];
+            for (int s = 0; s < SLD; ++s) R.phi[s] = A.phi_in[(goff[s] >= 0 ? goff[s] : 0) + kof];
+        }
         /* plane p+1 odd (= 2K+1) needs coarse planes K and K+1: K+1 is new */
         if (PROLONG && PAR == 0) coarse_plane(R.cn, ((p + 1) >> 1) + 1);
         const int kr = min(max(p, 0), n2 - 1), kb = min(max(p - 1, 0), n2 - 1);
@@ -690,7 +692,7 @@ struct ResidArgs {
     int inh;
     int ntx, nty, cz, remap;
     T* __restrict__ cb;          /* MODE 1: rhs of the z-coarsened level */
-    T* __restrict__ cphi;        /* MODE 1: its phi (set to 0) */
+    T* __restrict__ cphi;        /* MODE 1: its phi, set to 0 (null: left alone, the level's first sweep takes it as 0) */
     float* __restrict__ r32;     /* MODE 3: the residual in FP32 */
     double* __restrict__ part;   /* MODE 2, 3: max|r| and max|b| per workgroup */
 };
@@ -838,7 +840,7 @@ __global__ __launch_bounds__(256) void k_mg_resid_stream(ResidArgs<T> A) {
                     s += r;
                     const long long cc2 = cc[u] + (long long)plane * (k >> 1);
                     A.cb[cc2] = s / T(2);
-                    A.cphi[cc2] = T(0);
+                    if (A.cphi) A.cphi[cc2] = T(0);
                 }
             } else {
                 if (MODE == 3) A.r32[c] = (float)r;
@@ -2374,8 +2376,8 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
         hipLaunchKernelGGL(k_mg_smooth<T>, g, dim3(256), 0, s, l, bc, inh, 1, 0);
     }
 }
-void mg_sweep(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps, int allow_fused) {
-    sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, 0);
+void mg_sweep(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps, int allow_fused, int zero_first) {
+    sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, zero_first);
 }
 bool mg_sweep_first(hipStream_t s, const DLev& l, const DBC& bc, int inh, int allow_fused) {
     if (allow_fused != 1 || !stream_use(l, bc, allow_fused)) return false;
@@ -2448,12 +2450,14 @@ static void resid_stream_launch(hipStream_t s, const DLevT<T>& l, const ResidArg
 }
 
 template <class T>
-static void resid_restrict_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>& c, const DBC& bc, int inh) {
+static void resid_restrict_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>& c, const DBC& bc, int inh,
+                                bool zero_cphi) {
     const int zonly = f.crs[0] == 0 && f.crs[1] == 0 && f.crs[2] == 1 && c.n0 == f.n0 && c.n1 == f.n1 && (f.n2 % 2) == 0;
     if (zonly && resid_stream_ok(f, bc)) {
         int nblk;
         ResidArgs<T> A = resid_args(f, bc, inh, &nblk);
-        A.cb = c.b; A.cphi = c.phi;
+        A.cb = c.b;
+        A.cphi = zero_cphi ? c.phi : nullptr;
         resid_stream_launch<T, 1>(s, f, A, nblk);
         return;
     }
@@ -2487,11 +2491,11 @@ static void resid_restrict_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>
     }
 #undef RR
 }
-void mg_resid_restrict(hipStream_t s, const DLev& f, const DLev& c, const DBC& bc, int inh) {
-    resid_restrict_impl<double>(s, f, c, bc, inh);
+void mg_resid_restrict(hipStream_t s, const DLev& f, const DLev& c, const DBC& bc, int inh, bool zero_cphi) {
+    resid_restrict_impl<double>(s, f, c, bc, inh, zero_cphi);
 }
-void mg_resid_restrict(hipStream_t s, const DLevF& f, const DLevF& c, const DBC& bc) {
-    resid_restrict_impl<float>(s, f, c, bc, 0);
+void mg_resid_restrict(hipStream_t s, const DLevF& f, const DLevF& c, const DBC& bc, bool zero_cphi) {
+    resid_restrict_impl<float>(s, f, c, bc, 0, zero_cphi);
 }
 /* Block shape of one leg of level l (coarser level c): among 16x16, 16x8,
  * 8x8, 8x4 and 4x4 blocks that tile the level and whose LDS fits (down:
