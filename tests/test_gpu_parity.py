@@ -294,3 +294,27 @@ angle_theta.theta_function(x,y,z) = "15"
         e = relerr(g.get(name), o.get(name))
         print(name, e)
         assert e < TOL, name
+
+
+@pytest.mark.parametrize("tol", [1e-12, 1e-10])
+def test_throughput_workload_one_step(relerr, tol):
+    """One step of the 256x256x128 polycrystal with the solver's default
+    launch shapes: streamed sweeps on the three largest levels, block legs of
+    8x8 and 4x4 columns on the next two, the LDS-resident tail and the
+    explicit-inverse bottom.  At 1e-12 both sides converge far below the bar;
+    at the reference's 1e-10 they stop on the same V-cycle."""
+    inp = fx.from_text(configs.mfim_poly_256(nsteps=1), mg_tol_rel=tol)
+    f = configs.polycrystal_fields(inp, n_grains=400, fe_fraction=0.7, alpha_pe=1.0e9, seed=7)
+    g, o = fx.FerroX(inp), Oracle(inp)
+    configs.apply_fields(g, f)
+    configs.apply_fields(o, f)
+    assert g.levels() == o.levels()
+    sg, so = g.step(1), o.step(1)
+    print(tol, sg["total_vcycles"], so["total_vcycles"])
+    assert sg["total_vcycles"] == so["total_vcycles"]
+    for name in ("Pz", "Phi"):
+        e = relerr(g.get(name), o.get(name))
+        print(name, e)
+        assert e < (TOL if tol < 1e-11 else 1e-9), name
+    g.close()
+    o.close()
