@@ -306,6 +306,67 @@ __global__ void k_b_f2d(double* __restrict__ d, const float* __restrict__ s, lon
 
 constexpr int AXR = 16;        /* vectors per transform tile */
 
+/* One 16x16x4 matrix-core step per element type, C += A B with A 16x4 and B
+ * 4x16: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; of the 16x16
+ * result it holds four values, C[row(l, v)][l % 16] for v = 0..3 (the FP64
+ * instruction interleaves the rows, the FP32 one keeps four consecutive rows
+ * per lane group; both layouts were checked on the device by multiplying
+ * known matrices). */
+template <class T> struct Mfma16;
+template <> struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <> struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+/* The lateral transforms on the matrix cores: for the vectors of a 16-wide
+ * tile (16 rows j..j+15 of a device for AXIS 0; 16 consecutive i of plane k
+ * for AXIS 1) out_m = sum_q in_q M[q][m], as the 16x16 blocks
+ * C[m][j] = sum_q M[q][m] in_j[q], K = n in steps of 4.  Tile = (device,
+ * first vector, group of four 16-wide m blocks, plane); each of the four
+ * wavefronts of a workgroup takes one m block of the same vectors, so the
+ * vectors come from L2 once per group.  Every lateral extent is a multiple
+ * of 16 (fx_batch_create checks it; otherwise k_b_axis runs). */
+template <class T, int AXIS, int INV>
+__global__ __launch_bounds__(256) void k_b_axis_mfma(const BDev* __restrict__ devs, const BBasis* __restrict__ bases,
+                                                     const int4* __restrict__ tiles, int ntiles, const T* __restrict__ in,
+                                                     T* __restrict__ out) {
+    using MF = Mfma16<T>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int4 tl = tile_at(tiles, t);
+        const BDev& D = devs[tl.x];
+        const int nx = D.g.n[0], ny = D.g.n[1];
+        const BBasis& B = bases[AXIS == 0 ? D.bx : D.by];
+        const int n = B.n;
+        const int m0 = (tl.z * 4 + wave) * 16;
+        if (m0 >= n) continue;                       /* uniform in the wavefront */
+        const T* M;
+        if (sizeof(T) == 8) M = reinterpret_cast<const T*>(INV ? B.QT : B.Q);
+        else M = reinterpret_cast<const T*>(INV ? B.QT32 : B.Q32);
+        const int j = tl.y + li;                     /* this lane's vector */
+        const long long base = AXIS == 0 ? D.off + (long long)nx * j : D.off + j + (long long)nx * ny * tl.w;
+        const long long qs = AXIS == 0 ? 1 : nx;     /* stride between the elements of a vector */
+        typename MF::v4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll 4
+        for (int k0 = 0; k0 < n; k0 += 4) {
+            const int q = k0 + lk;
+            acc = MF::mma(M[(long long)q * n + m0 + li], in[base + qs * q], acc);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int m = m0 + MF::row(lane, v);
+            out[AXIS == 0 ? base + m : base + (long long)nx * m] = acc[v];
+        }
+    }
+}
+
 }  // namespace
 
 /* ------------------------------- host side -------------------------------- */
@@ -326,6 +387,7 @@ struct fx_batch {
     /* tile lists */
     int4 *t_cell = nullptr, *t_x = nullptr, *t_y = nullptr, *t_mode = nullptr;
     int n_cell = 0, n_x = 0, n_y = 0, n_mode = 0;
+    bool mfma = false;                   /* lateral transforms on the matrix cores (extents multiples of 16) */
     /* tuned launch shapes */
     int tx = 64;                         /* stencil tile width (height 256/tx) */
     int g_cell = 0, g_axis = 0, g_mode = 0;
@@ -375,9 +437,18 @@ static int build_fixed_tiles(fx_batch* b) {
     for (int d = 0; d < b->nd; ++d) {
         const Geom& g = b->hdev[d].g;
         const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-        for (int r = 0; r < ny * nz; r += AXR) tx.push_back(make_int4(d, r, 0, 0));
-        for (int k = 0; k < nz; ++k)
-            for (int i = 0; i < nx; i += AXR) ty.push_back(make_int4(d, i, k, 0));
+        if (b->mfma) {
+            /* (device, first vector, group of four 16-wide m blocks, plane) */
+            for (int r = 0; r < ny * nz; r += 16)
+                for (int g = 0; g < (nx + 63) / 64; ++g) tx.push_back(make_int4(d, r, g, 0));
+            for (int k = 0; k < nz; ++k)
+                for (int i = 0; i < nx; i += 16)
+                    for (int g = 0; g < (ny + 63) / 64; ++g) ty.push_back(make_int4(d, i, g, k));
+        } else {
+            for (int r = 0; r < ny * nz; r += AXR) tx.push_back(make_int4(d, r, 0, 0));
+            for (int k = 0; k < nz; ++k)
+                for (int i = 0; i < nx; i += AXR) ty.push_back(make_int4(d, i, k, 0));
+        }
         for (int m = 0; m < nx * ny; m += 256) tm.push_back(make_int4(d, m, 0, 0));
     }
     int rc;
@@ -418,10 +489,20 @@ static void launch_resid(fx_batch* b) {
     }
 }
 
-/* A^-1 src -> dst, with scratch t1 (all of type T, batch layout) */
+/* A^-1 src -> dst, with scratch t1, t2, scr (all of type T, batch layout);
+ * dst must not be t1: a transform never writes its own input (each
+ * wavefront reads whole vectors that other wavefronts are writing) */
 template <class T>
 static void spectral_solve(fx_batch* b, const T* src, T* t1, T* t2, T* scr, T* dst) {
     const unsigned gx = gsize(b->g_axis, b->n_x), gy = gsize(b->g_axis, b->n_y), gm = gsize(b->g_mode, b->n_mode);
+    if (b->mfma) {
+        hipLaunchKernelGGL((k_b_axis_mfma<T, 0, 0>), dim3(gx), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_x, b->n_x, src, t1);
+        hipLaunchKernelGGL((k_b_axis_mfma<T, 1, 0>), dim3(gy), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_y, b->n_y, t1, t2);
+        hipLaunchKernelGGL(k_b_tridiag<T>, dim3(gm), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_mode, b->n_mode, t2, scr);
+        hipLaunchKernelGGL((k_b_axis_mfma<T, 1, 1>), dim3(gy), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_y, b->n_y, t2, t1);
+        hipLaunchKernelGGL((k_b_axis_mfma<T, 0, 1>), dim3(gx), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_x, b->n_x, t1, dst);
+        return;
+    }
     hipLaunchKernelGGL((k_b_axis<T, AXR, 0, 0>), dim3(gx), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_x, b->n_x, src, t1);
     hipLaunchKernelGGL((k_b_axis<T, AXR, 1, 0>), dim3(gy), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_y, b->n_y, t1, t2);
     hipLaunchKernelGGL(k_b_tridiag<T>, dim3(gm), dim3(256), 0, b->s, b->ddev, b->dbas, b->t_mode, b->n_mode, t2, scr);
@@ -438,11 +519,11 @@ static void enqueue_step(fx_batch* b, int from) {
     launch_rhs(b, P);
     if (b->precision == FX_PREC_FP32_REFINE) {
         hipLaunchKernelGGL(k_b_d2f, dim3(flat_blocks(b->Ntot)), dim3(256), 0, b->s, b->r32, b->rhs, b->Ntot);
-        spectral_solve<float>(b, b->r32, b->f1, b->f2, b->fc, b->f1);
-        hipLaunchKernelGGL(k_b_f2d, dim3(flat_blocks(b->Ntot)), dim3(256), 0, b->s, b->phi, b->f1, b->Ntot, 0);
+        spectral_solve<float>(b, b->r32, b->f1, b->f2, b->fc, b->f2);
+        hipLaunchKernelGGL(k_b_f2d, dim3(flat_blocks(b->Ntot)), dim3(256), 0, b->s, b->phi, b->f2, b->Ntot, 0);
         launch_resid(b);
-        spectral_solve<float>(b, b->r32, b->f1, b->f2, b->fc, b->f1);
-        hipLaunchKernelGGL(k_b_f2d, dim3(flat_blocks(b->Ntot)), dim3(256), 0, b->s, b->phi, b->f1, b->Ntot, 1);
+        spectral_solve<float>(b, b->r32, b->f1, b->f2, b->fc, b->f2);
+        hipLaunchKernelGGL(k_b_f2d, dim3(flat_blocks(b->Ntot)), dim3(256), 0, b->s, b->phi, b->f2, b->Ntot, 1);
     } else {
         spectral_solve<double>(b, b->rhs, b->h1, b->h2, b->cp, b->phi);
     }
@@ -603,6 +684,9 @@ int fx_batch_create(int n, const fx_inputs* const* inputs, fx_batch** out) {
         return bail(rc);
     BCHK(b, hipMemcpy(b->dbas, hb.data(), hb.size() * sizeof(BBasis), hipMemcpyHostToDevice));
     BCHK(b, hipMemcpy(b->ddev, b->hdev.data(), n * sizeof(BDev), hipMemcpyHostToDevice));
+    b->mfma = true;
+    for (int d = 0; d < n; ++d)
+        if (b->hdev[d].g.n[0] % 16 || b->hdev[d].g.n[1] % 16) b->mfma = false;
     if ((rc = build_fixed_tiles(b)) || (rc = build_cell_tiles(b, 64))) return bail(rc);
     b->g_cell = b->g_axis = b->g_mode = 4 * b->ncu;
     if ((rc = capture(b))) return bail(rc);
@@ -720,8 +804,8 @@ int fx_batch_autotune(fx_batch* b, char* report, int report_len) {
             b->g_axis = fa * b->ncu;
             b->g_mode = fm * b->ncu;
             const double ms = time_ms([&] {
-                if (b->precision == FX_PREC_FP32_REFINE) spectral_solve<float>(b, b->r32, b->f1, b->f2, b->fc, b->f1);
-                else spectral_solve<double>(b, b->rhs, b->h1, b->h2, b->cp, b->h1);
+                if (b->precision == FX_PREC_FP32_REFINE) spectral_solve<float>(b, b->r32, b->f1, b->f2, b->fc, b->f2);
+                else spectral_solve<double>(b, b->rhs, b->h1, b->h2, b->cp, b->h2);
             });
             char buf[128];
             std::snprintf(buf, sizeof buf, "%s{\"axis_grid\": %d, \"mode_grid\": %d, \"ms\": %.5f}", first ? "" : ", ", b->g_axis,

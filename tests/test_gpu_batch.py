@@ -13,7 +13,9 @@ pytestmark = pytest.mark.gpu
 
 
 def small_texts(nsteps=10):
-    dims = [(32, 32, 28, 0.3), (48, 32, 30, -0.7), (32, 64, 36, 1.0), (64, 48, 44, -0.2)]
+    # the last device has extents above 64 (two groups of m blocks per
+    # transform tile on the matrix-core path)
+    dims = [(32, 32, 28, 0.3), (48, 32, 30, -0.7), (32, 64, 36, 1.0), (64, 48, 44, -0.2), (128, 96, 32, 0.5)]
     return [configs.mfim(nx=nx, ny=ny, nz=nz, lx=nx * 0.5e-9, ly=ny * 0.5e-9, lz=nz * 0.25e-9, de_top=4e-9,
                          fe_top=nz * 0.25e-9, nsteps=nsteps, seed=7 + q, bias=v)
             for q, (nx, ny, nz, v) in enumerate(dims)]
