@@ -328,15 +328,20 @@ template <> struct Mfma16<float> {
  * tile (16 rows j..j+15 of a device for AXIS 0; 16 consecutive i of plane k
  * for AXIS 1) out_m = sum_q in_q M[q][m], as the 16x16 blocks
  * C[m][j] = sum_q M[q][m] in_j[q], K = n in steps of 4.  Tile = (device,
- * first vector, group of four 16-wide m blocks, plane); each of the four
- * wavefronts of a workgroup takes one m block of the same vectors, so the
- * vectors come from L2 once per group.  Every lateral extent is a multiple
- * of 16 (fx_batch_create checks it; otherwise k_b_axis runs). */
+ * first vector, group of four 16-wide m blocks, plane); the tile's vectors
+ * are staged in LDS once with coalesced loads (as [q][j], padded to 17 so
+ * that the staging writes and the fragment reads spread over the banks), and
+ * each of the four wavefronts takes one m block.  AXIS 0 writes its result
+ * back through the same LDS so that whole rows are stored.  Every lateral
+ * extent is a multiple of 16 (fx_batch_create checks it; otherwise k_b_axis
+ * runs). */
+constexpr int MF_LD = 17;
 template <class T, int AXIS, int INV>
 __global__ __launch_bounds__(256) void k_b_axis_mfma(const BDev* __restrict__ devs, const BBasis* __restrict__ bases,
                                                      const int4* __restrict__ tiles, int ntiles, const T* __restrict__ in,
                                                      T* __restrict__ out) {
     using MF = Mfma16<T>;
+    __shared__ T sv[128 * MF_LD];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -346,23 +351,46 @@ __global__ __launch_bounds__(256) void k_b_axis_mfma(const BDev* __restrict__ de
         const BBasis& B = bases[AXIS == 0 ? D.bx : D.by];
         const int n = B.n;
         const int m0 = (tl.z * 4 + wave) * 16;
-        if (m0 >= n) continue;                       /* uniform in the wavefront */
         const T* M;
         if (sizeof(T) == 8) M = reinterpret_cast<const T*>(INV ? B.QT : B.Q);
         else M = reinterpret_cast<const T*>(INV ? B.QT32 : B.Q32);
-        const int j = tl.y + li;                     /* this lane's vector */
-        const long long base = AXIS == 0 ? D.off + (long long)nx * j : D.off + j + (long long)nx * ny * tl.w;
-        const long long qs = AXIS == 0 ? 1 : nx;     /* stride between the elements of a vector */
-        typename MF::v4 acc = {T(0), T(0), T(0), T(0)};
-#pragma unroll 4
-        for (int k0 = 0; k0 < n; k0 += 4) {
-            const int q = k0 + lk;
-            acc = MF::mma(M[(long long)q * n + m0 + li], in[base + qs * q], acc);
+        /* element q of vector j of the tile */
+        auto gidx = [&](int j, int q) -> long long {
+            return AXIS == 0 ? D.off + (long long)nx * (tl.y + j) + q : D.off + (tl.y + j) + (long long)nx * (q + (long long)ny * tl.w);
+        };
+        __syncthreads();                             /* the previous tile is done with sv */
+        for (int e = threadIdx.x; e < 16 * n; e += 256) {
+            int j, q;
+            if (AXIS == 0) { j = e / n; q = e - j * n; }   /* rows are contiguous */
+            else { j = e & 15; q = e >> 4; }              /* the 16 i of a row are contiguous */
+            sv[q * MF_LD + j] = in[gidx(j, q)];
         }
+        __syncthreads();
+        typename MF::v4 acc = {T(0), T(0), T(0), T(0)};
+        if (m0 < n) {                                /* uniform in the wavefront */
+#pragma unroll 4
+            for (int k0 = 0; k0 < n; k0 += 4) {
+                const int q = k0 + lk;
+                acc = MF::mma(M[(long long)q * n + m0 + li], sv[q * MF_LD + li], acc);
+            }
+        }
+        if (AXIS == 1) {
+            if (m0 < n) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int m = m0 + MF::row(lane, v);
-            out[AXIS == 0 ? base + m : base + (long long)nx * m] = acc[v];
+                for (int v = 0; v < 4; ++v) out[gidx(li, m0 + MF::row(lane, v))] = acc[v];
+            }
+        } else {
+            __syncthreads();                         /* every wavefront has read sv */
+            if (m0 < n) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sv[(m0 + MF::row(lane, v)) * MF_LD + li] = acc[v];
+            }
+            __syncthreads();
+            const int mlo = tl.z * 64, w = min(n, mlo + 64) - mlo;
+            for (int e = threadIdx.x; e < 16 * w; e += 256) {
+                const int j = e / w, m = mlo + (e - j * w);
+                out[gidx(j, m)] = sv[m * MF_LD + j];
+            }
         }
     }
 }
