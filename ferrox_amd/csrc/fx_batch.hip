@@ -202,9 +202,9 @@ __global__ __launch_bounds__(256) void k_b_tridiag(const BDev* __restrict__ devs
             else if (D.bclo[2] == BC_DIR) diag += T(2) * wl;
             if (k < nz - 1) { diag += wh; c = -wh; }
             else if (D.bchi[2] == BC_DIR) diag += T(2) * wh;
-            const T den = diag - a * cprev;
-            const T cq = c / den;
-            const T dq = (hc[k * plane] - a * dprev) / den;
+            const T rden = T(1) / (diag - a * cprev);
+            const T cq = c * rden;
+            const T dq = (hc[k * plane] - a * dprev) * rden;
             cc[k * plane] = cq;
             hc[k * plane] = dq;
             cprev = cq;
@@ -243,8 +243,13 @@ __global__ __launch_bounds__(256) void k_b_tdgl(const BDev* __restrict__ devs, c
         for (int q = 0; q < 6; ++q) pv.v[q] = D.bv[q];
         pv.ghost_face = D.ghost_face;
         const double Gam = D.G;
-        double fx, fy, fz;
-        grad_phi(pv, D.g, i, j, k, fx, fy, fz);
+        /* FE cells whose stencils are all the centred ones (MAT_INTERIOR_FE)
+         * read their neighbours directly, as k_tdgl does */
+        const bool inner = (raw & MAT_INTERIOR_FE) != 0;
+        const long long pl = (long long)nx * ny;
+        double fx = 0.0, fy = 0.0, fz;
+        if (inner) fz = (phi[c + pl] - phi[c - pl]) / (2. * D.g.dx[2]);
+        else grad_phi(pv, D.g, i, j, k, fx, fy, fz);
         double sE = 0.0;
         sE += 1.0 * fz;
         const double E2 = -sE;
@@ -252,9 +257,18 @@ __global__ __launch_bounds__(256) void k_b_tdgl(const BDev* __restrict__ devs, c
         const double r2 = r * r;
         const double dF = D.alpha * r + D.beta * (r2 * r) + D.gamma * (r2 * r2 * r);
         double L = -1.0 * Gam * dF;
-        const double Hxx = ddpd<0>(Fr, M, i, j, k, D.g.dx, D.pbc);
-        const double Hyy = ddpd<1>(Fr, M, i, j, k, D.g.dx, D.pbc);
-        const double Hzz = ddpd<2>(Fr, M, i, j, k, D.g.dx, D.pbc);
+        double Hxx, Hyy, Hzz;
+        if (inner) {
+            const NbrOff o(i, j, nx, ny);
+            const double h0 = D.g.dx[0], h1 = D.g.dx[1], h2 = D.g.dx[2];
+            Hxx = (P[c + o.xp] - 2. * r + P[c + o.xm]) / (h0 * h0);
+            Hyy = (P[c + o.yp] - 2. * r + P[c + o.ym]) / (h1 * h1);
+            Hzz = (P[c + o.zp] - 2. * r + P[c + o.zm]) / (h2 * h2);
+        } else {
+            Hxx = ddpd<0>(Fr, M, i, j, k, D.g.dx, D.pbc);
+            Hyy = ddpd<1>(Fr, M, i, j, k, D.g.dx, D.pbc);
+            Hzz = ddpd<2>(Fr, M, i, j, k, D.g.dx, D.pbc);
+        }
         const double gr = -D.g11 * (0.0 * Hxx + 0.0 * Hyy + 1.0 * Hzz)
                           - D.g44m * (1.0 * Hxx + 0.0 * Hyy + 0.0 * Hzz)
                           - D.g44m * (0.0 * Hxx + 1.0 * Hyy + 0.0 * Hzz);

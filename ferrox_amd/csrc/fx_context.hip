@@ -599,8 +599,7 @@ int build_coefficients(fx_ctx* x) {
  * the 18 face and edge neighbours are FE cells inside the domain or across a
  * periodic face, and k is neither the first nor the last plane (where the
  * z-derivative of phi is one-sided).  The host shadow keeps bits 0-2. */
-int upload_mat(fx_ctx* x) {
-    std::vector<uint8_t> dm(x->h_mat);
+void mark_interior_fe(const fx_ctx* x, std::vector<uint8_t>& dm) {
     const int nx = x->n[0], ny = x->n[1], nz = x->n[2];
     const bool px = x->in.is_periodic[0], py = x->in.is_periodic[1];
     for (int k = 1; k + 1 < nz; ++k)
@@ -621,6 +620,10 @@ int upload_mat(fx_ctx* x) {
                 if (ok) dm[g] |= fxd::MAT_INTERIOR_FE;
             }
         }
+}
+int upload_mat(fx_ctx* x) {
+    std::vector<uint8_t> dm(x->h_mat);
+    mark_interior_fe(x, dm);
     HIPCHK(x, hipMemcpyAsync(x->mat, dm.data(), dm.size(), hipMemcpyHostToDevice, x->s));
     x->has_sc = false;
     for (int k = 0; k < x->n[2]; ++k)
@@ -1812,6 +1815,7 @@ int fxi_export(fx_ctx* x, std::vector<uint8_t>& mat, int& msx, int& msy, fx_inpu
                std::vector<double>& epsk, std::vector<double>& bzk, double bv[6]) {
     if (!x) return FX_EINVAL;
     mat = x->h_mat; msx = x->msx; msy = x->msy; in = x->in;
+    mark_interior_fe(x, mat);   /* the device copy's bit 3, as upload_mat */
     const DLev& d = x->lev[0].d;
     for (int q = 0; q < 3; ++q) ih2[q] = d.ih2[q];
     if (!d.layered) return fail(x, FX_EINVAL, "batched devices must be laterally uniform (layered) stacks");
