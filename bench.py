@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+FP64-refinement leg")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 64x64x36 MFIM and 128x128x64 MFISM legs")
+    ap.add_argument("--mfism-steps", type=int, default=200, help="steps of the 128x128x64 MFISM leg (of its 2000-step ramp)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
     ap.add_argument("--batch-steps", type=int, default=1000)
@@ -124,6 +126,34 @@ def diff_leg(large: bool) -> dict:
                         "backward_s": round(t2 - t1, 3),
                         "checkpoint_MB": round((nsteps // K + 1 + 2 * K) * P0.size * 8 / 2**20, 1)}
         sim.close()
+    return out
+
+
+def config_legs(mfism_steps: int) -> dict:
+    """The two single-device FP64 configurations besides the throughput
+    workload: 64x64x36 MFIM (Al2O3 under HZO, 1000 Euler steps at 0 V) and
+    128x128x64 MFISM (p-Si, Boltzmann carriers, the nonlinear outer loop of
+    ComputePhi_Rho, gate ramped 0 -> 1 V over 2000 steps; the first
+    mfism_steps of the ramp are timed), both with the reference's Poisson
+    semantics (zero initial guess, 1e-10)."""
+    import ferrox_amd as fx
+    from ferrox_amd import configs
+    out = {}
+    for name, text, n in (("mfim_64", configs.mfim_64(nsteps=1000), 1000),
+                          ("mfism_128", configs.mfism_128(nsteps=2000), mfism_steps)):
+        g = fx.FerroX(fx.from_text(text))
+        g.sync()
+        t0 = time.perf_counter()
+        st = g.step(n)
+        g.sync()
+        secs = time.perf_counter() - t0
+        N = g.n[0] * g.n[1] * g.n[2]
+        out[name] = {"steps": n, "steps_per_s": round(n / secs, 2), "cell_updates_per_s": round(N * n / secs, 1),
+                     "vcycles_per_step": round(st["total_vcycles"] / max(1, st["steps_done"]), 3),
+                     "outer_iters_per_step": round(st["total_outer_iters"] / max(1, st["steps_done"]), 3),
+                     "gate_V_end": round(st["Phi_Bc_hi"], 4)}
+        g.close()
+    print(f"# configs: {out}", file=sys.stderr, flush=True)
     return out
 
 
@@ -265,6 +295,7 @@ def main():
                 "max_rel_err_Pz": eP, "max_rel_err_phi": eF}
         print(f"# warm start: {warm}", file=sys.stderr, flush=True)
 
+    other = None if a.no_configs else config_legs(a.mfism_steps)
     diff = None if a.no_diff else diff_leg(a.diff_large)
     batch = None if a.no_batch else batch_leg(a.batch_steps)
 
@@ -306,6 +337,7 @@ def main():
         "roofline": roof,
         "fp32_refine": fp32,
         "warm_start": warm,
+        "other_configs": other,
         "differentiation": diff,
         "batching": batch,
         "cpu_baseline": cpu,
