@@ -114,6 +114,13 @@ struct fx_ctx {
     DLev* d_tail64 = nullptr;
     DLevF* d_tail32 = nullptr;
     std::vector<float*> up32;     /* FP32 block levels: the up leg's output (Level::up) */
+    /* exact spectral solve of level spec0 (-1: none), used by the solves without a
+     * Helmholtz term in place of the sub-V-cycle below it */
+    int spec0 = -1;
+    fxd::SpecTail<double> spec64{};
+    fxd::SpecTail<float> spec32{};
+    double* spec_basis64 = nullptr;   /* Qx, Qy, mu_x, mu_y of level spec0 */
+    float* spec_basis32 = nullptr;
     int tail_ts64 = 0, tail_ts32 = 0;          /* first tail level staged in LDS */
     size_t tail_lds64 = 0, tail_lds32 = 0;     /* dynamic LDS of the tail kernel */
     /* FP32 correction hierarchy of the mixed-precision solve (precision = FP32_REFINE) */
@@ -559,6 +566,45 @@ int build_coefficients(fx_ctx* x) {
             break;
         }
     }
+    /* the exact spectral solve of the tail's first level: a laterally periodic
+     * hierarchy bounded in z with a Dirichlet face, the level's coefficients
+     * depending on z only (the choice the oracle makes from the same rule) */
+    x->spec0 = -1;
+    if (x->in.mg_exact_tail && x->tail_base >= 1 && x->in.is_periodic[0] && x->in.is_periodic[1] &&
+        !x->in.is_periodic[2] && (x->in.bc_lo[2] == FX_BC_DIRICHLET || x->in.bc_hi[2] == FX_BC_DIRICHLET)) {
+        const DLev& d = x->lev[x->tail_base].d;
+        if (d.layered && fxd::spec_tail_values(d.n0, d.n1, d.n2) * sizeof(double) <= fxd::SPEC_TAIL_MAX_BYTES) {
+            const size_t nb = (size_t)d.n0 * d.n0 + (size_t)d.n1 * d.n1 + d.n0 + d.n1;
+            std::vector<double> h(nb);
+            double* Qx = h.data();
+            double* Qy = Qx + (size_t)d.n0 * d.n0;
+            double* mux = Qy + (size_t)d.n1 * d.n1;
+            double* muy = mux + d.n0;
+            int rc = fx_batch_basis(d.n0, Qx, mux);
+            if (!rc) rc = fx_batch_basis(d.n1, Qy, muy);
+            if (rc) return fail(x, rc, "spectral tail: basis");
+            if (!x->spec_basis64 && (rc = dalloc(x, (void**)&x->spec_basis64, nb * sizeof(double)))) return rc;
+            HIPCHK(x, hipMemcpy(x->spec_basis64, h.data(), nb * sizeof(double), hipMemcpyHostToDevice));
+            if (!x->lev32.empty()) {
+                std::vector<float> hf(h.begin(), h.end());
+                if (!x->spec_basis32 && (rc = dalloc(x, (void**)&x->spec_basis32, nb * sizeof(float)))) return rc;
+                HIPCHK(x, hipMemcpy(x->spec_basis32, hf.data(), nb * sizeof(float), hipMemcpyHostToDevice));
+            }
+            auto fill = [&](auto& P, const auto* base) {
+                P.n0 = d.n0; P.n1 = d.n1; P.n2 = d.n2;
+                P.Qx = base;
+                P.Qy = base + (size_t)d.n0 * d.n0;
+                P.mux = P.Qy + (size_t)d.n1 * d.n1;
+                P.muy = P.mux + d.n0;
+                for (int q = 0; q < 3; ++q) P.ih2[q] = d.ih2[q];
+                P.dlo = x->in.bc_lo[2] == FX_BC_DIRICHLET;
+                P.dhi = x->in.bc_hi[2] == FX_BC_DIRICHLET;
+            };
+            fill(x->spec64, x->spec_basis64);
+            if (x->spec_basis32) fill(x->spec32, x->spec_basis32);
+            x->spec0 = x->tail_base;
+        }
+    }
     /* descriptors of the tail levels, read by the single-workgroup kernel */
     if (x->tail0 >= 0 && !x->tail_lean) {
         /* stage as many of the tail's levels (the smallest ones) in LDS as
@@ -770,6 +816,12 @@ int read_red(fx_ctx* x, int nq, double* out) { return wait_red(x, post_red(x, nq
 void vcycle(fx_ctx* x, size_t li, const DBC& bc, bool pre_done = false) {
     Level& L = x->lev[li];
     const int inh = li == 0;
+    if ((int)li == x->spec0 && !L.d.read_a) {
+        fxd::SpecTail<double> P = x->spec64;
+        P.b = L.d.b; P.phi = L.d.phi; P.bxk = L.d.bxk; P.byk = L.d.byk; P.bzk = L.d.bzk;
+        fxk::mg_tail_spec(x->s, P);
+        return;
+    }
     if ((int)li == x->tail0) {
         if (x->tail_lean) {
             if (!x->lu_ok64 && x->lu64) fxk::bottom_factor(x->s, x->lev.back().d, bc, x->lu64, x->perm64);
@@ -827,6 +879,12 @@ void set_read_a(fx_ctx* x, int mode) {
  * always do, their phi being cleared by the restriction. */
 void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true, bool pre_done = false) {
     DLevF& L = x->lev32[li];
+    if ((int)li == x->spec0 && !L.read_a && x->spec_basis32) {
+        fxd::SpecTail<float> P = x->spec32;
+        P.b = L.b; P.phi = L.phi; P.bxk = L.bxk; P.byk = L.byk; P.bzk = L.bzk;
+        fxk::mg_tail_spec(x->s, P);
+        return;
+    }
     if ((int)li == x->tail0) {
         if (x->tail_lean) {
             if (!x->lu_ok32 && x->lu32) fxk::bottom_factor(x->s, x->lev32.back(), bc, x->lu32, x->perm32);
@@ -1102,6 +1160,7 @@ void free_all(fx_ctx* x) {
         F(f.phi); F(f.b); F(f.a); F(f.bx); F(f.by); F(f.bz); F((void*)f.bxk); F((void*)f.byk); F((void*)f.bzk); F(f.phi2);
     }
     for (float* p : x->up32) F(p);
+    F(x->spec_basis64); F(x->spec_basis32);
     F(x->cg_work32); F(x->phi_alt); F(x->d_tail64); F(x->d_tail32);
     F(x->lu64); F(x->lu32); F(x->perm64); F(x->perm32);
     if (x->h_red) hipHostFree(x->h_red);

@@ -492,7 +492,7 @@ struct Applier {
         /* solver controls added by this build */
         D1(mg_tol_rel) I1(mg_max_iter) I1(mg_nu1) I1(mg_nu2) I1(mg_warm_start) I1(outer_max_iter)
         I1(phi_ghost_mode) I1(precision) I1(bias_ramp_steps) D1(bias_ramp_start) D1(bias_ramp_end) I1(verbose)
-        I1(mg_fused_sweep) I1(mg_block_levels) I1(mg_stream_min_cells) D1(mg_coarsen_ratio)
+        I1(mg_fused_sweep) I1(mg_block_levels) I1(mg_stream_min_cells) D1(mg_coarsen_ratio) I1(mg_exact_tail)
 #undef D1
 #undef D3
 #undef I1
@@ -583,6 +583,7 @@ int fx_inputs_default(fx_inputs* in) {
     in->mg_block_levels = 1;
     in->mg_stream_min_cells = 1 << 21;
     in->mg_coarsen_ratio = 1.9;
+    in->mg_exact_tail = 1;
     return FX_OK;
 }
 

@@ -2205,6 +2205,121 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
     }
 }
 
+/* ------------- exact spectral solve of a periodic layered level ------------ */
+/* A e = b on one level of a laterally periodic layered hierarchy, exactly (to
+ * rounding), in one 1024-thread workgroup with everything in LDS: the lateral
+ * transforms x_m = sum_i f_i Qx[i][m] and then along y, a Thomas solve in z
+ * per mode (mx, my) with diagonal bx_k mu_x/dx^2 + by_k mu_y/dy^2 plus the z
+ * couplings of the row operator (2w across a Dirichlet face, 0 across a
+ * Neumann one), and the inverse transforms (Q is orthogonal, so its inverse
+ * is its transpose).  Sums run over the basis index in increasing order. */
+template <class T>
+__global__ __launch_bounds__(1024) void k_mg_tail_spec(SpecTail<T> P) {
+    constexpr int BS = 1024;
+    extern __shared__ __align__(16) unsigned char spec_lds[];
+    const int n0 = P.n0, n1 = P.n1, n2 = P.n2, s = n0 * n1, N = s * n2;
+    T* const X = reinterpret_cast<T*>(spec_lds);
+    T* const Y = X + N;
+    T* const Qx = Y + N;
+    T* const QxT = Qx + n0 * n0;           /* its transpose: the inverse x transform reads it along a row */
+    T* const Qy = QxT + n0 * n0;
+    T* const mx = Qy + n1 * n1;
+    T* const my = mx + n0;
+    T* const wt = my + n1;                 /* per plane: bx/dx^2, by/dy^2, z lower, z upper, diagonal z part */
+    const int tid = threadIdx.x;
+    for (int c = tid; c < N; c += BS) X[c] = P.b[c];
+    for (int c = tid; c < n0 * n0; c += BS) {
+        const T v = P.Qx[c];
+        const int i = c / n0, m = c - i * n0;
+        Qx[c] = v;
+        QxT[m * n0 + i] = v;
+    }
+    for (int c = tid; c < n1 * n1; c += BS) Qy[c] = P.Qy[c];
+    for (int c = tid; c < n0; c += BS) mx[c] = P.mux[c];
+    for (int c = tid; c < n1; c += BS) my[c] = P.muy[c];
+    for (int k = tid; k < n2; k += BS) {
+        const T wzl = P.bzk[k] * P.ih2[2], wzh = P.bzk[k + 1] * P.ih2[2];
+        T* w = wt + 5 * k;
+        w[0] = P.bxk[k] * P.ih2[0];
+        w[1] = P.byk[k] * P.ih2[1];
+        w[2] = k > 0 ? wzl : T(0);
+        w[3] = k < n2 - 1 ? wzh : T(0);
+        T dz = T(0);
+        dz += k > 0 ? wzl : (P.dlo ? T(2) * wzl : T(0));
+        dz += k < n2 - 1 ? wzh : (P.dhi ? T(2) * wzh : T(0));
+        w[4] = dz;
+    }
+    __syncthreads();
+    /* The four transforms: out[e] = sum_t v[t * vs] * q[t * qs], with the
+     * vector v and the basis column q of output e given by at(e).  Each thread
+     * takes SPC outputs at a time and keeps their sums as independent chains;
+     * the basis index runs in increasing order for every output. */
+    constexpr int SPC = 4;
+    auto transform = [&](auto at, const int len, const int vs, const int qs, T* dst) {
+        for (int e0 = tid; e0 < N; e0 += SPC * BS) {
+            T acc[SPC];
+            const T* v[SPC];
+            const T* q[SPC];
+#pragma unroll
+            for (int u = 0; u < SPC; ++u) {
+                acc[u] = T(0);
+                const int e = min(e0 + u * BS, N - 1);
+                at(e, v[u], q[u]);
+            }
+#pragma unroll 4
+            for (int t = 0; t < len; ++t) {
+#pragma unroll
+                for (int u = 0; u < SPC; ++u) acc[u] += v[u][t * vs] * q[u][t * qs];
+            }
+#pragma unroll
+            for (int u = 0; u < SPC; ++u)
+                if (e0 + u * BS < N) dst[e0 + u * BS] = acc[u];
+        }
+    };
+    /* along x: Y[m, j, k] = sum_i X[i, j, k] Qx[i][m] */
+    transform([&](int e, const T*& v, const T*& q) { const int row = e / n0, m = e - row * n0; v = X + row * n0; q = Qx + m; },
+              n0, 1, n0, Y);
+    __syncthreads();
+    /* along y: X[m, q, k] = sum_j Y[m, j, k] Qy[j][q] */
+    transform([&](int e, const T*& v, const T*& q) {
+        const int k = e / s, r = e - k * s, qq = r / n0, m = r - qq * n0;
+        v = Y + k * s + m; q = Qy + qq; }, n1, n0, n1, X);
+    __syncthreads();
+    /* z: one Thomas solve per mode (m, q); Y holds the eliminated upper diagonal */
+    for (int mode = tid; mode < s; mode += BS) {
+        const int q = mode / n0, m = mode - q * n0;
+        T cprev = T(0), dprev = T(0);
+        for (int k = 0; k < n2; ++k) {
+            const T* w = wt + 5 * k;
+            T diag = w[0] * mx[m];
+            diag += w[1] * my[q];
+            diag += w[4];
+            const T a = -w[2], c = -w[3];
+            const T rden = T(1) / (diag - a * cprev);
+            const T cq = c * rden;
+            const T dq = (X[mode + k * s] - a * dprev) * rden;
+            Y[mode + k * s] = cq;
+            X[mode + k * s] = dq;
+            cprev = cq;
+            dprev = dq;
+        }
+        T xn = T(0);
+        for (int k = n2 - 1; k >= 0; --k) {
+            xn = X[mode + k * s] - Y[mode + k * s] * xn;
+            X[mode + k * s] = xn;
+        }
+    }
+    __syncthreads();
+    /* back along y: Y[m, j, k] = sum_q X[m, q, k] Qy[j][q] */
+    transform([&](int e, const T*& v, const T*& q) {
+        const int k = e / s, r = e - k * s, j = r / n0, m = r - j * n0;
+        v = X + k * s + m; q = Qy + j * n1; }, n1, n0, 1, Y);
+    __syncthreads();
+    /* back along x: phi[i, j, k] = sum_m Y[m, j, k] Qx[i][m] */
+    transform([&](int e, const T*& v, const T*& q) { const int row = e / n0, i = e - row * n0; v = Y + row * n0; q = QxT + i; },
+              n0, 1, n0, P.phi);
+}
+
 /* --------------------- FP64 refinement of an FP32 correction -------------- */
 /* phi += e (FP32 correction added into the FP64 solution). */
 __global__ void k_add_f2d(double* __restrict__ phi, const float* __restrict__ e, long long n) {
@@ -2692,6 +2807,19 @@ static void tail_lay_impl(hipStream_t s, const TailPlan<T>& P, const DBC& bc, in
     }
     hipLaunchKernelGGL(k_mg_tail_lay<T>, dim3(1), dim3(1024), lds, s, P, bc, read_a, nu1, nu2, work, tol, maxit, iters);
 }
+template <class T>
+static void tail_spec_impl(hipStream_t s, const SpecTail<T>& P) {
+    const size_t lds = spec_tail_values(P.n0, P.n1, P.n2) * sizeof(T);
+    static size_t attr_set = 0;
+    if (lds > attr_set) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail_spec<T>), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        attr_set = lds;
+    }
+    hipLaunchKernelGGL(k_mg_tail_spec<T>, dim3(1), dim3(1024), lds, s, P);
+}
+void mg_tail_spec(hipStream_t s, const SpecTail<double>& P) { tail_spec_impl<double>(s, P); }
+void mg_tail_spec(hipStream_t s, const SpecTail<float>& P) { tail_spec_impl<float>(s, P); }
 void mg_tail_lay(hipStream_t s, const TailPlan<double>& P, const DBC& bc, int read_a, int nu1, int nu2, double* work,
                  double tol, int maxit, int* iters) {
     tail_lay_impl<double>(s, P, bc, read_a, nu1, nu2, work, tol, maxit, iters);

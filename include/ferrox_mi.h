@@ -136,6 +136,8 @@ typedef struct fx_inputs {
     int    mg_block_levels;       /* 1: small layered levels run their down and up legs as one launch each, column blocks in LDS (default 1) */
     int    mg_stream_min_cells;   /* levels of at least this many cells use the streaming kernels (default 2097152) */
     double mg_coarsen_ratio;      /* direction d is coarsened when h_d < ratio * min(h) (default 1.9) */
+    int    mg_exact_tail;         /* 1 (default): laterally periodic layered hierarchies solve the tail's first level
+                                     exactly (lateral Fourier transforms + tridiagonal solves) instead of a sub-V-cycle */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;

@@ -423,9 +423,65 @@ struct MG {
                 }
     }
 
-    void vcycle(size_t li, const double* bv0) {
+    /* The level the V-cycle solves exactly instead of cycling below it (-1:
+     * none): the first level of the tail -- from the coarsest level up, the
+     * levels of at most 8192 cells, at least two of them -- when the hierarchy
+     * is periodic in x and y and bounded in z with a Dirichlet face, that
+     * level's face coefficients depend on z only, it has no Helmholtz term and
+     * the device's single-workgroup spectral solve of it fits its LDS budget
+     * (2N + 2 n0^2 + n1^2 + n0 + n1 + 5 n2 values of 8 bytes within 128 KiB).
+     * The device solves that level with lateral Fourier transforms and a
+     * tridiagonal solve per mode; here it is solved by repeated V-cycles of
+     * the levels below until the residual stops falling, which is the same
+     * solution to rounding. */
+    int exact_tail = 1;
+    int exact_level() const {
+        if (!exact_tail || bclo[0] != FX_BC_PERIODIC || bclo[1] != FX_BC_PERIODIC || bclo[2] == FX_BC_PERIODIC) return -1;
+        if (bclo[2] != FX_BC_DIRICHLET && bchi[2] != FX_BC_DIRICHLET) return -1;
+        int t = -1;
+        for (int l = (int)L.size() - 1; l >= 1 && L[l].N <= 8192; --l) t = l;
+        if (t < 0 || (int)L.size() - t < 2) return -1;
+        const Lev& v = L[t];
+        const size_t vals = 2 * v.N + 2 * (size_t)v.n[0] * v.n[0] + (size_t)v.n[1] * v.n[1] + v.n[0] + v.n[1] + 5 * (size_t)v.n[2];
+        if (vals * 8 > 131072) return -1;
+        for (size_t c = 0; c < v.N; ++c) if (v.a[c] != 0.0) return -1;
+        for (int k = 0; k < v.n[2]; ++k) {
+            const double bx0 = v.bx[v.fx(0, 0, k)], by0 = v.by[v.fy(0, 0, k)];
+            for (int j = 0; j < v.n[1]; ++j)
+                for (int i = 0; i <= v.n[0]; ++i) if (v.bx[v.fx(i, j, k)] != bx0) return -1;
+            for (int j = 0; j <= v.n[1]; ++j)
+                for (int i = 0; i < v.n[0]; ++i) if (v.by[v.fy(i, j, k)] != by0) return -1;
+        }
+        for (int k = 0; k <= v.n[2]; ++k) {
+            const double bz0 = v.bz[v.fz(0, 0, k)];
+            for (int j = 0; j < v.n[1]; ++j)
+                for (int i = 0; i < v.n[0]; ++i) if (v.bz[v.fz(i, j, k)] != bz0) return -1;
+        }
+        return t;
+    }
+    int exact_lvl = -1;          /* exact_level() of the current solve */
+    void exact_solve(size_t li) {
+        Lev& l = L[li];
+        std::fill(l.phi.begin(), l.phi.end(), 0.0);
+        const double bn = norm_inf(l.b);
+        if (bn == 0.0) return;
+        double prev = bn;
+        for (int it = 0; it < 100; ++it) {
+            vcycle(li, nullptr, false);
+            residual(l, nullptr);
+            const double r = norm_inf(l.r);
+            if (r <= 1e-15 * bn || r >= 0.5 * prev) break;
+            prev = r;
+        }
+    }
+
+    void vcycle(size_t li, const double* bv0, bool allow_exact = true) {
         Lev& l = L[li];
         const double* bv = (li == 0) ? bv0 : nullptr;
+        if (allow_exact && li > 0 && (int)li == exact_lvl) {
+            exact_solve(li);
+            return;
+        }
         if (li + 1 == L.size()) {
             if (li == 0) {     /* single-level hierarchy: correct the solution */
                 residual(l, bv);
@@ -442,13 +498,14 @@ struct MG {
         Lev& c = L[li + 1];
         restrict_cells(l, c, l.r.data(), c.b.data());
         std::fill(c.phi.begin(), c.phi.end(), 0.0);
-        vcycle(li + 1, bv0);
+        vcycle(li + 1, bv0, allow_exact);
         prolong(l, c);
         for (int s = 0; s < nu2; ++s) { smooth(l, 0, bv); smooth(l, 1, bv); }
     }
 
     /* MLMG-style solve on level 0: phi (in/out), b, Dirichlet values bv. */
     int solve(double tol_rel, int maxit, const double* bv, fx_solve_stats* st) {
+        exact_lvl = exact_level();
         Lev& l = L[0];
         residual(l, bv);
         double r0 = norm_inf(l.r), bn = norm_inf(l.b);
@@ -983,6 +1040,7 @@ fxo_state* fxo_create(const fx_inputs* in) {
     /* multigrid finest level */
     for (int d = 0; d < 3; ++d) { s->mg.bclo[d] = in->bc_lo[d]; s->mg.bchi[d] = in->bc_hi[d]; }
     s->mg.nu1 = in->mg_nu1; s->mg.nu2 = in->mg_nu2; s->mg.ratio = in->mg_coarsen_ratio;
+    s->mg.exact_tail = in->mg_exact_tail;
     Lev l0;
     for (int d = 0; d < 3; ++d) { l0.n[d] = s->n[d]; l0.h[d] = s->dx[d]; l0.ih2[d] = 1.0 / (s->dx[d] * s->dx[d]); l0.crs[d] = 0; }
     l0.N = N;
