@@ -119,7 +119,7 @@ struct fx_ctx {
     int spec0 = -1;
     fxd::SpecTail<double> spec64{};
     fxd::SpecTail<float> spec32{};
-    double* spec_basis64 = nullptr;   /* Qx, Qy, mu_x, mu_y of level spec0 */
+    double* spec_basis64 = nullptr;   /* Qx, Qy, mu_x, mu_y of level spec0, then two scratch arrays of its size */
     float* spec_basis32 = nullptr;
     int tail_ts64 = 0, tail_ts32 = 0;          /* first tail level staged in LDS */
     size_t tail_lds64 = 0, tail_lds32 = 0;     /* dynamic LDS of the tail kernel */
@@ -566,15 +566,25 @@ int build_coefficients(fx_ctx* x) {
             break;
         }
     }
-    /* the exact spectral solve of the tail's first level: a laterally periodic
-     * hierarchy bounded in z with a Dirichlet face, the level's coefficients
-     * depending on z only (the choice the oracle makes from the same rule) */
+    /* the exact spectral solve that ends the V-cycle: on a laterally periodic
+     * hierarchy bounded in z with a Dirichlet face, the finest level below the
+     * finest whose planes fit one workgroup (n0 n1 <= SPEC_MAX_PLANE, n0, n1 <=
+     * SPEC_MAX_N), of at most SPEC_MAX_CELLS cells, with coefficients that
+     * depend on z only (the choice the oracle makes from the same rule) */
     x->spec0 = -1;
-    if (x->in.mg_exact_tail && x->tail_base >= 1 && x->in.is_periodic[0] && x->in.is_periodic[1] &&
-        !x->in.is_periodic[2] && (x->in.bc_lo[2] == FX_BC_DIRICHLET || x->in.bc_hi[2] == FX_BC_DIRICHLET)) {
-        const DLev& d = x->lev[x->tail_base].d;
-        if (d.layered && fxd::spec_tail_values(d.n0, d.n1, d.n2) * sizeof(double) <= fxd::SPEC_TAIL_MAX_BYTES) {
+    if (x->in.mg_exact_tail && x->in.is_periodic[0] && x->in.is_periodic[1] && !x->in.is_periodic[2] &&
+        (x->in.bc_lo[2] == FX_BC_DIRICHLET || x->in.bc_hi[2] == FX_BC_DIRICHLET)) {
+        int lsel = -1;
+        for (size_t l = 1; l < x->lev.size() && lsel < 0; ++l) {
+            const DLev& d = x->lev[l].d;
+            if (d.n0 <= fxd::SPEC_MAX_N && d.n1 <= fxd::SPEC_MAX_N && d.n0 * d.n1 <= fxd::SPEC_MAX_PLANE &&
+                x->lev[l].N <= (size_t)fxd::SPEC_MAX_CELLS)
+                lsel = (int)l;
+        }
+        if (lsel >= 1 && x->lev[lsel].d.layered) {
+            const DLev& d = x->lev[lsel].d;
             const size_t nb = (size_t)d.n0 * d.n0 + (size_t)d.n1 * d.n1 + d.n0 + d.n1;
+            const size_t N = x->lev[lsel].N;
             std::vector<double> h(nb);
             double* Qx = h.data();
             double* Qy = Qx + (size_t)d.n0 * d.n0;
@@ -582,27 +592,30 @@ int build_coefficients(fx_ctx* x) {
             double* muy = mux + d.n0;
             int rc = fx_batch_basis(d.n0, Qx, mux);
             if (!rc) rc = fx_batch_basis(d.n1, Qy, muy);
-            if (rc) return fail(x, rc, "spectral tail: basis");
-            if (!x->spec_basis64 && (rc = dalloc(x, (void**)&x->spec_basis64, nb * sizeof(double)))) return rc;
+            if (rc) return fail(x, rc, "spectral coarse solve: basis");
+            /* basis, then the two scratch arrays */
+            if (!x->spec_basis64 && (rc = dalloc(x, (void**)&x->spec_basis64, (nb + 2 * N) * sizeof(double)))) return rc;
             HIPCHK(x, hipMemcpy(x->spec_basis64, h.data(), nb * sizeof(double), hipMemcpyHostToDevice));
             if (!x->lev32.empty()) {
                 std::vector<float> hf(h.begin(), h.end());
-                if (!x->spec_basis32 && (rc = dalloc(x, (void**)&x->spec_basis32, nb * sizeof(float)))) return rc;
+                if (!x->spec_basis32 && (rc = dalloc(x, (void**)&x->spec_basis32, (nb + 2 * N) * sizeof(float)))) return rc;
                 HIPCHK(x, hipMemcpy(x->spec_basis32, hf.data(), nb * sizeof(float), hipMemcpyHostToDevice));
             }
-            auto fill = [&](auto& P, const auto* base) {
+            auto fill = [&](auto& P, auto* base) {
                 P.n0 = d.n0; P.n1 = d.n1; P.n2 = d.n2;
                 P.Qx = base;
                 P.Qy = base + (size_t)d.n0 * d.n0;
                 P.mux = P.Qy + (size_t)d.n1 * d.n1;
                 P.muy = P.mux + d.n0;
+                P.S = base + nb;
+                P.C = base + nb + N;
                 for (int q = 0; q < 3; ++q) P.ih2[q] = d.ih2[q];
                 P.dlo = x->in.bc_lo[2] == FX_BC_DIRICHLET;
                 P.dhi = x->in.bc_hi[2] == FX_BC_DIRICHLET;
             };
             fill(x->spec64, x->spec_basis64);
             if (x->spec_basis32) fill(x->spec32, x->spec_basis32);
-            x->spec0 = x->tail_base;
+            x->spec0 = lsel;
         }
     }
     /* descriptors of the tail levels, read by the single-workgroup kernel */
@@ -819,7 +832,7 @@ void vcycle(fx_ctx* x, size_t li, const DBC& bc, bool pre_done = false) {
     if ((int)li == x->spec0 && !L.d.read_a) {
         fxd::SpecTail<double> P = x->spec64;
         P.b = L.d.b; P.phi = L.d.phi; P.bxk = L.d.bxk; P.byk = L.d.byk; P.bzk = L.d.bzk;
-        fxk::mg_tail_spec(x->s, P);
+        fxk::mg_spec_solve(x->s, P);
         return;
     }
     if ((int)li == x->tail0) {
@@ -882,7 +895,7 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true, bool pre_d
     if ((int)li == x->spec0 && !L.read_a && x->spec_basis32) {
         fxd::SpecTail<float> P = x->spec32;
         P.b = L.b; P.phi = L.phi; P.bxk = L.bxk; P.byk = L.byk; P.bzk = L.bzk;
-        fxk::mg_tail_spec(x->s, P);
+        fxk::mg_spec_solve(x->s, P);
         return;
     }
     if ((int)li == x->tail0) {

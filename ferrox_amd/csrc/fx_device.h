@@ -285,16 +285,20 @@ struct TailPlan {
 };
 
 /* The exact solve that ends the V-cycle of a laterally periodic layered
- * hierarchy (k_mg_tail_spec): one level, its operator block-diagonalised by
+ * hierarchy (mg_spec_solve): one level, its operator block-diagonalised by
  * the real Fourier basis of each periodic lateral direction (column m of the
  * n x n matrix Q is basis vector m, stored Q[i*n + m]; mu[m] its eigenvalue of
  * 2 f_i - f_i-1 - f_i+1), then one tridiagonal system in z per mode.
- * Homogeneous faces; dlo/dhi = 1 for a Dirichlet z face, 0 for Neumann. */
+ * Homogeneous faces; dlo/dhi = 1 for a Dirichlet z face, 0 for Neumann.
+ * S and C are scratch of the level's size (the transformed values and the
+ * eliminated upper diagonal of the z systems). */
 template <class T>
 struct SpecTail {
     int n0, n1, n2;
     const T* b;
     T* phi;
+    T* S;
+    T* C;
     const T* Qx;
     const T* Qy;
     const T* mux;
@@ -305,13 +309,14 @@ struct SpecTail {
     T ih2[3];
     int dlo, dhi;
 };
-/* Dynamic LDS values of k_mg_tail_spec for a level of n0 x n1 x n2 cells. */
-__host__ __device__ constexpr size_t spec_tail_values(int n0, int n1, int n2) {
-    return 2 * (size_t)n0 * n1 * n2 + 2 * (size_t)n0 * n0 + (size_t)n1 * n1 + (size_t)n0 + n1 + 5 * (size_t)n2;
+/* Plane kernels of the exact solve: dynamic LDS values (two planes and two
+ * basis matrices). */
+__host__ __device__ constexpr size_t spec_plane_values(int n0, int n1) {
+    return 2 * (size_t)n0 * n1 + (size_t)n0 * n0 + (size_t)n1 * n1;
 }
-/* The exact solve is used where its LDS fits this budget (shared with the oracle's
- * choice of the level). */
-constexpr size_t SPEC_TAIL_MAX_BYTES = 131072;
+/* A level is solved exactly where n0*n1 <= SPEC_MAX_PLANE, n0, n1 <= SPEC_MAX_N
+ * and it has at most SPEC_MAX_CELLS cells (the same rule as the oracle's). */
+constexpr int SPEC_MAX_PLANE = 4096, SPEC_MAX_N = 64, SPEC_MAX_CELLS = 65536;
 
 /* Cell permittivity of a material id: FE (t-phase or not), DE, SC; chosen
  * with selects on kernel-argument scalars rather than an indexed table. */

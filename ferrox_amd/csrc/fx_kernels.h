@@ -154,10 +154,10 @@ void mg_tail_lay(hipStream_t s, const fxd::TailPlan<double>& P, const fxd::DBC& 
                  double* work, double tol, int maxit, int* iters);
 void mg_tail_lay(hipStream_t s, const fxd::TailPlan<float>& P, const fxd::DBC& bc, int read_a, int nu1, int nu2,
                  float* work, double tol, int maxit, int* iters);
-/* exact solve of one laterally periodic layered level (homogeneous faces),
- * one workgroup, LDS of spec_tail_values(...) values */
-void mg_tail_spec(hipStream_t s, const fxd::SpecTail<double>& P);
-void mg_tail_spec(hipStream_t s, const fxd::SpecTail<float>& P);
+/* exact solve of one laterally periodic layered level (homogeneous faces):
+ * per-plane transforms, a tridiagonal solve per mode, inverse transforms */
+void mg_spec_solve(hipStream_t s, const fxd::SpecTail<double>& P);
+void mg_spec_solve(hipStream_t s, const fxd::SpecTail<float>& P);
 /* one launch per leg for a small layered level l above c: down = nu1 sweeps
  * from phi = 0 + residual restricted into c, written to l.phi; up =
  * prolongation from c + nu2 sweeps, read from l.phi and written to out, a

@@ -2207,117 +2207,116 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
 
 /* ------------- exact spectral solve of a periodic layered level ------------ */
 /* A e = b on one level of a laterally periodic layered hierarchy, exactly (to
- * rounding), in one 1024-thread workgroup with everything in LDS: the lateral
- * transforms x_m = sum_i f_i Qx[i][m] and then along y, a Thomas solve in z
+ * rounding), in three launches: every plane transformed along x and y by one
+ * workgroup (x_m = sum_i f_i Qx[i][m], then along y), one Thomas solve in z
  * per mode (mx, my) with diagonal bx_k mu_x/dx^2 + by_k mu_y/dy^2 plus the z
  * couplings of the row operator (2w across a Dirichlet face, 0 across a
- * Neumann one), and the inverse transforms (Q is orthogonal, so its inverse
- * is its transpose).  Sums run over the basis index in increasing order. */
-template <class T>
-__global__ __launch_bounds__(1024) void k_mg_tail_spec(SpecTail<T> P) {
-    constexpr int BS = 1024;
-    extern __shared__ __align__(16) unsigned char spec_lds[];
-    const int n0 = P.n0, n1 = P.n1, n2 = P.n2, s = n0 * n1, N = s * n2;
-    T* const X = reinterpret_cast<T*>(spec_lds);
-    T* const Y = X + N;
-    T* const Qx = Y + N;
-    T* const QxT = Qx + n0 * n0;           /* its transpose: the inverse x transform reads it along a row */
-    T* const Qy = QxT + n0 * n0;
-    T* const mx = Qy + n1 * n1;
-    T* const my = mx + n0;
-    T* const wt = my + n1;                 /* per plane: bx/dx^2, by/dy^2, z lower, z upper, diagonal z part */
-    const int tid = threadIdx.x;
-    for (int c = tid; c < N; c += BS) X[c] = P.b[c];
-    for (int c = tid; c < n0 * n0; c += BS) {
-        const T v = P.Qx[c];
-        const int i = c / n0, m = c - i * n0;
-        Qx[c] = v;
-        QxT[m * n0 + i] = v;
-    }
-    for (int c = tid; c < n1 * n1; c += BS) Qy[c] = P.Qy[c];
-    for (int c = tid; c < n0; c += BS) mx[c] = P.mux[c];
-    for (int c = tid; c < n1; c += BS) my[c] = P.muy[c];
-    for (int k = tid; k < n2; k += BS) {
-        const T wzl = P.bzk[k] * P.ih2[2], wzh = P.bzk[k + 1] * P.ih2[2];
-        T* w = wt + 5 * k;
-        w[0] = P.bxk[k] * P.ih2[0];
-        w[1] = P.byk[k] * P.ih2[1];
-        w[2] = k > 0 ? wzl : T(0);
-        w[3] = k < n2 - 1 ? wzh : T(0);
-        T dz = T(0);
-        dz += k > 0 ? wzl : (P.dlo ? T(2) * wzl : T(0));
-        dz += k < n2 - 1 ? wzh : (P.dhi ? T(2) * wzh : T(0));
-        w[4] = dz;
-    }
-    __syncthreads();
-    /* The four transforms: out[e] = sum_t v[t * vs] * q[t * qs], with the
-     * vector v and the basis column q of output e given by at(e).  Each thread
-     * takes SPC outputs at a time and keeps their sums as independent chains;
-     * the basis index runs in increasing order for every output. */
-    constexpr int SPC = 4;
-    auto transform = [&](auto at, const int len, const int vs, const int qs, T* dst) {
-        for (int e0 = tid; e0 < N; e0 += SPC * BS) {
-            T acc[SPC];
-            const T* v[SPC];
-            const T* q[SPC];
+ * Neumann one), and the inverse transforms (Q is orthogonal: its inverse is
+ * its transpose).  Sums run over the basis index in increasing order. */
+namespace {
+/* out[e] = sum_t v[t*vs] q[t*qs] for e < cnt, the vector v and basis column q
+ * of output e given by at(e, v, q); four outputs per thread at a time, kept as
+ * independent chains */
+template <class T, class At>
+__device__ __forceinline__ void spec_transform(int tid, int cnt, At at, int len, int vs, int qs, T* dst) {
+    constexpr int BS = 1024, SPC = 4;
+    for (int e0 = tid; e0 < cnt; e0 += SPC * BS) {
+        T acc[SPC];
+        const T* v[SPC];
+        const T* q[SPC];
 #pragma unroll
-            for (int u = 0; u < SPC; ++u) {
-                acc[u] = T(0);
-                const int e = min(e0 + u * BS, N - 1);
-                at(e, v[u], q[u]);
-            }
+        for (int u = 0; u < SPC; ++u) {
+            acc[u] = T(0);
+            at(min(e0 + u * BS, cnt - 1), v[u], q[u]);
+        }
 #pragma unroll 4
-            for (int t = 0; t < len; ++t) {
+        for (int t = 0; t < len; ++t) {
 #pragma unroll
-                for (int u = 0; u < SPC; ++u) acc[u] += v[u][t * vs] * q[u][t * qs];
-            }
+            for (int u = 0; u < SPC; ++u) acc[u] += v[u][t * vs] * q[u][t * qs];
+        }
 #pragma unroll
-            for (int u = 0; u < SPC; ++u)
-                if (e0 + u * BS < N) dst[e0 + u * BS] = acc[u];
-        }
-    };
-    /* along x: Y[m, j, k] = sum_i X[i, j, k] Qx[i][m] */
-    transform([&](int e, const T*& v, const T*& q) { const int row = e / n0, m = e - row * n0; v = X + row * n0; q = Qx + m; },
-              n0, 1, n0, Y);
-    __syncthreads();
-    /* along y: X[m, q, k] = sum_j Y[m, j, k] Qy[j][q] */
-    transform([&](int e, const T*& v, const T*& q) {
-        const int k = e / s, r = e - k * s, qq = r / n0, m = r - qq * n0;
-        v = Y + k * s + m; q = Qy + qq; }, n1, n0, n1, X);
-    __syncthreads();
-    /* z: one Thomas solve per mode (m, q); Y holds the eliminated upper diagonal */
-    for (int mode = tid; mode < s; mode += BS) {
-        const int q = mode / n0, m = mode - q * n0;
-        T cprev = T(0), dprev = T(0);
-        for (int k = 0; k < n2; ++k) {
-            const T* w = wt + 5 * k;
-            T diag = w[0] * mx[m];
-            diag += w[1] * my[q];
-            diag += w[4];
-            const T a = -w[2], c = -w[3];
-            const T rden = T(1) / (diag - a * cprev);
-            const T cq = c * rden;
-            const T dq = (X[mode + k * s] - a * dprev) * rden;
-            Y[mode + k * s] = cq;
-            X[mode + k * s] = dq;
-            cprev = cq;
-            dprev = dq;
-        }
-        T xn = T(0);
-        for (int k = n2 - 1; k >= 0; --k) {
-            xn = X[mode + k * s] - Y[mode + k * s] * xn;
-            X[mode + k * s] = xn;
-        }
+        for (int u = 0; u < SPC; ++u)
+            if (e0 + u * BS < cnt) dst[e0 + u * BS] = acc[u];
     }
+}
+}  // namespace
+
+/* plane k = blockIdx.x: b -> S, along x then y */
+template <class T>
+__global__ __launch_bounds__(1024) void k_spec_fwd(SpecTail<T> P) {
+    extern __shared__ __align__(16) unsigned char spec_lds[];
+    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, tid = threadIdx.x;
+    T* const A = reinterpret_cast<T*>(spec_lds);
+    T* const B = A + s;
+    T* const Qx = B + s;
+    T* const Qy = Qx + n0 * n0;
+    for (int c = tid; c < s; c += 1024) A[c] = P.b[(size_t)k * s + c];
+    for (int c = tid; c < n0 * n0; c += 1024) Qx[c] = P.Qx[c];
+    for (int c = tid; c < n1 * n1; c += 1024) Qy[c] = P.Qy[c];
     __syncthreads();
-    /* back along y: Y[m, j, k] = sum_q X[m, q, k] Qy[j][q] */
-    transform([&](int e, const T*& v, const T*& q) {
-        const int k = e / s, r = e - k * s, j = r / n0, m = r - j * n0;
-        v = X + k * s + m; q = Qy + j * n1; }, n1, n0, 1, Y);
+    /* B[m + n0 j] = sum_i A[i + n0 j] Qx[i][m] */
+    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int j = e / n0; v = A + j * n0; q = Qx + (e - j * n0); },
+                      n0, 1, n0, B);
     __syncthreads();
-    /* back along x: phi[i, j, k] = sum_m Y[m, j, k] Qx[i][m] */
-    transform([&](int e, const T*& v, const T*& q) { const int row = e / n0, i = e - row * n0; v = Y + row * n0; q = QxT + i; },
-              n0, 1, n0, P.phi);
+    /* S[m + n0 q] = sum_j B[m + n0 j] Qy[j][q] */
+    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int qq = e / n0; v = B + (e - qq * n0); q = Qy + qq; },
+                      n1, n0, n1, P.S + (size_t)k * s);
+}
+
+/* one thread per mode: the tridiagonal system in z (S in, S out, C scratch) */
+template <class T>
+__global__ __launch_bounds__(256) void k_spec_tri(SpecTail<T> P) {
+    const int n0 = P.n0, s = n0 * P.n1, n2 = P.n2;
+    const int mode = blockIdx.x * 256 + threadIdx.x;
+    if (mode >= s) return;
+    const int q = mode / n0, m = mode - q * n0;
+    const T mx = P.mux[m], my = P.muy[q];
+    T cprev = T(0), dprev = T(0);
+    for (int k = 0; k < n2; ++k) {
+        const T wzl = P.bzk[k] * P.ih2[2], wzh = P.bzk[k + 1] * P.ih2[2];
+        T diag = (P.bxk[k] * P.ih2[0]) * mx;
+        diag += (P.byk[k] * P.ih2[1]) * my;
+        diag += k > 0 ? wzl : (P.dlo ? T(2) * wzl : T(0));
+        diag += k < n2 - 1 ? wzh : (P.dhi ? T(2) * wzh : T(0));
+        const T a = k > 0 ? -wzl : T(0), c = k < n2 - 1 ? -wzh : T(0);
+        const T rden = T(1) / (diag - a * cprev);
+        const T cq = c * rden;
+        const T dq = (P.S[mode + (size_t)k * s] - a * dprev) * rden;
+        P.C[mode + (size_t)k * s] = cq;
+        P.S[mode + (size_t)k * s] = dq;
+        cprev = cq;
+        dprev = dq;
+    }
+    T xn = T(0);
+    for (int k = n2 - 1; k >= 0; --k) {
+        xn = P.S[mode + (size_t)k * s] - P.C[mode + (size_t)k * s] * xn;
+        P.S[mode + (size_t)k * s] = xn;
+    }
+}
+
+/* plane k = blockIdx.x: S -> phi, back along y then x */
+template <class T>
+__global__ __launch_bounds__(1024) void k_spec_inv(SpecTail<T> P) {
+    extern __shared__ __align__(16) unsigned char spec_lds[];
+    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, tid = threadIdx.x;
+    T* const A = reinterpret_cast<T*>(spec_lds);
+    T* const B = A + s;
+    T* const QxT = B + s;                  /* the transpose: the x sums read it along a row */
+    T* const Qy = QxT + n0 * n0;
+    for (int c = tid; c < s; c += 1024) A[c] = P.S[(size_t)k * s + c];
+    for (int c = tid; c < n0 * n0; c += 1024) {
+        const int i = c / n0, m = c - i * n0;
+        QxT[m * n0 + i] = P.Qx[c];
+    }
+    for (int c = tid; c < n1 * n1; c += 1024) Qy[c] = P.Qy[c];
+    __syncthreads();
+    /* B[m + n0 j] = sum_q A[m + n0 q] Qy[j][q] */
+    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int j = e / n0; v = A + (e - j * n0); q = Qy + j * n1; },
+                      n1, n0, 1, B);
+    __syncthreads();
+    /* phi[i + n0 j] = sum_m B[m + n0 j] Qx[i][m] */
+    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int j = e / n0; v = B + j * n0; q = QxT + (e - j * n0); },
+                      n0, 1, n0, P.phi + (size_t)k * s);
 }
 
 /* --------------------- FP64 refinement of an FP32 correction -------------- */
@@ -2808,18 +2807,21 @@ static void tail_lay_impl(hipStream_t s, const TailPlan<T>& P, const DBC& bc, in
     hipLaunchKernelGGL(k_mg_tail_lay<T>, dim3(1), dim3(1024), lds, s, P, bc, read_a, nu1, nu2, work, tol, maxit, iters);
 }
 template <class T>
-static void tail_spec_impl(hipStream_t s, const SpecTail<T>& P) {
-    const size_t lds = spec_tail_values(P.n0, P.n1, P.n2) * sizeof(T);
+static void spec_solve_impl(hipStream_t s, const SpecTail<T>& P) {
+    const size_t lds = spec_plane_values(P.n0, P.n1) * sizeof(T);
     static size_t attr_set = 0;
     if (lds > attr_set) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_tail_spec<T>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spec_fwd<T>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_spec_inv<T>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr_set = lds;
     }
-    hipLaunchKernelGGL(k_mg_tail_spec<T>, dim3(1), dim3(1024), lds, s, P);
+    const int np = P.n0 * P.n1;
+    hipLaunchKernelGGL(k_spec_fwd<T>, dim3(P.n2), dim3(1024), lds, s, P);
+    hipLaunchKernelGGL(k_spec_tri<T>, dim3((np + 255) / 256), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_spec_inv<T>, dim3(P.n2), dim3(1024), lds, s, P);
 }
-void mg_tail_spec(hipStream_t s, const SpecTail<double>& P) { tail_spec_impl<double>(s, P); }
-void mg_tail_spec(hipStream_t s, const SpecTail<float>& P) { tail_spec_impl<float>(s, P); }
+void mg_spec_solve(hipStream_t s, const SpecTail<double>& P) { spec_solve_impl<double>(s, P); }
+void mg_spec_solve(hipStream_t s, const SpecTail<float>& P) { spec_solve_impl<float>(s, P); }
 void mg_tail_lay(hipStream_t s, const TailPlan<double>& P, const DBC& bc, int read_a, int nu1, int nu2, double* work,
                  double tol, int maxit, int* iters) {
     tail_lay_impl<double>(s, P, bc, read_a, nu1, nu2, work, tol, maxit, iters);

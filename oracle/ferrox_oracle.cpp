@@ -424,26 +424,22 @@ struct MG {
     }
 
     /* The level the V-cycle solves exactly instead of cycling below it (-1:
-     * none): the first level of the tail -- from the coarsest level up, the
-     * levels of at most 8192 cells, at least two of them -- when the hierarchy
-     * is periodic in x and y and bounded in z with a Dirichlet face, that
-     * level's face coefficients depend on z only, it has no Helmholtz term and
-     * the device's single-workgroup spectral solve of it fits its LDS budget
-     * (2N + 2 n0^2 + n1^2 + n0 + n1 + 5 n2 values of 8 bytes within 128 KiB).
-     * The device solves that level with lateral Fourier transforms and a
-     * tridiagonal solve per mode; here it is solved by repeated V-cycles of
-     * the levels below until the residual stops falling, which is the same
-     * solution to rounding. */
+     * none): on a hierarchy periodic in x and y and bounded in z with a
+     * Dirichlet face, the finest level below the finest with n0 n1 <= 4096,
+     * n0, n1 <= 64 and at most 65536 cells, when its face coefficients depend
+     * on z only and it has no Helmholtz term.  The device solves that level
+     * with lateral Fourier transforms and a tridiagonal solve per mode; here
+     * it is solved by repeated V-cycles of the levels below until the
+     * residual stops falling, which is the same solution to rounding. */
     int exact_tail = 1;
     int exact_level() const {
         if (!exact_tail || bclo[0] != FX_BC_PERIODIC || bclo[1] != FX_BC_PERIODIC || bclo[2] == FX_BC_PERIODIC) return -1;
         if (bclo[2] != FX_BC_DIRICHLET && bchi[2] != FX_BC_DIRICHLET) return -1;
         int t = -1;
-        for (int l = (int)L.size() - 1; l >= 1 && L[l].N <= 8192; --l) t = l;
-        if (t < 0 || (int)L.size() - t < 2) return -1;
+        for (size_t l = 1; l < L.size() && t < 0; ++l)
+            if (L[l].n[0] <= 64 && L[l].n[1] <= 64 && L[l].n[0] * L[l].n[1] <= 4096 && L[l].N <= 65536) t = (int)l;
+        if (t < 0) return -1;
         const Lev& v = L[t];
-        const size_t vals = 2 * v.N + 2 * (size_t)v.n[0] * v.n[0] + (size_t)v.n[1] * v.n[1] + v.n[0] + v.n[1] + 5 * (size_t)v.n[2];
-        if (vals * 8 > 131072) return -1;
         for (size_t c = 0; c < v.N; ++c) if (v.a[c] != 0.0) return -1;
         for (int k = 0; k < v.n[2]; ++k) {
             const double bx0 = v.bx[v.fx(0, 0, k)], by0 = v.by[v.fy(0, 0, k)];
