@@ -320,24 +320,6 @@ __global__ void k_b_f2d(double* __restrict__ d, const float* __restrict__ s, lon
 
 constexpr int AXR = 16;        /* vectors per transform tile */
 
-/* One 16x16x4 matrix-core step per element type, C += A B with A 16x4 and B
- * 4x16: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; of the 16x16
- * result it holds four values, C[row(l, v)][l % 16] for v = 0..3 (the FP64
- * instruction interleaves the rows, the FP32 one keeps four consecutive rows
- * per lane group; both layouts were checked on the device by multiplying
- * known matrices). */
-template <class T> struct Mfma16;
-template <> struct Mfma16<double> {
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
-};
-template <> struct Mfma16<float> {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-
 /* The lateral transforms on the matrix cores: for the vectors of a 16-wide
  * tile (16 rows j..j+15 of a device for AXIS 0; 16 consecutive i of plane k
  * for AXIS 1) out_m = sum_q in_q M[q][m], as the 16x16 blocks

@@ -284,6 +284,24 @@ struct TailPlan {
     const int* perm;
 };
 
+/* One 16x16x4 matrix-core step per element type, C += A B with A 16x4 and B
+ * 4x16: lane l holds A[l % 16][l / 16] and B[l / 16][l % 16]; of the 16x16
+ * result it holds four values, C[row(l, v)][l % 16] for v = 0..3 (the FP64
+ * instruction interleaves the rows, the FP32 one keeps four consecutive rows
+ * per lane group; both layouts were checked on the device by multiplying
+ * known matrices). */
+template <class T> struct Mfma16;
+template <> struct Mfma16<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * v + (lane >> 4); }
+};
+template <> struct Mfma16<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
 /* The exact solve that ends the V-cycle of a laterally periodic layered
  * hierarchy (mg_spec_solve): one level, its operator block-diagonalised by
  * the real Fourier basis of each periodic lateral direction (column m of the
@@ -310,9 +328,10 @@ struct SpecTail {
     int dlo, dhi;
 };
 /* Plane kernels of the exact solve: dynamic LDS values (two planes and two
- * basis matrices). */
+ * basis matrices, every row padded by one value so that the matrix-core
+ * operand loads, 16 rows at a time, fall in distinct banks). */
 __host__ __device__ constexpr size_t spec_plane_values(int n0, int n1) {
-    return 2 * (size_t)n0 * n1 + (size_t)n0 * n0 + (size_t)n1 * n1;
+    return 2 * (size_t)n1 * (n0 + 1) + (size_t)n0 * (n0 + 1) + (size_t)n1 * (n1 + 1);
 }
 /* A level is solved exactly where n0*n1 <= SPEC_MAX_PLANE, n0, n1 <= SPEC_MAX_N
  * and it has at most SPEC_MAX_CELLS cells (the same rule as the oracle's). */

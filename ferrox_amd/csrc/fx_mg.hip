@@ -2208,35 +2208,51 @@ __global__ __launch_bounds__(1024) void k_mg_tail_lay(TailPlan<T> P, DBC bc, int
 /* ------------- exact spectral solve of a periodic layered level ------------ */
 /* A e = b on one level of a laterally periodic layered hierarchy, exactly (to
  * rounding), in three launches: every plane transformed along x and y by one
- * workgroup (x_m = sum_i f_i Qx[i][m], then along y), one Thomas solve in z
- * per mode (mx, my) with diagonal bx_k mu_x/dx^2 + by_k mu_y/dy^2 plus the z
- * couplings of the row operator (2w across a Dirichlet face, 0 across a
- * Neumann one), and the inverse transforms (Q is orthogonal: its inverse is
- * its transpose).  Sums run over the basis index in increasing order. */
+ * workgroup on the matrix cores (with the plane as an n1 x n0 matrix F,
+ * S = Qy^T F Qx), one Thomas solve in z per mode (mx, my) with diagonal
+ * bx_k mu_x/dx^2 + by_k mu_y/dy^2 plus the z couplings of the row operator
+ * (2w across a Dirichlet face, 0 across a Neumann one), and the inverse
+ * transforms (Q is orthogonal: F = Qy S Qx^T). */
 namespace {
-/* out[e] = sum_t v[t*vs] q[t*qs] for e < cnt, the vector v and basis column q
- * of output e given by at(e, v, q); four outputs per thread at a time, kept as
- * independent chains */
-template <class T, class At>
-__device__ __forceinline__ void spec_transform(int tid, int cnt, At at, int len, int vs, int qs, T* dst) {
-    constexpr int BS = 1024, SPC = 4;
-    for (int e0 = tid; e0 < cnt; e0 += SPC * BS) {
-        T acc[SPC];
-        const T* v[SPC];
-        const T* q[SPC];
-#pragma unroll
-        for (int u = 0; u < SPC; ++u) {
-            acc[u] = T(0);
-            at(min(e0 + u * BS, cnt - 1), v[u], q[u]);
+/* C = X Y for an R x K and a K x N operand, R, K, N <= 64, on the matrix
+ * cores: X[r][k] = x[r*xr + k*xk], Y[k][c] = y[k*yk + c*yc], each entry of C
+ * handed to put(r, c, value).  The 16x16 output tiles are shared out over the
+ * workgroup's waves; rows and columns past R, K, N read as zero. */
+template <class T, class Put>
+__device__ __forceinline__ void spec_mm(int R, int K, int N, const T* x, int xr, int xk, const T* y, int yk, int yc, Put put) {
+    using MF = Mfma16<T>;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int tr = (R + 15) >> 4, tc = (N + 15) >> 4;
+    for (int t = wave; t < tr * tc; t += nw) {
+        const int r0 = (t / tc) * 16, c0 = (t - (t / tc) * tc) * 16;
+        const int r = r0 + li, c = c0 + li;
+        typename MF::v4 acc = {T(0), T(0), T(0), T(0)};
+        for (int k0 = 0; k0 < K; k0 += 4) {
+            const int k = k0 + lk;
+            const T a = (r < R && k < K) ? x[r * xr + k * xk] : T(0);
+            const T b = (k < K && c < N) ? y[k * yk + c * yc] : T(0);
+            acc = MF::mma(a, b, acc);
         }
-#pragma unroll 4
-        for (int t = 0; t < len; ++t) {
 #pragma unroll
-            for (int u = 0; u < SPC; ++u) acc[u] += v[u][t * vs] * q[u][t * qs];
+        for (int v = 0; v < 4; ++v) {
+            const int rr = r0 + MF::row(lane, v);
+            if (rr < R && c < N) put(rr, c, acc[v]);
         }
-#pragma unroll
-        for (int u = 0; u < SPC; ++u)
-            if (e0 + u * BS < cnt) dst[e0 + u * BS] = acc[u];
+    }
+}
+
+/* the basis matrices into LDS with padded rows (lx = n0 + 1, ly = n1 + 1) */
+template <class T>
+__device__ __forceinline__ void spec_load_basis(const SpecTail<T>& P, T* Qx, T* Qy) {
+    const int n0 = P.n0, n1 = P.n1;
+    for (int c = threadIdx.x; c < n0 * n0; c += blockDim.x) {
+        const int i = c / n0;
+        Qx[c + i] = P.Qx[c];
+    }
+    for (int c = threadIdx.x; c < n1 * n1; c += blockDim.x) {
+        const int j = c / n1;
+        Qy[c + j] = P.Qy[c];
     }
 }
 }  // namespace
@@ -2245,22 +2261,23 @@ __device__ __forceinline__ void spec_transform(int tid, int cnt, At at, int len,
 template <class T>
 __global__ __launch_bounds__(1024) void k_spec_fwd(SpecTail<T> P) {
     extern __shared__ __align__(16) unsigned char spec_lds[];
-    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, tid = threadIdx.x;
+    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, lx = n0 + 1, ly = n1 + 1;
     T* const A = reinterpret_cast<T*>(spec_lds);
-    T* const B = A + s;
-    T* const Qx = B + s;
-    T* const Qy = Qx + n0 * n0;
-    for (int c = tid; c < s; c += 1024) A[c] = P.b[(size_t)k * s + c];
-    for (int c = tid; c < n0 * n0; c += 1024) Qx[c] = P.Qx[c];
-    for (int c = tid; c < n1 * n1; c += 1024) Qy[c] = P.Qy[c];
+    T* const B = A + n1 * lx;
+    T* const Qx = B + n1 * lx;
+    T* const Qy = Qx + n0 * lx;
+    for (int c = threadIdx.x; c < s; c += blockDim.x) {
+        const int j = c / n0;
+        A[c + j] = P.b[(size_t)k * s + c];
+    }
+    spec_load_basis(P, Qx, Qy);
     __syncthreads();
-    /* B[m + n0 j] = sum_i A[i + n0 j] Qx[i][m] */
-    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int j = e / n0; v = A + j * n0; q = Qx + (e - j * n0); },
-                      n0, 1, n0, B);
+    /* B[j][m] = sum_i A[j][i] Qx[i][m] */
+    spec_mm<T>(n1, n0, n0, A, lx, 1, Qx, lx, 1, [&](int r, int c, T v) { B[r * lx + c] = v; });
     __syncthreads();
-    /* S[m + n0 q] = sum_j B[m + n0 j] Qy[j][q] */
-    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int qq = e / n0; v = B + (e - qq * n0); q = Qy + qq; },
-                      n1, n0, n1, P.S + (size_t)k * s);
+    /* S[q][m] = sum_j Qy[j][q] B[j][m] */
+    T* const S = P.S + (size_t)k * s;
+    spec_mm<T>(n1, n1, n0, Qy, 1, ly, B, lx, 1, [&](int r, int c, T v) { S[r * n0 + c] = v; });
 }
 
 /* one thread per mode: the tridiagonal system in z (S in, S out, C scratch) */
@@ -2298,25 +2315,23 @@ __global__ __launch_bounds__(256) void k_spec_tri(SpecTail<T> P) {
 template <class T>
 __global__ __launch_bounds__(1024) void k_spec_inv(SpecTail<T> P) {
     extern __shared__ __align__(16) unsigned char spec_lds[];
-    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, tid = threadIdx.x;
+    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, lx = n0 + 1, ly = n1 + 1;
     T* const A = reinterpret_cast<T*>(spec_lds);
-    T* const B = A + s;
-    T* const QxT = B + s;                  /* the transpose: the x sums read it along a row */
-    T* const Qy = QxT + n0 * n0;
-    for (int c = tid; c < s; c += 1024) A[c] = P.S[(size_t)k * s + c];
-    for (int c = tid; c < n0 * n0; c += 1024) {
-        const int i = c / n0, m = c - i * n0;
-        QxT[m * n0 + i] = P.Qx[c];
+    T* const B = A + n1 * lx;
+    T* const Qx = B + n1 * lx;
+    T* const Qy = Qx + n0 * lx;
+    for (int c = threadIdx.x; c < s; c += blockDim.x) {
+        const int j = c / n0;
+        A[c + j] = P.S[(size_t)k * s + c];
     }
-    for (int c = tid; c < n1 * n1; c += 1024) Qy[c] = P.Qy[c];
+    spec_load_basis(P, Qx, Qy);
     __syncthreads();
-    /* B[m + n0 j] = sum_q A[m + n0 q] Qy[j][q] */
-    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int j = e / n0; v = A + (e - j * n0); q = Qy + j * n1; },
-                      n1, n0, 1, B);
+    /* B[j][m] = sum_q Qy[j][q] A[q][m] */
+    spec_mm<T>(n1, n1, n0, Qy, ly, 1, A, lx, 1, [&](int r, int c, T v) { B[r * lx + c] = v; });
     __syncthreads();
-    /* phi[i + n0 j] = sum_m B[m + n0 j] Qx[i][m] */
-    spec_transform<T>(tid, s, [&](int e, const T*& v, const T*& q) { const int j = e / n0; v = B + j * n0; q = QxT + (e - j * n0); },
-                      n0, 1, n0, P.phi + (size_t)k * s);
+    /* phi[j][i] = sum_m B[j][m] Qx[i][m] */
+    T* const phi = P.phi + (size_t)k * s;
+    spec_mm<T>(n1, n0, n0, B, lx, 1, Qx, 1, lx, [&](int r, int c, T v) { phi[r * n0 + c] = v; });
 }
 
 /* --------------------- FP64 refinement of an FP32 correction -------------- */
