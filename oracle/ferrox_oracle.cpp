@@ -466,7 +466,7 @@ struct MG {
             vcycle(li, nullptr, false);
             residual(l, nullptr);
             const double r = norm_inf(l.r);
-            if (r <= 1e-15 * bn || r >= 0.5 * prev) break;
+            if (r <= 1e-15 * bn || r >= prev) break;
             prev = r;
         }
     }
