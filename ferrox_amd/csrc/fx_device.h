@@ -327,11 +327,13 @@ struct SpecTail {
     T ih2[3];
     int dlo, dhi;
 };
-/* Plane kernels of the exact solve: dynamic LDS values (two planes and two
- * basis matrices, every row padded by one value so that the matrix-core
- * operand loads, 16 rows at a time, fall in distinct banks). */
+/* Plane kernels of the exact solve (one workgroup per plane and block of 16
+ * columns): dynamic LDS values, the plane and the y basis with every row
+ * padded by one value so that the matrix-core operand loads, 16 rows at a
+ * time, fall in distinct banks, and two n0 x 17 blocks (16 columns of the x
+ * basis, 16 columns of the intermediate product). */
 __host__ __device__ constexpr size_t spec_plane_values(int n0, int n1) {
-    return 2 * (size_t)n1 * (n0 + 1) + (size_t)n0 * (n0 + 1) + (size_t)n1 * (n1 + 1);
+    return (size_t)n1 * (n0 + 1) + (size_t)n1 * (n1 + 1) + 2 * (size_t)(n0 > n1 ? n0 : n1) * 17;
 }
 /* A level is solved exactly where n0*n1 <= SPEC_MAX_PLANE, n0, n1 <= SPEC_MAX_N
  * and it has at most SPEC_MAX_CELLS cells (the same rule as the oracle's). */

@@ -2242,96 +2242,144 @@ __device__ __forceinline__ void spec_mm(int R, int K, int N, const T* x, int xr,
     }
 }
 
-/* the basis matrices into LDS with padded rows (lx = n0 + 1, ly = n1 + 1) */
-template <class T>
-__device__ __forceinline__ void spec_load_basis(const SpecTail<T>& P, T* Qx, T* Qy) {
-    const int n0 = P.n0, n1 = P.n1;
-    for (int c = threadIdx.x; c < n0 * n0; c += blockDim.x) {
-        const int i = c / n0;
-        Qx[c + i] = P.Qx[c];
-    }
-    for (int c = threadIdx.x; c < n1 * n1; c += blockDim.x) {
-        const int j = c / n1;
-        Qy[c + j] = P.Qy[c];
-    }
-}
 }  // namespace
 
-/* plane k = blockIdx.x: b -> S, along x then y */
+/* The plane kernels split each plane by blocks of 16 output columns, one
+ * workgroup per (column block, plane): column block c of
+ * S = Qy^T (F Qx) needs only columns c of Qx, and column block c of
+ * F = Qy (S Qx^T) only rows c of Qx.  LDS: the plane and Qy with rows padded
+ * to n + 1 values, the 16 columns of the block (padded to 17) for Qx and for
+ * the intermediate product. */
+constexpr int SPEC_CB = 16, SPEC_CBL = SPEC_CB + 1, SPEC_THR = 1024;   /* 16 waves to stage, 4 to multiply */
+
+/* plane k = blockIdx.y, output columns m0.. = 16 blockIdx.x: b -> S */
 template <class T>
-__global__ __launch_bounds__(1024) void k_spec_fwd(SpecTail<T> P) {
+__global__ __launch_bounds__(SPEC_THR) void k_spec_fwd(SpecTail<T> P) {
     extern __shared__ __align__(16) unsigned char spec_lds[];
-    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, lx = n0 + 1, ly = n1 + 1;
+    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.y, lx = n0 + 1, ly = n1 + 1;
+    const int m0 = blockIdx.x * SPEC_CB, nb = min(SPEC_CB, n0 - m0);
     T* const A = reinterpret_cast<T*>(spec_lds);
-    T* const B = A + n1 * lx;
-    T* const Qx = B + n1 * lx;
-    T* const Qy = Qx + n0 * lx;
-    for (int c = threadIdx.x; c < s; c += blockDim.x) {
+    T* const Qy = A + n1 * lx;
+    T* const Qc = Qy + n1 * ly;            /* Qc[i][mm] = Qx[i][m0 + mm] */
+    T* const Bc = Qc + n0 * SPEC_CBL;      /* Bc[j][mm] = (F Qx)[j][m0 + mm] */
+    for (int c = threadIdx.x; c < s; c += SPEC_THR) {
         const int j = c / n0;
         A[c + j] = P.b[(size_t)k * s + c];
     }
-    spec_load_basis(P, Qx, Qy);
+    for (int c = threadIdx.x; c < n1 * n1; c += SPEC_THR) {
+        const int j = c / n1;
+        Qy[c + j] = P.Qy[c];
+    }
+    for (int c = threadIdx.x; c < n0 * SPEC_CB; c += SPEC_THR) {
+        const int i = c / SPEC_CB, mm = c - i * SPEC_CB;
+        Qc[i * SPEC_CBL + mm] = mm < nb ? P.Qx[i * n0 + m0 + mm] : T(0);
+    }
     __syncthreads();
-    /* B[j][m] = sum_i A[j][i] Qx[i][m] */
-    spec_mm<T>(n1, n0, n0, A, lx, 1, Qx, lx, 1, [&](int r, int c, T v) { B[r * lx + c] = v; });
+    /* Bc[j][mm] = sum_i A[j][i] Qx[i][m0 + mm] */
+    spec_mm<T>(n1, n0, nb, A, lx, 1, Qc, SPEC_CBL, 1, [&](int r, int c, T v) { Bc[r * SPEC_CBL + c] = v; });
     __syncthreads();
-    /* S[q][m] = sum_j Qy[j][q] B[j][m] */
-    T* const S = P.S + (size_t)k * s;
-    spec_mm<T>(n1, n1, n0, Qy, 1, ly, B, lx, 1, [&](int r, int c, T v) { S[r * n0 + c] = v; });
+    /* S[q][m0 + mm] = sum_j Qy[j][q] Bc[j][mm] */
+    T* const S = P.S + (size_t)k * s + m0;
+    spec_mm<T>(n1, n1, nb, Qy, 1, ly, Bc, SPEC_CBL, 1, [&](int r, int c, T v) { S[r * n0 + c] = v; });
 }
 
-/* one thread per mode: the tridiagonal system in z (S in, S out, C scratch) */
-template <class T>
-__global__ __launch_bounds__(256) void k_spec_tri(SpecTail<T> P) {
+/* one thread per mode: the tridiagonal system in z, its column of S held in
+ * registers (n2 <= NZ; NZ = 0: any n2, through global memory with C as
+ * scratch) */
+template <class T, int NZ>
+__global__ __launch_bounds__(64) void k_spec_tri(SpecTail<T> P) {
     const int n0 = P.n0, s = n0 * P.n1, n2 = P.n2;
-    const int mode = blockIdx.x * 256 + threadIdx.x;
+    const int mode = blockIdx.x * 64 + threadIdx.x;
     if (mode >= s) return;
     const int q = mode / n0, m = mode - q * n0;
     const T mx = P.mux[m], my = P.muy[q];
-    T cprev = T(0), dprev = T(0);
-    for (int k = 0; k < n2; ++k) {
+    /* row k of the system: a_k x_k-1 + d_k x_k + c_k x_k+1 = S_k */
+    auto coef = [&](int k, T& a, T& d, T& c) {
         const T wzl = P.bzk[k] * P.ih2[2], wzh = P.bzk[k + 1] * P.ih2[2];
         T diag = (P.bxk[k] * P.ih2[0]) * mx;
         diag += (P.byk[k] * P.ih2[1]) * my;
         diag += k > 0 ? wzl : (P.dlo ? T(2) * wzl : T(0));
         diag += k < n2 - 1 ? wzh : (P.dhi ? T(2) * wzh : T(0));
-        const T a = k > 0 ? -wzl : T(0), c = k < n2 - 1 ? -wzh : T(0);
-        const T rden = T(1) / (diag - a * cprev);
-        const T cq = c * rden;
-        const T dq = (P.S[mode + (size_t)k * s] - a * dprev) * rden;
-        P.C[mode + (size_t)k * s] = cq;
-        P.S[mode + (size_t)k * s] = dq;
-        cprev = cq;
-        dprev = dq;
-    }
-    T xn = T(0);
-    for (int k = n2 - 1; k >= 0; --k) {
-        xn = P.S[mode + (size_t)k * s] - P.C[mode + (size_t)k * s] * xn;
-        P.S[mode + (size_t)k * s] = xn;
+        a = k > 0 ? -wzl : T(0);
+        c = k < n2 - 1 ? -wzh : T(0);
+        d = diag;
+    };
+    if constexpr (NZ > 0) {
+        T sv[NZ], cv[NZ];
+#pragma unroll
+        for (int k = 0; k < NZ; ++k)
+            if (k < n2) sv[k] = P.S[mode + (size_t)k * s];
+        T cprev = T(0), dprev = T(0);
+#pragma unroll
+        for (int k = 0; k < NZ; ++k) {
+            if (k < n2) {
+                T a, d, c;
+                coef(k, a, d, c);
+                const T rden = T(1) / (d - a * cprev);
+                cv[k] = c * rden;
+                sv[k] = (sv[k] - a * dprev) * rden;
+                cprev = cv[k];
+                dprev = sv[k];
+            }
+        }
+        T xn = T(0);
+#pragma unroll
+        for (int k = NZ - 1; k >= 0; --k) {
+            if (k < n2) {
+                xn = sv[k] - cv[k] * xn;
+                P.S[mode + (size_t)k * s] = xn;
+            }
+        }
+    } else {
+        T cprev = T(0), dprev = T(0);
+        for (int k = 0; k < n2; ++k) {
+            T a, d, c;
+            coef(k, a, d, c);
+            const T rden = T(1) / (d - a * cprev);
+            const T cq = c * rden;
+            const T dq = (P.S[mode + (size_t)k * s] - a * dprev) * rden;
+            P.C[mode + (size_t)k * s] = cq;
+            P.S[mode + (size_t)k * s] = dq;
+            cprev = cq;
+            dprev = dq;
+        }
+        T xn = T(0);
+        for (int k = n2 - 1; k >= 0; --k) {
+            xn = P.S[mode + (size_t)k * s] - P.C[mode + (size_t)k * s] * xn;
+            P.S[mode + (size_t)k * s] = xn;
+        }
     }
 }
 
-/* plane k = blockIdx.x: S -> phi, back along y then x */
+/* plane k = blockIdx.y, output columns i0.. = 16 blockIdx.x: S -> phi */
 template <class T>
-__global__ __launch_bounds__(1024) void k_spec_inv(SpecTail<T> P) {
+__global__ __launch_bounds__(SPEC_THR) void k_spec_inv(SpecTail<T> P) {
     extern __shared__ __align__(16) unsigned char spec_lds[];
-    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.x, lx = n0 + 1, ly = n1 + 1;
+    const int n0 = P.n0, n1 = P.n1, s = n0 * n1, k = blockIdx.y, lx = n0 + 1, ly = n1 + 1;
+    const int i0 = blockIdx.x * SPEC_CB, nb = min(SPEC_CB, n0 - i0);
     T* const A = reinterpret_cast<T*>(spec_lds);
-    T* const B = A + n1 * lx;
-    T* const Qx = B + n1 * lx;
-    T* const Qy = Qx + n0 * lx;
-    for (int c = threadIdx.x; c < s; c += blockDim.x) {
+    T* const Qy = A + n1 * lx;
+    T* const Qr = Qy + n1 * ly;            /* Qr[m][ii] = Qx[i0 + ii][m] */
+    T* const Tc = Qr + n0 * SPEC_CBL;      /* Tc[q][ii] = (S Qx^T)[q][i0 + ii] */
+    for (int c = threadIdx.x; c < s; c += SPEC_THR) {
         const int j = c / n0;
         A[c + j] = P.S[(size_t)k * s + c];
     }
-    spec_load_basis(P, Qx, Qy);
+    for (int c = threadIdx.x; c < n1 * n1; c += SPEC_THR) {
+        const int j = c / n1;
+        Qy[c + j] = P.Qy[c];
+    }
+    for (int c = threadIdx.x; c < SPEC_CB * n0; c += SPEC_THR) {
+        const int ii = c / n0, m = c - ii * n0;
+        Qr[m * SPEC_CBL + ii] = ii < nb ? P.Qx[(i0 + ii) * n0 + m] : T(0);
+    }
     __syncthreads();
-    /* B[j][m] = sum_q Qy[j][q] A[q][m] */
-    spec_mm<T>(n1, n1, n0, Qy, ly, 1, A, lx, 1, [&](int r, int c, T v) { B[r * lx + c] = v; });
+    /* Tc[q][ii] = sum_m A[q][m] Qx[i0 + ii][m] */
+    spec_mm<T>(n1, n0, nb, A, lx, 1, Qr, SPEC_CBL, 1, [&](int r, int c, T v) { Tc[r * SPEC_CBL + c] = v; });
     __syncthreads();
-    /* phi[j][i] = sum_m B[j][m] Qx[i][m] */
-    T* const phi = P.phi + (size_t)k * s;
-    spec_mm<T>(n1, n0, n0, B, lx, 1, Qx, 1, lx, [&](int r, int c, T v) { phi[r * n0 + c] = v; });
+    /* phi[j][i0 + ii] = sum_q Qy[j][q] Tc[q][ii] */
+    T* const phi = P.phi + (size_t)k * s + i0;
+    spec_mm<T>(n1, n1, nb, Qy, ly, 1, Tc, SPEC_CBL, 1, [&](int r, int c, T v) { phi[r * n0 + c] = v; });
 }
 
 /* --------------------- FP64 refinement of an FP32 correction -------------- */
@@ -2831,9 +2879,11 @@ static void spec_solve_impl(hipStream_t s, const SpecTail<T>& P) {
         attr_set = lds;
     }
     const int np = P.n0 * P.n1;
-    hipLaunchKernelGGL(k_spec_fwd<T>, dim3(P.n2), dim3(1024), lds, s, P);
-    hipLaunchKernelGGL(k_spec_tri<T>, dim3((np + 255) / 256), dim3(256), 0, s, P);
-    hipLaunchKernelGGL(k_spec_inv<T>, dim3(P.n2), dim3(1024), lds, s, P);
+    const dim3 g((P.n0 + SPEC_CB - 1) / SPEC_CB, P.n2);
+    hipLaunchKernelGGL(k_spec_fwd<T>, g, dim3(SPEC_THR), lds, s, P);
+    if (P.n2 <= 16) hipLaunchKernelGGL((k_spec_tri<T, 16>), dim3((np + 63) / 64), dim3(64), 0, s, P);
+    else hipLaunchKernelGGL((k_spec_tri<T, 0>), dim3((np + 63) / 64), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_spec_inv<T>, g, dim3(SPEC_THR), lds, s, P);
 }
 void mg_spec_solve(hipStream_t s, const SpecTail<double>& P) { spec_solve_impl<double>(s, P); }
 void mg_spec_solve(hipStream_t s, const SpecTail<float>& P) { spec_solve_impl<float>(s, P); }

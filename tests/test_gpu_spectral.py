@@ -10,8 +10,9 @@ FP32 + refinement solve, and the exact solve must not need more V-cycles.
 
 The cases cover lateral sizes that are multiples of the 16x16 matrix-core
 tiles (the 64 x 64 stack) and sizes that are not (80 x 40: its exact level
-is 40 x 20 x 9, so every transform runs with padded tiles), and a stack with
-a Neumann top face.
+is 40 x 20 x 9, so every transform runs with padded tiles and a partial
+column block), a stack with a Neumann top face, and an exact level of 32
+planes (the tridiagonal solves through global memory).
 """
 import ctypes as C
 
@@ -38,6 +39,9 @@ CASES = {
     "mfim_64": lambda: configs.mfim_64(nsteps=1),
     "padded_80x40": lambda: configs.mfim(nx=80, ny=40, nz=36, lx=40e-9, ly=20e-9),
     "neumann_top": neumann_top,
+    # cubic cells: the exact level is 32 x 32 x 32, more planes than the
+    # register-resident tridiagonal solve takes (16), so the general one runs
+    "tall_32": lambda: configs.mfim(nx=64, ny=64, nz=64, lx=32e-9, ly=32e-9, lz=32e-9, fe_top=12e-9),
 }
 
 
