@@ -825,7 +825,7 @@ int read_red(fx_ctx* x, int nq, double* out) { return wait_red(x, post_red(x, nq
 
 /* --------------------------------- multigrid ------------------------------ */
 /* pre_done: the first pre-smoothing sweep of level 0 has already run
- * (phi -> phi2, queued speculatively behind the previous convergence check) */
+ * (phi -> phi2, queued speculatively with the previous convergence check) */
 void vcycle(fx_ctx* x, size_t li, const DBC& bc, bool pre_done = false) {
     Level& L = x->lev[li];
     const int inh = li == 0;
@@ -938,15 +938,30 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     const DBC bc = dbc_of(x, bv);
     Level& L0 = x->lev[0];
     double nr[2];
-    /* The first pre-smoothing sweep of the next V-cycle only reads phi (it
-     * writes phi2), so it is queued right behind the kernel that publishes
-     * each convergence check: it runs while the host reads the norms and
-     * queues the rest of the next V-cycle behind it.  When the check stops
-     * the solve, phi is the answer and phi2 is simply not used. */
+    /* Each convergence check reads the residual of phi, and the first
+     * pre-smoothing sweep of the next V-cycle only reads phi (it writes
+     * phi2): it is queued with the check, so it runs while the host waits for
+     * the norms and queues the rest of the cycle.  Where level 0 streams, the
+     * check is folded into that sweep (its residual is of its input; 58 us
+     * against 41 + 27 us for the sweep and a separate residual pass).  When
+     * the check stops the solve, phi is the answer and phi2 is not used.
+     * (Running the second pre-smoothing sweep speculatively as well, into a
+     * third buffer, hid the host's reaction time but was slower overall: the
+     * third 67 MB buffer of the 256x256x128 level pushes the sweeps' working
+     * set out of the 256 MB last-level cache.) */
     const bool spec = x->lev.size() > 1 && x->tail0 != 0 && x->in.mg_nu1 >= 1;
-    fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
-    unsigned v = post_red(x, 2);
-    bool pre = spec && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
+    bool pre = false;   /* the next V-cycle's first pre-smoothing sweep is queued */
+    auto check = [&](bool speculate) -> unsigned {
+        if (speculate && fxk::mg_sweep_first_norm(x->s, L0.d, bc, 1, x->in.mg_fused_sweep, x->red, x->red_out)) {
+            pre = true;
+            return post_red(x, 2);
+        }
+        fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
+        const unsigned q = post_red(x, 2);
+        pre = speculate && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
+        return q;
+    };
+    unsigned v = check(spec);
     int rc = wait_red(x, v, 2, nr);
     if (rc) return rc;
     const double r0 = nr[0], bn = nr[1];
@@ -956,9 +971,7 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
     if (r0 <= target) return FX_OK;
     for (int it = 1; it <= maxit; ++it) {
         vcycle(x, 0, bc, pre);
-        fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
-        v = post_red(x, 2);
-        pre = spec && it < maxit && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
+        v = check(spec && it < maxit);
         if ((rc = wait_red(x, v, 2, nr))) return rc;
         st->mg_resid = nr[0]; st->vcycles = it;
         if (!(nr[0] == nr[0])) return fail(x, FX_ENAN, "multigrid produced a non-finite residual");

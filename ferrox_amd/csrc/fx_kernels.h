@@ -108,6 +108,12 @@ bool fused_ok(const fxd::DLev& l, const fxd::DBC& bc);
  * as it was; false when the level is not streamed one sweep per pass), and
  * the remaining sweeps from phi2, the result in phi */
 bool mg_sweep_first(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int allow_fused);
+/* mg_sweep_first that also leaves max|r| and max|b| of its input phi in
+ * out[0], out[1] (mg_resid_norm's values, folded into the sweep; part holds
+ * two partials per workgroup); false, with nothing queued, where the level
+ * has a Helmholtz term or is not streamed */
+bool mg_sweep_first_norm(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int allow_fused, double* part,
+                         double* out);
 void mg_sweep_rest(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused);
 bool fused_ok32(const fxd::DLevF& l, const fxd::DBC& bc);
 /* residual of f restricted into c's rhs; c's phi is set to 0 unless zero_cphi

@@ -102,6 +102,7 @@ struct StreamArgs {
     int ntx, nty, cz, remap;
     const T* __restrict__ ec;   /* PROLONG: coarse correction (z-coarsened level, same n0 x n1) */
     int cn2;                    /* its number of planes */
+    double* part;               /* NORM: max|r| and max|b| of the input phi, per workgroup */
 };
 
 namespace {
@@ -112,6 +113,7 @@ struct StreamRegs {
     T b[3];          /* rhs of red cells u = 0, 1 and of the black cell */
     T a[3];          /* Helmholtz term of the same cells */
     uint8_t m[3];    /* their material id */
+    typename Vec2<T>::type bp;   /* NORM: rhs of the thread's pair of cells on the step's plane */
 };
 /* geometry of one of this thread's cells, fixed for the whole sweep */
 struct StreamCell {
@@ -150,10 +152,11 @@ struct StreamCell {
  * workgroups round-robin) works on one contiguous range of tiles and the
  * in-plane halos are shared in that XCD's L2.  Requires: z not periodic, n0
  * even, n0 >= 64, n1 >= 8, periodic extents even, cz even. */
-template <class T, int AMODE, int PROLONG>
+template <class T, int AMODE, int PROLONG, int NORM>
 __global__ __launch_bounds__(SGeo<T>::THR) void k_mg_gsrb_stream(StreamArgs<T> A) {
     using G = SGeo<T>;
     constexpr int STX = G::TX, STY = G::TY, STHR = G::THR, SHX = G::HX, SPL = G::PL, SLD = G::LD, NRED = G::NRED;
+    static_assert(!NORM || (AMODE == 0 && !PROLONG), "the residual norm is folded into plain sweeps only");
     __shared__ T sp[SNS * SPL];
     __shared__ T scw[NPW * (SCZ_MAX + 2)];
     const int n0 = A.n0, n1 = A.n1, n2 = A.n2;
@@ -272,6 +275,7 @@ __global__ __launch_bounds__(SGeo<T>::THR) void k_mg_gsrb_stream(StreamArgs<T> A
         R.b[2] = A.b[c];
         if (AMODE) R.a[2] = ap[c];
         if (AMODE == 1) R.m[2] = mp[blk[1 - PAR].g2 + gplane * (kb + 1)];
+        if constexpr (NORM != 0) R.bp = *reinterpret_cast<const typename Vec2<T>::type*>(A.b + (bstore ? bi0 + n0 * bj : 0) + plane * kr);
     };
     auto a_use = [&](const StreamRegs<T>& R, int u) -> T {
         if (AMODE == 0) return T(0);
@@ -338,6 +342,54 @@ __global__ __launch_bounds__(SGeo<T>::THR) void k_mg_gsrb_stream(StreamArgs<T> A
         mg_acc(lv, A.bc, A.inh != 0, cl.i, cl.j, k, av, wc, nb, D, S);
         return (bv + S) / D;
     };
+    /* NORM: r = b - A phi of the input phi on the thread's pair of cells of
+     * plane p, from the old values in LDS before the red phase of step p
+     * replaces them -- except the red value below the black cell, which the
+     * red phase of step p-1 has replaced: the pair's old red value is kept in
+     * a register from one step to the next (the cell below a black cell is
+     * the red cell of the same pair one plane down).  The arithmetic of the
+     * row operator as in k_mg_resid_stream. */
+    T red_below = T(0);
+    double mr = 0.0, mb = 0.0;
+    auto norm_pair = [&](const StreamRegs<T>& R, int p) {
+        if (!(p >= 0 && p < n2 && bstore)) return;
+        const int ob = (bj + p + 1) & 1;          /* 1: the pair's first cell is red */
+        const int q0 = (2 + 2 * bx2) + SHX * bhy;
+        const int qr = q0 + (ob ? 0 : 1), qb = q0 + (ob ? 1 : 0);
+        const T* pl = slot(p - 1);
+        const T* pm = slot(p);
+        const T* pu = slot(p + 1);
+        const T rb = red_below;
+        red_below = pm[qr];
+        if (p < z0 || p >= z1) return;
+        const T* t = scw + NPW * (p - z0 + 1);
+        const T wx = t[0], wy = t[1], wzl = t[2], wzh = t[3];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            const int q = e == 0 ? qr : qb;
+            const int ci = bi0 + (q - q0);
+            const T n0v = pm[q - 1], n1v = pm[q + 1], n2v = pm[q - SHX], n3v = pm[q + SHX];
+            const T n4v = e == 0 ? pl[q] : rb, n5v = pu[q];
+            const T pc = pm[q];
+            const bool lat = (perx || (ci > 0 && ci < n0 - 1)) && (pery || (bj > 0 && bj < n1 - 1));
+            T D, S;
+            if (lat && p > 0 && p < n2 - 1) {
+                S = T(0);
+                S += wx * n0v; S += wx * n1v; S += wy * n2v; S += wy * n3v; S += wzl * n4v; S += wzh * n5v;
+                D = t[4];
+            } else {
+                DLevT<T> lv;
+                lv.n0 = n0; lv.n1 = n1; lv.n2 = n2;
+                const T wc[6] = {wx, wx, wy, wy, wzl, wzh};
+                const T nb[6] = {n0v, n1v, n2v, n3v, n4v, n5v};
+                mg_acc(lv, A.bc, A.inh != 0, ci, bj, p, T(0), wc, nb, D, S);
+            }
+            const T bv = (q - q0) ? R.bp.y : R.bp.x;
+            const T r = bv - (D * pc - S);
+            mr = nmax(mr, fabs((double)r));
+            mb = nmax(mb, fabs((double)bv));
+        }
+    };
     /* the black plane computed by the previous step, stored one step late so
      * that its completion is never waited for right after it is issued */
     using T2 = typename Vec2<T>::type;
@@ -355,6 +407,10 @@ __global__ __launch_bounds__(SGeo<T>::THR) void k_mg_gsrb_stream(StreamArgs<T> A
         issue(nxt, p + 1, std::integral_constant<int, 1 - PAR>{});
         if (pend) *reinterpret_cast<T2*>(pptr) = pv;
         __syncthreads();
+        if constexpr (NORM != 0) {   /* the old values are read before any red cell of plane p changes */
+            norm_pair(cur, p);
+            __syncthreads();
+        }
         if (p >= 0 && p < n2) {       /* red on plane p */
             const T* pl = slot(p - 1);
             T* pm = slot(p);
@@ -398,6 +454,12 @@ __global__ __launch_bounds__(SGeo<T>::THR) void k_mg_gsrb_stream(StreamArgs<T> A
         if (p + 1 <= z1) step(p + 1, R1, R0, P0{});
     }
     if (pend) *reinterpret_cast<T2*>(pptr) = pv;
+    if constexpr (NORM != 0) {
+        __shared__ double shn[STHR / 64 + 1];
+        mr = block_max<STHR>(mr, shn);
+        mb = block_max<STHR>(mb, shn);
+        if (tid == 0) { A.part[blockIdx.x] = mr; A.part[gridDim.x + blockIdx.x] = mb; }
+    }
 }
 
 /* -------------- two red-black sweeps in one streaming pass ---------------- */
@@ -2475,7 +2537,8 @@ static void prolong_impl(hipStream_t s, const DLevT<T>& f, const DLevT<T>& c, co
  * by k_mg_prolong_blk. */
 template <class T>
 static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh, int nsweeps, int allow_fused, int zero_first,
-                       const DLevT<T>* coarse = nullptr, bool start_alt = false, bool keep_alt = false) {
+                       const DLevT<T>* coarse = nullptr, bool start_alt = false, bool keep_alt = false,
+                       double* norm_part = nullptr, double* norm_out = nullptr) {
     const bool zonly = coarse && l.crs[0] == 0 && l.crs[1] == 0 && l.crs[2] == 1 && coarse->n0 == l.n0 && coarse->n1 == l.n1;
     if (coarse && !(zonly && nsweeps > 0 && stream_use(l, bc, allow_fused))) {
         prolong_impl<T>(s, l, *coarse, bc);
@@ -2532,8 +2595,13 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
             A.phi_out = nxt;
             A.ec = pro ? coarse->phi : nullptr;
             A.cn2 = pro ? coarse->n2 : 0;
-#define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr>), dim3(nblk), dim3(SGeo<T>::THR), 0, s, A)
-            if (pro) {
+#define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr, 0>), dim3(nblk), dim3(SGeo<T>::THR), 0, s, A)
+            if (norm_part && it == 0 && !pro && l.read_a == 0) {
+                /* the first sweep also reduces max|r| and max|b| of its input */
+                A.part = norm_part;
+                hipLaunchKernelGGL((k_mg_gsrb_stream<T, 0, 0, 1>), dim3(nblk), dim3(SGeo<T>::THR), 0, s, A);
+                hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, norm_part, nblk, 2, 1, norm_out);
+            } else if (pro) {
                 switch (l.read_a) { case 0: SWEEP(0, 1); break; case 1: SWEEP(1, 1); break; default: SWEEP(2, 1); break; }
             } else {
                 switch (l.read_a) { case 0: SWEEP(0, 0); break; case 1: SWEEP(1, 0); break; default: SWEEP(2, 0); break; }
@@ -2559,6 +2627,11 @@ void mg_sweep(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps,
 bool mg_sweep_first(hipStream_t s, const DLev& l, const DBC& bc, int inh, int allow_fused) {
     if (allow_fused != 1 || !stream_use(l, bc, allow_fused)) return false;
     sweep_impl<double>(s, l, bc, inh, 1, allow_fused, 0, nullptr, false, true);
+    return true;
+}
+bool mg_sweep_first_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, int allow_fused, double* part, double* out) {
+    if (allow_fused != 1 || !stream_use(l, bc, allow_fused) || l.read_a != 0) return false;
+    sweep_impl<double>(s, l, bc, inh, 1, allow_fused, 0, nullptr, false, true, part, out);
     return true;
 }
 void mg_sweep_rest(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps, int allow_fused) {
