@@ -934,7 +934,7 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true, bool pre_d
 /* MLMG-style solve (stopping rule ||r||_inf <= tol*max(||b||_inf, ||r0||_inf),
  * tol_abs ignored as in MLMG::solve(..., 1e-10, -1)).  Level 0 holds phi
  * (initial guess), rhs and the A-coefficient. */
-int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false) {
     const DBC bc = dbc_of(x, bv);
     Level& L0 = x->lev[0];
     double nr[2];
@@ -951,27 +951,33 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
      * set out of the 256 MB last-level cache.) */
     const bool spec = x->lev.size() > 1 && x->tail0 != 0 && x->in.mg_nu1 >= 1;
     bool pre = false;   /* the next V-cycle's first pre-smoothing sweep is queued */
-    auto check = [&](bool speculate) -> unsigned {
-        if (speculate && fxk::mg_sweep_first_norm(x->s, L0.d, bc, 1, x->in.mg_fused_sweep, x->red, x->red_out)) {
+    /* zero: phi_zero's first check, which takes phi as 0 without reading it */
+    auto check = [&](bool speculate, bool zero) -> unsigned {
+        if (speculate && fxk::mg_sweep_first_norm(x->s, L0.d, bc, 1, x->in.mg_fused_sweep, x->red, x->red_out, zero)) {
             pre = true;
             return post_red(x, 2);
         }
+        if (zero) fxk::fill(x->s, L0.d.phi, (long long)L0.N, 0.0);
         fxk::mg_resid_norm(x->s, L0.d, bc, 1, nullptr, x->red, x->red_out);
         const unsigned q = post_red(x, 2);
         pre = speculate && fxk::mg_sweep_first(x->s, L0.d, bc, 1, x->in.mg_fused_sweep);
         return q;
     };
-    unsigned v = check(spec);
+    unsigned v = check(spec, phi_zero);
+    const bool zero_left = phi_zero && pre;   /* phi itself was never written */
     int rc = wait_red(x, v, 2, nr);
     if (rc) return rc;
     const double r0 = nr[0], bn = nr[1];
     const double target = tol * std::max(bn, r0);
     st->mg_resid0 = r0; st->mg_bnorm = bn; st->mg_resid = r0; st->vcycles = 0;
     if (!(r0 == r0) || !(bn == bn)) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
-    if (r0 <= target) return FX_OK;
+    if (r0 <= target) {
+        if (zero_left) fxk::fill(x->s, L0.d.phi, (long long)L0.N, 0.0);
+        return FX_OK;
+    }
     for (int it = 1; it <= maxit; ++it) {
         vcycle(x, 0, bc, pre);
-        v = check(spec && it < maxit);
+        v = check(spec && it < maxit, false);
         if ((rc = wait_red(x, v, 2, nr))) return rc;
         st->mg_resid = nr[0]; st->vcycles = it;
         if (!(nr[0] == nr[0])) return fail(x, FX_ENAN, "multigrid produced a non-finite residual");
@@ -1036,8 +1042,10 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
     return fail(x, FX_ENOCONV_MG, "mixed-precision multigrid did not converge in mg_max_iter V-cycles");
 }
 
-int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+/* phi_zero: the solve starts from phi = 0, whatever phi holds */
+int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false) {
     if (x->in.precision == FX_PREC_FP32_REFINE) {
+        if (phi_zero) fxk::fill(x->s, x->phi, (long long)x->N, 0.0);
         /* FP32 copy of the A-coefficient for the correction levels */
         if (x->lev32[0].read_a) {
             fxk::d2f(x->s, x->lev32[0].a, x->alpha_cc, (long long)x->N);
@@ -1047,7 +1055,7 @@ int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
         }
         return mg_solve_mixed(x, bv, tol, maxit, st);
     }
-    return mg_solve(x, bv, tol, maxit, st);
+    return mg_solve(x, bv, tol, maxit, st, phi_zero);
 }
 
 void restrict_acoef(fx_ctx* x) {
@@ -1077,9 +1085,8 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
         fxk::rhs_newton(x->s, rhs_args(x, P));
         set_read_a(x, x->has_sc ? 1 : 0);
         if (x->has_sc) restrict_acoef(x);
-        if (!x->in.mg_warm_start) fxk::fill(x->s, x->phi, (long long)x->N, 0.0);
         fx_solve_stats one{};
-        int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one);
+        int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one, !x->in.mg_warm_start);
         st->vcycles += one.vcycles;
         st->mg_resid0 = one.mg_resid0; st->mg_resid = one.mg_resid; st->mg_bnorm = one.mg_bnorm;
         if (rc) { st->outer_iters = iter + 1; return rc; }

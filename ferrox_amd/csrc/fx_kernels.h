@@ -110,10 +110,11 @@ bool fused_ok(const fxd::DLev& l, const fxd::DBC& bc);
 bool mg_sweep_first(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int allow_fused);
 /* mg_sweep_first that also leaves max|r| and max|b| of its input phi in
  * out[0], out[1] (mg_resid_norm's values, folded into the sweep; part holds
- * two partials per workgroup); false, with nothing queued, where the level
- * has a Helmholtz term or is not streamed */
+ * two partials per workgroup); zero: the input is phi = 0, not read; false,
+ * with nothing queued, where the level has a Helmholtz term or is not
+ * streamed */
 bool mg_sweep_first_norm(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int allow_fused, double* part,
-                         double* out);
+                         double* out, bool zero = false);
 void mg_sweep_rest(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int nsweeps, int allow_fused);
 bool fused_ok32(const fxd::DLevF& l, const fxd::DBC& bc);
 /* residual of f restricted into c's rhs; c's phi is set to 0 unless zero_cphi

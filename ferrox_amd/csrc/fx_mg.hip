@@ -2596,7 +2596,7 @@ static void sweep_impl(hipStream_t s, const DLevT<T>& l, const DBC& bc, int inh,
             A.ec = pro ? coarse->phi : nullptr;
             A.cn2 = pro ? coarse->n2 : 0;
 #define SWEEP(am, pr) hipLaunchKernelGGL((k_mg_gsrb_stream<T, am, pr, 0>), dim3(nblk), dim3(SGeo<T>::THR), 0, s, A)
-            if (norm_part && it == 0 && !pro && l.read_a == 0) {
+            if (norm_part && it == 0 && !pro && l.read_a == 0) {   /* zero_init as set above */
                 /* the first sweep also reduces max|r| and max|b| of its input */
                 A.part = norm_part;
                 hipLaunchKernelGGL((k_mg_gsrb_stream<T, 0, 0, 1>), dim3(nblk), dim3(SGeo<T>::THR), 0, s, A);
@@ -2629,9 +2629,10 @@ bool mg_sweep_first(hipStream_t s, const DLev& l, const DBC& bc, int inh, int al
     sweep_impl<double>(s, l, bc, inh, 1, allow_fused, 0, nullptr, false, true);
     return true;
 }
-bool mg_sweep_first_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, int allow_fused, double* part, double* out) {
+bool mg_sweep_first_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, int allow_fused, double* part, double* out,
+                         bool zero) {
     if (allow_fused != 1 || !stream_use(l, bc, allow_fused) || l.read_a != 0) return false;
-    sweep_impl<double>(s, l, bc, inh, 1, allow_fused, 0, nullptr, false, true, part, out);
+    sweep_impl<double>(s, l, bc, inh, 1, allow_fused, zero ? 1 : 0, nullptr, false, true, part, out);
     return true;
 }
 void mg_sweep_rest(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsweeps, int allow_fused) {
