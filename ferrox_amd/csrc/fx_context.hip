@@ -76,6 +76,11 @@ struct fx_ctx {
     double* e_den = nullptr;
     double* h_den = nullptr;
     double* alpha_cc = nullptr;
+    /* rho (alpha_cc) holds 0 in every cell, written so by this library and
+     * never handed out as a device pointer: without semiconductor cells the
+     * per-step rewrite with zeros is skipped */
+    bool rho_clean = false, alpha_clean = false;
+    bool rho_exposed = false, alpha_exposed = false;
     double* E[3] = {nullptr, nullptr, nullptr};
     double* GL[3] = {nullptr, nullptr, nullptr};
     double* GL1[3] = {nullptr, nullptr, nullptr};
@@ -1082,7 +1087,10 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
     std::memcpy(x->bc_solve, bv, sizeof(bv));
     st->outer_iters = 0; st->vcycles = 0; st->outer_err = 0.0;
     while (err > OUTER_TOL) {
-        fxk::rhs_newton(x->s, rhs_args(x, P));
+        RhsArgs ra0 = rhs_args(x, P);
+        if (!x->has_sc && x->alpha_clean) ra0.alpha = nullptr;   /* alpha is 0 outside SC cells: written once */
+        fxk::rhs_newton(x->s, ra0);
+        x->alpha_clean = !x->has_sc && !x->alpha_exposed;
         set_read_a(x, x->has_sc ? 1 : 0);
         if (x->has_sc) restrict_acoef(x);
         fx_solve_stats one{};
@@ -1094,7 +1102,9 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
         for (int d = 0; d < 3; ++d) ra.n[d] = x->n[d];
         ra.M = mview_of(x); ra.phi = x->phi; ra.rho = x->rho; ra.e_den = x->e_den; ra.h_den = x->h_den;
         ra.phi_prev = x->phi_prev; ra.part = x->red; ra.want_err = x->has_sc; ra.sc = x->sc;
-        const int nbr = fxk::rho_update(x->s, ra);
+        int nbr = 0;
+        if (x->has_sc || !x->rho_clean) nbr = fxk::rho_update(x->s, ra);   /* rho = 0 everywhere without SC cells */
+        x->rho_clean = !x->has_sc && !x->rho_exposed;
         if (!x->has_sc) {
             err = 0.;
             st->outer_iters = 1;
@@ -1435,6 +1445,7 @@ int fx_set_field(fx_ctx* x, int f, const double* host) {
         HIPCHK(x, hipStreamSynchronize(x->s));
         return FX_OK;
     case FX_FIELD_CHARGE:
+        x->rho_clean = false;
         HIPCHK(x, hipMemcpyAsync(x->rho, host, B, hipMemcpyHostToDevice, x->s));
         HIPCHK(x, hipStreamSynchronize(x->s));
         return FX_OK;
@@ -1609,6 +1620,7 @@ int fx_poisson_solve(fx_ctx* x, const double* rhs, const double* acoef, double* 
     if (!x || !rhs || !phi) return FX_EINVAL;
     const size_t B = x->N * sizeof(double);
     HIPCHK(x, hipMemcpyAsync(x->rhs, rhs, B, hipMemcpyHostToDevice, x->s));
+    x->alpha_clean = false;
     if (acoef) HIPCHK(x, hipMemcpyAsync(x->alpha_cc, acoef, B, hipMemcpyHostToDevice, x->s));
     else fxk::fill(x->s, x->alpha_cc, (long long)x->N, 0.0);
     set_read_a(x, acoef ? 2 : 0);
@@ -1628,6 +1640,8 @@ int fx_field_device_ptr(fx_ctx* x, int f, void** dptr) {
     if (!x || !dptr) return FX_EINVAL;
     double* d = field_dev(x, f);
     if (!d) return fail(x, FX_EINVAL, "field has no device array");
+    if (d == x->rho) x->rho_exposed = true, x->rho_clean = false;
+    if (d == x->alpha_cc) x->alpha_exposed = true, x->alpha_clean = false;
     *dptr = d;
     return FX_OK;
 }

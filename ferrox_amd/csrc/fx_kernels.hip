@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void k_rhs_newton(RhsArgs a) {
         al = 0.0;
     }
     a.rhs[c] = rhs;
-    a.alpha[c] = al;
+    if (a.alpha) a.alpha[c] = al;   /* null: alpha is known to be 0 everywhere (no SC cells) */
 }
 
 /* rho(phi) in SC cells (0 elsewhere), optional L1 partial sums of
