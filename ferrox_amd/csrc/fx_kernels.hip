@@ -240,14 +240,15 @@ __device__ __forceinline__ double grad_r_interior(const double* __restrict__ P, 
     const double Hxx = (P[c + o.xp] - 2. * pc + P[c + o.xm]) * (ih[0] * ih[0]);
     const double Hyy = (P[c + o.yp] - 2. * pc + P[c + o.ym]) * (ih[1] * ih[1]);
     const double Hzz = (P[c + o.zp] - 2. * pc + P[c + o.zm]) * (ih[2] * ih[2]);
-    bool nxy, nyz, nxz;
-    mixed_needed(R, nxy, nyz, nxz);
-    const double Hxy = nxy ? ((P[c + o.xp + o.yp] - P[c + o.xp + o.ym]) - (P[c + o.xm + o.yp] - P[c + o.xm + o.ym])) *
-                                 (0.25 * ih[0] * ih[1]) : 0.0;
-    const double Hyz = nyz ? ((P[c + o.yp + o.zp] - P[c + o.yp + o.zm]) - (P[c + o.ym + o.zp] - P[c + o.ym + o.zm])) *
-                                 (0.25 * ih[1] * ih[2]) : 0.0;
-    const double Hxz = nxz ? ((P[c + o.xp + o.zp] - P[c + o.xp + o.zm]) - (P[c + o.xm + o.zp] - P[c + o.xm + o.zm])) *
-                                 (0.25 * ih[0] * ih[2]) : 0.0;
+    /* the mixed entries are formed whether or not R couples them (a zero
+     * product of R entries then multiplies a finite value), so every load
+     * can be issued before R is known */
+    const double Hxy = ((P[c + o.xp + o.yp] - P[c + o.xp + o.ym]) - (P[c + o.xm + o.yp] - P[c + o.xm + o.ym])) *
+                       (0.25 * ih[0] * ih[1]);
+    const double Hyz = ((P[c + o.yp + o.zp] - P[c + o.yp + o.zm]) - (P[c + o.ym + o.zp] - P[c + o.ym + o.zm])) *
+                       (0.25 * ih[1] * ih[2]);
+    const double Hxz = ((P[c + o.xp + o.zp] - P[c + o.xp + o.zm]) - (P[c + o.xm + o.zp] - P[c + o.xm + o.zm])) *
+                       (0.25 * ih[0] * ih[2]);
     return grad_r_of(R, Hxx, Hyy, Hzz, Hxy, Hyz, Hxz, g11, g44m);
 }
 
