@@ -93,6 +93,41 @@ def test_mfim_steps(relerr, ghost, fused):
         assert e < TOL, (name, e)
 
 
+def bounded_mfim(**kw):
+    """64 x 48 lateral cells with Neumann and Dirichlet lateral faces."""
+    text = configs.mfim(nx=64, ny=48, nz=36, lx=32e-9, ly=24e-9, **kw)
+    text = text.replace("domain.is_periodic = 1 1 0", "domain.is_periodic = 0 0 0")
+    text = text.replace("boundary.lo = per per dir(0.0)", "boundary.lo = neu dir(0.0) dir(0.0)")
+    return text.replace("boundary.hi = per per dir(", "boundary.hi = dir(0.0) neu dir(")
+
+
+@pytest.mark.parametrize("case", ["periodic", "bounded"])
+def test_fused_convergence_check(relerr, case):
+    """With level 0 streamed and no Helmholtz term, each convergence check is
+    folded into the next V-cycle's first sweep: its max|r| must be the
+    oracle's separate residual evaluation (the initial one to rounding, both
+    sides stopping on the same V-cycle at the reference tolerance), and a
+    run of steps, whose solves start from phi = 0 without reading phi, must
+    match the oracle."""
+    text = small_mfim(nsteps=10) if case == "periodic" else bounded_mfim(nsteps=10, bias=0.3)
+    g, o = pair(text, mg_tol_rel=1e-10, mg_fused_sweep=1)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal(g.shape) * 1e6
+    pg, sg = g.poisson_solve(b)
+    po, so = o.poisson_solve(b)
+    print(sg, so)
+    assert abs(sg["mg_resid0"] - so["mg_resid0"]) <= 1e-12 * so["mg_resid0"]
+    assert sg["mg_bnorm"] == so["mg_bnorm"]
+    assert sg["vcycles"] == so["vcycles"]
+    assert relerr(pg, po) < 1e-9
+    g2, o2 = pair(text, mg_fused_sweep=1)
+    assert g2.step(10)["steps_done"] == o2.step(10)["steps_done"] == 10
+    for name in ("Pz", "Phi"):
+        e = relerr(g2.get(name), o2.get(name))
+        print(name, e)
+        assert e < TOL, (name, e)
+
+
 @pytest.mark.parametrize("fused", [0, 1, 2])
 def test_mfim_reference_stopping_rule(relerr, fused):
     g, o = pair(small_mfim(nsteps=20), mg_tol_rel=1e-10, mg_fused_sweep=fused)
