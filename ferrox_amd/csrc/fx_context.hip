@@ -81,6 +81,10 @@ struct fx_ctx {
      * per-step rewrite with zeros is skipped */
     bool rho_clean = false, alpha_clean = false;
     bool rho_exposed = false, alpha_exposed = false;
+    /* the steady-state check's max|phi| and max|phi - phi_old| (device), and
+     * whether the host has yet to read the last of them */
+    double* steady_dev = nullptr;
+    bool steady_pending = false;
     double* E[3] = {nullptr, nullptr, nullptr};
     double* GL[3] = {nullptr, nullptr, nullptr};
     double* GL1[3] = {nullptr, nullptr, nullptr};
@@ -800,9 +804,9 @@ DBC dbc_of(const fx_ctx* x, const double v[6]) {
  * synchronisation wakes the host much later than the GPU finishes); the
  * stream is polled now and then so that an error ends the wait, and a wait
  * of several seconds falls back to the synchronisation. */
-unsigned post_red(fx_ctx* x, int nq) {
+unsigned post_red(fx_ctx* x, int nq, const double* src = nullptr) {
     const unsigned v = ++x->pub_seq;
-    fxk::publish(x->s, x->red_out, nq, x->d_red_host, reinterpret_cast<unsigned*>(x->d_red_host + 8), v);
+    fxk::publish(x->s, src ? src : x->red_out, nq, x->d_red_host, reinterpret_cast<unsigned*>(x->d_red_host + 8), v);
     return v;
 }
 /* Waits for the values published under sequence number v (work queued after
@@ -826,7 +830,7 @@ int wait_red(fx_ctx* x, unsigned v, int nq, double* out) {
     for (int q = 0; q < nq; ++q) out[q] = x->h_red[q];
     return FX_OK;
 }
-int read_red(fx_ctx* x, int nq, double* out) { return wait_red(x, post_red(x, nq), nq, out); }
+int read_red(fx_ctx* x, int nq, double* out, const double* src = nullptr) { return wait_red(x, post_red(x, nq, src), nq, out); }
 
 /* --------------------------------- multigrid ------------------------------ */
 /* pre_done: the first pre-smoothing sweep of level 0 has already run
@@ -1170,15 +1174,27 @@ int update(fx_ctx* x, int order, fx_solve_stats* st2) {
     return FX_OK;
 }
 
-/* CheckSteadyState, ElectrostaticSolver.cpp:526-560. */
-int steady_check(fx_ctx* x, int step, double* metric) {
-    fxk::maxabs2(x->s, x->phi, x->phi_old, (long long)x->N, x->red, x->red_out);
+/* CheckSteadyState, ElectrostaticSolver.cpp:526-560.  phidiff divides by
+ * max|phi| read from device memory.  With phi_tolerance <= 0 the value can
+ * never stop or advance the run (val < tol is false for val >= 0 and for
+ * NaN), so unless it is printed the host does not wait for it: fx_step reads
+ * the last one when it ends (steady_read). */
+int steady_read(fx_ctx* x, double* metric) {
     double m[2];
-    int rc = read_red(x, 2, m);
+    int rc = read_red(x, 2, m, x->steady_dev);
     if (rc) return rc;
-    const double phi_max = m[0];
-    const double val = (phi_max == 0.0) ? NAN : m[1] / phi_max;
-    fxk::phidiff(x->s, x->phi, x->phi_old, x->phidiff, (long long)x->N, phi_max);
+    x->steady_pending = false;
+    *metric = (m[0] == 0.0) ? NAN : m[1] / m[0];
+    return FX_OK;
+}
+int steady_check(fx_ctx* x, int step, double* metric) {
+    fxk::maxabs2(x->s, x->phi, x->phi_old, (long long)x->N, x->red, x->steady_dev);
+    fxk::phidiff(x->s, x->phi, x->phi_old, x->phidiff, (long long)x->N, x->steady_dev);
+    x->steady_pending = true;
+    if (!(x->in.phi_tolerance > 0.0) && !x->in.verbose) return FX_OK;
+    double val;
+    int rc = steady_read(x, &val);
+    if (rc) return rc;
     if (step > 1 && val < x->in.phi_tolerance) { x->steady_state_step = step; x->inc_step = step; }
     if (x->in.verbose) std::printf("Steady state check : (phi(t) - phi(t-1)).norm0() = %g\n", val);
     *metric = val;
@@ -1192,7 +1208,7 @@ void free_all(fx_ctx* x) {
     F(x->alpha_cc); F(x->phidiff); F(x->eps); F(x->mat); F(x->rot_idx); F(x->rot_tab); F(x->rot_atab);
     F(x->diff.ck); F(x->diff.segP); F(x->diff.segPhi); F(x->diff.lam[0]); F(x->diff.lam[1]); F(x->diff.s);
     F(x->diff.fin); F(x->diff.acc); F(x->diff.m); F(x->diff.part);
-    F(x->red); F(x->red_out); F(x->d_iters); F(x->cg_work); F(x->tmp);
+    F(x->red); F(x->red_out); F(x->steady_dev); F(x->d_iters); F(x->cg_work); F(x->tmp);
     for (size_t l = 0; l < x->lev.size(); ++l) {
         DLev& d = x->lev[l].d;
         F(d.bx); F(d.by); F(d.bz); F(x->lev[l].up);
@@ -1275,6 +1291,7 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
     x->nred = std::max(RED_NB, ((x->n[0] + 63) / 64) * ((x->n[1] + 3) / 4) * x->n[2]);
     ALLOC(x->red, sizeof(double) * 4 * (size_t)x->nred);
     ALLOC(x->red_out, sizeof(double) * 8);
+    ALLOC(x->steady_dev, sizeof(double) * 2);
     ALLOC(x->d_iters, sizeof(int) * 4);
 #undef ALLOC
     if (hipHostMalloc((void**)&x->h_red, 8 * sizeof(double) + 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
@@ -1601,6 +1618,10 @@ int fx_step(fx_ctx* x, int nsteps, fx_step_stats* out) {
         if (in.voltage_sweep == 1 && x->Phi_Bc_hi > 0. && x->Phi_Bc_hi - in.Phi_Bc_hi_max > tiny) break;
         if (in.voltage_sweep == 1 && x->Phi_Bc_hi < 0. && -x->Phi_Bc_hi - in.Phi_Bc_hi_max > tiny) break;
         if (in.voltage_sweep == 1 && x->num_Vapp == in.num_Vapp_max) break;
+    }
+    if (x->steady_pending) {
+        const int rc2 = steady_read(x, &st.last_phi_change);
+        if (!rc) rc = rc2;
     }
     hipError_t e = hipStreamSynchronize(x->s);
     if (!rc && e != hipSuccess) rc = fail(x, FX_EHIP, hipGetErrorString(e));

@@ -86,7 +86,8 @@ void add(hipStream_t s, double* d, const double* src, long long n);
 void final_reduce(hipStream_t s, const double* part, int nb, int nq, int is_max, double* out);
 /* out[0] = max|a|, out[1] = max|a-b| (b may be null); part holds 2*2048 doubles */
 void maxabs2(hipStream_t s, const double* a, const double* b, long long n, double* part, double* out);
-void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, double phi_max);
+/* diff = |phi - phi_old| / *phi_max (device), then phi_old = phi */
+void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, const double* phi_max);
 void rhs_newton(hipStream_t s, const RhsArgs& a);
 /* returns the number of per-block partials written ([num...][den...]) */
 int rho_update(hipStream_t s, const RhoArgs& a);

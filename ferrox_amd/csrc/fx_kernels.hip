@@ -69,7 +69,8 @@ __global__ __launch_bounds__(256) void k_maxabs2(const double* __restrict__ a, c
 /* PhiDiff = |phi - phi_old| / phi_max, then phi_old = phi (CheckSteadyState,
  * ElectrostaticSolver.cpp:531-556). */
 __global__ void k_phidiff(const double* __restrict__ phi, double* __restrict__ phi_old, double* __restrict__ diff,
-                          long long n, double phi_max) {
+                          long long n, const double* __restrict__ phi_max_p) {
+    const double phi_max = *phi_max_p;
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x) {
         diff[c] = fabs(phi[c] - phi_old[c]) / phi_max;
         phi_old[c] = phi[c];
@@ -458,7 +459,7 @@ void maxabs2(hipStream_t s, const double* a, const double* b, long long n, doubl
     hipLaunchKernelGGL(k_maxabs2, dim3(2048), dim3(256), 0, s, a, b, n, part);
     final1024(s, part, 2048, 2, 1, out);
 }
-void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, double phi_max) {
+void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, long long n, const double* phi_max) {
     unsigned b = nblk(n, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;
     hipLaunchKernelGGL(k_phidiff, dim3(b), dim3(256), 0, s, phi, phi_old, diff, n, phi_max);
 }

@@ -87,10 +87,12 @@ def test_mfim_steps(relerr, ghost, fused):
     g, o = pair(small_mfim(nsteps=20), phi_ghost_mode=ghost, mg_fused_sweep=fused)
     sg, so = g.step(20), o.step(20)
     assert sg["steps_done"] == so["steps_done"] == 20
-    for name in ("Pz", "Phi"):
+    for name in ("Pz", "Phi", "PhiDiff"):
         e = relerr(g.get(name), o.get(name))
         print(name, e)
         assert e < TOL, (name, e)
+    # phi_tolerance = 0: the steady-state value is read once, after the last step
+    assert abs(sg["last_phi_change"] - so["last_phi_change"]) <= TOL * abs(so["last_phi_change"])
 
 
 def bounded_mfim(**kw):
