@@ -1006,11 +1006,14 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
  * stop at the FP32 limit (a correction solved to 1e-6 of its right-hand
  * side).  As in mg_solve, the first sweep of the next correction solve is
  * queued right behind the kernel that publishes each check. */
-int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false) {
     const DBC bc = dbc_of(x, bv);
     double nr[2];
     const bool spec = x->lev32.size() > 1 && x->tail0 != 0 && x->in.mg_nu1 >= 1;
-    fxk::resid_to32(x->s, x->lev[0].d, bc, x->lev32[0].b, x->red, x->red_out);
+    /* phi_zero: the first residual takes phi as 0 without reading it, and the
+     * first correction is stored rather than added */
+    bool zero = phi_zero;
+    fxk::resid_to32(x->s, x->lev[0].d, bc, x->lev32[0].b, x->red, x->red_out, zero);
     unsigned v = post_red(x, 2);
     bool pre = spec && fxk::mg_sweep_first(x->s, x->lev32[0], bc, x->in.mg_fused_sweep);
     int rc = wait_red(x, v, 2, nr);
@@ -1019,7 +1022,10 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
     const double target = tol * std::max(bn, r0);
     st->mg_resid0 = r0; st->mg_bnorm = bn; st->mg_resid = r0; st->vcycles = 0;
     if (!(r0 == r0) || !(bn == bn)) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
-    if (r0 <= target) return FX_OK;
+    if (r0 <= target) {
+        if (zero) fxk::fill(x->s, x->phi, (long long)x->N, 0.0);
+        return FX_OK;
+    }
     int cycles = 0;
     double r = r0, rho = 0.0;   /* measured contraction per FP32 cycle (0: none yet) */
     while (cycles < maxit) {
@@ -1037,7 +1043,8 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
         nA = std::min(nA, maxit - cycles);
         for (int c = 0; c < nA; ++c) vcycle32(x, 0, bc, c == 0, pre && c == 0);
         cycles += nA;
-        fxk::add_f2d(x->s, x->phi, x->lev32[0].phi, (long long)x->N);
+        fxk::add_f2d(x->s, x->phi, x->lev32[0].phi, (long long)x->N, zero);
+        zero = false;
         fxk::resid_to32(x->s, x->lev[0].d, bc, x->lev32[0].b, x->red, x->red_out);
         v = post_red(x, 2);
         pre = spec && cycles < maxit && fxk::mg_sweep_first(x->s, x->lev32[0], bc, x->in.mg_fused_sweep);
@@ -1054,7 +1061,6 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
 /* phi_zero: the solve starts from phi = 0, whatever phi holds */
 int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false) {
     if (x->in.precision == FX_PREC_FP32_REFINE) {
-        if (phi_zero) fxk::fill(x->s, x->phi, (long long)x->N, 0.0);
         /* FP32 copy of the A-coefficient for the correction levels */
         if (x->lev32[0].read_a) {
             fxk::d2f(x->s, x->lev32[0].a, x->alpha_cc, (long long)x->N);
@@ -1062,7 +1068,7 @@ int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
                 fxk::mg_restrict_cells(x->s, x->lev32[l], x->lev32[l + 1], x->lev32[l].a, x->lev32[l + 1].a);
             x->lu_ok32 = false;
         }
-        return mg_solve_mixed(x, bv, tol, maxit, st);
+        return mg_solve_mixed(x, bv, tol, maxit, st, phi_zero);
     }
     return mg_solve(x, bv, tol, maxit, st, phi_zero);
 }

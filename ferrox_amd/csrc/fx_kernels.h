@@ -144,7 +144,8 @@ void mg_restrict_cells(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, 
 void bottom_cg(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, const float* rhs, float* e, float* work,
                double tol, int maxit, int* iters);
 void d2f(hipStream_t s, float* d, const double* src, long long n);
-void add_f2d(hipStream_t s, double* phi, const float* e, long long n);
+/* phi += e (set: phi = e, phi being 0) */
+void add_f2d(hipStream_t s, double* phi, const float* e, long long n, bool set = false);
 /* whole sub-V-cycle of the levels levs[0..nl-1] (device array) in one workgroup */
 void mg_tail(hipStream_t s, const fxd::DLev* levs, int nl, const fxd::DBC& bc, int read_a, int nu1, int nu2, double* work,
              double tol, int maxit, int* iters, int ts, size_t lds_bytes);
@@ -187,7 +188,7 @@ size_t tail_lay_lds(const int (*n)[3], int nl, int read_a, size_t elem);
 void bottom_factor(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, double* inv, int* perm);
 void bottom_factor(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, float* inv, int* perm);
 /* r32 = (float)(b - A phi) in FP64 (inhomogeneous BCs); out[0] = max|r|, out[1] = max|b| */
-void resid_to32(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, float* r32, double* part, double* out);
+void resid_to32(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, float* r32, double* part, double* out, bool zero = false);
 /* ------------------------------- adjoint ---------------------------------- */
 /* One backward step of the discrete adjoint (fx_adjoint.hip). */
 struct AdjArgs {

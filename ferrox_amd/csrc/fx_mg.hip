@@ -756,6 +756,7 @@ struct ResidArgs {
     T* __restrict__ cb;          /* MODE 1: rhs of the z-coarsened level */
     T* __restrict__ cphi;        /* MODE 1: its phi, set to 0 (null: left alone, the level's first sweep takes it as 0) */
     float* __restrict__ r32;     /* MODE 3: the residual in FP32 */
+    int zero_phi;                /* MODE 3: phi is taken as 0 and not read */
     double* __restrict__ part;   /* MODE 2, 3: max|r| and max|b| per workgroup */
 };
 
@@ -843,7 +844,7 @@ __global__ __launch_bounds__(256) void k_mg_resid_stream(ResidArgs<T> A) {
     auto load_plane = [&](ResidRegs<T>& R, int k) {
         const int kof = plane * min(max(k, 0), n2 - 1);
 #pragma unroll
-        for (int s = 0; s < RLD; ++s) R.phi[s] = A.phi[(goff[s] >= 0 ? goff[s] : 0) + kof];
+        for (int s = 0; s < RLD; ++s) R.phi[s] = (MODE == 3 && A.zero_phi) ? T(0) : A.phi[(goff[s] >= 0 ? goff[s] : 0) + kof];
     };
     auto load_cells = [&](ResidRegs<T>& R, int k) {
         const int kk = min(max(k, 0), n2 - 1);
@@ -2446,9 +2447,9 @@ __global__ __launch_bounds__(SPEC_THR) void k_spec_inv(SpecTail<T> P) {
 
 /* --------------------- FP64 refinement of an FP32 correction -------------- */
 /* phi += e (FP32 correction added into the FP64 solution). */
-__global__ void k_add_f2d(double* __restrict__ phi, const float* __restrict__ e, long long n) {
+__global__ void k_add_f2d(double* __restrict__ phi, const float* __restrict__ e, long long n, int set) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < n; c += (long long)gridDim.x * blockDim.x)
-        phi[c] += (double)e[c];
+        phi[c] = set ? (double)e[c] : phi[c] + (double)e[c];   /* set: phi was 0 (0 + e is e exactly) */
 }
 
 __global__ void k_d2f(float* __restrict__ d, const double* __restrict__ s, long long n) {
@@ -2691,6 +2692,7 @@ static ResidArgs<T> resid_args(const DLevT<T>& l, const DBC& bc, int inh, int* n
     *nblk = A.ntx * A.nty * ((l.n2 + A.cz - 1) / A.cz);
     A.remap = (*nblk % 8) == 0 ? 1 : 0;
     A.cb = nullptr; A.cphi = nullptr; A.r32 = nullptr; A.part = nullptr;
+    A.zero_phi = 0;
     return A;
 }
 template <class T, int MODE>
@@ -2884,21 +2886,26 @@ void mg_resid_norm(hipStream_t s, const DLev& l, const DBC& bc, int inh, double*
     else hipLaunchKernelGGL((k_mg_resid_max<0, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, inh, nullptr, nullptr, part);
     hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, RESID_NB, 2, 1, out);
 }
-void add_f2d(hipStream_t s, double* phi, const float* e, long long n) {
+void add_f2d(hipStream_t s, double* phi, const float* e, long long n, bool set) {
     unsigned b = (unsigned)((n + 255) / 256); if (b > 8192) b = 8192; if (b == 0) b = 1;
-    hipLaunchKernelGGL(k_add_f2d, dim3(b), dim3(256), 0, s, phi, e, n);
+    hipLaunchKernelGGL(k_add_f2d, dim3(b), dim3(256), 0, s, phi, e, n, set ? 1 : 0);
 }
-void resid_to32(hipStream_t s, const DLev& l, const DBC& bc, float* r32, double* part, double* out) {
+void resid_to32(hipStream_t s, const DLev& l, const DBC& bc, float* r32, double* part, double* out, bool zero) {
     if (resid_stream_ok(l, bc)) {
         int nblk;
         ResidArgs<double> A = resid_args(l, bc, 1, &nblk);
         if (nblk <= RESID_NB) {
             A.part = part;
             A.r32 = r32;
+            A.zero_phi = zero ? 1 : 0;
             resid_stream_launch<double, 3>(s, l, A, nblk);
             hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, nblk, 2, 1, out);
             return;
         }
+    }
+    if (zero) {
+        const long long n = (long long)l.n0 * l.n1 * l.n2;
+        fill(s, l.phi, n, 0.0);
     }
     hipLaunchKernelGGL((k_mg_resid_max<2, RESID_ZC>), dim3(RESID_NB), dim3(256), 0, s, l, bc, 1, nullptr, r32, part);
     hipLaunchKernelGGL(k_final1024, dim3(1), dim3(1024), 0, s, part, RESID_NB, 2, 1, out);
