@@ -4,19 +4,27 @@
  * (ElectrostaticSolver.cpp:562-611, 713-720).
  *
  *   k_mg_gsrb_stream a full red-black Gauss-Seidel sweep in one pass: each
- *                    workgroup owns a 64x8 column tile and a z-chunk, keeps five
+ *                    workgroup owns a 64x16 column tile and a z-chunk, keeps five
  *                    z-planes of phi (with a 2-cell in-plane halo) in LDS and
  *                    streams upward, updating red on plane p and black on plane
  *                    p-1 -- the same values as a red sweep followed by a black
- *                    sweep, with phi and rhs read once and phi written once.
+ *                    sweep, with phi and rhs read once and phi written once;
+ *                    variants fold in the z-prolongation (PROLONG) or the
+ *                    convergence check's residual norm of the input (NORM).
+ *   k_mg_resid_stream the residual streamed the same way: z-restriction, norm,
+ *                    FP32 residual of the refinement
  *   k_mg_smooth      one colour of the sweep (small levels, periodic z)
+ *   k_mg_block_down/up  one leg of a small layered level in one launch
  *   k_mg_resid_restrict   residual averaged into the coarse rhs; coarse phi = 0
  *   k_mg_prolong_blk cell-centred linear prolongation added to the fine phi
  *   k_mg_resid_max   residual with max|r| and max|b| per workgroup (fixed grid)
  *   k_bottom         exact (dense) or Jacobi-PCG bottom solve in one workgroup
  *   k_mg_tail        the coarse end of the V-cycle in one workgroup, staged in LDS
- * Launches use 3-D grids (x tiles of 64, y tiles of 4, z) so no thread does
- * 64-bit index division.
+ *   k_spec_fwd/tri/inv  exact solve of a laterally periodic layered level
+ *                    (plane transforms on the matrix cores, a tridiagonal
+ *                    solve per lateral mode)
+ * Per-level launches use 3-D grids (x tiles of 64, y tiles of 4, z) so no
+ * thread does 64-bit index division.
  */
 #include "fx_device.h"
 #include "fx_kernels.h"
@@ -129,7 +137,7 @@ struct StreamCell {
 /* One full red-black Gauss-Seidel sweep, phi_in -> phi_out (two buffers, so a
  * workgroup never reads values another workgroup has already updated).
  *
- * A workgroup owns a 64x8 column tile and the planes [z0, z1) (z0 even), and
+ * A workgroup owns a 64x16 column tile and the planes [z0, z1) (z0 even), and
  * marches upward: at step p it updates red on plane p over the tile and a
  * 1-cell ring (from old black values), then black on plane p-1 over the tile
  * (from the new red values of planes p-2, p-1, p), and stores plane p-1.  The
