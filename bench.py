@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=6, help="steps of the CPU oracle sample (0 = skip)")
+    ap.add_argument("--cpu-single-steps", type=int, default=1,
+                    help="steps of the one-thread CPU oracle sample (0 = skip)")
     ap.add_argument("--warm-start", action="store_true", help="previous phi as the multigrid initial guess")
     ap.add_argument("--no-kernel-timing", action="store_true")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+FP64-refinement leg")
@@ -301,7 +303,7 @@ def main():
 
     cpu = None
     if a.cpu_steps > 0:
-        from oracle.oracle import Oracle, threads
+        from oracle.oracle import Oracle, threads, set_threads
         o = Oracle(fx.from_text(text))
         configs.apply_fields(o, grains)
         c0 = time.perf_counter()
@@ -312,6 +314,19 @@ def main():
                "kind": "port",
                "sample": f"{a.cpu_steps} step(s) of the same 256x256x128 polycrystal workload, FP64 oracle (OpenMP), "
                          f"{c1 - c0:.1f} s"}
+        if a.cpu_single_steps > 0:
+            # the same oracle on one host thread
+            n_all = threads()
+            set_threads(1)
+            o = Oracle(fx.from_text(text))
+            configs.apply_fields(o, grains)
+            c0 = time.perf_counter()
+            o.step(a.cpu_single_steps)
+            c1 = time.perf_counter()
+            o.close()
+            set_threads(n_all)
+            cpu["single_core"] = {"value": round(a.cpu_single_steps / (c1 - c0), 5), "unit": "steps/s", "cores": 1,
+                                  "sample": f"{a.cpu_single_steps} step(s), {c1 - c0:.1f} s"}
 
     out = {
         "metric": "TDGL+Poisson steps/s at 256x256x128 (FP64)",

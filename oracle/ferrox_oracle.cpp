@@ -1286,6 +1286,17 @@ int fxo_threads(void) {
 #endif
 }
 
+/* threads of the parallel loops from here on (timing samples); returns the count in effect */
+int fxo_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
 double fxo_uniform01(unsigned long long seed, unsigned long long idx) { return uniform01(seed, idx); }
 
 }  // extern "C"

@@ -53,6 +53,7 @@ def lib():
             "fxo_apply_op": (C.c_int, [P, DP, DP, DP, C.c_int]),
             "fxo_get_faces": (C.c_int, [P, C.c_int, DP]),
             "fxo_threads": (C.c_int, []),
+            "fxo_set_threads": (C.c_int, [C.c_int]),
             "fxo_uniform01": (C.c_double, [C.c_ulonglong, C.c_ulonglong]),
         }
         for name, (res, args) in sig.items():
@@ -173,6 +174,11 @@ class Oracle:
 
 def threads() -> int:
     return lib().fxo_threads()
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's parallel loops from now on; returns the count in effect."""
+    return lib().fxo_set_threads(int(n))
 
 
 def uniform01(seed: int, idx: int) -> float:
