@@ -143,10 +143,15 @@ __global__ __launch_bounds__(256) void k_rhs_newton(RhsArgs a) {
         if (a.pxy_zero && (a.M.raw(i, j, k) & MAT_INTERIOR_FE)) {   /* centred DPDx/y/z */
             const NbrOff o(i, j, nx, ny);
             const double* P = a.P[2];
-            /* half reciprocal spacings multiplied in: the same differences to rounding */
-            if (R[6] != 0.0) x += R[6] * ((P[c + o.xp] - P[c + o.xm]) * (0.5 / dx[0]));
-            if (R[7] != 0.0) x += R[7] * ((P[c + o.yp] - P[c + o.ym]) * (0.5 / dx[1]));
-            x += R[8] * ((P[c + o.zp] - P[c + o.zm]) * (0.5 / dx[2]));
+            /* half reciprocal spacings multiplied in: the same differences to
+             * rounding; all three formed before R is known, so their loads
+             * issue beside the rotation's */
+            const double gx = (P[c + o.xp] - P[c + o.xm]) * (0.5 / dx[0]);
+            const double gy = (P[c + o.yp] - P[c + o.ym]) * (0.5 / dx[1]);
+            const double gz = (P[c + o.zp] - P[c + o.zm]) * (0.5 / dx[2]);
+            if (R[6] != 0.0) x += R[6] * gx;
+            if (R[7] != 0.0) x += R[7] * gy;
+            x += R[8] * gz;
             rhs = -x;
         } else if (a.pxy_zero) {
             if (R[6] != 0.0) x += R[6] * dpd<0>(Fr, a.M, i, j, k, dx, a.pbc);
