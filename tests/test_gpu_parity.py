@@ -103,7 +103,7 @@ def bounded_mfim(**kw):
     return text.replace("boundary.hi = per per dir(", "boundary.hi = dir(0.0) neu dir(")
 
 
-@pytest.mark.parametrize("case", ["periodic", "bounded"])
+@pytest.mark.parametrize("case", ["periodic", "bounded", "partial_tiles"])
 def test_fused_convergence_check(relerr, case):
     """With level 0 streamed and no Helmholtz term, each convergence check is
     folded into the next V-cycle's first sweep: its max|r| must be the
@@ -111,7 +111,12 @@ def test_fused_convergence_check(relerr, case):
     sides stopping on the same V-cycle at the reference tolerance), and a
     run of steps, whose solves start from phi = 0 without reading phi, must
     match the oracle."""
-    text = small_mfim(nsteps=10) if case == "periodic" else bounded_mfim(nsteps=10, bias=0.3)
+    if case == "periodic":
+        text = small_mfim(nsteps=10)
+    elif case == "bounded":
+        text = bounded_mfim(nsteps=10, bias=0.3)
+    else:   # 80 x 40 lateral cells: partial 64 x 16 tiles in x and y
+        text = configs.mfim(nx=80, ny=40, nz=36, lx=40e-9, ly=20e-9, nsteps=10)
     g, o = pair(text, mg_tol_rel=1e-10, mg_fused_sweep=1)
     rng = np.random.default_rng(5)
     b = rng.standard_normal(g.shape) * 1e6
