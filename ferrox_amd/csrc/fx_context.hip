@@ -911,11 +911,11 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true, bool pre_d
         if (x->tail_lean) {
             if (!x->lu_ok32 && x->lu32) fxk::bottom_factor(x->s, x->lev32.back(), bc, x->lu32, x->perm32);
             x->lu_ok32 = true;
-            fxk::mg_tail_lay(x->s, x->plan32, bc, L.read_a ? 1 : 0, x->in.mg_nu1, x->in.mg_nu2, x->cg_work32, 1e-5,
+            fxk::mg_tail_lay(x->s, x->plan32, bc, L.read_a ? 1 : 0, x->in.mg_refine_nu1, x->in.mg_refine_nu2, x->cg_work32, 1e-5,
                              BOTTOM_MAXIT, x->d_iters);
             return;
         }
-        fxk::mg_tail(x->s, x->d_tail32, (int)(x->lev32.size() - li), bc, L.read_a, x->in.mg_nu1, x->in.mg_nu2,
+        fxk::mg_tail(x->s, x->d_tail32, (int)(x->lev32.size() - li), bc, L.read_a, x->in.mg_refine_nu1, x->in.mg_refine_nu2,
                      x->cg_work32, 1e-5, BOTTOM_MAXIT, x->d_iters, x->tail_ts32, x->tail_lds32);
         return;
     }
@@ -925,19 +925,19 @@ void vcycle32(fx_ctx* x, size_t li, const DBC& bc, bool zero0 = true, bool pre_d
     }
     DLevF& C = x->lev32[li + 1];
     if (li >= 1 && x->in.mg_block_levels && x->up32[li] && !(L.stream && x->in.mg_fused_sweep && fxk::fused_ok32(L, bc)) &&
-        fxk::mg_block_ok(L, C, bc, x->in.mg_nu1, x->in.mg_nu2)) {
-        fxk::mg_block_down(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2);
+        fxk::mg_block_ok(L, C, bc, x->in.mg_refine_nu1, x->in.mg_refine_nu2)) {
+        fxk::mg_block_down(x->s, L, C, bc, x->in.mg_refine_nu1, x->in.mg_refine_nu2);
         vcycle32(x, li + 1, bc, true);
-        fxk::mg_block_up(x->s, L, C, bc, x->in.mg_nu1, x->in.mg_nu2, x->up32[li]);
+        fxk::mg_block_up(x->s, L, C, bc, x->in.mg_refine_nu1, x->in.mg_refine_nu2, x->up32[li]);
         std::swap(L.phi, x->up32[li]);
         return;
     }
-    if (pre_done) fxk::mg_sweep_rest(x->s, L, bc, x->in.mg_nu1 - 1, x->in.mg_fused_sweep);
-    else fxk::mg_sweep(x->s, L, bc, x->in.mg_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
+    if (pre_done) fxk::mg_sweep_rest(x->s, L, bc, x->in.mg_refine_nu1 - 1, x->in.mg_fused_sweep);
+    else fxk::mg_sweep(x->s, L, bc, x->in.mg_refine_nu1, (li > 0 || zero0) ? 1 : 0, x->in.mg_fused_sweep);
     const bool plain_next = (int)(li + 1) != x->tail0 && li + 2 < x->lev32.size();
     fxk::mg_resid_restrict(x->s, L, C, bc, !plain_next);
     vcycle32(x, li + 1, bc, true);
-    fxk::mg_prolong_sweep(x->s, L, C, bc, x->in.mg_nu2, x->in.mg_fused_sweep);
+    fxk::mg_prolong_sweep(x->s, L, C, bc, x->in.mg_refine_nu2, x->in.mg_fused_sweep);
 }
 
 /* MLMG-style solve (stopping rule ||r||_inf <= tol*max(||b||_inf, ||r0||_inf),
@@ -1009,7 +1009,7 @@ int mg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stat
 int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false) {
     const DBC bc = dbc_of(x, bv);
     double nr[2];
-    const bool spec = x->lev32.size() > 1 && x->tail0 != 0 && x->in.mg_nu1 >= 1;
+    const bool spec = x->lev32.size() > 1 && x->tail0 != 0 && x->in.mg_refine_nu1 >= 1;
     /* phi_zero: the first residual takes phi as 0 without reading it, and the
      * first correction is stored rather than added */
     bool zero = phi_zero;

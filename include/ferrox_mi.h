@@ -138,6 +138,8 @@ typedef struct fx_inputs {
     double mg_coarsen_ratio;      /* direction d is coarsened when h_d < ratio * min(h) (default 1.9) */
     int    mg_exact_tail;         /* 1 (default): laterally periodic layered hierarchies solve the tail's first level
                                      exactly (lateral Fourier transforms + tridiagonal solves) instead of a sub-V-cycle */
+    int    mg_refine_nu1;         /* precision = 1: pre- and post-smoothing sweeps of the FP32 correction V-cycles */
+    int    mg_refine_nu2;         /* (defaults 1 and 2; the FP64 V-cycles use mg_nu1, mg_nu2) */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;
