@@ -134,8 +134,8 @@ void bottom_cg(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const doub
                double tol, int maxit, int* iters);
 /* FP32 correction hierarchy (homogeneous BCs) */
 void mg_sweep(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int nsweeps, int zero_first, int allow_fused);
-/* the zero-initial first sweep of a correction solve on its own, into phi2,
- * and the remaining sweeps from phi2 (as the FP64 pair above) */
+/* the zero-initial first sweep of a correction solve on its own, into phi
+ * (its input is not read), and the remaining sweeps from phi */
 bool mg_sweep_first(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int allow_fused);
 void mg_sweep_rest(hipStream_t s, const fxd::DLevF& l, const fxd::DBC& bc, int nsweeps, int allow_fused);
 void mg_resid_restrict(hipStream_t s, const fxd::DLevF& f, const fxd::DLevF& c, const fxd::DBC& bc, bool zero_cphi = true);

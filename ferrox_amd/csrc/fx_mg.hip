@@ -2655,18 +2655,17 @@ void mg_sweep_rest(hipStream_t s, const DLev& l, const DBC& bc, int inh, int nsw
 void mg_sweep(hipStream_t s, const DLevF& l, const DBC& bc, int nsweeps, int zero_first, int allow_fused) {
     sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, zero_first);
 }
+/* The FP32 first sweep starts from e = 0 and does not read its input, so it
+ * writes phi directly (the input named is phi2, never read): the correction
+ * is never the answer a convergence check might keep. */
 bool mg_sweep_first(hipStream_t s, const DLevF& l, const DBC& bc, int allow_fused) {
     if (allow_fused != 1 || !stream_use(l, bc, allow_fused)) return false;
-    sweep_impl<float>(s, l, bc, 0, 1, allow_fused, 1, nullptr, false, true);
+    sweep_impl<float>(s, l, bc, 0, 1, allow_fused, 1, nullptr, true, false);
     return true;
 }
 void mg_sweep_rest(hipStream_t s, const DLevF& l, const DBC& bc, int nsweeps, int allow_fused) {
-    if (nsweeps <= 0) {
-        const long long n = (long long)l.n0 * l.n1 * l.n2;
-        hipLaunchKernelGGL(k_copyT<float>, dim3((unsigned)std::min<long long>((n + 255) / 256, 4096)), dim3(256), 0, s, l.phi, l.phi2, n);
-        return;
-    }
-    sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, 0, nullptr, true, false);
+    if (nsweeps <= 0) return;   /* the first sweep's result is in phi */
+    sweep_impl<float>(s, l, bc, 0, nsweeps, allow_fused, 0, nullptr, false, false);
 }
 void mg_prolong_sweep(hipStream_t s, const DLev& l, const DLev& c, const DBC& bc, int inh, int nsweeps, int allow_fused) {
     sweep_impl<double>(s, l, bc, inh, nsweeps, allow_fused, 0, &c);
