@@ -1180,6 +1180,9 @@ int fxo_compute_phi_rho(fxo_state* s, fx_solve_stats* st) {
     return compute_phi_rho(s, s->P, st ? st : &tmp);
 }
 
+/* ComputeRho (ChargeDensity.cpp:15-102) at the current phi, without a solve. */
+int fxo_compute_rho(fxo_state* s) { compute_rho(s, s->phi); return FX_OK; }
+
 int fxo_compute_E(fxo_state* s) { compute_E(s); return FX_OK; }
 
 int fxo_tdgl_rhs(fxo_state* s) {

@@ -46,6 +46,7 @@ def lib():
             "fxo_set_bias_waveform": (C.c_int, [P, DP, C.c_int]),
             "fxo_compute_phi_rho": (C.c_int, [P, C.POINTER(FxSolveStats)]),
             "fxo_compute_E": (C.c_int, [P]),
+            "fxo_compute_rho": (C.c_int, [P]),
             "fxo_tdgl_rhs": (C.c_int, [P]),
             "fxo_update": (C.c_int, [P, C.c_int]),
             "fxo_step": (C.c_int, [P, C.c_int, C.POINTER(FxStepStats)]),
@@ -124,6 +125,10 @@ class Oracle:
 
     def ComputeEfromPhi(self):
         self._check(self._L.fxo_compute_E(self._h))
+
+    def ComputeRho(self):
+        """rho, electrons and holes of the current phi (no solve)."""
+        self._check(self._L.fxo_compute_rho(self._h))
 
     def CalculateTDGL_RHS(self):
         self._check(self._L.fxo_tdgl_rhs(self._h))
