@@ -130,6 +130,14 @@ struct fx_ctx {
     fxd::SpecTail<float> spec32{};
     double* spec_basis64 = nullptr;   /* Qx, Qy, mu_x, mu_y of level spec0, then two scratch arrays of its size */
     float* spec_basis32 = nullptr;
+    /* direct solve of the finest level when it is laterally periodic and
+     * z-layered (fx_fft.hip), used by the solves without a Helmholtz term */
+    bool direct = false;
+    fxd::FftPlan<double> fft64{};
+    fxd::FftPlan<float> fft32{};
+    double* fft_mem64 = nullptr;      /* twiddles, eigenvalues, S, checkpoints */
+    float* fft_mem32 = nullptr;
+    double fft_w2lo = 0.0, fft_w2hi = 0.0;   /* 2 w of the first / last plane's Dirichlet face (0: Neumann) */
     int tail_ts64 = 0, tail_ts32 = 0;          /* first tail level staged in LDS */
     size_t tail_lds64 = 0, tail_lds32 = 0;     /* dynamic LDS of the tail kernel */
     /* FP32 correction hierarchy of the mixed-precision solve (precision = FP32_REFINE) */
@@ -373,6 +381,8 @@ void fill_plan(fxd::TailPlan<T>& P, const fx_ctx* x, int t0, F lev, const T* lu,
 /* Permittivity with its ghost rule (InitializePermittivity,
  * ElectrostaticSolver.cpp:205-299), face averages (eXstaticUtil.cpp:105-143)
  * and the face means of every coarser level; uploaded to the device. */
+int build_direct(fx_ctx* x, double bz_lo, double bz_hi);
+
 int build_coefficients(fx_ctx* x) {
     const fx_inputs& in = x->in;
     const int nx = x->n[0], ny = x->n[1], nz = x->n[2];
@@ -659,6 +669,64 @@ int build_coefficients(fx_ctx* x) {
             HIPCHK(x, hipMemcpy(x->d_tail32, t32.data(), nt * sizeof(DLevF), hipMemcpyHostToDevice));
         }
     }
+    return build_direct(x, fz[0][0], fz[0][(size_t)nx * ny * nz]);
+}
+
+/* The direct solve of the finest level (fx_fft.hip) where it applies: x and y
+ * periodic, z bounded with a Dirichlet face, the face coefficients a
+ * function of the plane only, lateral extents powers of two (4..1024), and
+ * the poisson_direct input set.  bz_lo, bz_hi: the z-face coefficients of
+ * the first and last plane's boundary faces. */
+int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
+    const fx_inputs& in = x->in;
+    const DLev& d = x->lev[0].d;
+    const bool dlo = in.bc_lo[2] == FX_BC_DIRICHLET, dhi = in.bc_hi[2] == FX_BC_DIRICHLET;
+    x->direct = in.poisson_direct && in.is_periodic[0] && in.is_periodic[1] && !in.is_periodic[2] && (dlo || dhi) &&
+                d.layered && fxk::fft_size_ok(d.n0) && fxk::fft_size_ok(d.n1);
+    if (!x->direct) return FX_OK;
+    const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
+    const size_t ntab = 2 * (size_t)n0 + 2 * (size_t)n1;
+    const size_t nck = (size_t)n0 * n1 * ((n2 + 15) / 16);
+    const size_t total = ntab + x->N + nck;
+    std::vector<double> tab(ntab);
+    double* tw0 = tab.data();
+    double* tw1 = tw0 + n0;
+    double* mux = tw1 + n1;
+    double* muy = mux + n0;
+    const double pi = 3.14159265358979323846;
+    auto fill_dir = [&](int n, double* tw, double* mu) {
+        for (int q = 0; q < n / 2; ++q) { tw[2 * q] = std::cos(2.0 * pi * q / n); tw[2 * q + 1] = -std::sin(2.0 * pi * q / n); }
+        for (int m = 0; m < n; ++m) { const double sn = std::sin(pi * m / n); mu[m] = 4.0 * sn * sn; }   /* 2 - 2 cos(2 pi m / n) */
+    };
+    fill_dir(n0, tw0, mux);
+    fill_dir(n1, tw1, muy);
+    int rc;
+    if (!x->fft_mem64 && (rc = dalloc(x, (void**)&x->fft_mem64, total * sizeof(double)))) return rc;
+    HIPCHK(x, hipMemcpy(x->fft_mem64, tab.data(), ntab * sizeof(double), hipMemcpyHostToDevice));
+    const bool fp32 = in.precision == FX_PREC_FP32_REFINE && !x->lev32.empty() && x->lev32[0].bxk;
+    if (fp32) {
+        std::vector<float> tf(tab.begin(), tab.end());
+        if (!x->fft_mem32 && (rc = dalloc(x, (void**)&x->fft_mem32, total * sizeof(float)))) return rc;
+        HIPCHK(x, hipMemcpy(x->fft_mem32, tf.data(), ntab * sizeof(float), hipMemcpyHostToDevice));
+    }
+    int l0 = 0, l1 = 0;
+    while ((1 << l0) < n0) ++l0;
+    while ((1 << l1) < n1) ++l1;
+    auto fill = [&](auto& P, auto* base, auto* bxk, auto* byk, auto* bzk) {
+        P.n0 = n0; P.n1 = n1; P.n2 = n2; P.l0 = l0; P.l1 = l1;
+        P.tw0 = base; P.tw1 = base + n0; P.mux = base + n0 + n1; P.muy = base + 2 * n0 + n1;
+        P.bxk = bxk; P.byk = byk; P.bzk = bzk;
+        for (int q = 0; q < 3; ++q) P.ih2[q] = d.ih2[q];
+        P.dlo = dlo; P.dhi = dhi;
+        P.S = base + ntab;
+        P.ck = base + ntab + x->N;
+    };
+    fill(x->fft64, x->fft_mem64, d.bxk, d.byk, d.bzk);
+    if (fp32) fill(x->fft32, x->fft_mem32, x->lev32[0].bxk, x->lev32[0].byk, x->lev32[0].bzk);
+    /* the inhomogeneous face term of the row operator is (2 w) v, w = beta_face / hz^2 */
+    x->fft_w2lo = dlo ? 2.0 * (bz_lo * d.ih2[2]) : 0.0;
+    x->fft_w2hi = dhi ? 2.0 * (bz_hi * d.ih2[2]) : 0.0;
+    if (in.verbose) std::printf("direct Poisson solve: %d x %d lateral Hartley transforms, %d planes\n", n0, n1, n2);
     return FX_OK;
 }
 
@@ -1058,8 +1126,61 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
     return fail(x, FX_ENOCONV_MG, "mixed-precision multigrid did not converge in mg_max_iter V-cycles");
 }
 
+/* Direct solve (fx_fft.hip) of the finest level without a Helmholtz term.
+ * check: form the FP64 residual afterwards and apply the multigrid stopping
+ * rule to it, ||r|| <= tol max(||b||, ||r0||), with ||r0|| (the zero
+ * guess's residual, b plus the Dirichlet face terms) replaced by its lower
+ * bound max(||b||, |face term| - ||b||), so the test is never looser; without
+ * it the stats carry no norms (0). */
+int direct_solve(fx_ctx* x, const double bv[6], double tol, fx_solve_stats* st, bool check) {
+    const double blo = x->fft_w2lo * bv[2], bhi = x->fft_w2hi * bv[5];
+    fxk::fft_solve(x->s, x->fft64, x->rhs, x->phi, blo, bhi);
+    st->vcycles = 0; st->mg_resid0 = 0.0; st->mg_bnorm = 0.0; st->mg_resid = 0.0;
+    if (!check) return FX_OK;
+    fxk::mg_resid_norm(x->s, x->lev[0].d, dbc_of(x, bv), 1, nullptr, x->red, x->red_out);
+    double nr[2];
+    int rc = read_red(x, 2, nr);
+    if (rc) return rc;
+    const double r0 = std::max({nr[1], std::fabs(blo) - nr[1], std::fabs(bhi) - nr[1]});
+    st->mg_resid = nr[0]; st->mg_bnorm = nr[1]; st->mg_resid0 = r0;
+    if (!(nr[0] == nr[0]) || !(nr[1] == nr[1])) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
+    if (nr[0] > tol * r0) return fail(x, FX_ENOCONV_MG, "direct Poisson solve: residual above the stopping rule's bound");
+    return FX_OK;
+}
+
+/* Precision form of the direct solve: lateral transforms and tridiagonal
+ * solves in FP32, FP64 iterative refinement.  The first solve takes the FP64
+ * right-hand side (with the face terms) and stores phi in FP64; then, until
+ * the FP64 residual meets the stopping rule (with ||r0|| bounded below as in
+ * direct_solve), r = b - A phi is formed in FP64, stored in FP32, and
+ * phi += A^-1 r in FP32.  st->vcycles counts the FP32 solves. */
+int direct_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+    const DBC bc = dbc_of(x, bv);
+    const double blo = x->fft_w2lo * bv[2], bhi = x->fft_w2hi * bv[5];
+    fxk::fft_solve(x->s, x->fft32, x->rhs, x->phi, blo, bhi);
+    st->vcycles = 1; st->mg_resid0 = 0.0; st->mg_bnorm = 0.0; st->mg_resid = 0.0;
+    for (;;) {
+        fxk::resid_to32(x->s, x->lev[0].d, bc, x->lev32[0].b, x->red, x->red_out);
+        double nr[2];
+        int rc = read_red(x, 2, nr);
+        if (rc) return rc;
+        const double r0 = std::max({nr[1], std::fabs(blo) - nr[1], std::fabs(bhi) - nr[1]});
+        st->mg_resid = nr[0]; st->mg_bnorm = nr[1]; st->mg_resid0 = r0;
+        if (!(nr[0] == nr[0]) || !(nr[1] == nr[1])) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
+        if (nr[0] <= tol * r0) return FX_OK;
+        if (st->vcycles >= maxit) return fail(x, FX_ENOCONV_MG, "FP32 direct solve with refinement did not converge");
+        fxk::fft_solve_add(x->s, x->fft32, x->lev32[0].b, x->phi);
+        ++st->vcycles;
+    }
+}
+
 /* phi_zero: the solve starts from phi = 0, whatever phi holds */
-int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false) {
+int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false,
+              bool check = false) {
+    if (x->direct && x->lev[0].d.read_a == 0) {
+        if (x->in.precision == FX_PREC_FP32_REFINE && x->fft_mem32) return direct_solve_mixed(x, bv, tol, maxit, st);
+        return direct_solve(x, bv, tol, st, check || x->in.poisson_direct_check);
+    }
     if (x->in.precision == FX_PREC_FP32_REFINE) {
         /* FP32 copy of the A-coefficient for the correction levels */
         if (x->lev32[0].read_a) {
@@ -1225,7 +1346,7 @@ void free_all(fx_ctx* x) {
         F(f.phi); F(f.b); F(f.a); F(f.bx); F(f.by); F(f.bz); F((void*)f.bxk); F((void*)f.byk); F((void*)f.bzk); F(f.phi2);
     }
     for (float* p : x->up32) F(p);
-    F(x->spec_basis64); F(x->spec_basis32);
+    F(x->spec_basis64); F(x->spec_basis32); F(x->fft_mem64); F(x->fft_mem32);
     F(x->cg_work32); F(x->phi_alt); F(x->d_tail64); F(x->d_tail32);
     F(x->lu64); F(x->lu32); F(x->perm64); F(x->perm32);
     if (x->h_red) hipHostFree(x->h_red);
@@ -1657,7 +1778,7 @@ int fx_poisson_solve(fx_ctx* x, const double* rhs, const double* acoef, double* 
     bc_values(x, bv);
     std::memcpy(x->bc_solve, bv, sizeof(bv));
     fx_solve_stats tmp{};
-    int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, st ? st : &tmp);
+    int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, st ? st : &tmp, false, true);
     HIPCHK(x, hipMemcpyAsync(phi, x->phi, B, hipMemcpyDeviceToHost, x->s));
     HIPCHK(x, hipStreamSynchronize(x->s));
     return rc;

@@ -327,6 +327,27 @@ struct SpecTail {
     T ih2[3];
     int dlo, dhi;
 };
+/* Direct solve of a laterally periodic, z-layered level (fx_fft.hip): the
+ * Hartley twiddles and eigenvalues of each periodic direction (n0, n1
+ * powers of two), the per-plane face coefficients, the spectrum buffer S
+ * (n0 n1 n2 values) and the tridiagonal solve's checkpoints ck (n0 n1
+ * ceil(n2 / 16) values). */
+template <class T>
+struct FftPlan {
+    int n0, n1, n2;
+    int l0, l1;            /* log2 n0, log2 n1 */
+    const T* tw0;          /* cos(2 pi q / n0), -sin(2 pi q / n0), q < n0/2, interleaved */
+    const T* tw1;
+    const T* mux;          /* 2 - 2 cos(2 pi m / n0) */
+    const T* muy;
+    const T* bxk;
+    const T* byk;
+    const T* bzk;
+    T ih2[3];
+    int dlo, dhi;
+    T* S;
+    T* ck;
+};
 /* Plane kernels of the exact solve (one workgroup per plane and block of 16
  * columns): dynamic LDS values, the plane and the y basis with every row
  * padded by one value so that the matrix-core operand loads, 16 rows at a

@@ -78,6 +78,14 @@ struct EArgs {
 
 namespace fxk {
 void fill(hipStream_t s, double* p, long long n, double v);
+/* direct solve of a laterally periodic z-layered operator (fx_fft.hip):
+ * phi = A^-1 (b + the Dirichlet face terms blo, bhi of the first and last
+ * plane); the FP32 plan transforms in FP32 and stores phi in FP64;
+ * fft_solve_add: phi += A^-1 r (homogeneous faces) */
+bool fft_size_ok(int n);
+void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi);
+void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi);
+void fft_solve_add(hipStream_t s, const fxd::FftPlan<float>& P, const float* r, double* phi);
 /* host[0..nq) = src[0..nq), then *seq = v (host and seq: device views of
  * coherent pinned host memory) */
 void publish(hipStream_t s, const double* src, int nq, double* host, unsigned* seq, unsigned v);

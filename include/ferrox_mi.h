@@ -140,6 +140,12 @@ typedef struct fx_inputs {
                                      exactly (lateral Fourier transforms + tridiagonal solves) instead of a sub-V-cycle */
     int    mg_refine_nu1;         /* precision = 1: pre- and post-smoothing sweeps of the FP32 correction V-cycles */
     int    mg_refine_nu2;         /* (defaults 1 and 2; the FP64 V-cycles use mg_nu1, mg_nu2) */
+    int    poisson_direct;        /* 1 (default): a laterally periodic stack whose permittivity depends on z only,
+                                     with a Dirichlet z face, lateral extents powers of two (4..1024) and no
+                                     Helmholtz term is solved directly (lateral Hartley transforms + one
+                                     tridiagonal solve per mode) instead of by multigrid; 0: multigrid always */
+    int    poisson_direct_check;  /* 1: after each direct solve, form the FP64 residual and apply the multigrid
+                                     stopping rule to it (FX_ENOCONV_MG if it fails); 0 (default): no check */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;

@@ -36,7 +36,7 @@ def _dp(a):
 def cycles(text, ncyc, block, rhs, acoef=None, **kw):
     """phi after exactly ncyc V-cycles from phi = 0 (a zero tolerance stops
     the solve at mg_max_iter)."""
-    inp = fx.from_text(text, mg_tol_rel=0.0, mg_max_iter=ncyc, mg_block_levels=block, **kw)
+    inp = fx.from_text(text, mg_tol_rel=0.0, mg_max_iter=ncyc, mg_block_levels=block, poisson_direct=0, **kw)
     sim = fx.FerroX(inp)
     b = np.ascontiguousarray(rhs, dtype=np.float64)
     a = None if acoef is None else np.ascontiguousarray(acoef, dtype=np.float64)
@@ -82,7 +82,7 @@ CASES = {
 def test_block_legs_match_per_level_kernels(case, helmholtz):
     make, kw = CASES[case]
     text = make()
-    probe = fx.FerroX(fx.from_text(text))
+    probe = fx.FerroX(fx.from_text(text, poisson_direct=0))
     shape = probe.shape
     probe.close()
     rng = np.random.default_rng(17)
@@ -119,7 +119,7 @@ STREAM_CASES = {
 @pytest.mark.parametrize("helmholtz", [False, True])
 def test_two_sweep_pass_matches_single_sweeps(case, helmholtz):
     text = STREAM_CASES[case]()
-    probe = fx.FerroX(fx.from_text(text))
+    probe = fx.FerroX(fx.from_text(text, poisson_direct=0))
     shape = probe.shape
     probe.close()
     rng = np.random.default_rng(23)

@@ -17,7 +17,11 @@ TOL = 1e-10
 
 
 def pair(text, **kw):
+    """GPU and oracle on the same inputs.  These are the multigrid's parity
+    cases, so the direct solve of laterally periodic layered stacks is off
+    unless asked for (tests/test_gpu_direct.py covers it)."""
     kw.setdefault("mg_tol_rel", 1e-13)
+    kw.setdefault("poisson_direct", 0)
     if kw.get("mg_fused_sweep", 0):
         # the streaming sweeps on every level they can serve, not only on
         # levels of 2M cells and more
@@ -345,7 +349,7 @@ def test_throughput_workload_one_step(relerr, tol):
     8x8 and 4x4 columns on the next two, the LDS-resident tail and the
     explicit-inverse bottom.  At 1e-12 both sides converge far below the bar;
     at the reference's 1e-10 they stop on the same V-cycle."""
-    inp = fx.from_text(configs.mfim_poly_256(nsteps=1), mg_tol_rel=tol)
+    inp = fx.from_text(configs.mfim_poly_256(nsteps=1), mg_tol_rel=tol, poisson_direct=0)
     f = configs.polycrystal_fields(inp, n_grains=400, fe_fraction=0.7, alpha_pe=1.0e9, seed=7)
     g, o = fx.FerroX(inp), Oracle(inp)
     configs.apply_fields(g, f)

@@ -46,7 +46,8 @@ CASES = {
 
 
 def solve(text, exact, precision, rhs):
-    inp = fx.from_text(text, mg_tol_rel=1e-12, mg_max_iter=200, mg_exact_tail=exact, precision=precision)
+    inp = fx.from_text(text, mg_tol_rel=1e-12, mg_max_iter=200, mg_exact_tail=exact, precision=precision,
+                      poisson_direct=0)
     sim = fx.FerroX(inp)
     b = np.ascontiguousarray(rhs, dtype=np.float64)
     phi = np.zeros(sim.shape)
@@ -61,7 +62,7 @@ def solve(text, exact, precision, rhs):
 @pytest.mark.parametrize("case", sorted(CASES))
 def test_exact_solve_matches_multigrid_tail(case, precision):
     text = CASES[case]()
-    sim = fx.FerroX(fx.from_text(text))
+    sim = fx.FerroX(fx.from_text(text, poisson_direct=0))
     shape = sim.shape
     sim.close()
     rng = np.random.default_rng(7)
