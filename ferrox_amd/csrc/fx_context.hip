@@ -85,6 +85,7 @@ struct fx_ctx {
      * whether the host has yet to read the last of them */
     double* steady_dev = nullptr;
     bool steady_pending = false;
+    bool phi_old_stale = false;    /* steps skipped the check: phi_old is not the previous step's phi */
     double* E[3] = {nullptr, nullptr, nullptr};
     double* GL[3] = {nullptr, nullptr, nullptr};
     double* GL1[3] = {nullptr, nullptr, nullptr};
@@ -1712,8 +1713,20 @@ int fx_step(fx_ctx* x, int nsteps, fx_step_stats* out) {
     };
     /* plotfile of the initial data (main.cpp:228-234) */
     if (in.plot_int > 0 && x->step == 0 && (rc = plot(0))) return fail(x, rc, "writing plt00000000 failed");
+    /* With phi_tolerance <= 0 and no voltage sweep, CheckSteadyState
+     * (ElectrostaticSolver.cpp:526-560) can neither stop nor advance the run:
+     * its value is seen only after the last step of this call and through
+     * PhiDiff at plot steps.  It is then formed only at those steps, with
+     * phi_old = phi copied at the start of such a step when the step before
+     * skipped it (the check itself copies phi_old = phi). */
+    const bool lazy = !(in.phi_tolerance > 0.0) && !in.verbose && in.voltage_sweep == 0;
     for (int it = 0; it < nsteps; ++it) {
         const int step = ++x->step;
+        const bool check_now = !lazy || it == nsteps - 1 || (in.plot_int > 0 && step % in.plot_int == 0);
+        if (check_now && x->phi_old_stale) {
+            fxk::copy(x->s, x->phi_old, x->phi, (long long)x->N);
+            x->phi_old_stale = false;
+        }
         if (!x->waveform.empty()) x->Phi_Bc_hi = x->waveform[std::min((size_t)step, x->waveform.size()) - 1];
         fx_solve_stats ss{};
         rc = compute_phi_rho(x, x->P, &ss);
@@ -1723,7 +1736,11 @@ int fx_step(fx_ctx* x, int nsteps, fx_step_stats* out) {
         rc = update(x, in.TimeIntegratorOrder, &ss2);
         st.total_vcycles += ss2.vcycles; st.total_outer_iters += ss2.outer_iters; st.total_solves += ss2.outer_iters;
         if (rc) break;
-        if ((rc = steady_check(x, step, &st.last_phi_change))) break;
+        if (check_now) {
+            if ((rc = steady_check(x, step, &st.last_phi_change))) break;
+        } else {
+            x->phi_old_stale = true;
+        }
         x->time = x->time + in.dt;
         st.steps_done++;
         if (in.verbose) std::printf("Advanced step %d\n", step);
