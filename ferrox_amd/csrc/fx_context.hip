@@ -86,6 +86,21 @@ struct fx_ctx {
     double* steady_dev = nullptr;
     bool steady_pending = false;
     bool phi_old_stale = false;    /* steps skipped the check: phi_old is not the previous step's phi */
+    /* the TDGL update of the last step also formed the Poisson RHS of the P
+     * it produced (k_tdgl_rhs_stream): rhs holds RHS(P) for the current P and
+     * the next solve needs no RHS pass.  Anything else that writes P, rhs,
+     * the materials or the rotations clears it. */
+    bool rhs_next = false;
+    int fe_k0 = -1, fe_k1 = -1;    /* first and last plane holding an FE cell (-1: none) */
+    int* face_list = nullptr;      /* FE cells without centred stencils (global indices) */
+    int nface = 0;
+    /* the fused update writes the next RHS into rhs_nx, which the next solve
+     * swaps in (rhs keeps the RHS of the last solve, the PoissonRHS the
+     * reference writes at plot time); the fused kernel writes only planes
+     * fe_k0..fe_k1, so each buffer records whether it holds 0 in every
+     * non-FE cell */
+    double* rhs_nx = nullptr;
+    bool rhs_clean = false, rhs_nx_clean = true, rhs_exposed = false;
     double* E[3] = {nullptr, nullptr, nullptr};
     double* GL[3] = {nullptr, nullptr, nullptr};
     double* GL1[3] = {nullptr, nullptr, nullptr};
@@ -762,11 +777,36 @@ int upload_mat(fx_ctx* x) {
     std::vector<uint8_t> dm(x->h_mat);
     mark_interior_fe(x, dm);
     HIPCHK(x, hipMemcpyAsync(x->mat, dm.data(), dm.size(), hipMemcpyHostToDevice, x->s));
+    /* FE cells whose stencils are not all centred: the fused TDGL + RHS
+     * update forms them on their own (k_tdgl_face_cells, k_rhs_face_cells) */
+    std::vector<int> face;
+    for (int k = 0; k < x->n[2]; ++k)
+        for (int j = 0; j < x->n[1]; ++j)
+            for (int i = 0; i < x->n[0]; ++i) {
+                const uint8_t r = dm[gidx(x, i, j, k)];
+                if ((r & 3) == 0 && !(r & fxd::MAT_INTERIOR_FE)) face.push_back(i + x->n[0] * (j + x->n[1] * k));
+            }
+    if (x->face_list) { (void)hipFree(x->face_list); x->face_list = nullptr; }
+    x->nface = (int)face.size();
+    if (!face.empty()) {
+        HIPCHK(x, hipMalloc((void**)&x->face_list, face.size() * sizeof(int)));
+        HIPCHK(x, hipMemcpyAsync(x->face_list, face.data(), face.size() * sizeof(int), hipMemcpyHostToDevice, x->s));
+    }
     x->has_sc = false;
     for (int k = 0; k < x->n[2]; ++k)
         for (int j = 0; j < x->n[1]; ++j)
             for (int i = 0; i < x->n[0]; ++i)
                 if ((x->h_mat[gidx(x, i, j, k)] & 3) >= 2) x->has_sc = true;
+    x->fe_k0 = x->fe_k1 = -1;
+    for (int k = 0; k < x->n[2]; ++k)
+        for (int j = 0; j < x->n[1]; ++j)
+            for (int i = 0; i < x->n[0]; ++i)
+                if ((x->h_mat[gidx(x, i, j, k)] & 3) == 0) {
+                    if (x->fe_k0 < 0) x->fe_k0 = k;
+                    x->fe_k1 = k;
+                }
+    x->rhs_next = false;
+    x->rhs_clean = x->rhs_nx_clean = false;   /* the FE cells moved: the zeros outside them are not known */
     HIPCHK(x, hipStreamSynchronize(x->s));
     return FX_OK;
 }
@@ -1207,6 +1247,7 @@ RhsArgs rhs_args(fx_ctx* x, double* const* P) {
     for (int d = 0; d < 3; ++d) a.P[d] = P[d];
     a.phi = x->phi; a.rho = x->rho; a.rhs = x->rhs; a.alpha = x->alpha_cc;
     a.delta = x->in.delta; a.pxy_zero = x->pxy_zero; a.sc = x->sc;
+    a.kz0 = 0; a.kz1 = x->n[2] - 1;
     return a;
 }
 
@@ -1221,7 +1262,25 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
     while (err > OUTER_TOL) {
         RhsArgs ra0 = rhs_args(x, P);
         if (!x->has_sc && x->alpha_clean) ra0.alpha = nullptr;   /* alpha is 0 outside SC cells: written once */
-        fxk::rhs_newton(x->s, ra0);
+        /* the RHS of this P was formed by the last TDGL update (no SC cells) */
+        if (x->rhs_next && !x->has_sc && x->alpha_clean && P == x->P) {
+            std::swap(x->rhs, x->rhs_nx);
+            std::swap(x->rhs_clean, x->rhs_nx_clean);
+            x->lev[0].d.b = x->rhs;
+        } else {
+            ra0.rhs = x->rhs;
+            /* without SC cells the RHS is 0 outside FE cells: once the buffer
+             * holds those zeros (and alpha is not written), only the planes
+             * with FE cells are formed */
+            const bool narrow = !x->has_sc && x->rhs_clean && !x->rhs_exposed && !ra0.alpha;
+            if (narrow) {
+                ra0.kz0 = x->fe_k0 >= 0 ? x->fe_k0 : 1;
+                ra0.kz1 = x->fe_k0 >= 0 ? x->fe_k1 : 0;
+            }
+            fxk::rhs_newton(x->s, ra0);
+            x->rhs_clean = !x->has_sc;
+        }
+        x->rhs_next = false;
         x->alpha_clean = !x->has_sc && !x->alpha_exposed;
         set_read_a(x, x->has_sc ? 1 : 0);
         if (x->has_sc) restrict_acoef(x);
@@ -1267,6 +1326,7 @@ TdglArgs tdgl_args(fx_ctx* x, double* const* P) {
     a.BigGamma = in.BigGamma; a.g11 = in.g11; a.g44 = in.g44; a.g12 = in.g12; a.g44_p = in.g44_p; a.dt = in.dt;
     a.incL = in.include_Landau; a.incG = in.include_Grad; a.incE = in.include_Elec;
     a.scalar = in.is_polarization_scalar == 1;
+    a.kz0 = 0; a.kz1 = x->n[2] - 1;
     return a;
 }
 
@@ -1276,9 +1336,32 @@ bool scalar_fast(const fx_ctx* x) { return x->in.is_polarization_scalar == 1 && 
 int update(fx_ctx* x, int order, fx_solve_stats* st2) {
     const bool fast = scalar_fast(x);
     TdglArgs a = tdgl_args(x, x->P);
+    x->rhs_next = false;
     if (order == 1) {
         for (int d = 0; d < 3; ++d) a.Pout[d] = (fast && d < 2) ? nullptr : x->Pn[d];
-        fxk::tdgl(x->s, a, fast);
+        /* scalar P changes only in FE cells (Gamma = 0 elsewhere) */
+        if (fast) {
+            a.kz0 = x->fe_k0 >= 0 ? x->fe_k0 : 1;
+            a.kz1 = x->fe_k0 >= 0 ? x->fe_k1 : 0;
+        }
+        if (fast && !x->has_sc && x->in.tdgl_fused_rhs && !x->rhs_exposed && x->fe_k0 >= 0 && x->n[0] >= 4 &&
+            x->n[1] >= 4) {
+            if (!x->rhs_nx_clean) {
+                fxk::fill(x->s, x->rhs_nx, (long long)x->N, 0.0);
+                x->rhs_nx_clean = true;
+            }
+            TdglRhsArgs f;
+            f.t = a;
+            f.rhs = x->rhs_nx;
+            f.k0 = x->fe_k0;
+            f.k1 = x->fe_k1;
+            f.face = x->face_list;
+            f.nface = x->nface;
+            fxk::tdgl_rhs_stream(x->s, f);
+            x->rhs_next = true;
+        } else {
+            fxk::tdgl(x->s, a, fast);
+        }
         if (fast) std::swap(x->P[2], x->Pn[2]);
         else for (int d = 0; d < 3; ++d) std::swap(x->P[d], x->Pn[d]);
         return FX_OK;
@@ -1332,7 +1415,7 @@ int steady_check(fx_ctx* x, int step, double* metric) {
 void free_all(fx_ctx* x) {
     auto F = [](void* p) { if (p) hipFree(p); };
     for (int d = 0; d < 3; ++d) { F(x->P[d]); F(x->Pn[d]); F(x->E[d]); F(x->GL[d]); F(x->GL1[d]); }
-    F(x->phi); F(x->phi_old); F(x->phi_prev); F(x->rhs); F(x->rho); F(x->e_den); F(x->h_den);
+    F(x->phi); F(x->phi_old); F(x->phi_prev); F(x->rhs); F(x->rhs_nx); F(x->face_list); F(x->rho); F(x->e_den); F(x->h_den);
     F(x->alpha_cc); F(x->phidiff); F(x->eps); F(x->mat); F(x->rot_idx); F(x->rot_tab); F(x->rot_atab);
     F(x->diff.ck); F(x->diff.segP); F(x->diff.segPhi); F(x->diff.lam[0]); F(x->diff.lam[1]); F(x->diff.s);
     F(x->diff.fin); F(x->diff.acc); F(x->diff.m); F(x->diff.part);
@@ -1412,7 +1495,7 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
     const size_t B = x->N * sizeof(double);
 #define ALLOC(p, b) do { if ((rc = dalloc(x, (void**)&(p), (b)))) { free_all(x); std::fprintf(stderr, "fx_create: %s\n", x->err.c_str()); delete x; return rc; } } while (0)
     for (int d = 0; d < 3; ++d) { ALLOC(x->P[d], B); ALLOC(x->Pn[d], B); ALLOC(x->E[d], B); ALLOC(x->GL[d], B); ALLOC(x->GL1[d], B); }
-    ALLOC(x->phi, B); ALLOC(x->phi_old, B); ALLOC(x->phi_prev, B); ALLOC(x->rhs, B); ALLOC(x->rho, B);
+    ALLOC(x->phi, B); ALLOC(x->phi_old, B); ALLOC(x->phi_prev, B); ALLOC(x->rhs, B); ALLOC(x->rhs_nx, B); ALLOC(x->rho, B);
     ALLOC(x->e_den, B); ALLOC(x->h_den, B); ALLOC(x->alpha_cc, B); ALLOC(x->phidiff, B); ALLOC(x->eps, B);
     ALLOC(x->tmp, 2 * B);
     ALLOC(x->mat, (size_t)x->msy * (x->n[2] + 2));
@@ -1563,6 +1646,7 @@ int fx_get_field(fx_ctx* x, int f, double* host) {
 
 int fx_set_field(fx_ctx* x, int f, const double* host) {
     if (!x || !host) return FX_EINVAL;
+    x->rhs_next = false;
     const size_t N = x->N, B = N * sizeof(double);
     switch (f) {
     case FX_FIELD_PX: case FX_FIELD_PY: case FX_FIELD_PZ: {
@@ -1783,8 +1867,10 @@ int fx_set_plot_dir(fx_ctx* x, const char* dir) {
 
 int fx_poisson_solve(fx_ctx* x, const double* rhs, const double* acoef, double* phi, fx_solve_stats* st) {
     if (!x || !rhs || !phi) return FX_EINVAL;
+    x->rhs_next = false;
     const size_t B = x->N * sizeof(double);
     HIPCHK(x, hipMemcpyAsync(x->rhs, rhs, B, hipMemcpyHostToDevice, x->s));
+    x->rhs_clean = false;
     x->alpha_clean = false;
     if (acoef) HIPCHK(x, hipMemcpyAsync(x->alpha_cc, acoef, B, hipMemcpyHostToDevice, x->s));
     else fxk::fill(x->s, x->alpha_cc, (long long)x->N, 0.0);
@@ -1806,6 +1892,7 @@ int fx_field_device_ptr(fx_ctx* x, int f, void** dptr) {
     double* d = field_dev(x, f);
     if (!d) return fail(x, FX_EINVAL, "field has no device array");
     if (d == x->rho) x->rho_exposed = true, x->rho_clean = false;
+    if (d == x->rhs) x->rhs_exposed = true;   /* the buffers are not swapped from then on */
     if (d == x->alpha_cc) x->alpha_exposed = true, x->alpha_clean = false;
     *dptr = d;
     return FX_OK;
@@ -1877,6 +1964,7 @@ static int diff_reserve(fx_ctx* x, int nsteps, int K) {
 
 /* restore P (both buffers: the scalar step leaves non-FE cells in place) */
 static void diff_set_P(fx_ctx* x, const double* src) {
+    x->rhs_next = false;
     fxk::copy(x->s, x->P[2], src, (long long)x->N);
     fxk::copy(x->s, x->Pn[2], src, (long long)x->N);
 }
@@ -1891,6 +1979,7 @@ static int diff_step(fx_ctx* x, double V) {
 
 int fx_diff_set_params(fx_ctx* x, const double* th) {
     if (!x || !th) return FX_EINVAL;
+    x->rhs_next = false;
     fx_inputs& in = x->in;
     const double a_old = in.alpha;
     in.alpha = th[0]; in.beta = th[1]; in.gamma = th[2]; in.g11 = th[3]; in.g44 = th[4];
@@ -1940,6 +2029,7 @@ int fx_diff_forward(fx_ctx* x, const double* V, int nsteps, int K, double* m_out
 }
 
 int fx_diff_backward(fx_ctx* x, const double* dL_dm, double* g_theta, double* g_V) {
+    if (x) x->rhs_next = false;
     if (!x || !dL_dm) return FX_EINVAL;
     fx_ctx::Diff& D = x->diff;
     if (!D.valid) return fail(x, FX_EINVAL, "fx_diff_backward needs a preceding fx_diff_forward with the same parameters");
@@ -1987,6 +2077,7 @@ int fx_diff_backward(fx_ctx* x, const double* dL_dm, double* g_theta, double* g_
             fxk::adj_fzT(x->s, a);
             /* psi = A^-1 (Fz^T nu): the symmetric operator, homogeneous BCs */
             fxk::copy(x->s, x->rhs, D.s, (long long)N);
+            x->rhs_clean = false;
             fxk::fill(x->s, x->phi, (long long)N, 0.0);
             set_read_a(x, 0);
             const double bz[6] = {0, 0, 0, 0, 0, 0};

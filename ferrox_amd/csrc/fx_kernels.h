@@ -33,6 +33,7 @@ struct RhsArgs {
     double delta;
     int pxy_zero;
     SCParams sc;
+    int kz0, kz1;   /* planes formed (the caller narrows them to the FE planes when the rest is known to be 0) */
 };
 
 struct RhoArgs {
@@ -59,6 +60,19 @@ struct TdglArgs {
     double* GLout[3];
     double alpha, beta, gamma, a12, a112, a123, BigGamma, g11, g44, g12, g44_p, dt;
     int incL, incG, incE, scalar;
+    int kz0, kz1;   /* planes updated (P changes only where Gamma != 0, i.e. in FE cells) */
+};
+
+/* TDGL update fused with the next step's Poisson RHS (scalar P, forward
+ * Euler, no semiconductor cells): t's Pout[2] receives P_new, rhs the RHS of
+ * P_new in planes k0..k1 (the planes holding FE cells); face lists the FE
+ * cells next to a material or domain face (global cell indices) */
+struct TdglRhsArgs {
+    TdglArgs t;
+    double* rhs;
+    int k0, k1;
+    const int* face;   /* FE cells whose stencils are not all centred (no MAT_INTERIOR_FE) */
+    int nface;
 };
 
 struct HeunArgs {
@@ -100,6 +114,7 @@ void rhs_newton(hipStream_t s, const RhsArgs& a);
 /* returns the number of per-block partials written ([num...][den...]) */
 int rho_update(hipStream_t s, const RhoArgs& a);
 void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast);
+void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a);
 void heun(hipStream_t s, const HeunArgs& a);
 void compute_E(hipStream_t s, const EArgs& a);
 void mg_smooth(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int color);

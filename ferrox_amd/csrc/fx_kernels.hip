@@ -115,12 +115,22 @@ __device__ __forceinline__ void rot_of(const RotArgs& r, int c, double R[9]) {
     }
 }
 
+/* R_31, R_32, R_33 into R[6..8] (the rest left unset) */
+__device__ __forceinline__ void rot_row3(const RotArgs& r, int c, double R[9]) {
+    if (r.idx) {
+        const double* t = r.tab + 9 * r.idx[c];
+        R[6] = t[6]; R[7] = t[7]; R[8] = t[8];
+    } else {
+        R[6] = r.Ru[6]; R[7] = r.Ru[7]; R[8] = r.Ru[8];
+    }
+}
+
 /* Poisson RHS + Newton linearisation, one thread per cell. */
 __global__ __launch_bounds__(256) void k_rhs_newton(RhsArgs a) {
     const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
+    const int k = a.kz0 + blockIdx.z;
     if (i >= nx || j >= ny || k >= nz) return;
     const int c = i + nx * (j + ny * k);
     const int code = a.M.code(i, j, k);
@@ -136,7 +146,8 @@ __global__ __launch_bounds__(256) void k_rhs_newton(RhsArgs a) {
         al = 0.;
     } else {
         double R[9];
-        rot_of(a.rot, c, R);
+        if (a.pxy_zero) rot_row3(a.rot, c, R);   /* scalar P: only R_3 enters */
+        else rot_of(a.rot, c, R);
         const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
         const double* dx = a.g.dx;
         double x = 0.0;
@@ -240,23 +251,99 @@ __device__ __forceinline__ double grad_r(const PView& Fr, const MView& M, int i,
  * so the neighbours are read directly.  The spacings enter as the reciprocals
  * ih = 1/dx, multiplied in (the masked stencils divide): the same differences
  * to rounding, three FP64 divisions per cell instead of fifteen. */
-__device__ __forceinline__ double grad_r_interior(const double* __restrict__ P, int c, const NbrOff& o,
-                                                  const double* ih, const double R[9], double g11, double g44m) {
-    const double pc = P[c];
-    const double Hxx = (P[c + o.xp] - 2. * pc + P[c + o.xm]) * (ih[0] * ih[0]);
-    const double Hyy = (P[c + o.yp] - 2. * pc + P[c + o.ym]) * (ih[1] * ih[1]);
-    const double Hzz = (P[c + o.zp] - 2. * pc + P[c + o.zm]) * (ih[2] * ih[2]);
+template <class Acc>
+__device__ __forceinline__ double grad_r_interior_acc(const Acc& F, const double* ih, const double R[9], double g11,
+                                                      double g44m) {
+    const double pc = F(0, 0, 0);
+    const double Hxx = (F(1, 0, 0) - 2. * pc + F(-1, 0, 0)) * (ih[0] * ih[0]);
+    const double Hyy = (F(0, 1, 0) - 2. * pc + F(0, -1, 0)) * (ih[1] * ih[1]);
+    const double Hzz = (F(0, 0, 1) - 2. * pc + F(0, 0, -1)) * (ih[2] * ih[2]);
     /* the mixed entries are formed whether or not R couples them (a zero
      * product of R entries then multiplies a finite value), so every load
      * can be issued before R is known */
-    const double Hxy = ((P[c + o.xp + o.yp] - P[c + o.xp + o.ym]) - (P[c + o.xm + o.yp] - P[c + o.xm + o.ym])) *
-                       (0.25 * ih[0] * ih[1]);
-    const double Hyz = ((P[c + o.yp + o.zp] - P[c + o.yp + o.zm]) - (P[c + o.ym + o.zp] - P[c + o.ym + o.zm])) *
-                       (0.25 * ih[1] * ih[2]);
-    const double Hxz = ((P[c + o.xp + o.zp] - P[c + o.xp + o.zm]) - (P[c + o.xm + o.zp] - P[c + o.xm + o.zm])) *
-                       (0.25 * ih[0] * ih[2]);
+    const double Hxy = ((F(1, 1, 0) - F(1, -1, 0)) - (F(-1, 1, 0) - F(-1, -1, 0))) * (0.25 * ih[0] * ih[1]);
+    const double Hyz = ((F(0, 1, 1) - F(0, 1, -1)) - (F(0, -1, 1) - F(0, -1, -1))) * (0.25 * ih[1] * ih[2]);
+    const double Hxz = ((F(1, 0, 1) - F(1, 0, -1)) - (F(-1, 0, 1) - F(-1, 0, -1))) * (0.25 * ih[0] * ih[2]);
     return grad_r_of(R, Hxx, Hyy, Hzz, Hxy, Hyz, Hxz, g11, g44m);
 }
+
+/* P of an interior cell's neighbourhood from global memory (periodic wrap
+ * laterally through the offsets). */
+struct GlobalNbr {
+    const double* __restrict__ P;
+    int c;
+    NbrOff o;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        return P[c + (di > 0 ? o.xp : di < 0 ? o.xm : 0) + (dj > 0 ? o.yp : dj < 0 ? o.ym : 0) +
+                 (dk > 0 ? o.zp : dk < 0 ? o.zm : 0)];
+    }
+};
+
+__device__ __forceinline__ double grad_r_interior(const double* __restrict__ P, int c, const NbrOff& o,
+                                                  const double* ih, const double R[9], double g11, double g44m) {
+    return grad_r_interior_acc(GlobalNbr{P, c, o}, ih, R, g11, g44m);
+}
+
+/* The scalar TDGL update of one FE cell (SCALAR_FAST: is_polarization_scalar
+ * with P_p = P_q = 0): GL_r and the new P_r.  Interior cells take the Hessian
+ * of P from F (neighbour offsets -1..1) and grad(phi) from the centred
+ * differences; the others the masked stencils on global memory. */
+template <class Acc>
+__device__ __forceinline__ double tdgl_scalar_cell(const TdglArgs& a, int i, int j, int k, int c, int raw,
+                                                   const Acc& F, double& g_out) {
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const bool tph = (raw & 4) != 0;
+    const double Gam = a.BigGamma;
+    const double* dx = a.g.dx;
+    double R[9];
+    rot_of(a.rot, c, R);
+    const bool inner = (raw & MAT_INTERIOR_FE) != 0;
+    const NbrOff o(i, j, nx, ny);
+    double ih[3] = {0.0, 0.0, 0.0};
+    if (inner) { ih[0] = 1.0 / dx[0]; ih[1] = 1.0 / dx[1]; ih[2] = 1.0 / dx[2]; }
+    double E2 = 0.0;
+    if (a.incE == 1) {
+        double fx, fy, fz;
+        if (inner) {
+            const double* ph = a.phv.p;
+            fx = (ph[c + o.xp] - ph[c + o.xm]) * (0.5 * ih[0]);
+            fy = (ph[c + o.yp] - ph[c + o.ym]) * (0.5 * ih[1]);
+            fz = (ph[c + o.zp] - ph[c + o.zm]) * (0.5 * ih[2]);
+        } else {
+            grad_phi(a.phv, a.g, i, j, k, fx, fy, fz);
+        }
+        double s = 0.0;
+        if (R[6] != 0.0) s += R[6] * fx;
+        if (R[7] != 0.0) s += R[7] * fy;
+        s += R[8] * fz;
+        E2 = -s;
+    }
+    const double r = F(0, 0, 0);
+    double L = 0.0, G = 0.0, El = 0.0;
+    if (a.incL == 1) {
+        const double r2 = r * r;
+        const double alpha = a.rot.idx ? a.rot.atab[a.rot.idx[c]] : a.alpha;
+        const double dF = alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
+        L = -1.0 * Gam * dF;
+    }
+    if (a.incG == 1) {
+        if (inner) {
+            G = -1.0 * Gam * grad_r_interior_acc(F, ih, R, a.g11, a.g44 - a.g44_p);
+        } else {
+            const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+            G = -1.0 * Gam * grad_r(Fr, a.M, i, j, k, dx, a.pbc, R, a.g11, a.g44 - a.g44_p);
+        }
+    }
+    if (a.incE == 1) El = -1.0 * Gam * (-E2);
+    if (tph) { L = 0.0; G = 0.0; El = 0.0; }
+    double g = 0.0;
+    if (a.incL == 1) g += L;
+    if (a.incG == 1) g += G;
+    if (a.incE == 1) g += El;
+    g_out = g;
+    return r + a.dt * g;
+}
+
 
 /* SCALAR_FAST: is_polarization_scalar with P_p = P_q = 0 everywhere (only P_r
  * is read or written).  Non-FE cells have Gamma = 0, so P never changes there;
@@ -266,7 +353,7 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
     const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
     const int i = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
+    const int k = a.kz0 + blockIdx.z;
     if (i >= nx || j >= ny || k >= nz) return;
     const int c = i + nx * (j + ny * k);
     const int raw = a.M.raw(i, j, k);
@@ -277,6 +364,13 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
         }
         return;
     }
+    if (SCALAR_FAST) {
+        double g;
+        const double rn = tdgl_scalar_cell(a, i, j, k, c, raw, GlobalNbr{a.P[2], c, NbrOff(i, j, nx, ny)}, g);
+        if (a.GLout[2]) a.GLout[2][c] = g;
+        if (a.Pout[2]) a.Pout[2][c] = rn;
+        return;
+    }
     const bool tph = (raw & 4) != 0;
     const double Gam = a.BigGamma;
     const double* dx = a.g.dx;
@@ -284,58 +378,14 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
     rot_of(a.rot, c, R);
     const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
     double GL[3] = {0.0, 0.0, 0.0};
-
-    const bool inner = SCALAR_FAST && (raw & MAT_INTERIOR_FE);
-    const NbrOff o(i, j, nx, ny);
-    double ih[3] = {0.0, 0.0, 0.0};   /* interior cells: reciprocal spacings */
-    if (inner) { ih[0] = 1.0 / dx[0]; ih[1] = 1.0 / dx[1]; ih[2] = 1.0 / dx[2]; }
     /* electric field E = -R grad(phi) */
     double E[3] = {0.0, 0.0, 0.0};
     if (a.incE == 1) {
         double fx, fy, fz;
-        if (inner) {   /* centred in every direction (DFDx, DFDy, DphiDz interior) */
-            const double* ph = a.phv.p;
-            fx = (ph[c + o.xp] - ph[c + o.xm]) * (0.5 * ih[0]);
-            fy = (ph[c + o.yp] - ph[c + o.ym]) * (0.5 * ih[1]);
-            fz = (ph[c + o.zp] - ph[c + o.zm]) * (0.5 * ih[2]);
-        } else {
-            grad_phi(a.phv, a.g, i, j, k, fx, fy, fz);
-        }
-        if (SCALAR_FAST) {
-            double s = 0.0;
-            if (R[6] != 0.0) s += R[6] * fx;
-            if (R[7] != 0.0) s += R[7] * fy;
-            s += R[8] * fz;
-            E[2] = -s;
-        } else {
-            E[0] = -(R[0] * fx + R[1] * fy + R[2] * fz);
-            E[1] = -(R[3] * fx + R[4] * fy + R[5] * fz);
-            E[2] = -(R[6] * fx + R[7] * fy + R[8] * fz);
-        }
-    }
-
-    if (SCALAR_FAST) {
-        const double r = a.P[2][c];
-        double L = 0.0, G = 0.0, El = 0.0;
-        if (a.incL == 1) {
-            const double r2 = r * r;
-            const double alpha = a.rot.idx ? a.rot.atab[a.rot.idx[c]] : a.alpha;
-            const double dF = alpha * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
-            L = -1.0 * Gam * dF;
-        }
-        if (a.incG == 1)
-            G = -1.0 * Gam * (inner ? grad_r_interior(a.P[2], c, o, ih, R, a.g11, a.g44 - a.g44_p)
-                                    : grad_r(Fr, a.M, i, j, k, dx, a.pbc, R, a.g11, a.g44 - a.g44_p));
-        if (a.incE == 1) El = -1.0 * Gam * (-E[2]);
-        if (tph) { L = 0.0; G = 0.0; El = 0.0; }
-        double g = 0.0;
-        if (a.incL == 1) g += L;
-        if (a.incG == 1) g += G;
-        if (a.incE == 1) g += El;
-        GL[2] = g;
-        if (a.GLout[2]) a.GLout[2][c] = g;
-        if (a.Pout[2]) a.Pout[2][c] = r + a.dt * g;
-        return;
+        grad_phi(a.phv, a.g, i, j, k, fx, fy, fz);
+        E[0] = -(R[0] * fx + R[1] * fy + R[2] * fz);
+        E[1] = -(R[3] * fx + R[4] * fy + R[5] * fz);
+        E[2] = -(R[6] * fx + R[7] * fy + R[8] * fz);
     }
 
     const PView Fp{a.P[0], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
@@ -408,6 +458,184 @@ __global__ __launch_bounds__(256) void k_tdgl(TdglArgs a) {
     }
 }
 
+/* ------------- TDGL update fused with the next step's Poisson RHS ---------- */
+/* Scalar P (P_p = P_q = 0), forward Euler, no semiconductor cells: the
+ * Poisson right-hand side of the next step (ComputePoissonRHS,
+ * ElectrostaticSolver.cpp:8-83: -R grad P_new in FE cells, 0 elsewhere)
+ * depends on P_new only, so it is formed by the workgroup that computes
+ * P_new, from a window of P_new in LDS: P is read once and written once per
+ * step, and the separate RHS pass (which reads P again) is gone.
+ *
+ * Three launches.  FE cells next to a material face or a domain face (a
+ * list, built with the material ids) take the masked stencils of
+ * k_tdgl<true> on global memory first (k_tdgl_face_cells).  Then
+ * k_tdgl_rhs_stream: a workgroup owns a TRS_TX x TRS_TY column tile and a
+ * chunk of TRS_Z planes of [k0, k1] (the planes holding FE cells; elsewhere P
+ * never changes and the RHS stays 0) and marches upward.  Old P is staged
+ * plane by plane in LDS with a 2-cell lateral halo (three planes, the next
+ * one loaded into registers while the current one is computed); P_new is
+ * formed on the tile and a 1-cell ring (one thread per cell of tile and
+ * ring; the ring is also formed by the neighbouring workgroups) for the
+ * interior FE cells, taken from the first launch's output for the face
+ * cells and from P elsewhere, and kept for three planes; the RHS of plane
+ * p-1 (interior FE cells, 0 in the others) is formed once P_new of planes
+ * p-2 .. p is in LDS.  Last, the face cells' RHS with the masked stencils
+ * on P_new (k_rhs_face_cells).  The arithmetic is that of k_tdgl<true> and
+ * k_rhs_newton. */
+constexpr int TRS_TX = 30, TRS_TY = 14, TRS_Z = 16;
+constexpr int TRS_HX = TRS_TX + 4, TRS_HY = TRS_TY + 4;     /* old P: tile + 2-cell halo */
+constexpr int TRS_RX = TRS_TX + 2, TRS_RY = TRS_TY + 2;     /* P_new: tile + 1-cell ring */
+constexpr int TRS_THR = TRS_RX * TRS_RY;                     /* one thread per cell of tile and ring: 512 */
+constexpr int TRS_PV = (TRS_HX * TRS_HY + TRS_THR - 1) / TRS_THR;   /* old-P values per thread and plane */
+
+__device__ __forceinline__ int slot3(int q) { return ((q % 3) + 3) % 3; }
+
+/* old P around window position o of plane slots (k-1, k, k+1) */
+struct LdsNbr {
+    const double* w0;
+    const double* w1;
+    const double* w2;
+    int o;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        const double* w = dk < 0 ? w0 : dk > 0 ? w2 : w1;
+        return w[o + di + dj * TRS_HX];
+    }
+};
+
+__global__ __launch_bounds__(256) void k_tdgl_face_cells(TdglRhsArgs A) {
+    const TdglArgs& a = A.t;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= A.nface) return;
+    const int nx = a.g.n[0], ny = a.g.n[1];
+    const int c = A.face[q];
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    double g;
+    a.Pout[2][c] = tdgl_scalar_cell(a, i, j, k, c, a.M.raw(i, j, k), GlobalNbr{a.P[2], c, NbrOff(i, j, nx, ny)}, g);
+}
+
+__global__ __launch_bounds__(256) void k_rhs_face_cells(TdglRhsArgs A) {
+    const TdglArgs& a = A.t;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= A.nface) return;
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int c = A.face[q];
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    double R[9];
+    rot_of(a.rot, c, R);
+    const PView Fr{a.Pout[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+    const double* dx = a.g.dx;
+    double x = 0.0;
+    if (R[6] != 0.0) x += R[6] * dpd<0>(Fr, a.M, i, j, k, dx, a.pbc);
+    if (R[7] != 0.0) x += R[7] * dpd<1>(Fr, a.M, i, j, k, dx, a.pbc);
+    x += R[8] * dpd<2>(Fr, a.M, i, j, k, dx, a.pbc);
+    A.rhs[c] = -x;
+}
+
+__global__ __launch_bounds__(TRS_THR, 2) void k_tdgl_rhs_stream(TdglRhsArgs A) {
+    __shared__ double Pw[3 * TRS_HX * TRS_HY];
+    __shared__ double Nw[3 * TRS_RX * TRS_RY];
+    const TdglArgs& a = A.t;
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int per0 = a.g.per[0], per1 = a.g.per[1];
+    const int i0 = blockIdx.x * TRS_TX, j0 = blockIdx.y * TRS_TY;
+    const int ks = A.k0 + blockIdx.z * TRS_Z;
+    const int ke = min(ks + TRS_Z - 1, A.k1);
+    const double* __restrict__ P = a.P[2];
+    const double* __restrict__ Pn = a.Pout[2];
+    /* global index of a window column/row, -1 across a non-periodic face */
+    auto wrap = [](int v, int n, int per) -> int {
+        if (v < 0) return per ? v + n : -1;
+        if (v >= n) return per ? v - n : -1;
+        return v;
+    };
+    /* this thread's share of a halo plane of old P */
+    long long src[TRS_PV];
+#pragma unroll
+    for (int r = 0; r < TRS_PV; ++r) {
+        const int t = threadIdx.x + r * TRS_THR;
+        src[r] = -1;
+        if (t < TRS_HX * TRS_HY) {
+            const int v = t / TRS_HX, u = t - v * TRS_HX;
+            const int i = wrap(i0 + u - 2, nx, per0), j = wrap(j0 + v - 2, ny, per1);
+            if (i >= 0 && j >= 0) src[r] = i + (long long)nx * j;
+        }
+    }
+    const long long plane = (long long)nx * ny;
+    auto fetch = [&](int q, double (&vals)[TRS_PV]) {
+#pragma unroll
+        for (int r = 0; r < TRS_PV; ++r)
+            vals[r] = (q >= 0 && q < nz && src[r] >= 0) ? P[src[r] + plane * q] : 0.0;
+    };
+    auto stash = [&](int q, const double (&vals)[TRS_PV]) {
+        double* dst = Pw + slot3(q) * (TRS_HX * TRS_HY);
+#pragma unroll
+        for (int r = 0; r < TRS_PV; ++r) {
+            const int t = threadIdx.x + r * TRS_THR;
+            if (t < TRS_HX * TRS_HY) dst[t] = vals[r];
+        }
+    };
+    const int tu = threadIdx.x % TRS_RX - 1, tv = threadIdx.x / TRS_RX - 1;
+    const int ci = wrap(i0 + tu, nx, per0), cj = wrap(j0 + tv, ny, per1);
+    const bool in_tile = tu >= 0 && tu < TRS_TX && tv >= 0 && tv < TRS_TY && i0 + tu < nx && j0 + tv < ny;
+    const int wo = (tv + 2) * TRS_HX + (tu + 2);
+    {
+        double v0[TRS_PV], v1[TRS_PV], v2[TRS_PV];
+        fetch(ks - 2, v0); fetch(ks - 1, v1); fetch(ks, v2);
+        stash(ks - 2, v0); stash(ks - 1, v1); stash(ks, v2);
+    }
+    __syncthreads();
+    for (int p = ks - 1; p <= ke + 1; ++p) {
+        double nxt[TRS_PV];
+        fetch(p + 2, nxt);                             /* in flight while plane p is formed */
+        /* P_new of plane p at this thread's tile or ring position */
+        double pn = 0.0;
+        if (p >= 0 && p < nz && ci >= 0 && cj >= 0) {
+            const int c = ci + nx * (cj + ny * p);
+            const int raw = a.M.raw(ci, cj, p);
+            const double* w1 = Pw + slot3(p) * (TRS_HX * TRS_HY);
+            if ((raw & 3) != 0) {
+                pn = w1[wo];                           /* Gamma = 0: P does not change */
+            } else if (!(raw & MAT_INTERIOR_FE)) {
+                pn = Pn[c];                            /* formed by k_tdgl_face_cells */
+            } else {
+                const LdsNbr F{Pw + slot3(p - 1) * (TRS_HX * TRS_HY), w1, Pw + slot3(p + 1) * (TRS_HX * TRS_HY), wo};
+                double g;
+                pn = tdgl_scalar_cell(a, ci, cj, p, c, raw, F, g);
+                if (in_tile && p >= ks && p <= ke) a.Pout[2][c] = pn;
+            }
+        }
+        Nw[slot3(p) * (TRS_RX * TRS_RY) + threadIdx.x] = pn;
+        __syncthreads();
+        stash(p + 2, nxt);                             /* over plane p-1, no longer read */
+        /* Poisson RHS of plane p - 1 from P_new of planes p-2 .. p */
+        const int q = p - 1;
+        if (in_tile && q >= ks && q <= ke) {
+            const int c = ci + nx * (cj + ny * q);
+            const int raw = a.M.raw(ci, cj, q);
+            if ((raw & 3) != 0) {
+                A.rhs[c] = 0.0;
+            } else if (raw & MAT_INTERIOR_FE) {
+                double R[9];
+                rot_of(a.rot, c, R);
+                const double* dx = a.g.dx;
+                const int o = (tv + 1) * TRS_RX + (tu + 1);
+                const double* n0 = Nw + slot3(q - 1) * (TRS_RX * TRS_RY);
+                const double* n1 = Nw + slot3(q) * (TRS_RX * TRS_RY);
+                const double* n2 = Nw + slot3(q + 1) * (TRS_RX * TRS_RY);
+                const double gx = (n1[o + 1] - n1[o - 1]) * (0.5 / dx[0]);
+                const double gy = (n1[o + TRS_RX] - n1[o - TRS_RX]) * (0.5 / dx[1]);
+                const double gz = (n2[o] - n0[o]) * (0.5 / dx[2]);
+                double x = 0.0;
+                if (R[6] != 0.0) x += R[6] * gx;
+                if (R[7] != 0.0) x += R[7] * gy;
+                x += R[8] * gz;
+                A.rhs[c] = -x;
+            }
+        }
+        __syncthreads();
+    }
+}
+
 /* Heun combination, main.cpp:336-341: P <- P + dt*(0.5*f0 + 0.5*f1) in FE cells. */
 __global__ void k_heun(HeunArgs a) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < a.N; c += (long long)gridDim.x * blockDim.x) {
@@ -469,7 +697,9 @@ void phidiff(hipStream_t s, const double* phi, double* phi_old, double* diff, lo
     hipLaunchKernelGGL(k_phidiff, dim3(b), dim3(256), 0, s, phi, phi_old, diff, n, phi_max);
 }
 void rhs_newton(hipStream_t s, const RhsArgs& a) {
-    hipLaunchKernelGGL(k_rhs_newton, grid_cells(a.g.n), dim3(256), 0, s, a);
+    dim3 g = grid_cells(a.g.n);
+    g.z = a.kz1 - a.kz0 + 1;
+    if (a.kz1 >= a.kz0) hipLaunchKernelGGL(k_rhs_newton, g, dim3(256), 0, s, a);
 }
 int rho_update(hipStream_t s, const RhoArgs& a) {
     const dim3 g = grid_cells(a.n);
@@ -477,8 +707,19 @@ int rho_update(hipStream_t s, const RhoArgs& a) {
     return (int)(g.x * g.y * g.z);
 }
 void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast) {
-    if (scalar_fast) hipLaunchKernelGGL(k_tdgl<true>, grid_cells(a.g.n), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(k_tdgl<false>, grid_cells(a.g.n), dim3(256), 0, s, a);
+    dim3 g = grid_cells(a.g.n);
+    g.z = a.kz1 - a.kz0 + 1;
+    if (a.kz1 < a.kz0) return;
+    if (scalar_fast) hipLaunchKernelGGL(k_tdgl<true>, g, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_tdgl<false>, g, dim3(256), 0, s, a);
+}
+void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a) {
+    const unsigned nb = (unsigned)((a.nface + 255) / 256);
+    if (nb) hipLaunchKernelGGL(k_tdgl_face_cells, dim3(nb), dim3(256), 0, s, a);
+    const dim3 g((a.t.g.n[0] + TRS_TX - 1) / TRS_TX, (a.t.g.n[1] + TRS_TY - 1) / TRS_TY,
+                 (a.k1 - a.k0 + TRS_Z) / TRS_Z);
+    hipLaunchKernelGGL(k_tdgl_rhs_stream, g, dim3(TRS_THR), 0, s, a);
+    if (nb) hipLaunchKernelGGL(k_rhs_face_cells, dim3(nb), dim3(256), 0, s, a);
 }
 void heun(hipStream_t s, const HeunArgs& a) {
     unsigned b = nblk(a.N, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;

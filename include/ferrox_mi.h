@@ -146,6 +146,10 @@ typedef struct fx_inputs {
                                      tridiagonal solve per mode) instead of by multigrid; 0: multigrid always */
     int    poisson_direct_check;  /* 1: after each direct solve, form the FP64 residual and apply the multigrid
                                      stopping rule to it (FX_ENOCONV_MG if it fails); 0 (default): no check */
+    int    tdgl_fused_rhs;        /* 1: with scalar P, forward Euler and no semiconductor cells, the TDGL update
+                                     also forms the next step's Poisson RHS (one pass over P per step, z-streamed
+                                     through LDS); 0 (default, measured faster on MI355X): separate TDGL and RHS
+                                     passes */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;
