@@ -120,6 +120,7 @@ SIGNATURES = {
     "fx_update": (C.c_int, [_P, C.c_int]),
     "fx_step": (C.c_int, [_P, C.c_int, C.POINTER(FxStepStats)]),
     "fx_poisson_solve": (C.c_int, [_P, _DP, _DP, _DP, C.POINTER(FxSolveStats)]),
+    "fx_poisson_solve_noalloc": (C.c_int, [_P, C.c_void_p, C.c_void_p]),
     "fx_field_device_ptr": (C.c_int, [_P, C.c_int, C.POINTER(_P)]),
     "fx_sync": (C.c_int, [_P]),
     "fx_stream": (_P, [_P]),

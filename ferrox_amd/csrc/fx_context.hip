@@ -22,6 +22,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <mutex>
 #include <vector>
 #include <unordered_map>
 
@@ -94,6 +95,7 @@ struct fx_ctx {
     int fe_k0 = -1, fe_k1 = -1;    /* first and last plane holding an FE cell (-1: none) */
     int* face_list = nullptr;      /* FE cells without centred stencils (global indices) */
     int nface = 0;
+    double* noalloc_scratch = nullptr;   /* device address of the static workspace on this context's GPU */
     /* the fused update writes the next RHS into rhs_nx, which the next solve
      * swaps in (rhs keeps the RHS of the last solve, the PoissonRHS the
      * reference writes at plot time); the fused kernel writes only planes
@@ -1464,6 +1466,38 @@ double* field_dev(fx_ctx* x, int f) {
 }  // namespace
 
 /* ---------------------------------- C ABI --------------------------------- */
+/* ---- workspace of fx_poisson_solve_noalloc -------------------------------
+ * The direct solve's spectrum and tridiagonal checkpoints live in this
+ * file-scope device array instead of library-allocated memory, for callers
+ * that cannot allocate.  In host code the name is only a stand-in: its device
+ * address on each GPU comes from hipGetSymbolAddress, taken once per device
+ * at context set-up (fx_create) with that device current, and only that
+ * address is handed to kernels. */
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+/* device address of `scratch` on the current device, looked up once */
+int scratch_address(fx_ctx* x) {
+    int dev = 0;
+    HIPCHK(x, hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(x, FX_EINVAL, "device index beyond the workspace table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(x, FX_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    x->noalloc_scratch = g_scratch_ptr[dev];
+    return FX_OK;
+}
+}  // namespace
+
 extern "C" {
 
 int fx_create(const fx_inputs* in_, fx_ctx** out) {
@@ -1492,6 +1526,12 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
             x->waveform[t - 1] = in.bias_ramp_start + (in.bias_ramp_end - in.bias_ramp_start) * ((double)t / in.bias_ramp_steps);
     }
     if (hipStreamCreate(&x->s) != hipSuccess) { delete x; return FX_EHIP; }
+    if ((rc = scratch_address(x))) {
+        std::fprintf(stderr, "fx_create: %s\n", x->err.c_str());
+        (void)hipStreamDestroy(x->s);
+        delete x;
+        return rc;
+    }
     const size_t B = x->N * sizeof(double);
 #define ALLOC(p, b) do { if ((rc = dalloc(x, (void**)&(p), (b)))) { free_all(x); std::fprintf(stderr, "fx_create: %s\n", x->err.c_str()); delete x; return rc; } } while (0)
     for (int d = 0; d < 3; ++d) { ALLOC(x->P[d], B); ALLOC(x->Pn[d], B); ALLOC(x->E[d], B); ALLOC(x->GL[d], B); ALLOC(x->GL1[d], B); }
@@ -1885,6 +1925,29 @@ int fx_poisson_solve(fx_ctx* x, const double* rhs, const double* acoef, double* 
     HIPCHK(x, hipMemcpyAsync(phi, x->phi, B, hipMemcpyDeviceToHost, x->s));
     HIPCHK(x, hipStreamSynchronize(x->s));
     return rc;
+}
+
+int fx_poisson_solve_noalloc(fx_ctx* x, const double* rhs_dev, double* phi_dev) {
+    if (!x || !rhs_dev || !phi_dev) return FX_EINVAL;
+    if (!x->direct) return fail(x, FX_EINVAL, "fx_poisson_solve_noalloc: the context's operator has no direct solve "
+                                             "(needs x, y periodic, z-layered permittivity, a Dirichlet z face, "
+                                             "lateral extents powers of two)");
+    const FftPlan<double>& P0 = x->fft64;
+    const size_t need = (x->N + (size_t)P0.n0 * P0.n1 * ((P0.n2 + 15) / 16)) * sizeof(double);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(x, FX_EINVAL, "fx_poisson_solve_noalloc: " + std::to_string(x->N) + " cells need " + std::to_string(need) +
+                                      " bytes of workspace, more than the " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                      " bytes of the static array");
+    if (!x->noalloc_scratch) return fail(x, FX_EHIP, "fx_poisson_solve_noalloc: workspace address unknown");
+    FftPlan<double> P = P0;
+    P.S = x->noalloc_scratch;
+    P.ck = x->noalloc_scratch + x->N;
+    double bv[6];
+    bc_values(x, bv);
+    fxk::fft_solve(x->s, P, rhs_dev, phi_dev, x->fft_w2lo * bv[2], x->fft_w2hi * bv[5]);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(x, FX_EHIP, hipGetErrorString(e));
+    return FX_OK;
 }
 
 int fx_field_device_ptr(fx_ctx* x, int f, void** dptr) {

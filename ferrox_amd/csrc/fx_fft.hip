@@ -498,9 +498,12 @@ template <class T, int NZ>
 void tri_launch(hipStream_t s, const FftPlan<T>& P, T scale) {
     constexpr int LB = NZ > TRI_REG ? NZ - TRI_REG : 0;
     const size_t lds = ((size_t)LB * TRI_THR + 5 * (size_t)P.n2) * sizeof(T);
-    if (lds > 64 * 1024)
+    static bool attr = false;   /* the LDS limit is raised once per kernel */
+    if (lds > 64 * 1024 && !attr) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_tri<T, NZ>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)lds);
+                                  160 * 1024);
+        attr = true;
+    }
     hipLaunchKernelGGL((k_fft_tri<T, NZ>), dim3((P.n0 * P.n1 + TRI_THR - 1) / TRI_THR), dim3(TRI_THR), lds, s, P, scale);
 }
 

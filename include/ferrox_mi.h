@@ -284,6 +284,20 @@ int fx_step(fx_ctx* ctx, int nsteps, fx_step_stats* stats);
 int fx_poisson_solve(fx_ctx* ctx, const double* rhs, const double* acoef,
                      double* phi, fx_solve_stats* stats);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): phi_dev = A^-1 rhs_dev with the context's operator and
+ * current bias, on device arrays of N doubles (dense, i fastest), queued on
+ * the context's stream (fx_stream) with no host wait and no allocation.
+ * Only where fx_poisson_solve takes the direct path (x and y periodic,
+ * permittivity depending on z only, a Dirichlet z face, lateral extents
+ * powers of two, no Helmholtz term).  The transform workspace,
+ * N + n0 n1 ceil(n2/16) doubles, lives in a static device array of 64 MiB
+ * per GPU (about 7.9M cells at most), shared by every call on that GPU and
+ * left holding the last call's values: calls that overlap on one GPU race.
+ * FX_EINVAL off the direct path or beyond the limit. */
+int fx_poisson_solve_noalloc(fx_ctx* ctx, const double* rhs_dev, double* phi_dev);
+
 /* AMReX single-level plotfile of the current state into directory dir
  * (WritePlotfile, Plotfile.cpp:5-177; variables gated by the plot_* inputs
  * flags).  fx_step writes plt%08d itself when plot_int > 0, at step 0 and
