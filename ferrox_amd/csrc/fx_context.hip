@@ -66,6 +66,10 @@ struct fx_ctx {
     size_t N;
     double dx[3];
     hipStream_t s = nullptr;
+    /* second stream for the face-cell kernels beside the streamed passes,
+     * and the two events that order it (default priority, like s) */
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
     /* device fields, dense FP64 over the valid box */
     double* P[3] = {nullptr, nullptr, nullptr};
     double* Pn[3] = {nullptr, nullptr, nullptr};
@@ -113,6 +117,7 @@ struct fx_ctx {
     int* rot_idx = nullptr;
     double* rot_tab = nullptr;
     double* rot_atab = nullptr;
+    int nrot = 0;                  /* rows of rot_tab / rot_atab */
     double Ru[9];
     /* reductions */
     double* red = nullptr;
@@ -826,6 +831,7 @@ int refresh_rotation(fx_ctx* x) {
     if (x->rot_idx) { (void)hipFree(x->rot_idx); x->rot_idx = nullptr; }
     if (x->rot_tab) { (void)hipFree(x->rot_tab); x->rot_tab = nullptr; }
     if (x->rot_atab) { (void)hipFree(x->rot_atab); x->rot_atab = nullptr; }
+    x->nrot = 0;
     if (uniform && x->h_lalpha[0] == x->in.alpha) return FX_OK;
     /* rows in order of first appearance, keyed on the bit patterns */
     struct Key {
@@ -863,6 +869,7 @@ int refresh_rotation(fx_ctx* x) {
     HIPCHK(x, hipMemcpyAsync(x->rot_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
     HIPCHK(x, hipMemcpyAsync(x->rot_atab, atab.data(), atab.size() * sizeof(double), hipMemcpyHostToDevice, x->s));
     HIPCHK(x, hipStreamSynchronize(x->s));
+    x->nrot = (int)atab.size();
     return FX_OK;
 }
 
@@ -884,7 +891,7 @@ PBC pbc_of(const fx_ctx* x) {
 }
 RotArgs rot_of(const fx_ctx* x) {
     RotArgs r;
-    r.idx = x->rot_idx; r.tab = x->rot_tab; r.atab = x->rot_atab;
+    r.idx = x->rot_idx; r.tab = x->rot_tab; r.atab = x->rot_atab; r.nrows = x->nrot;
     std::memcpy(r.Ru, x->Ru, sizeof(r.Ru));
     return r;
 }
@@ -1279,7 +1286,10 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
                 ra0.kz0 = x->fe_k0 >= 0 ? x->fe_k0 : 1;
                 ra0.kz1 = x->fe_k0 >= 0 ? x->fe_k1 : 0;
             }
-            fxk::rhs_newton(x->s, ra0);
+            if (narrow && x->pxy_zero && x->in.tdgl_stream && x->fe_k0 >= 0)
+                fxk::rhs_stream(x->s, x->s2, x->ev0, x->ev1, ra0, x->fe_k0, x->fe_k1, x->face_list, x->nface);
+            else
+                fxk::rhs_newton(x->s, ra0);
             x->rhs_clean = !x->has_sc;
         }
         x->rhs_next = false;
@@ -1361,6 +1371,15 @@ int update(fx_ctx* x, int order, fx_solve_stats* st2) {
             f.nface = x->nface;
             fxk::tdgl_rhs_stream(x->s, f);
             x->rhs_next = true;
+        } else if (fast && x->in.tdgl_stream && x->fe_k0 >= 0) {
+            TdglRhsArgs f;
+            f.t = a;
+            f.rhs = nullptr;
+            f.k0 = x->fe_k0;
+            f.k1 = x->fe_k1;
+            f.face = x->face_list;
+            f.nface = x->nface;
+            fxk::tdgl_stream(x->s, x->s2, x->ev0, x->ev1, f);
         } else {
             fxk::tdgl(x->s, a, fast);
         }
@@ -1437,7 +1456,10 @@ void free_all(fx_ctx* x) {
     F(x->lu64); F(x->lu32); F(x->perm64); F(x->perm32);
     if (x->h_red) hipHostFree(x->h_red);
     for (size_t l = 0; l < x->lev.size(); ++l) { F((void*)x->lev[l].d.bxk); F((void*)x->lev[l].d.byk); F((void*)x->lev[l].d.bzk); F(x->lev[l].d.phi2); }
-    if (x->s) hipStreamDestroy(x->s);
+    if (x->ev0) (void)hipEventDestroy(x->ev0);
+    if (x->ev1) (void)hipEventDestroy(x->ev1);
+    if (x->s2) (void)hipStreamDestroy(x->s2);
+    if (x->s) (void)hipStreamDestroy(x->s);
 }
 
 double* field_dev(fx_ctx* x, int f) {
@@ -1526,9 +1548,15 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
             x->waveform[t - 1] = in.bias_ramp_start + (in.bias_ramp_end - in.bias_ramp_start) * ((double)t / in.bias_ramp_steps);
     }
     if (hipStreamCreate(&x->s) != hipSuccess) { delete x; return FX_EHIP; }
+    if (hipStreamCreate(&x->s2) != hipSuccess || hipEventCreateWithFlags(&x->ev0, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&x->ev1, hipEventDisableTiming) != hipSuccess) {
+        free_all(x);
+        delete x;
+        return FX_EHIP;
+    }
     if ((rc = scratch_address(x))) {
         std::fprintf(stderr, "fx_create: %s\n", x->err.c_str());
-        (void)hipStreamDestroy(x->s);
+        free_all(x);
         delete x;
         return rc;
     }

@@ -18,6 +18,7 @@ struct RotArgs {
     const double* tab;     /* 9 doubles of R per row */
     const double* atab;    /* alpha per row */
     double Ru[9];
+    int nrows;             /* rows of tab and atab */
 };
 
 struct RhsArgs {
@@ -115,6 +116,15 @@ void rhs_newton(hipStream_t s, const RhsArgs& a);
 int rho_update(hipStream_t s, const RhoArgs& a);
 void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast);
 void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a);
+/* k_tdgl<true> with the stencil operands streamed through LDS, the face
+ * cells by list on the second stream s2 (events e0, e1 order it after the
+ * work queued on s and s after it); a.rhs is not used */
+void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const TdglRhsArgs& a);
+/* the Poisson RHS of scalar P in planes k0..k1 without SC cells (the RHS
+ * buffer 0 in every non-FE cell), streamed through LDS, face cells by list
+ * on s2 */
+void rhs_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const RhsArgs& a, int k0, int k1,
+                const int* face, int nface);
 void heun(hipStream_t s, const HeunArgs& a);
 void compute_E(hipStream_t s, const EArgs& a);
 void mg_smooth(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int color);

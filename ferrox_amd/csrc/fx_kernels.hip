@@ -636,6 +636,299 @@ __global__ __launch_bounds__(TRS_THR, 2) void k_tdgl_rhs_stream(TdglRhsArgs A) {
     }
 }
 
+/* ----------------- TDGL update streamed through LDS (scalar P) ------------ */
+/* k_tdgl<true> for the interior FE cells, with the stencil operands from LDS
+ * instead of per-cell gathers: a workgroup owns a TDS_TX x TDS_TY column tile
+ * and a chunk of TDS_Z planes of [k0, k1] and marches upward, P and phi of
+ * three planes (tile + 1-cell halo) in LDS, the plane after next loaded into
+ * registers while the current one is updated.  FE cells next to a face
+ * (the list of k_tdgl_face_cells) are updated by that kernel; other cells
+ * keep their P (Gamma = 0). */
+constexpr int TDS_TX = 64, TDS_TY = 8, TDS_Z = 16;
+constexpr int TDS_HX = TDS_TX + 2, TDS_HY = TDS_TY + 2;
+constexpr int TDS_THR = TDS_TX * TDS_TY;                                  /* 512 */
+constexpr int TDS_PV = (TDS_HX * TDS_HY + TDS_THR - 1) / TDS_THR;       /* halo-plane values per thread */
+constexpr int TDS_MAXROWS = 512;   /* rotation / alpha tables up to this many rows are staged in LDS */
+
+/* old P (or phi) around window position o of plane slots (k-1, k, k+1) */
+struct WinNbr {
+    const double* w0;
+    const double* w1;
+    const double* w2;
+    int o;
+    __device__ __forceinline__ double operator()(int di, int dj, int dk) const {
+        const double* w = dk < 0 ? w0 : dk > 0 ? w2 : w1;
+        return w[o + di + dj * TDS_HX];
+    }
+};
+
+template <bool TAB>
+__global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A) {
+    __shared__ double Pw[3 * TDS_HX * TDS_HY];
+    __shared__ double Fw[3 * TDS_HX * TDS_HY];
+    extern __shared__ __align__(16) unsigned char tds_dyn[];
+    double* const Rt = reinterpret_cast<double*>(tds_dyn);   /* TAB: 9 R values, then alpha, per row */
+    const TdglArgs& a = A.t;
+    if (TAB) {
+        const int nr = a.rot.nrows;
+        for (int t = threadIdx.x; t < 10 * nr; t += TDS_THR)
+            Rt[t] = t < 9 * nr ? a.rot.tab[t] : a.rot.atab[t - 9 * nr];
+    }
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int per0 = a.g.per[0], per1 = a.g.per[1];
+    const int i0 = blockIdx.x * TDS_TX, j0 = blockIdx.y * TDS_TY;
+    const int ks = A.k0 + blockIdx.z * TDS_Z;
+    const int ke = min(ks + TDS_Z - 1, A.k1);
+    const double* __restrict__ P = a.P[2];
+    const double* __restrict__ ph = a.phv.p;
+    auto wrap = [](int v, int n, int per) -> int {
+        if (v < 0) return per ? v + n : -1;
+        if (v >= n) return per ? v - n : -1;
+        return v;
+    };
+    long long src[TDS_PV];
+#pragma unroll
+    for (int r = 0; r < TDS_PV; ++r) {
+        const int t = threadIdx.x + r * TDS_THR;
+        src[r] = -1;
+        if (t < TDS_HX * TDS_HY) {
+            const int v = t / TDS_HX, u = t - v * TDS_HX;
+            const int i = wrap(i0 + u - 1, nx, per0), j = wrap(j0 + v - 1, ny, per1);
+            if (i >= 0 && j >= 0) src[r] = i + (long long)nx * j;
+        }
+    }
+    const long long plane = (long long)nx * ny;
+    /* halo planes outside the domain (or across a non-periodic side) read 0;
+     * interior FE cells never read them */
+    auto fetch = [&](int q, double (&vp)[TDS_PV], double (&vf)[TDS_PV]) {
+#pragma unroll
+        for (int r = 0; r < TDS_PV; ++r) {
+            const bool ok = q >= 0 && q < nz && src[r] >= 0;
+            vp[r] = ok ? P[src[r] + plane * q] : 0.0;
+            vf[r] = ok ? ph[src[r] + plane * q] : 0.0;
+        }
+    };
+    auto stash = [&](int q, const double (&vp)[TDS_PV], const double (&vf)[TDS_PV]) {
+        double* dp = Pw + slot3(q) * (TDS_HX * TDS_HY);
+        double* df = Fw + slot3(q) * (TDS_HX * TDS_HY);
+#pragma unroll
+        for (int r = 0; r < TDS_PV; ++r) {
+            const int t = threadIdx.x + r * TDS_THR;
+            if (t < TDS_HX * TDS_HY) { dp[t] = vp[r]; df[t] = vf[r]; }
+        }
+    };
+    const int tu = threadIdx.x % TDS_TX, tv = threadIdx.x / TDS_TX;
+    const int ci = i0 + tu, cj = j0 + tv;
+    const bool in_dom = ci < nx && cj < ny;
+    const int wo = (tv + 1) * TDS_HX + (tu + 1);
+    {
+        double vp[TDS_PV], vf[TDS_PV];
+        fetch(ks - 1, vp, vf); stash(ks - 1, vp, vf);
+        fetch(ks, vp, vf); stash(ks, vp, vf);
+        fetch(ks + 1, vp, vf); stash(ks + 1, vp, vf);
+    }
+    __syncthreads();
+    const double Gam = a.BigGamma;
+    const double* dx = a.g.dx;
+    const double ih[3] = {1.0 / dx[0], 1.0 / dx[1], 1.0 / dx[2]};
+    /* material id and table row of this thread's cell, one plane ahead */
+    const int* __restrict__ ridx = a.rot.idx;
+    int raw_c = 0, row_c = 0;
+    if (in_dom) {
+        raw_c = a.M.raw(ci, cj, ks);
+        if (ridx) row_c = ridx[ci + nx * (cj + ny * ks)];
+    }
+    for (int p = ks; p <= ke; ++p) {
+        double np_[TDS_PV], nf[TDS_PV];
+        fetch(p + 2, np_, nf);                         /* in flight while plane p is updated */
+        int raw_n = 0, row_n = 0;
+        if (in_dom && p < ke) {
+            raw_n = a.M.raw(ci, cj, p + 1);
+            if (ridx) row_n = ridx[ci + nx * (cj + ny * (p + 1))];
+        }
+        if (in_dom) {
+            const int c = ci + nx * (cj + ny * p);
+            const int raw = raw_c;
+            if ((raw & 3) == 0 && (raw & MAT_INTERIOR_FE)) {
+                const double* p0 = Pw + slot3(p - 1) * (TDS_HX * TDS_HY);
+                const double* p1 = Pw + slot3(p) * (TDS_HX * TDS_HY);
+                const double* p2 = Pw + slot3(p + 1) * (TDS_HX * TDS_HY);
+                const double* f0 = Fw + slot3(p - 1) * (TDS_HX * TDS_HY);
+                const double* f1 = Fw + slot3(p) * (TDS_HX * TDS_HY);
+                const double* f2 = Fw + slot3(p + 1) * (TDS_HX * TDS_HY);
+                const WinNbr F{p0, p1, p2, wo};
+                double R[9];
+                double alpha_c = a.alpha;
+                if (TAB) {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) R[q] = Rt[9 * row_c + q];
+                    alpha_c = Rt[9 * a.rot.nrows + row_c];
+                } else if (ridx) {
+                    const double* t = a.rot.tab + 9 * row_c;
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) R[q] = t[q];
+                    alpha_c = a.rot.atab[row_c];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) R[q] = a.rot.Ru[q];
+                }
+                const bool tph = (raw & 4) != 0;
+                double E2 = 0.0;
+                if (a.incE == 1) {
+                    const double fx = (f1[wo + 1] - f1[wo - 1]) * (0.5 * ih[0]);
+                    const double fy = (f1[wo + TDS_HX] - f1[wo - TDS_HX]) * (0.5 * ih[1]);
+                    const double fz = (f2[wo] - f0[wo]) * (0.5 * ih[2]);
+                    double sE = 0.0;
+                    if (R[6] != 0.0) sE += R[6] * fx;
+                    if (R[7] != 0.0) sE += R[7] * fy;
+                    sE += R[8] * fz;
+                    E2 = -sE;
+                }
+                const double r = p1[wo];
+                double L = 0.0, G = 0.0, El = 0.0;
+                if (a.incL == 1) {
+                    const double r2 = r * r;
+                    const double dF = alpha_c * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
+                    L = -1.0 * Gam * dF;
+                }
+                if (a.incG == 1) G = -1.0 * Gam * grad_r_interior_acc(F, ih, R, a.g11, a.g44 - a.g44_p);
+                if (a.incE == 1) El = -1.0 * Gam * (-E2);
+                if (tph) { L = 0.0; G = 0.0; El = 0.0; }
+                double g = 0.0;
+                if (a.incL == 1) g += L;
+                if (a.incG == 1) g += G;
+                if (a.incE == 1) g += El;
+                a.Pout[2][c] = r + a.dt * g;
+            }
+        }
+        raw_c = raw_n;
+        row_c = row_n;
+        __syncthreads();
+        stash(p + 2, np_, nf);                         /* over plane p-1, no longer read */
+        __syncthreads();
+    }
+}
+
+/* ------------------- Poisson RHS streamed through LDS (scalar P) ----------- */
+/* k_rhs_newton's centred branch for the interior FE cells of planes
+ * [k0, k1] (no SC cells, the RHS buffer already 0 in every non-FE cell, the
+ * Newton coefficient not written): P of three planes (tile + 1-cell halo)
+ * in LDS, the plane after next in registers, R_3 from an LDS copy of the
+ * rotation table; FE cells next to a face by list (k_rhs_face_list). */
+template <bool TAB>
+__global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, int k1) {
+    __shared__ double Pw[3 * TDS_HX * TDS_HY];
+    extern __shared__ __align__(16) unsigned char rs_dyn[];
+    double* const Rt = reinterpret_cast<double*>(rs_dyn);   /* TAB: R_31, R_32, R_33 per row */
+    if (TAB) {
+        const int nr = a.rot.nrows;
+        for (int t = threadIdx.x; t < 3 * nr; t += TDS_THR) Rt[t] = a.rot.tab[9 * (t / 3) + 6 + t % 3];
+    }
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int per0 = a.g.per[0], per1 = a.g.per[1];
+    const int i0 = blockIdx.x * TDS_TX, j0 = blockIdx.y * TDS_TY;
+    const int ks = k0 + blockIdx.z * TDS_Z;
+    const int ke = min(ks + TDS_Z - 1, k1);
+    const double* __restrict__ P = a.P[2];
+    auto wrap = [](int v, int n, int per) -> int {
+        if (v < 0) return per ? v + n : -1;
+        if (v >= n) return per ? v - n : -1;
+        return v;
+    };
+    long long src[TDS_PV];
+#pragma unroll
+    for (int r = 0; r < TDS_PV; ++r) {
+        const int t = threadIdx.x + r * TDS_THR;
+        src[r] = -1;
+        if (t < TDS_HX * TDS_HY) {
+            const int v = t / TDS_HX, u = t - v * TDS_HX;
+            const int i = wrap(i0 + u - 1, nx, per0), j = wrap(j0 + v - 1, ny, per1);
+            if (i >= 0 && j >= 0) src[r] = i + (long long)nx * j;
+        }
+    }
+    const long long plane = (long long)nx * ny;
+    auto fetch = [&](int q, double (&vp)[TDS_PV]) {
+#pragma unroll
+        for (int r = 0; r < TDS_PV; ++r) vp[r] = (q >= 0 && q < nz && src[r] >= 0) ? P[src[r] + plane * q] : 0.0;
+    };
+    auto stash = [&](int q, const double (&vp)[TDS_PV]) {
+        double* dp = Pw + slot3(q) * (TDS_HX * TDS_HY);
+#pragma unroll
+        for (int r = 0; r < TDS_PV; ++r) {
+            const int t = threadIdx.x + r * TDS_THR;
+            if (t < TDS_HX * TDS_HY) dp[t] = vp[r];
+        }
+    };
+    const int tu = threadIdx.x % TDS_TX, tv = threadIdx.x / TDS_TX;
+    const int ci = i0 + tu, cj = j0 + tv;
+    const bool in_dom = ci < nx && cj < ny;
+    const int wo = (tv + 1) * TDS_HX + (tu + 1);
+    {
+        double vp[TDS_PV];
+        fetch(ks - 1, vp); stash(ks - 1, vp);
+        fetch(ks, vp); stash(ks, vp);
+        fetch(ks + 1, vp); stash(ks + 1, vp);
+    }
+    __syncthreads();
+    const double* dx = a.g.dx;
+    const int* __restrict__ ridx = a.rot.idx;
+    int raw_c = 0, row_c = 0;
+    if (in_dom) {
+        raw_c = a.M.raw(ci, cj, ks);
+        if (ridx) row_c = ridx[ci + nx * (cj + ny * ks)];
+    }
+    for (int p = ks; p <= ke; ++p) {
+        double np_[TDS_PV];
+        fetch(p + 2, np_);
+        int raw_n = 0, row_n = 0;
+        if (in_dom && p < ke) {
+            raw_n = a.M.raw(ci, cj, p + 1);
+            if (ridx) row_n = ridx[ci + nx * (cj + ny * (p + 1))];
+        }
+        if (in_dom && (raw_c & 3) == 0 && (raw_c & MAT_INTERIOR_FE)) {
+            const int c = ci + nx * (cj + ny * p);
+            double R6, R7, R8;
+            if (TAB) { R6 = Rt[3 * row_c]; R7 = Rt[3 * row_c + 1]; R8 = Rt[3 * row_c + 2]; }
+            else if (ridx) { const double* t = a.rot.tab + 9 * row_c; R6 = t[6]; R7 = t[7]; R8 = t[8]; }
+            else { R6 = a.rot.Ru[6]; R7 = a.rot.Ru[7]; R8 = a.rot.Ru[8]; }
+            const double* p0 = Pw + slot3(p - 1) * (TDS_HX * TDS_HY);
+            const double* p1 = Pw + slot3(p) * (TDS_HX * TDS_HY);
+            const double* p2 = Pw + slot3(p + 1) * (TDS_HX * TDS_HY);
+            const double gx = (p1[wo + 1] - p1[wo - 1]) * (0.5 / dx[0]);
+            const double gy = (p1[wo + TDS_HX] - p1[wo - TDS_HX]) * (0.5 / dx[1]);
+            const double gz = (p2[wo] - p0[wo]) * (0.5 / dx[2]);
+            double x = 0.0;
+            if (R6 != 0.0) x += R6 * gx;
+            if (R7 != 0.0) x += R7 * gy;
+            x += R8 * gz;
+            a.rhs[c] = -x;
+        }
+        raw_c = raw_n;
+        row_c = row_n;
+        __syncthreads();
+        stash(p + 2, np_);
+        __syncthreads();
+    }
+}
+
+/* the masked branch of k_rhs_newton (scalar P) for the FE cells of a list */
+__global__ __launch_bounds__(256) void k_rhs_face_list(RhsArgs a, const int* __restrict__ face, int nface) {
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= nface) return;
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int c = face[q];
+    const int i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+    double R[9];
+    rot_row3(a.rot, c, R);
+    const PView Fr{a.P[2], nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]};
+    const double* dx = a.g.dx;
+    double x = 0.0;
+    if (R[6] != 0.0) x += R[6] * dpd<0>(Fr, a.M, i, j, k, dx, a.pbc);
+    if (R[7] != 0.0) x += R[7] * dpd<1>(Fr, a.M, i, j, k, dx, a.pbc);
+    x += R[8] * dpd<2>(Fr, a.M, i, j, k, dx, a.pbc);
+    a.rhs[c] = -x;
+}
+
 /* Heun combination, main.cpp:336-341: P <- P + dt*(0.5*f0 + 0.5*f1) in FE cells. */
 __global__ void k_heun(HeunArgs a) {
     for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < a.N; c += (long long)gridDim.x * blockDim.x) {
@@ -720,6 +1013,39 @@ void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a) {
                  (a.k1 - a.k0 + TRS_Z) / TRS_Z);
     hipLaunchKernelGGL(k_tdgl_rhs_stream, g, dim3(TRS_THR), 0, s, a);
     if (nb) hipLaunchKernelGGL(k_rhs_face_cells, dim3(nb), dim3(256), 0, s, a);
+}
+void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const TdglRhsArgs& a) {
+    const unsigned nb = (unsigned)((a.nface + 255) / 256);
+    /* the face cells on the second stream, beside the streamed interior */
+    if (nb) {
+        (void)hipEventRecord(e0, s);
+        (void)hipStreamWaitEvent(s2, e0, 0);
+        hipLaunchKernelGGL(k_tdgl_face_cells, dim3(nb), dim3(256), 0, s2, a);
+        (void)hipEventRecord(e1, s2);
+    }
+    const dim3 g((a.t.g.n[0] + TDS_TX - 1) / TDS_TX, (a.t.g.n[1] + TDS_TY - 1) / TDS_TY,
+                 (a.k1 - a.k0 + TDS_Z) / TDS_Z);
+    if (a.t.rot.idx && a.t.rot.nrows <= TDS_MAXROWS)
+        hipLaunchKernelGGL(k_tdgl_stream<true>, g, dim3(TDS_THR), (size_t)10 * a.t.rot.nrows * sizeof(double), s, a);
+    else
+        hipLaunchKernelGGL(k_tdgl_stream<false>, g, dim3(TDS_THR), 0, s, a);
+    if (nb) (void)hipStreamWaitEvent(s, e1, 0);
+}
+void rhs_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const RhsArgs& a, int k0, int k1,
+                const int* face, int nface) {
+    const unsigned nb = (unsigned)((nface + 255) / 256);
+    if (nb) {
+        (void)hipEventRecord(e0, s);
+        (void)hipStreamWaitEvent(s2, e0, 0);
+        hipLaunchKernelGGL(k_rhs_face_list, dim3(nb), dim3(256), 0, s2, a, face, nface);
+        (void)hipEventRecord(e1, s2);
+    }
+    const dim3 g((a.g.n[0] + TDS_TX - 1) / TDS_TX, (a.g.n[1] + TDS_TY - 1) / TDS_TY, (k1 - k0 + TDS_Z) / TDS_Z);
+    if (a.rot.idx && a.rot.nrows <= 3 * TDS_MAXROWS)
+        hipLaunchKernelGGL(k_rhs_stream<true>, g, dim3(TDS_THR), (size_t)3 * a.rot.nrows * sizeof(double), s, a, k0, k1);
+    else
+        hipLaunchKernelGGL(k_rhs_stream<false>, g, dim3(TDS_THR), 0, s, a, k0, k1);
+    if (nb) (void)hipStreamWaitEvent(s, e1, 0);
 }
 void heun(hipStream_t s, const HeunArgs& a) {
     unsigned b = nblk(a.N, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;

@@ -105,3 +105,18 @@ def test_fused_then_host_writes():
         print("host writes", name, e, flush=True)
         assert e < 1e-13, (name, e)
     a.close(); b.close()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_streamed_update_equals_gathered(case):
+    """k_tdgl_stream (operands through LDS) against k_tdgl<true> (gathered)."""
+    text, poly = CASES[case]
+    _, a, _ = make(text, poly, tdgl_stream=1)
+    _, b, _ = make(text, poly, tdgl_stream=0)
+    a.step(8)
+    b.step(8)
+    for name in ("Pz", "Phi", "PoissonRHS"):
+        e = rel(a.get(name), b.get(name))
+        print(case, name, "streamed vs gathered", e, flush=True)
+        assert e < 1e-13, (name, e)
+    a.close(); b.close()
