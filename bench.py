@@ -5,17 +5,20 @@ Workload: 256x256x128-cell polycrystalline MFIM (dx = dy = 0.5 nm, dz =
 0.1 nm; 4 nm Al2O3 under an 8 nm HZO-like layer and a 0.8 nm DE spacer; the
 layer tiled by 400 seeded Voronoi grains, each with a Haar-random rotation of
 its polar axis, 70 % ferroelectric and 30 % paraelectric (alpha = +1e9 m/F);
-0.5 V bias; random initial Pz), FP64 throughout, forward Euler, the
-reference's Poisson semantics (zero initial guess each step, MLMG stopping rule
-||r||_inf <= 1e-10 max(||b||, ||r0||)).
-A step is one pass of main.cpp:261-623: Newton RHS, multigrid solve, fused
-TDGL update, steady-state reduction.
+0.5 V bias; random initial Pz), FP64 throughout, forward Euler.  A step is
+one pass of main.cpp:261-623: Poisson RHS, Poisson solve (zero initial guess,
+answer within the MLMG stopping rule ||r||_inf <= 1e-10 max(||b||, ||r0||);
+this stack is laterally periodic with a z-layered permittivity, so the
+library solves it directly by lateral Hartley transforms and one tridiagonal
+solve per mode), TDGL update, steady-state bookkeeping.
 
 Prints one JSON line.  `value` is steps/s with every field resident in HBM;
-`roofline` is measured live on the dominant kernel with HIP events on the
-stream it runs on; `cpu_baseline` times the FP64 CPU oracle (the build's
-restatement of the reference algorithm, OpenMP) on a bounded sample of the
-same workload on this host.
+`roofline` is measured live on the dominant kernel (the TDGL update) with HIP
+events on the stream it runs on; `kernels` gives the same for every pass of
+the step; `multigrid` repeats the workload on the multigrid (FP64 and FP32
+V-cycles inside FP64 refinement); `cpu_baseline` times the FP64 CPU oracle
+(the build's restatement of the reference algorithm, OpenMP) on a bounded
+sample of the same workload on this host.
 """
 from __future__ import annotations
 
@@ -39,25 +42,27 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=6, help="steps of the CPU oracle sample (0 = skip)")
     ap.add_argument("--cpu-single-steps", type=int, default=1,
                     help="steps of the one-thread CPU oracle sample (0 = skip)")
-    ap.add_argument("--warm-start", action="store_true", help="previous phi as the multigrid initial guess")
     ap.add_argument("--no-kernel-timing", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32+FP64-refinement leg")
-    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--multigrid", action="store_true", help="the headline leg on the multigrid (poisson_direct = 0)")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 legs (direct and multigrid)")
+    ap.add_argument("--no-mg", action="store_true", help="skip the multigrid legs")
+    ap.add_argument("--mg-steps", type=int, default=0, help="steps of the multigrid legs (0: --steps, which the "
+                    "comparison with the direct run needs)")
     ap.add_argument("--no-configs", action="store_true", help="skip the 64x64x36 MFIM and 128x128x64 MFISM legs")
-    ap.add_argument("--mfism-steps", type=int, default=200, help="steps of the 128x128x64 MFISM leg (of its 2000-step ramp)")
+    ap.add_argument("--mfism-steps", type=int, default=2000, help="steps of the 128x128x64 MFISM leg (its 0 -> 1 V ramp)")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiation-form leg")
+    ap.add_argument("--no-diff-large", action="store_true", help="skip the 128x128x64 repeat of the differentiation form")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form leg")
     ap.add_argument("--batch-steps", type=int, default=1000)
-    ap.add_argument("--diff-large", action="store_true", help="also time the adjoint at 128x128x64")
-    ap.add_argument("--fused", type=int, default=1, help="streamed levels: one red+black sweep per pass (1), two sweeps per pass (2), or per-colour kernels everywhere (0)")
+    ap.add_argument("--fused", type=int, default=1, help="multigrid legs: one red+black sweep per pass (1), two sweeps per pass (2), or per-colour kernels everywhere (0)")
     return ap.parse_args()
 
 
-def measured_traffic():
-    """HBM bytes per launch of the finest-level sweep from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-kernel calibrated; the
-    counters cannot be read inside this process)."""
-    path = os.path.join(ROOT, "profiles", "traffic_mg_sweep_poly256.md")
+def measured_traffic(name="traffic_tdgl_poly256.md"):
+    """HBM bytes per launch of a kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (copy-kernel calibrated; the counters cannot
+    be read inside this process)."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -194,111 +199,135 @@ def batch_leg(nsteps: int) -> dict:
     return out
 
 
+def timed_run(text, grains, steps, warmup, **kw):
+    """steps/s of `steps` steps after `warmup` untimed ones, with the stats"""
+    import ferrox_amd as fx
+    from ferrox_amd import configs
+    g = fx.FerroX(fx.from_text(text, **kw))
+    configs.apply_fields(g, grains)
+    if warmup:
+        g.step(warmup)
+    g.sync()
+    t0 = time.perf_counter()
+    st = g.step(steps)
+    g.sync()
+    secs = time.perf_counter() - t0
+    return g, st, secs
+
+
+def mg_legs(text, grains, steps, warmup, fused, P64, F64, fp32: bool) -> dict:
+    """The same workload on the multigrid (poisson_direct = 0): FP64, and FP32
+    V-cycles inside FP64 refinement (the precision form), with the roofline
+    of the finest-level sweep."""
+    import numpy as np
+    out = {}
+    for prec, name in ((0, "fp64"), (1, "fp32_refine")):
+        if prec and not fp32:
+            continue
+        g, st, secs = timed_run(text, grains, steps, warmup, poisson_direct=0, precision=prec, mg_fused_sweep=fused)
+        n_v = st["total_vcycles"] / max(1, st["steps_done"])
+        leg = {"steps": steps, "steps_per_s": round(steps / secs, 3), "vcycles_per_step": round(n_v, 3)}
+        if prec == 0:
+            t_s = g.bench_kernel("smooth0", 50)
+            N = g.n[0] * g.n[1] * g.n[2]
+            ach = 24.0 * N / (t_s * 1e-3) / 1e9
+            leg["smooth0"] = {"kernel": "k_mg_gsrb_stream (finest level, red+black in one pass)", "avg_launch_ms": t_s,
+                              "achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                              "traffic": measured_traffic("traffic_mg_sweep_poly256.md")}
+        if P64 is not None:
+            leg["max_rel_err_Pz_vs_direct_fp64"] = float(np.abs(g.get("Pz") - P64).max() / np.abs(P64).max())
+            leg["max_rel_err_phi_vs_direct_fp64"] = float(np.abs(g.get("Phi") - F64).max() / np.abs(F64).max())
+        g.close()
+        out[name] = leg
+    if "fp32_refine" in out:
+        out["fp32_refine"]["speedup_vs_fp64"] = round(out["fp32_refine"]["steps_per_s"] / out["fp64"]["steps_per_s"], 3)
+    print(f"# multigrid: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {a.gpus} is not supported "
               "(multi-GPU slab decomposition is a design note, not built)", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     import ferrox_amd as fx
     from ferrox_amd import configs
 
     text = configs.mfim_poly_256(nsteps=a.steps + a.warmup)
-    inp = fx.from_text(text, mg_warm_start=int(a.warm_start), mg_fused_sweep=a.fused)
+    inp = fx.from_text(text)
     nx, ny, nz = inp.n_cell
     N = nx * ny * nz
     grains = configs.polycrystal_fields(inp, n_grains=400, fe_fraction=0.7, alpha_pe=1.0e9, seed=7)
-    g = fx.FerroX(inp)
-    configs.apply_fields(g, grains)
+    main_kw = {"poisson_direct": 0, "mg_fused_sweep": a.fused} if a.multigrid else {}
+    g, st, secs = timed_run(text, grains, a.steps, a.warmup, **main_kw)
     print(f"# levels {g.levels()}", file=sys.stderr, flush=True)
-    if a.warmup:
-        g.step(a.warmup)
-    g.sync()
-    t0 = time.perf_counter()
-    st = g.step(a.steps)
-    g.sync()
-    t1 = time.perf_counter()
-    secs = t1 - t0
     steps_per_s = a.steps / secs
     n_v = st["total_vcycles"] / max(1, st["steps_done"])
     n_outer = st["total_outer_iters"] / max(1, st["steps_done"])
-    print(f"# {a.steps} steps in {secs:.3f} s, {n_v:.2f} V-cycles/step, {n_outer:.2f} outer/step",
+    N_FE = int((g.get("mask") == 0).sum())
+    print(f"# {a.steps} steps in {secs:.4f} s, {n_v:.2f} V-cycles/step, {n_outer:.2f} outer/step, {N_FE} FE cells",
           file=sys.stderr, flush=True)
 
     # algorithmic bytes per step (FP64, scalar P, 1-byte material id, 4-byte
-    # grain-table index read by the RHS and TDGL kernels)
-    B_step = N * (25 + 17 + 8 + n_v * 154 + n_outer * 58)
+    # grain-table index): the RHS pass reads P, the material id and the grain
+    # index and writes the RHS of each FE cell (21 B); the TDGL update reads
+    # P, phi, the material id and the grain index and writes P (29 B per FE
+    # cell); the direct solve makes five passes that each read and write the
+    # field once (x, y, tridiagonal z, y, x: 80 B per cell); V-cycles (154 B
+    # per cell) where the multigrid runs instead
+    direct = n_v == 0
+    B_step = N_FE * (21 + 29) + N * ((80.0 if direct else 0.0) + n_v * 154 + (n_outer - 1) * 58)
+
     roof = None
+    kernels = None
     if not a.no_kernel_timing:
         reps = 50
-        t_s = g.bench_kernel("smooth0", reps)         # ms per launch of the finest-level smoother
-        t_t = g.bench_kernel("tdgl", reps)
-        t_c = g.bench_kernel("copy", reps)
-        # algorithmic bytes of one red-black sweep of the finest level: read phi and rhs,
-        # write phi = 24 B/cell (no Helmholtz term in this workload, so neither the
-        # material id nor the A-coefficient is read); one launch of k_mg_smooth is one
-        # colour, i.e. half a sweep
-        fused = bool(inp.mg_fused_sweep)
-        alg = (24.0 if fused else 12.0) * N
-        ach = alg / (t_s * 1e-3) / 1e9
-        kname = "k_mg_gsrb_stream (finest level, red+black in one pass)" if fused else "k_mg_smooth (finest level, one colour)"
-        roof = {"bound": "hbm", "kernel": kname,
-                "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic() if fused else None,
-                "alg_bytes_per_launch": alg, "avg_launch_ms": t_s,
-                # TDGL update: read Pz, phi, the material id and the grain index, write Pz
-                "tdgl_kernel_GBs": round(29.0 * N / (t_t * 1e-3) / 1e9, 1), "tdgl_avg_launch_ms": t_t,
-                "copy_GBs": round(16.0 * N / (t_c * 1e-3) / 1e9, 1)}
+        kernels = {}
+        # (name, algorithmic bytes per launch, description)
+        table = [("smooth0", 24.0 * N, "k_mg_gsrb_stream: one red+black sweep of the finest multigrid level")] if not direct else []
+        table += [("tdgl", 29.0 * N_FE, "TDGL update: k_tdgl_stream (interior FE cells) + k_tdgl_face_cells (second stream)"),
+                 ("rhs", 21.0 * N_FE, "Poisson RHS: k_rhs_stream + k_rhs_face_list (second stream)"),
+                 ("fft_x", 16.0 * N, "k_fft_rows forward (Hartley along x)"),
+                 ("fft_y", 16.0 * N, "k_fft_cols (Hartley along y, in place)"),
+                 ("fft_tri", 16.0 * N, "k_fft_tri (Thomas solve in z per mode, column in registers/LDS)"),
+                 ("fft_xinv", 16.0 * N, "k_fft_rows inverse"),
+                 ("fft_solve", 80.0 * N, "the whole direct solve (five launches)"),
+                 ("copy", 16.0 * N, "plain copy (device bandwidth reference)")]
+        if not direct:
+            table = [t for t in table if not t[0].startswith("fft")]
+        for name, alg, desc in table:
+            t = g.bench_kernel(name, reps)
+            kernels[name] = {"what": desc, "avg_launch_ms": round(t, 5), "alg_bytes": alg,
+                             "achieved_GBs": round(alg / (t * 1e-3) / 1e9, 1),
+                             "frac": round(alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        t = kernels["tdgl"]["avg_launch_ms"]
+        ach = kernels["tdgl"]["achieved_GBs"]
+        roof = {"bound": "hbm", "kernel": "TDGL update (k_tdgl_stream + k_tdgl_face_cells), the largest launch of the step",
+                "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(), "alg_bytes_per_launch": 29.0 * N_FE, "avg_launch_ms": t,
+                "alg_bytes_per_FE_cell": 29}
     P64, F64 = g.get("Pz"), g.get("Phi")
     g.close()
 
-    # precision form: FP32 V-cycles inside FP64 iterative refinement, same
-    # workload and same number of steps, compared with the FP64 run
+    # precision form on the direct path: FP32 transforms and tridiagonal
+    # solves inside FP64 refinement to the same stopping rule
     fp32 = None
     if not a.no_fp32:
-        g2 = fx.FerroX(fx.from_text(text, mg_warm_start=int(a.warm_start), precision=1, mg_fused_sweep=a.fused))
-        configs.apply_fields(g2, grains)
-        if a.warmup:
-            g2.step(a.warmup)
-        g2.sync()
-        t0 = time.perf_counter()
-        st2 = g2.step(a.steps)
-        g2.sync()
-        s2 = time.perf_counter() - t0
-        import numpy as np
+        g2, st2, s2 = timed_run(text, grains, a.steps, a.warmup, precision=1)
         eP = float(np.abs(g2.get("Pz") - P64).max() / np.abs(P64).max())
         eF = float(np.abs(g2.get("Phi") - F64).max() / np.abs(F64).max())
         g2.close()
         fp32 = {"steps_per_s": round(a.steps / s2, 3), "speedup_vs_fp64": round((a.steps / s2) / steps_per_s, 3),
-                "vcycles_per_step": round(st2["total_vcycles"] / max(1, st2["steps_done"]), 3),
+                "fp32_solves_per_step": round(st2["total_vcycles"] / max(1, st2["steps_done"]), 3),
                 "max_rel_err_Pz": eP, "max_rel_err_phi": eF}
-        print(f"# fp32+refine: {fp32}", file=sys.stderr, flush=True)
+        print(f"# fp32+refine (direct): {fp32}", file=sys.stderr, flush=True)
 
-    # the same FP64 workload with the previous step's phi as the initial guess
-    # (an option of this build: the reference zeroes phi before every solve);
-    # the stopping rule is unchanged, so the answer meets the same residual bound
-    warm = None
-    if not a.no_warm and not a.warm_start:
-        g3 = fx.FerroX(fx.from_text(text, mg_warm_start=1, mg_fused_sweep=a.fused))
-        configs.apply_fields(g3, grains)
-        if a.warmup:
-            g3.step(a.warmup)
-        g3.sync()
-        t0 = time.perf_counter()
-        st3 = g3.step(a.steps)
-        g3.sync()
-        s3 = time.perf_counter() - t0
-        import numpy as np
-        eP = float(np.abs(g3.get("Pz") - P64).max() / np.abs(P64).max())
-        eF = float(np.abs(g3.get("Phi") - F64).max() / np.abs(F64).max())
-        g3.close()
-        warm = {"steps_per_s": round(a.steps / s3, 3), "speedup_vs_zero_guess": round((a.steps / s3) / steps_per_s, 3),
-                "vcycles_per_step": round(st3["total_vcycles"] / max(1, st3["steps_done"]), 3),
-                "max_rel_err_Pz": eP, "max_rel_err_phi": eF}
-        print(f"# warm start: {warm}", file=sys.stderr, flush=True)
-
+    mg = None if a.no_mg else mg_legs(text, grains, a.mg_steps or a.steps, a.warmup, a.fused,
+                                      P64 if not a.mg_steps or a.mg_steps == a.steps else None, F64, not a.no_fp32)
     other = None if a.no_configs else config_legs(a.mfism_steps)
-    diff = None if a.no_diff else diff_leg(a.diff_large)
+    diff = None if a.no_diff else diff_leg(not a.no_diff_large)
     batch = None if a.no_batch else batch_leg(a.batch_steps)
 
     cpu = None
@@ -312,8 +341,8 @@ def main():
         o.close()
         cpu = {"value": round(a.cpu_steps / (c1 - c0), 5), "unit": "steps/s", "cores": threads(),
                "kind": "port",
-               "sample": f"{a.cpu_steps} step(s) of the same 256x256x128 polycrystal workload, FP64 oracle (OpenMP), "
-                         f"{c1 - c0:.1f} s"}
+               "sample": f"{a.cpu_steps} step(s) of the same 256x256x128 polycrystal workload, FP64 oracle (OpenMP, "
+                         f"multigrid to 1e-10), {c1 - c0:.1f} s"}
         if a.cpu_single_steps > 0:
             # the same oracle on one host thread
             n_all = threads()
@@ -343,15 +372,18 @@ def main():
         "data": "synthetic (seeded random initial Pz, seeded Voronoi grains and rotations)",
         "config": {"workload": "poly_256: 256x256x128 polycrystalline MFIM, 400 Voronoi grains (70% FE, "
                                "30% PE alpha=+1e9), 4 nm Al2O3 / 8 nm FE / 0.8 nm DE spacer, 0.5 V, "
-                               "zero-guess Poisson to 1e-10" + (" (warm start)" if a.warm_start else ""),
-                   "cells": N, "vcycles_per_step": round(n_v, 3), "outer_iters_per_step": round(n_outer, 3)},
+                               "zero-guess Poisson within the 1e-10 MLMG rule"
+                               + (" (direct solve)" if direct else ""),
+                   "cells": N, "fe_cells": N_FE, "poisson": "direct" if direct else "multigrid",
+                   "vcycles_per_step": round(n_v, 3), "outer_iters_per_step": round(n_outer, 3)},
         "cell_updates_per_s": round(N * steps_per_s, 1),
         "step_bytes_algorithmic": B_step,
         "step_hbm_GBs_algorithmic": round(B_step * steps_per_s / 1e9, 1),
         "step_roofline_frac": round(B_step * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roof,
+        "kernels": kernels,
         "fp32_refine": fp32,
-        "warm_start": warm,
+        "multigrid": mg,
         "other_configs": other,
         "differentiation": diff,
         "batching": batch,
