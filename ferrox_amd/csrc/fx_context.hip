@@ -2195,6 +2195,9 @@ int fx_diff_backward(fx_ctx* x, const double* dL_dm, double* g_theta, double* g_
  *   "resid0"   residual + restriction from the finest level
  *   "tdgl"     fused TDGL right-hand side + forward-Euler update (state unchanged)
  *   "rhs"      Poisson right-hand side + Newton coefficient
+ *   "fft_x", "fft_y", "fft_tri", "fft_xinv", "fft_solve"
+ *              passes of the direct solve (forward x transform, y transform,
+ *              tridiagonal solves, inverse x transform) and the whole solve
  *   "copy"     plain copy of N doubles (bandwidth calibration)
  * Kernels that write state write it into scratch or re-smooth phi; use after
  * the measured run. */
@@ -2221,14 +2224,40 @@ int fx_bench_kernel(fx_ctx* x, const char* which, int reps, double* avg_ms) {
         } else if (w == "residmax0") {
             fxk::mg_resid_norm(x->s, x->lev[0].d, bc, 1, nullptr, x->red, x->red_out);
         } else if (w == "tdgl") {
+            /* the launches update() makes for an Euler step, into the second P buffer */
             const bool fast = scalar_fast(x);
             TdglArgs a = tdgl_args(x, x->P);
             for (int d = 0; d < 3; ++d) a.Pout[d] = (fast && d < 2) ? nullptr : x->Pn[d];
-            fxk::tdgl(x->s, a, fast);
+            if (fast && x->fe_k0 >= 0) { a.kz0 = x->fe_k0; a.kz1 = x->fe_k1; }
+            if (fast && x->in.tdgl_stream && x->fe_k0 >= 0) {
+                TdglRhsArgs f;
+                f.t = a; f.rhs = nullptr; f.k0 = x->fe_k0; f.k1 = x->fe_k1; f.face = x->face_list; f.nface = x->nface;
+                fxk::tdgl_stream(x->s, x->s2, x->ev0, x->ev1, f);
+            } else {
+                fxk::tdgl(x->s, a, fast);
+            }
         } else if (w == "rhs") {
+            /* the launches compute_phi_rho() makes, into scratch */
             RhsArgs a = rhs_args(x, x->P);
-            a.rhs = x->tmp; a.alpha = x->tmp + x->N;
-            fxk::rhs_newton(x->s, a);
+            const bool narrow = !x->has_sc && x->fe_k0 >= 0;
+            a.rhs = x->tmp; a.alpha = narrow ? nullptr : x->tmp + x->N;
+            if (narrow) { a.kz0 = x->fe_k0; a.kz1 = x->fe_k1; }
+            if (narrow && x->pxy_zero && x->in.tdgl_stream)
+                fxk::rhs_stream(x->s, x->s2, x->ev0, x->ev1, a, x->fe_k0, x->fe_k1, x->face_list, x->nface);
+            else
+                fxk::rhs_newton(x->s, a);
+        } else if (w.compare(0, 4, "fft_") == 0) {
+            /* passes of the direct solve; the inverse x transform and the
+             * whole solve write into scratch, the others into the solve's
+             * own workspace */
+            if (!x->direct) return false;
+            const double blo = x->fft_w2lo * bv[2], bhi = x->fft_w2hi * bv[5];
+            if (w == "fft_x") fxk::fft_pass(x->s, x->fft64, 0, x->rhs, nullptr, blo, bhi);
+            else if (w == "fft_y") fxk::fft_pass(x->s, x->fft64, 1, nullptr, nullptr, 0.0, 0.0);
+            else if (w == "fft_tri") fxk::fft_pass(x->s, x->fft64, 2, nullptr, nullptr, 0.0, 0.0);
+            else if (w == "fft_xinv") fxk::fft_pass(x->s, x->fft64, 3, nullptr, x->tmp, 0.0, 0.0);
+            else if (w == "fft_solve") fxk::fft_solve(x->s, x->fft64, x->rhs, x->tmp, blo, bhi);
+            else return false;
         } else if (w == "copy") fxk::copy(x->s, x->tmp, x->phi, (long long)x->N);
         else return false;
         return true;

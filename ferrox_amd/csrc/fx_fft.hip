@@ -542,4 +542,14 @@ void fft_solve_add(hipStream_t s, const fxd::FftPlan<float>& P, const float* r, 
     fft_solve_impl<float, float, double>(s, P, r, phi, 0.0, 0.0, 1);
 }
 
+void fft_pass(hipStream_t s, const fxd::FftPlan<double>& P, int pass, const double* b, double* phi, double blo,
+              double bhi) {
+    switch (pass) {
+    case 0: rows<double, double, double>(s, P, true, b, P.S, blo, bhi, 0); break;
+    case 1: cols<double>(s, P); break;
+    case 2: tri<double>(s, P, 1.0 / (double)(P.n0 * P.n1)); break;
+    default: rows<double, double, double>(s, P, false, P.S, phi, 0.0, 0.0, 0); break;
+    }
+}
+
 }  // namespace fxk

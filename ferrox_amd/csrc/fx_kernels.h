@@ -101,6 +101,11 @@ bool fft_size_ok(int n);
 void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi);
 void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi);
 void fft_solve_add(hipStream_t s, const fxd::FftPlan<float>& P, const float* r, double* phi);
+/* one pass of the FP64 direct solve: 0 forward x transform (b -> S), 1 y
+ * transform of S in place (forward and inverse are the same), 2 tridiagonal
+ * solves in z, 3 inverse x transform (S -> phi) */
+void fft_pass(hipStream_t s, const fxd::FftPlan<double>& P, int pass, const double* b, double* phi, double blo,
+              double bhi);
 /* host[0..nq) = src[0..nq), then *seq = v (host and seq: device views of
  * coherent pinned host memory) */
 void publish(hipStream_t s, const double* src, int nq, double* host, unsigned* seq, unsigned v);
