@@ -97,8 +97,9 @@ struct fx_ctx {
      * the materials or the rotations clears it. */
     bool rhs_next = false;
     int fe_k0 = -1, fe_k1 = -1;    /* first and last plane holding an FE cell (-1: none) */
-    int* face_list = nullptr;      /* FE cells without centred stencils (global indices) */
+    int* face_list = nullptr;      /* FE cells without centred stencils (global indices), then the planes holding them */
     int nface = 0;
+    int nface_planes = 0;
     double* noalloc_scratch = nullptr;   /* device address of the static workspace on this context's GPU */
     /* the fused update writes the next RHS into rhs_nx, which the next solve
      * swaps in (rhs keeps the RHS of the last solve, the PoissonRHS the
@@ -795,9 +796,13 @@ int upload_mat(fx_ctx* x) {
             }
     if (x->face_list) { (void)hipFree(x->face_list); x->face_list = nullptr; }
     x->nface = (int)face.size();
+    const int plane = x->n[0] * x->n[1];
+    x->nface_planes = 0;
+    for (size_t q = 0, n = face.size(); q < n; ++q)   /* face is in plane order */
+        if (q == 0 || face[q] / plane != face[q - 1] / plane) { face.push_back(face[q] / plane); ++x->nface_planes; }
     if (!face.empty()) {
         HIPCHK(x, hipMalloc((void**)&x->face_list, face.size() * sizeof(int)));
-        HIPCHK(x, hipMemcpyAsync(x->face_list, face.data(), face.size() * sizeof(int), hipMemcpyHostToDevice, x->s));
+        HIPCHK(x, hipMemcpy(x->face_list, face.data(), face.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     x->has_sc = false;
     for (int k = 0; k < x->n[2]; ++k)
@@ -1260,6 +1265,13 @@ RhsArgs rhs_args(fx_ctx* x, double* const* P) {
     return a;
 }
 
+fxk::FaceSet face_set(const fx_ctx* x) {
+    fxk::FaceSet f;
+    f.cells = x->face_list; f.ncells = x->nface;
+    f.planes = x->face_list ? x->face_list + x->nface : nullptr; f.nplanes = x->nface_planes;
+    return f;
+}
+
 /* ComputePhi_Rho (ElectrostaticSolver.cpp:678-747). */
 int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
     double err = 1.0;
@@ -1287,7 +1299,7 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
                 ra0.kz1 = x->fe_k0 >= 0 ? x->fe_k1 : 0;
             }
             if (narrow && x->pxy_zero && x->in.tdgl_stream && x->fe_k0 >= 0)
-                fxk::rhs_stream(x->s, x->s2, x->ev0, x->ev1, ra0, x->fe_k0, x->fe_k1, x->face_list, x->nface);
+                fxk::rhs_stream(x->s, x->s2, x->ev0, x->ev1, ra0, x->fe_k0, x->fe_k1, face_set(x), x->in.tdgl_stream);
             else
                 fxk::rhs_newton(x->s, ra0);
             x->rhs_clean = !x->has_sc;
@@ -1379,7 +1391,7 @@ int update(fx_ctx* x, int order, fx_solve_stats* st2) {
             f.k1 = x->fe_k1;
             f.face = x->face_list;
             f.nface = x->nface;
-            fxk::tdgl_stream(x->s, x->s2, x->ev0, x->ev1, f);
+            fxk::tdgl_stream(x->s, x->s2, x->ev0, x->ev1, f, face_set(x), x->in.tdgl_stream);
         } else {
             fxk::tdgl(x->s, a, fast);
         }
@@ -2232,7 +2244,7 @@ int fx_bench_kernel(fx_ctx* x, const char* which, int reps, double* avg_ms) {
             if (fast && x->in.tdgl_stream && x->fe_k0 >= 0) {
                 TdglRhsArgs f;
                 f.t = a; f.rhs = nullptr; f.k0 = x->fe_k0; f.k1 = x->fe_k1; f.face = x->face_list; f.nface = x->nface;
-                fxk::tdgl_stream(x->s, x->s2, x->ev0, x->ev1, f);
+                fxk::tdgl_stream(x->s, x->s2, x->ev0, x->ev1, f, face_set(x), x->in.tdgl_stream);
             } else {
                 fxk::tdgl(x->s, a, fast);
             }
@@ -2243,7 +2255,7 @@ int fx_bench_kernel(fx_ctx* x, const char* which, int reps, double* avg_ms) {
             a.rhs = x->tmp; a.alpha = narrow ? nullptr : x->tmp + x->N;
             if (narrow) { a.kz0 = x->fe_k0; a.kz1 = x->fe_k1; }
             if (narrow && x->pxy_zero && x->in.tdgl_stream)
-                fxk::rhs_stream(x->s, x->s2, x->ev0, x->ev1, a, x->fe_k0, x->fe_k1, x->face_list, x->nface);
+                fxk::rhs_stream(x->s, x->s2, x->ev0, x->ev1, a, x->fe_k0, x->fe_k1, face_set(x), x->in.tdgl_stream);
             else
                 fxk::rhs_newton(x->s, a);
         } else if (w.compare(0, 4, "fft_") == 0) {

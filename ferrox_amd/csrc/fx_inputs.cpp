@@ -642,6 +642,11 @@ int fx_inputs_finalize(fx_inputs* in, char* err, int errlen) {
         set_err(err, errlen, "bad multigrid controls (mg_nu1 and mg_refine_nu1 must be at least 1)");
         return FX_EINVAL;
     }
+    if (in->tdgl_stream < 0 || in->tdgl_stream > 3) {
+        set_err(err, errlen, "tdgl_stream must be 0 (gathered), 1 (streamed), 2 (streamed, face cells by list) "
+                             "or 3 (streamed, face cells by plane)");
+        return FX_EINVAL;
+    }
     return FX_OK;
 }
 

@@ -121,15 +121,26 @@ void rhs_newton(hipStream_t s, const RhsArgs& a);
 int rho_update(hipStream_t s, const RhoArgs& a);
 void tdgl(hipStream_t s, const TdglArgs& a, bool scalar_fast);
 void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a);
-/* k_tdgl<true> with the stencil operands streamed through LDS, the face
- * cells by list on the second stream s2 (events e0, e1 order it after the
- * work queued on s and s after it); a.rhs is not used */
-void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const TdglRhsArgs& a);
+/* FE cells whose stencils are not all centred: their global indices and the
+ * planes that hold them (device arrays) */
+struct FaceSet {
+    const int* cells;
+    int ncells;
+    const int* planes;
+    int nplanes;
+};
+/* k_tdgl<true> with the stencil operands streamed through LDS; the cells
+ * next to a face beside it on the second stream s2 (events e0, e1 order it
+ * after the work queued on s and s after it): mode 1, by the streamed kernel
+ * over their planes when those are no more than a quarter of [k0, k1], else
+ * by list; 2 always by list; 3 always over their planes.  a.rhs is not used */
+void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const TdglRhsArgs& a,
+                 const FaceSet& fs, int mode);
 /* the Poisson RHS of scalar P in planes k0..k1 without SC cells (the RHS
- * buffer 0 in every non-FE cell), streamed through LDS, face cells by list
- * on s2 */
+ * buffer 0 in every non-FE cell), streamed through LDS, the cells next to a
+ * face as in tdgl_stream */
 void rhs_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const RhsArgs& a, int k0, int k1,
-                const int* face, int nface);
+                const FaceSet& fs, int mode);
 void heun(hipStream_t s, const HeunArgs& a);
 void compute_E(hipStream_t s, const EArgs& a);
 void mg_smooth(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, int inh, int color);

@@ -233,7 +233,8 @@ __device__ __forceinline__ double grad_r_of(const double R[9], double Hxx, doubl
            - g44m * (R11 * R11 * Hxx + R12 * R12 * Hyy + R13 * R13 * Hzz + 2. * R11 * R12 * Hxy + 2. * R12 * R13 * Hyz + 2. * R13 * R11 * Hxz)
            - g44m * (R21 * R21 * Hxx + R22 * R22 * Hyy + R23 * R23 * Hzz + 2. * R21 * R22 * Hxy + 2. * R22 * R23 * Hyz + 2. * R23 * R21 * Hxz);
 }
-__device__ __forceinline__ double grad_r(const PView& Fr, const MView& M, int i, int j, int k, const double* dx,
+template <class V = PView>
+__device__ __forceinline__ double grad_r(const V& Fr, const MView& M, int i, int j, int k, const double* dx,
                                          const PBC& bc, const double R[9], double g11, double g44m) {
     const double Hxx = ddpd<0>(Fr, M, i, j, k, dx, bc);
     const double Hyy = ddpd<1>(Fr, M, i, j, k, dx, bc);
@@ -282,6 +283,30 @@ struct GlobalNbr {
 __device__ __forceinline__ double grad_r_interior(const double* __restrict__ P, int c, const NbrOff& o,
                                                   const double* ih, const double R[9], double g11, double g44m) {
     return grad_r_interior_acc(GlobalNbr{P, c, o}, ih, R, g11, g44m);
+}
+
+/* The gradient and field terms of the scalar update of an interior FE cell
+ * as fixed weights of the raw differences, per rotation: with
+ * D_xx = P(i+1) - 2P + P(i-1) (likewise y, z) and D_xy = (P(+1,+1) -
+ * P(+1,-1)) - (P(-1,+1) - P(-1,-1)) (likewise yz, xz),
+ *   grad_r_of(R, H) = -(w0 D_xx + w1 D_yy + w2 D_zz + w3 D_xy + w4 D_yz + w5 D_xz),
+ *   w0 = (g11 R31^2 + g44m (R11^2 + R21^2)) / dx^2,
+ *   w3 = 2 (g11 R31 R32 + g44m (R11 R12 + R21 R22)) / (4 dx dy), ...;
+ * and the field term E2 = -(w6 d_x + w7 d_y + w8 d_z) with d_x = phi(i+1) -
+ * phi(i-1), w6 = R31 / (2 dx), ...  The same sums as grad_r_interior_acc and
+ * the centred E, reassociated (rounding-level differences). */
+__device__ __forceinline__ void interior_weights(const double R[9], double g11, double g44m, const double ih[3],
+                                                 double w[9]) {
+    const double R11 = R[0], R12 = R[1], R13 = R[2], R21 = R[3], R22 = R[4], R23 = R[5], R31 = R[6], R32 = R[7], R33 = R[8];
+    w[0] = (g11 * R31 * R31 + g44m * (R11 * R11 + R21 * R21)) * (ih[0] * ih[0]);
+    w[1] = (g11 * R32 * R32 + g44m * (R12 * R12 + R22 * R22)) * (ih[1] * ih[1]);
+    w[2] = (g11 * R33 * R33 + g44m * (R13 * R13 + R23 * R23)) * (ih[2] * ih[2]);
+    w[3] = 2. * (g11 * R31 * R32 + g44m * (R11 * R12 + R21 * R22)) * (0.25 * ih[0] * ih[1]);
+    w[4] = 2. * (g11 * R32 * R33 + g44m * (R12 * R13 + R22 * R23)) * (0.25 * ih[1] * ih[2]);
+    w[5] = 2. * (g11 * R33 * R31 + g44m * (R13 * R11 + R23 * R21)) * (0.25 * ih[0] * ih[2]);
+    w[6] = R31 * (0.5 * ih[0]);
+    w[7] = R32 * (0.5 * ih[1]);
+    w[8] = R33 * (0.5 * ih[2]);
 }
 
 /* The scalar TDGL update of one FE cell (SCALAR_FAST: is_polarization_scalar
@@ -543,10 +568,16 @@ __global__ __launch_bounds__(TRS_THR, 2) void k_tdgl_rhs_stream(TdglRhsArgs A) {
     const double* __restrict__ P = a.P[2];
     const double* __restrict__ Pn = a.Pout[2];
     /* global index of a window column/row, -1 across a non-periodic face */
+    /* in-plane source of a window column/row: the cell, its periodic image
+     * one cell past either side, or -1 (across a non-periodic side, or past
+     * the domain where the tile is wider than it) */
     auto wrap = [](int v, int n, int per) -> int {
-        if (v < 0) return per ? v + n : -1;
-        if (v >= n) return per ? v - n : -1;
-        return v;
+        if (v >= 0 && v < n) return v;
+        if (v == -1) return per ? n - 1 : -1;
+        if (v == n) return per ? 0 : -1;
+        if (v == -2) return per && n >= 2 ? n - 2 : -1;
+        if (v == n + 1) return per && n >= 2 ? 1 : -1;
+        return -1;
     };
     /* this thread's share of a halo plane of old P */
     long long src[TRS_PV];
@@ -639,16 +670,23 @@ __global__ __launch_bounds__(TRS_THR, 2) void k_tdgl_rhs_stream(TdglRhsArgs A) {
 /* ----------------- TDGL update streamed through LDS (scalar P) ------------ */
 /* k_tdgl<true> for the interior FE cells, with the stencil operands from LDS
  * instead of per-cell gathers: a workgroup owns a TDS_TX x TDS_TY column tile
- * and a chunk of TDS_Z planes of [k0, k1] and marches upward, P and phi of
- * three planes (tile + 1-cell halo) in LDS, the plane after next loaded into
- * registers while the current one is updated.  FE cells next to a face
- * (the list of k_tdgl_face_cells) are updated by that kernel; other cells
- * keep their P (Gamma = 0). */
-constexpr int TDS_TX = 64, TDS_TY = 8, TDS_Z = 16;
+ * and a chunk of zc planes of [k0, k1] and marches upward.  P and phi of
+ * each plane (tile + 1-cell halo) go to a ring of four LDS slots: while
+ * plane p is updated from slots p-1, p, p+1, plane p+2 (loaded into
+ * registers at the top of the step) is stored into the fourth slot, so one
+ * barrier per plane suffices.  Every global load of a step is issued
+ * unconditionally (clamped addresses, the value selected afterwards), so
+ * the update of plane p never waits for the loads in flight.  FE cells next
+ * to a face (the list of k_tdgl_face_cells) are updated by that kernel;
+ * other cells keep their P (Gamma = 0). */
+constexpr int TDS_TX = 64, TDS_TY = 8;
 constexpr int TDS_HX = TDS_TX + 2, TDS_HY = TDS_TY + 2;
+constexpr int TDS_HP = TDS_HX * TDS_HY;                                   /* values of a halo plane */
 constexpr int TDS_THR = TDS_TX * TDS_TY;                                  /* 512 */
-constexpr int TDS_PV = (TDS_HX * TDS_HY + TDS_THR - 1) / TDS_THR;       /* halo-plane values per thread */
-constexpr int TDS_MAXROWS = 512;   /* rotation / alpha tables up to this many rows are staged in LDS */
+constexpr int TDS_PV = (TDS_HP + TDS_THR - 1) / TDS_THR;                  /* halo-plane values per thread */
+constexpr int TDS_SLOTS = 4;
+constexpr int TDS_MAXROWS = 416;   /* rotation / alpha tables up to this many rows are staged in LDS (two
+                                      workgroups per CU: 2 x (42 KB ring + 6 KB ids + 80 B per row) <= 160 KB) */
 
 /* old P (or phi) around window position o of plane slots (k-1, k, k+1) */
 struct WinNbr {
@@ -662,238 +700,411 @@ struct WinNbr {
     }
 };
 
-template <bool TAB>
-__global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A) {
-    __shared__ double Pw[3 * TDS_HX * TDS_HY];
-    __shared__ double Fw[3 * TDS_HX * TDS_HY];
-    extern __shared__ __align__(16) unsigned char tds_dyn[];
-    double* const Rt = reinterpret_cast<double*>(tds_dyn);   /* TAB: 9 R values, then alpha, per row */
-    const TdglArgs& a = A.t;
-    if (TAB) {
-        const int nr = a.rot.nrows;
-        for (int t = threadIdx.x; t < 10 * nr; t += TDS_THR)
-            Rt[t] = t < 9 * nr ? a.rot.tab[t] : a.rot.atab[t - 9 * nr];
-    }
-    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    const int per0 = a.g.per[0], per1 = a.g.per[1];
-    const int i0 = blockIdx.x * TDS_TX, j0 = blockIdx.y * TDS_TY;
-    const int ks = A.k0 + blockIdx.z * TDS_Z;
-    const int ke = min(ks + TDS_Z - 1, A.k1);
-    const double* __restrict__ P = a.P[2];
-    const double* __restrict__ ph = a.phv.p;
-    auto wrap = [](int v, int n, int per) -> int {
-        if (v < 0) return per ? v + n : -1;
-        if (v >= n) return per ? v - n : -1;
-        return v;
-    };
-    long long src[TDS_PV];
-#pragma unroll
-    for (int r = 0; r < TDS_PV; ++r) {
-        const int t = threadIdx.x + r * TDS_THR;
-        src[r] = -1;
-        if (t < TDS_HX * TDS_HY) {
-            const int v = t / TDS_HX, u = t - v * TDS_HX;
-            const int i = wrap(i0 + u - 1, nx, per0), j = wrap(j0 + v - 1, ny, per1);
-            if (i >= 0 && j >= 0) src[r] = i + (long long)nx * j;
+/* P at (i, j, k) with the ghost semantics of PView, from the LDS window of
+ * the streamed passes around (ci, cj, p) (slots of planes p-1, p, p+1, tile
+ * offset wo) where it reaches, else from global memory: the masked stencils
+ * of the cells next to a face read at most one cell away, except the
+ * two-cell one-sided second differences (P_BC_flag 4). */
+struct WinPView {
+    const double* w0;
+    const double* w1;
+    const double* w2;
+    int ci, cj, p, wo;
+    PView g;
+    __device__ __forceinline__ double operator()(int i, int j, int k) const {
+        const int di = i - ci, dj = j - cj, dk = k - p;
+        if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1) {
+            const double* w = dk < 0 ? w0 : dk > 0 ? w2 : w1;
+            return w[wo + di + dj * TDS_HX];
         }
+        return g(i, j, k);
     }
-    const long long plane = (long long)nx * ny;
-    /* halo planes outside the domain (or across a non-periodic side) read 0;
-     * interior FE cells never read them */
-    auto fetch = [&](int q, double (&vp)[TDS_PV], double (&vf)[TDS_PV]) {
-#pragma unroll
-        for (int r = 0; r < TDS_PV; ++r) {
-            const bool ok = q >= 0 && q < nz && src[r] >= 0;
-            vp[r] = ok ? P[src[r] + plane * q] : 0.0;
-            vf[r] = ok ? ph[src[r] + plane * q] : 0.0;
+};
+
+/* phi at (i, j, k) with the ghost semantics of PhiView: from the window
+ * where the cell is inside the domain (or across a periodic side, whose
+ * image the window holds), else PhiView's Dirichlet / Neumann ghost */
+struct WinPhi {
+    const double* w0;
+    const double* w1;
+    const double* w2;
+    int ci, cj, p, wo;
+    PhiView g;
+    __device__ __forceinline__ double at(int i, int j, int k) const {
+        const int di = i - ci, dj = j - cj, dk = k - p;
+        const bool in_i = (i >= 0 && i < g.n[0]) || g.lo[0] == BC_PER;
+        const bool in_j = (j >= 0 && j < g.n[1]) || g.lo[1] == BC_PER;
+        if (di >= -1 && di <= 1 && dj >= -1 && dj <= 1 && dk >= -1 && dk <= 1 && in_i && in_j && k >= 0 &&
+            k < g.n[2]) {
+            const double* w = dk < 0 ? w0 : dk > 0 ? w2 : w1;
+            return w[wo + di + dj * TDS_HX];
         }
-    };
-    auto stash = [&](int q, const double (&vp)[TDS_PV], const double (&vf)[TDS_PV]) {
-        double* dp = Pw + slot3(q) * (TDS_HX * TDS_HY);
-        double* df = Fw + slot3(q) * (TDS_HX * TDS_HY);
+        return g.at(i, j, k);
+    }
+};
+
+/* In-plane source of window column (or row) v of a domain of n cells:
+ * cells 0 .. n-1 themselves, -1 and n their periodic images (or nothing
+ * across a non-periodic side); positions further out, which arise when the
+ * tile is wider than the domain, are read by no cell of the domain and have
+ * no source.  Returns -1 for no source. */
+__device__ __forceinline__ int halo_src(int v, int n, int per) {
+    if (v >= 0 && v < n) return v;
+    if (v == -1) return per ? n - 1 : -1;
+    if (v == n) return per ? 0 : -1;
+    return -1;
+}
+
+/* The halo plane a thread stages: for r < TDS_PV, window position t =
+ * threadIdx.x + r TDS_THR (if below TDS_HP), its in-plane source index
+ * (0, the plane's first cell, where it has none) and whether the source
+ * exists.  Every offset lies inside the plane. */
+struct HaloSrc {
+    int off[TDS_PV];
+    bool ok[TDS_PV];
+    __device__ __forceinline__ HaloSrc(int i0, int j0, int nx, int ny, int per0, int per1) {
 #pragma unroll
         for (int r = 0; r < TDS_PV; ++r) {
             const int t = threadIdx.x + r * TDS_THR;
-            if (t < TDS_HX * TDS_HY) { dp[t] = vp[r]; df[t] = vf[r]; }
+            const int v = t / TDS_HX, u = t - v * TDS_HX;
+            const int i = halo_src(i0 + u - 1, nx, per0), j = halo_src(j0 + v - 1, ny, per1);
+            const bool good = t < TDS_HP && i >= 0 && j >= 0;
+            ok[r] = good;
+            off[r] = good ? i + nx * j : 0;
         }
-    };
+    }
+};
+
+/* What a streamed pass loads for plane q: the halo plane of P (and of phi
+ * for the update) and the material id and table row of this thread's tile
+ * cell.  Every load is issued unconditionally (the plane index clamped to
+ * [0, qmax], halo sources off the domain read the plane's first value), so
+ * the compiler's wait counting never merges paths with different loads in
+ * flight; what is off the domain is replaced by 0 when stored to LDS. */
+template <bool PHI>
+struct PlaneRegs {
+    double p[TDS_PV];
+    double f[TDS_PV];
+    int raw, row;
+};
+
+template <bool PHI, int ROT>
+__device__ __forceinline__ void plane_fetch(PlaneRegs<PHI>& R, int q, int qmax, const double* __restrict__ P,
+                                            const double* __restrict__ ph, const HaloSrc& hs, long long plane,
+                                            const MView& M, int cic, int cjc, const int* __restrict__ ridx,
+                                            long long cc0) {
+    const int qc = q < 0 ? 0 : q > qmax ? qmax : q;
+    const long long base = plane * (long long)qc;
+#pragma unroll
+    for (int r = 0; r < TDS_PV; ++r) {
+        R.p[r] = P[base + hs.off[r]];
+        if (PHI) R.f[r] = ph[base + hs.off[r]];
+    }
+    R.raw = M.raw(cic, cjc, qc);
+    R.row = ROT ? ridx[cc0 + base] : 0;
+}
+
+/* LDS ring of a streamed pass: four slots of the P (and phi) halo planes,
+ * three of the tile's packed material id | table row << 8. */
+template <bool PHI>
+__device__ __forceinline__ void plane_stash(const PlaneRegs<PHI>& R, int q, int nz, const HaloSrc& hs, double* Pw,
+                                            double* Fw, int* Rr) {
+    const bool qok = q >= 0 && q < nz;
+    double* dp = Pw + (q & 3) * TDS_HP;
+    double* df = Fw + (q & 3) * TDS_HP;
+#pragma unroll
+    for (int r = 0; r < TDS_PV; ++r) {
+        const int t = threadIdx.x + r * TDS_THR;
+        if (t < TDS_HP) {
+            const bool ok = qok && hs.ok[r];
+            dp[t] = ok ? R.p[r] : 0.0;
+            if (PHI) df[t] = ok ? R.f[r] : 0.0;
+        }
+    }
+    Rr[((q + 3) % 3) * TDS_THR + threadIdx.x] = R.raw | (R.row << 8);
+}
+
+/* ROT: 0 uniform rotation (no table), 1 rotation/alpha table staged in LDS,
+ * 2 table read from global memory.  FACES false: the interior FE cells of
+ * chunks of zc planes of [k0, k1]; true: the FE cells next to a face (masked
+ * stencils on the window) of plane zlist[blockIdx.z] (the planes that hold
+ * such cells), a separate launch because the masked stencils need twice the
+ * registers.
+ *
+ * The march: planes ks-1 .. ks+1 are staged first and ks+2 is loaded into
+ * registers; step p loads plane p+3 into the other register set, updates
+ * plane p from LDS, stores plane p+2 (loaded one step earlier) into the
+ * ring slot of plane p-2 and ends with the plane's one barrier.  The loop
+ * is unrolled by two so that the two register sets stay fixed; planes past
+ * ke + 1 are loaded as ke + 1 again (cache hits) and not used. */
+template <int ROT, bool FACES>
+__global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int zc, const int* __restrict__ zlist) {
+    __shared__ double Pw[TDS_SLOTS * TDS_HP];
+    __shared__ double Fw[TDS_SLOTS * TDS_HP];
+    __shared__ int Rr[3 * TDS_THR];
+    extern __shared__ __align__(16) unsigned char tds_dyn[];
+    /* ROT 1: per row, the interior weights w0..w8 and alpha (interior pass)
+     * or the 9 R values and alpha (face pass) */
+    double* const Rt = reinterpret_cast<double*>(tds_dyn);
+    const TdglArgs& a = A.t;
+    const double* dx = a.g.dx;
+    const double ih[3] = {1.0 / dx[0], 1.0 / dx[1], 1.0 / dx[2]};
+    const double g44m = a.g44 - a.g44_p;
+    if (ROT == 1) {
+        const int nr = a.rot.nrows;
+        if (FACES) {
+            for (int t = threadIdx.x; t < 10 * nr; t += TDS_THR)
+                Rt[t] = t < 9 * nr ? a.rot.tab[t] : a.rot.atab[t - 9 * nr];
+        } else {
+            for (int q = threadIdx.x; q < nr; q += TDS_THR) {
+                double R[9], w[9];
+#pragma unroll
+                for (int m = 0; m < 9; ++m) R[m] = a.rot.tab[9 * q + m];
+                interior_weights(R, a.g11, g44m, ih, w);
+#pragma unroll
+                for (int m = 0; m < 9; ++m) Rt[10 * q + m] = w[m];
+                Rt[10 * q + 9] = a.rot.atab[q];
+            }
+        }
+    }
+    double wu[9];   /* ROT 0: the uniform rotation's weights */
+    if (ROT == 0 && !FACES) interior_weights(a.rot.Ru, a.g11, g44m, ih, wu);
+    const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
+    const int i0 = blockIdx.x * TDS_TX, j0 = blockIdx.y * TDS_TY;
+    const int ks = FACES ? zlist[blockIdx.z] : A.k0 + blockIdx.z * zc;
+    const int ke = FACES ? ks : min(ks + zc - 1, A.k1);
+    const int qmax = min(ke + 1, nz - 1);
+    const double* __restrict__ P = a.P[2];
+    const double* __restrict__ ph = a.phv.p;
+    const HaloSrc hs(i0, j0, nx, ny, a.g.per[0], a.g.per[1]);
+    const long long plane = (long long)nx * ny;
     const int tu = threadIdx.x % TDS_TX, tv = threadIdx.x / TDS_TX;
     const int ci = i0 + tu, cj = j0 + tv;
     const bool in_dom = ci < nx && cj < ny;
+    const int cic = in_dom ? ci : 0, cjc = in_dom ? cj : 0;   /* clamped: every load is in bounds */
     const int wo = (tv + 1) * TDS_HX + (tu + 1);
-    {
-        double vp[TDS_PV], vf[TDS_PV];
-        fetch(ks - 1, vp, vf); stash(ks - 1, vp, vf);
-        fetch(ks, vp, vf); stash(ks, vp, vf);
-        fetch(ks + 1, vp, vf); stash(ks + 1, vp, vf);
-    }
+    const int* __restrict__ ridx = a.rot.idx;
+    const long long cc0 = cic + (long long)nx * cjc;
+    auto fetch = [&](PlaneRegs<true>& R, int q) { plane_fetch<true, ROT>(R, q, qmax, P, ph, hs, plane, a.M, cic, cjc, ridx, cc0); };
+    auto stash = [&](const PlaneRegs<true>& R, int q) { plane_stash<true>(R, q, nz, hs, Pw, Fw, Rr); };
+    PlaneRegs<true> RA, RB;
+    fetch(RA, ks - 1); stash(RA, ks - 1);
+    fetch(RA, ks); stash(RA, ks);
+    fetch(RA, ks + 1); stash(RA, ks + 1);
+    fetch(RA, ks + 2);
     __syncthreads();
     const double Gam = a.BigGamma;
-    const double* dx = a.g.dx;
-    const double ih[3] = {1.0 / dx[0], 1.0 / dx[1], 1.0 / dx[2]};
-    /* material id and table row of this thread's cell, one plane ahead */
-    const int* __restrict__ ridx = a.rot.idx;
-    int raw_c = 0, row_c = 0;
-    if (in_dom) {
-        raw_c = a.M.raw(ci, cj, ks);
-        if (ridx) row_c = ridx[ci + nx * (cj + ny * ks)];
-    }
-    for (int p = ks; p <= ke; ++p) {
-        double np_[TDS_PV], nf[TDS_PV];
-        fetch(p + 2, np_, nf);                         /* in flight while plane p is updated */
-        int raw_n = 0, row_n = 0;
-        if (in_dom && p < ke) {
-            raw_n = a.M.raw(ci, cj, p + 1);
-            if (ridx) row_n = ridx[ci + nx * (cj + ny * (p + 1))];
-        }
-        if (in_dom) {
+
+    auto update = [&](int p) {
+        const int rr = Rr[((p + 3) % 3) * TDS_THR + threadIdx.x];
+        const int raw_c = rr & 255, row_c = rr >> 8;
+        const int s0 = ((p - 1) & 3) * TDS_HP, s1 = (p & 3) * TDS_HP, s2 = ((p + 1) & 3) * TDS_HP;
+        if (!FACES && in_dom && (raw_c & 3) == 0 && (raw_c & MAT_INTERIOR_FE)) {
             const int c = ci + nx * (cj + ny * p);
-            const int raw = raw_c;
-            if ((raw & 3) == 0 && (raw & MAT_INTERIOR_FE)) {
-                const double* p0 = Pw + slot3(p - 1) * (TDS_HX * TDS_HY);
-                const double* p1 = Pw + slot3(p) * (TDS_HX * TDS_HY);
-                const double* p2 = Pw + slot3(p + 1) * (TDS_HX * TDS_HY);
-                const double* f0 = Fw + slot3(p - 1) * (TDS_HX * TDS_HY);
-                const double* f1 = Fw + slot3(p) * (TDS_HX * TDS_HY);
-                const double* f2 = Fw + slot3(p + 1) * (TDS_HX * TDS_HY);
-                const WinNbr F{p0, p1, p2, wo};
+            const double* f0 = Fw + s0;
+            const double* f1 = Fw + s1;
+            const double* f2 = Fw + s2;
+            const WinNbr F{Pw + s0, Pw + s1, Pw + s2, wo};
+            double w[9];
+            double alpha_c = a.alpha;
+            if (ROT == 1) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) w[q] = Rt[10 * row_c + q];
+                alpha_c = Rt[10 * row_c + 9];
+            } else if (ROT == 2) {
                 double R[9];
-                double alpha_c = a.alpha;
-                if (TAB) {
+                const double* t = a.rot.tab + 9 * row_c;
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) R[q] = Rt[9 * row_c + q];
-                    alpha_c = Rt[9 * a.rot.nrows + row_c];
-                } else if (ridx) {
-                    const double* t = a.rot.tab + 9 * row_c;
+                for (int q = 0; q < 9; ++q) R[q] = t[q];
+                interior_weights(R, a.g11, g44m, ih, w);
+                alpha_c = a.rot.atab[row_c];
+            } else {
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) R[q] = t[q];
-                    alpha_c = a.rot.atab[row_c];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 9; ++q) R[q] = a.rot.Ru[q];
-                }
-                const bool tph = (raw & 4) != 0;
-                double E2 = 0.0;
-                if (a.incE == 1) {
-                    const double fx = (f1[wo + 1] - f1[wo - 1]) * (0.5 * ih[0]);
-                    const double fy = (f1[wo + TDS_HX] - f1[wo - TDS_HX]) * (0.5 * ih[1]);
-                    const double fz = (f2[wo] - f0[wo]) * (0.5 * ih[2]);
-                    double sE = 0.0;
-                    if (R[6] != 0.0) sE += R[6] * fx;
-                    if (R[7] != 0.0) sE += R[7] * fy;
-                    sE += R[8] * fz;
-                    E2 = -sE;
-                }
-                const double r = p1[wo];
-                double L = 0.0, G = 0.0, El = 0.0;
-                if (a.incL == 1) {
-                    const double r2 = r * r;
-                    const double dF = alpha_c * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
-                    L = -1.0 * Gam * dF;
-                }
-                if (a.incG == 1) G = -1.0 * Gam * grad_r_interior_acc(F, ih, R, a.g11, a.g44 - a.g44_p);
-                if (a.incE == 1) El = -1.0 * Gam * (-E2);
-                if (tph) { L = 0.0; G = 0.0; El = 0.0; }
-                double g = 0.0;
-                if (a.incL == 1) g += L;
-                if (a.incG == 1) g += G;
-                if (a.incE == 1) g += El;
-                a.Pout[2][c] = r + a.dt * g;
+                for (int q = 0; q < 9; ++q) w[q] = wu[q];
             }
+            const bool tph = (raw_c & 4) != 0;
+            const double r = Pw[s1 + wo];
+            double L = 0.0, G = 0.0, El = 0.0;
+            if (a.incL == 1) {
+                const double r2 = r * r;
+                const double dF = alpha_c * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
+                L = -1.0 * Gam * dF;
+            }
+            if (a.incG == 1) {
+                const double Dxx = (F(1, 0, 0) - 2. * r) + F(-1, 0, 0);
+                const double Dyy = (F(0, 1, 0) - 2. * r) + F(0, -1, 0);
+                const double Dzz = (F(0, 0, 1) - 2. * r) + F(0, 0, -1);
+                const double Dxy = (F(1, 1, 0) - F(1, -1, 0)) - (F(-1, 1, 0) - F(-1, -1, 0));
+                const double Dyz = (F(0, 1, 1) - F(0, 1, -1)) - (F(0, -1, 1) - F(0, -1, -1));
+                const double Dxz = (F(1, 0, 1) - F(1, 0, -1)) - (F(-1, 0, 1) - F(-1, 0, -1));
+                const double gr = w[0] * Dxx + w[1] * Dyy + w[2] * Dzz + w[3] * Dxy + w[4] * Dyz + w[5] * Dxz;
+                G = Gam * gr;                          /* -Gam * grad_r_of, grad_r_of = -gr */
+            }
+            if (a.incE == 1) {
+                const double sE = w[6] * (f1[wo + 1] - f1[wo - 1]) + w[7] * (f1[wo + TDS_HX] - f1[wo - TDS_HX]) +
+                                  w[8] * (f2[wo] - f0[wo]);
+                El = -Gam * sE;                        /* -Gam * (-E2), E2 = -sE */
+            }
+            if (tph) { L = 0.0; G = 0.0; El = 0.0; }
+            double g = 0.0;
+            if (a.incL == 1) g += L;
+            if (a.incG == 1) g += G;
+            if (a.incE == 1) g += El;
+            a.Pout[2][c] = r + a.dt * g;
+        } else if (FACES && in_dom && (raw_c & 3) == 0 && !(raw_c & MAT_INTERIOR_FE)) {
+            /* an FE cell next to a face: the masked stencils (tdgl_scalar_cell),
+             * operands from the window */
+            const int c = ci + nx * (cj + ny * p);
+            const WinPView Fr{Pw + s0, Pw + s1, Pw + s2, ci, cj, p, wo,
+                              PView{P, nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]}};
+            const WinPhi Fp{Fw + s0, Fw + s1, Fw + s2, ci, cj, p, wo, a.phv};
+            double R[9];
+            double alpha_c = a.alpha;
+            if (ROT == 1) {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) R[q] = Rt[9 * row_c + q];
+                alpha_c = Rt[9 * a.rot.nrows + row_c];
+            } else if (ROT == 2) {
+                const double* t = a.rot.tab + 9 * row_c;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) R[q] = t[q];
+                alpha_c = a.rot.atab[row_c];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 9; ++q) R[q] = a.rot.Ru[q];
+            }
+            const bool tph = (raw_c & 4) != 0;
+            double E2 = 0.0;
+            if (a.incE == 1) {
+                double fx, fy, fz;
+                grad_phi(Fp, a.g, ci, cj, p, fx, fy, fz);
+                double sE = 0.0;
+                if (R[6] != 0.0) sE += R[6] * fx;
+                if (R[7] != 0.0) sE += R[7] * fy;
+                sE += R[8] * fz;
+                E2 = -sE;
+            }
+            const double r = Pw[s1 + wo];
+            double L = 0.0, G = 0.0, El = 0.0;
+            if (a.incL == 1) {
+                const double r2 = r * r;
+                const double dF = alpha_c * r + a.beta * (r2 * r) + a.gamma * (r2 * r2 * r);
+                L = -1.0 * Gam * dF;
+            }
+            if (a.incG == 1) G = -1.0 * Gam * grad_r(Fr, a.M, ci, cj, p, dx, a.pbc, R, a.g11, a.g44 - a.g44_p);
+            if (a.incE == 1) El = -1.0 * Gam * (-E2);
+            if (tph) { L = 0.0; G = 0.0; El = 0.0; }
+            double g = 0.0;
+            if (a.incL == 1) g += L;
+            if (a.incG == 1) g += G;
+            if (a.incE == 1) g += El;
+            a.Pout[2][c] = r + a.dt * g;
         }
-        raw_c = raw_n;
-        row_c = row_n;
+    };
+    for (int p = ks; p <= ke; p += 2) {
+        fetch(RB, p + 3);
+        update(p);
+        stash(RA, p + 2);                              /* the slot of plane p-2, read by no step after p-1 */
         __syncthreads();
-        stash(p + 2, np_, nf);                         /* over plane p-1, no longer read */
+        fetch(RA, p + 4);
+        if (p + 1 <= ke) update(p + 1);
+        stash(RB, p + 3);
         __syncthreads();
     }
+}
+
+/* compute units of the current device (256 if it cannot be read) */
+static int num_cus() {
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+            ncu = 256;
+    }
+    return ncu;
+}
+
+/* Workgroups of the streamed passes resident per CU: the update's LDS (the
+ * four-slot P and phi ring, the id ring and the rotation table, 80 KB)
+ * holds it to two; the RHS would fit four, but its extra warm-up planes
+ * cost more than the occupancy gains (measured). */
+#ifndef FX_TDGL_WG_PER_CU
+#define FX_TDGL_WG_PER_CU 2
+#endif
+#ifndef FX_RHS_WG_PER_CU
+#define FX_RHS_WG_PER_CU 2
+#endif
+
+/* z-chunk length of the streamed passes over nk planes with nxy column
+ * tiles: the fewest chunks that keep the grid within whole rounds of wpc
+ * workgroups per CU, measured by (rounds) x (chunk + 2 warm-up planes) */
+int stream_chunk(int nxy, int nk, int ncu, int wpc) {
+    const int slots = wpc * (ncu > 0 ? ncu : 256);
+    int best = nk, best_cost = 1 << 30;
+    for (int nch = 1; nch <= nk; ++nch) {
+        const int zc = (nk + nch - 1) / nch;
+        const int rounds = (nxy * ((nk + zc - 1) / zc) + slots - 1) / slots;
+        const int cost = rounds * (zc + 2);
+        if (cost < best_cost) { best_cost = cost; best = zc; }
+    }
+    return best;
 }
 
 /* ------------------- Poisson RHS streamed through LDS (scalar P) ----------- */
 /* k_rhs_newton's centred branch for the interior FE cells of planes
  * [k0, k1] (no SC cells, the RHS buffer already 0 in every non-FE cell, the
- * Newton coefficient not written): P of three planes (tile + 1-cell halo)
- * in LDS, the plane after next in registers, R_3 from an LDS copy of the
- * rotation table; FE cells next to a face by list (k_rhs_face_list). */
-template <bool TAB>
-__global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, int k1) {
-    __shared__ double Pw[3 * TDS_HX * TDS_HY];
+ * Newton coefficient not written): P of each plane (tile + 1-cell halo) in
+ * the four-slot LDS ring of k_tdgl_stream, the plane after next in
+ * registers, R_3 from an LDS copy of the rotation table; FE cells next to a
+ * face by list (k_rhs_face_list). */
+template <int ROT, bool FACES>
+__global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, int k1, int zc,
+                                                           const int* __restrict__ zlist) {
+    __shared__ double Pw[TDS_SLOTS * TDS_HP];
+    __shared__ int Rr[3 * TDS_THR];
     extern __shared__ __align__(16) unsigned char rs_dyn[];
-    double* const Rt = reinterpret_cast<double*>(rs_dyn);   /* TAB: R_31, R_32, R_33 per row */
-    if (TAB) {
+    double* const Rt = reinterpret_cast<double*>(rs_dyn);   /* ROT 1: R_31, R_32, R_33 per row */
+    if (ROT == 1) {
         const int nr = a.rot.nrows;
         for (int t = threadIdx.x; t < 3 * nr; t += TDS_THR) Rt[t] = a.rot.tab[9 * (t / 3) + 6 + t % 3];
     }
     const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    const int per0 = a.g.per[0], per1 = a.g.per[1];
     const int i0 = blockIdx.x * TDS_TX, j0 = blockIdx.y * TDS_TY;
-    const int ks = k0 + blockIdx.z * TDS_Z;
-    const int ke = min(ks + TDS_Z - 1, k1);
+    const int ks = FACES ? zlist[blockIdx.z] : k0 + blockIdx.z * zc;
+    const int ke = FACES ? ks : min(ks + zc - 1, k1);
+    const int qmax = min(ke + 1, nz - 1);
     const double* __restrict__ P = a.P[2];
-    auto wrap = [](int v, int n, int per) -> int {
-        if (v < 0) return per ? v + n : -1;
-        if (v >= n) return per ? v - n : -1;
-        return v;
-    };
-    long long src[TDS_PV];
-#pragma unroll
-    for (int r = 0; r < TDS_PV; ++r) {
-        const int t = threadIdx.x + r * TDS_THR;
-        src[r] = -1;
-        if (t < TDS_HX * TDS_HY) {
-            const int v = t / TDS_HX, u = t - v * TDS_HX;
-            const int i = wrap(i0 + u - 1, nx, per0), j = wrap(j0 + v - 1, ny, per1);
-            if (i >= 0 && j >= 0) src[r] = i + (long long)nx * j;
-        }
-    }
+    const HaloSrc hs(i0, j0, nx, ny, a.g.per[0], a.g.per[1]);
     const long long plane = (long long)nx * ny;
-    auto fetch = [&](int q, double (&vp)[TDS_PV]) {
-#pragma unroll
-        for (int r = 0; r < TDS_PV; ++r) vp[r] = (q >= 0 && q < nz && src[r] >= 0) ? P[src[r] + plane * q] : 0.0;
-    };
-    auto stash = [&](int q, const double (&vp)[TDS_PV]) {
-        double* dp = Pw + slot3(q) * (TDS_HX * TDS_HY);
-#pragma unroll
-        for (int r = 0; r < TDS_PV; ++r) {
-            const int t = threadIdx.x + r * TDS_THR;
-            if (t < TDS_HX * TDS_HY) dp[t] = vp[r];
-        }
-    };
     const int tu = threadIdx.x % TDS_TX, tv = threadIdx.x / TDS_TX;
     const int ci = i0 + tu, cj = j0 + tv;
     const bool in_dom = ci < nx && cj < ny;
+    const int cic = in_dom ? ci : 0, cjc = in_dom ? cj : 0;
     const int wo = (tv + 1) * TDS_HX + (tu + 1);
-    {
-        double vp[TDS_PV];
-        fetch(ks - 1, vp); stash(ks - 1, vp);
-        fetch(ks, vp); stash(ks, vp);
-        fetch(ks + 1, vp); stash(ks + 1, vp);
-    }
+    const int* __restrict__ ridx = a.rot.idx;
+    const long long cc0 = cic + (long long)nx * cjc;
+    auto fetch = [&](PlaneRegs<false>& R, int q) { plane_fetch<false, ROT>(R, q, qmax, P, nullptr, hs, plane, a.M, cic, cjc, ridx, cc0); };
+    auto stash = [&](const PlaneRegs<false>& R, int q) { plane_stash<false>(R, q, nz, hs, Pw, nullptr, Rr); };
+    PlaneRegs<false> RA, RB;
+    fetch(RA, ks - 1); stash(RA, ks - 1);
+    fetch(RA, ks); stash(RA, ks);
+    fetch(RA, ks + 1); stash(RA, ks + 1);
+    fetch(RA, ks + 2);
     __syncthreads();
     const double* dx = a.g.dx;
-    const int* __restrict__ ridx = a.rot.idx;
-    int raw_c = 0, row_c = 0;
-    if (in_dom) {
-        raw_c = a.M.raw(ci, cj, ks);
-        if (ridx) row_c = ridx[ci + nx * (cj + ny * ks)];
-    }
-    for (int p = ks; p <= ke; ++p) {
-        double np_[TDS_PV];
-        fetch(p + 2, np_);
-        int raw_n = 0, row_n = 0;
-        if (in_dom && p < ke) {
-            raw_n = a.M.raw(ci, cj, p + 1);
-            if (ridx) row_n = ridx[ci + nx * (cj + ny * (p + 1))];
-        }
-        if (in_dom && (raw_c & 3) == 0 && (raw_c & MAT_INTERIOR_FE)) {
+    auto rhs_of = [&](int p) {
+        const int rr = Rr[((p + 3) % 3) * TDS_THR + threadIdx.x];
+        const int raw_c = rr & 255, row_c = rr >> 8;
+        const double* p0 = Pw + ((p - 1) & 3) * TDS_HP;
+        const double* p1 = Pw + (p & 3) * TDS_HP;
+        const double* p2 = Pw + ((p + 1) & 3) * TDS_HP;
+        if (!FACES && in_dom && (raw_c & 3) == 0 && (raw_c & MAT_INTERIOR_FE)) {
             const int c = ci + nx * (cj + ny * p);
             double R6, R7, R8;
-            if (TAB) { R6 = Rt[3 * row_c]; R7 = Rt[3 * row_c + 1]; R8 = Rt[3 * row_c + 2]; }
-            else if (ridx) { const double* t = a.rot.tab + 9 * row_c; R6 = t[6]; R7 = t[7]; R8 = t[8]; }
+            if (ROT == 1) { R6 = Rt[3 * row_c]; R7 = Rt[3 * row_c + 1]; R8 = Rt[3 * row_c + 2]; }
+            else if (ROT == 2) { const double* t = a.rot.tab + 9 * row_c; R6 = t[6]; R7 = t[7]; R8 = t[8]; }
             else { R6 = a.rot.Ru[6]; R7 = a.rot.Ru[7]; R8 = a.rot.Ru[8]; }
-            const double* p0 = Pw + slot3(p - 1) * (TDS_HX * TDS_HY);
-            const double* p1 = Pw + slot3(p) * (TDS_HX * TDS_HY);
-            const double* p2 = Pw + slot3(p + 1) * (TDS_HX * TDS_HY);
             const double gx = (p1[wo + 1] - p1[wo - 1]) * (0.5 / dx[0]);
             const double gy = (p1[wo + TDS_HX] - p1[wo - TDS_HX]) * (0.5 / dx[1]);
             const double gz = (p2[wo] - p0[wo]) * (0.5 / dx[2]);
@@ -902,11 +1113,29 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, in
             if (R7 != 0.0) x += R7 * gy;
             x += R8 * gz;
             a.rhs[c] = -x;
+        } else if (FACES && in_dom && (raw_c & 3) == 0 && !(raw_c & MAT_INTERIOR_FE)) {
+            /* an FE cell next to a face: the masked branch (k_rhs_face_list) on the window */
+            const int c = ci + nx * (cj + ny * p);
+            double R6, R7, R8;
+            if (ROT == 1) { R6 = Rt[3 * row_c]; R7 = Rt[3 * row_c + 1]; R8 = Rt[3 * row_c + 2]; }
+            else if (ROT == 2) { const double* t = a.rot.tab + 9 * row_c; R6 = t[6]; R7 = t[7]; R8 = t[8]; }
+            else { R6 = a.rot.Ru[6]; R7 = a.rot.Ru[7]; R8 = a.rot.Ru[8]; }
+            const WinPView Fr{p0, p1, p2, ci, cj, p, wo, PView{P, nx, ny, nz, a.g.per[0], a.g.per[1], a.g.per[2]}};
+            double x = 0.0;
+            if (R6 != 0.0) x += R6 * dpd<0>(Fr, a.M, ci, cj, p, dx, a.pbc);
+            if (R7 != 0.0) x += R7 * dpd<1>(Fr, a.M, ci, cj, p, dx, a.pbc);
+            x += R8 * dpd<2>(Fr, a.M, ci, cj, p, dx, a.pbc);
+            a.rhs[c] = -x;
         }
-        raw_c = raw_n;
-        row_c = row_n;
+    };
+    for (int p = ks; p <= ke; p += 2) {
+        fetch(RB, p + 3);
+        rhs_of(p);
+        stash(RA, p + 2);
         __syncthreads();
-        stash(p + 2, np_);
+        fetch(RA, p + 4);
+        if (p + 1 <= ke) rhs_of(p + 1);
+        stash(RB, p + 3);
         __syncthreads();
     }
 }
@@ -1014,38 +1243,80 @@ void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a) {
     hipLaunchKernelGGL(k_tdgl_rhs_stream, g, dim3(TRS_THR), 0, s, a);
     if (nb) hipLaunchKernelGGL(k_rhs_face_cells, dim3(nb), dim3(256), 0, s, a);
 }
-void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const TdglRhsArgs& a) {
-    const unsigned nb = (unsigned)((a.nface + 255) / 256);
-    /* the face cells on the second stream, beside the streamed interior */
-    if (nb) {
+/* The cells next to a face: with no more than a quarter of the planes
+ * holding such cells, the streamed kernel over those planes, queued after
+ * the interior pass on s (it needs twice the interior kernel's registers,
+ * so running beside it would halve the interior pass's occupancy); else one
+ * thread per listed cell beside the interior pass on the second stream s2
+ * (events e0, e1 order it after the work queued on s, and s after it). */
+#ifndef FX_FACE_PLANES_BESIDE
+#define FX_FACE_PLANES_BESIDE 0
+#endif
+static bool use_face_planes(int nplanes, int nk, int mode) { return nplanes > 0 && (mode == 3 || (mode == 1 && 4 * nplanes <= nk)); }
+
+void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const TdglRhsArgs& a,
+                 const FaceSet& fs, int mode) {
+    const int gx = (a.t.g.n[0] + TDS_TX - 1) / TDS_TX, gy = (a.t.g.n[1] + TDS_TY - 1) / TDS_TY;
+    const int nk = a.k1 - a.k0 + 1;
+    const bool planes = use_face_planes(fs.nplanes, nk, mode);
+    const size_t tab = (size_t)10 * a.t.rot.nrows * sizeof(double);
+    const int rot = !a.t.rot.idx ? 0 : a.t.rot.nrows <= TDS_MAXROWS ? 1 : 2;
+    const bool beside = fs.ncells && (!planes || FX_FACE_PLANES_BESIDE);
+    auto faces = [&](hipStream_t q) {
+        if (planes) {
+            const dim3 gf(gx, gy, fs.nplanes);
+            if (rot == 0) hipLaunchKernelGGL((k_tdgl_stream<0, true>), gf, dim3(TDS_THR), 0, q, a, 1, fs.planes);
+            else if (rot == 1) hipLaunchKernelGGL((k_tdgl_stream<1, true>), gf, dim3(TDS_THR), tab, q, a, 1, fs.planes);
+            else hipLaunchKernelGGL((k_tdgl_stream<2, true>), gf, dim3(TDS_THR), 0, q, a, 1, fs.planes);
+        } else {
+            hipLaunchKernelGGL(k_tdgl_face_cells, dim3((fs.ncells + 255) / 256), dim3(256), 0, q, a);
+        }
+    };
+    if (beside) {
         (void)hipEventRecord(e0, s);
         (void)hipStreamWaitEvent(s2, e0, 0);
-        hipLaunchKernelGGL(k_tdgl_face_cells, dim3(nb), dim3(256), 0, s2, a);
+        faces(s2);
         (void)hipEventRecord(e1, s2);
     }
-    const dim3 g((a.t.g.n[0] + TDS_TX - 1) / TDS_TX, (a.t.g.n[1] + TDS_TY - 1) / TDS_TY,
-                 (a.k1 - a.k0 + TDS_Z) / TDS_Z);
-    if (a.t.rot.idx && a.t.rot.nrows <= TDS_MAXROWS)
-        hipLaunchKernelGGL(k_tdgl_stream<true>, g, dim3(TDS_THR), (size_t)10 * a.t.rot.nrows * sizeof(double), s, a);
-    else
-        hipLaunchKernelGGL(k_tdgl_stream<false>, g, dim3(TDS_THR), 0, s, a);
-    if (nb) (void)hipStreamWaitEvent(s, e1, 0);
+    const int zc = stream_chunk(gx * gy, nk, num_cus(), FX_TDGL_WG_PER_CU);
+    const dim3 g(gx, gy, (nk + zc - 1) / zc);
+    if (rot == 0) hipLaunchKernelGGL((k_tdgl_stream<0, false>), g, dim3(TDS_THR), 0, s, a, zc, nullptr);
+    else if (rot == 1) hipLaunchKernelGGL((k_tdgl_stream<1, false>), g, dim3(TDS_THR), tab, s, a, zc, nullptr);
+    else hipLaunchKernelGGL((k_tdgl_stream<2, false>), g, dim3(TDS_THR), 0, s, a, zc, nullptr);
+    if (beside) (void)hipStreamWaitEvent(s, e1, 0);
+    else if (fs.ncells) faces(s);
 }
 void rhs_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, const RhsArgs& a, int k0, int k1,
-                const int* face, int nface) {
-    const unsigned nb = (unsigned)((nface + 255) / 256);
-    if (nb) {
+                const FaceSet& fs, int mode) {
+    const int gx = (a.g.n[0] + TDS_TX - 1) / TDS_TX, gy = (a.g.n[1] + TDS_TY - 1) / TDS_TY;
+    const int nk = k1 - k0 + 1;
+    const bool planes = use_face_planes(fs.nplanes, nk, mode);
+    const size_t tab = (size_t)3 * a.rot.nrows * sizeof(double);
+    const int rot = !a.rot.idx ? 0 : a.rot.nrows <= 3 * TDS_MAXROWS ? 1 : 2;
+    const bool beside = fs.ncells && (!planes || FX_FACE_PLANES_BESIDE);
+    auto faces = [&](hipStream_t q) {
+        if (planes) {
+            const dim3 gf(gx, gy, fs.nplanes);
+            if (rot == 0) hipLaunchKernelGGL((k_rhs_stream<0, true>), gf, dim3(TDS_THR), 0, q, a, k0, k1, 1, fs.planes);
+            else if (rot == 1) hipLaunchKernelGGL((k_rhs_stream<1, true>), gf, dim3(TDS_THR), tab, q, a, k0, k1, 1, fs.planes);
+            else hipLaunchKernelGGL((k_rhs_stream<2, true>), gf, dim3(TDS_THR), 0, q, a, k0, k1, 1, fs.planes);
+        } else {
+            hipLaunchKernelGGL(k_rhs_face_list, dim3((fs.ncells + 255) / 256), dim3(256), 0, q, a, fs.cells, fs.ncells);
+        }
+    };
+    if (beside) {
         (void)hipEventRecord(e0, s);
         (void)hipStreamWaitEvent(s2, e0, 0);
-        hipLaunchKernelGGL(k_rhs_face_list, dim3(nb), dim3(256), 0, s2, a, face, nface);
+        faces(s2);
         (void)hipEventRecord(e1, s2);
     }
-    const dim3 g((a.g.n[0] + TDS_TX - 1) / TDS_TX, (a.g.n[1] + TDS_TY - 1) / TDS_TY, (k1 - k0 + TDS_Z) / TDS_Z);
-    if (a.rot.idx && a.rot.nrows <= 3 * TDS_MAXROWS)
-        hipLaunchKernelGGL(k_rhs_stream<true>, g, dim3(TDS_THR), (size_t)3 * a.rot.nrows * sizeof(double), s, a, k0, k1);
-    else
-        hipLaunchKernelGGL(k_rhs_stream<false>, g, dim3(TDS_THR), 0, s, a, k0, k1);
-    if (nb) (void)hipStreamWaitEvent(s, e1, 0);
+    const int zc = stream_chunk(gx * gy, nk, num_cus(), FX_RHS_WG_PER_CU);
+    const dim3 g(gx, gy, (nk + zc - 1) / zc);
+    if (rot == 0) hipLaunchKernelGGL((k_rhs_stream<0, false>), g, dim3(TDS_THR), 0, s, a, k0, k1, zc, nullptr);
+    else if (rot == 1) hipLaunchKernelGGL((k_rhs_stream<1, false>), g, dim3(TDS_THR), tab, s, a, k0, k1, zc, nullptr);
+    else hipLaunchKernelGGL((k_rhs_stream<2, false>), g, dim3(TDS_THR), 0, s, a, k0, k1, zc, nullptr);
+    if (beside) (void)hipStreamWaitEvent(s, e1, 0);
+    else if (fs.ncells) faces(s);
 }
 void heun(hipStream_t s, const HeunArgs& a) {
     unsigned b = nblk(a.N, 256); if (b > 4096) b = 4096; if (b == 0) b = 1;

@@ -150,8 +150,12 @@ typedef struct fx_inputs {
                                      also forms the next step's Poisson RHS (one pass over P per step, z-streamed
                                      through LDS); 0 (default, measured faster on MI355X): separate TDGL and RHS
                                      passes */
-    int    tdgl_stream;           /* 1 (default): the scalar-P forward-Euler update streams P and phi through LDS
-                                     (k_tdgl_stream); 0: one thread per cell with gathered operands (k_tdgl) */
+    int    tdgl_stream;           /* 1 (default): the scalar-P forward-Euler update and the Poisson RHS stream
+                                     P and phi through LDS (k_tdgl_stream, k_rhs_stream); the cells next to a
+                                     face beside them on a second stream, streamed over the planes that hold
+                                     them when those are at most a quarter of the FE planes, else by list;
+                                     2: always by list; 3: always over their planes; 0: one thread per cell
+                                     with gathered operands (k_tdgl, k_rhs_newton) */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;

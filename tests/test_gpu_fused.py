@@ -47,7 +47,7 @@ def make(text, poly, **kw):
     g = fx.FerroX(inp)
     f = None
     if poly:
-        f = configs.polycrystal_fields(inp, n_grains=12, seed=3)
+        f = configs.polycrystal_fields(inp, n_grains=600 if poly == "poly_600" else 12, seed=3)
         configs.apply_fields(g, f)
     return inp, g, f
 
@@ -107,11 +107,23 @@ def test_fused_then_host_writes():
     a.close(); b.close()
 
 
-@pytest.mark.parametrize("case", list(CASES))
-def test_streamed_update_equals_gathered(case):
-    """k_tdgl_stream (operands through LDS) against k_tdgl<true> (gathered)."""
-    text, poly = CASES[case]
-    _, a, _ = make(text, poly, tdgl_stream=1)
+STREAM_CASES = dict(CASES, poly_600=(CASES["poly"][0], "poly_600"))
+
+
+@pytest.mark.parametrize("listed", [1, 2, 3])
+@pytest.mark.parametrize("case", list(STREAM_CASES))
+def test_streamed_update_equals_gathered(case, listed):
+    """k_tdgl_stream and k_rhs_stream (operands through LDS) against k_tdgl<true>
+    and k_rhs_newton (gathered): uniform rotation (mfim_64, fe_block), a
+    12-grain rotation table staged in LDS (poly) and a 600-grain one, longer
+    than the LDS copy, read from global memory by the update (poly_600); the
+    cells next to a face by the streamed kernels over the planes holding them
+    (masked stencils on the LDS window; tdgl_stream = 1 when those planes are
+    few, as the FE planes next to the dielectric, and 3 always: fe_block has
+    lateral FE faces with P_BC_flag 2 and 4, whose two-cell stencils reach
+    past the window) or by list (tdgl_stream = 2, and 1 for fe_block)."""
+    text, poly = STREAM_CASES[case]
+    _, a, _ = make(text, poly, tdgl_stream=listed)
     _, b, _ = make(text, poly, tdgl_stream=0)
     a.step(8)
     b.step(8)
