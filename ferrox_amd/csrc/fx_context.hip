@@ -710,8 +710,8 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
     if (!x->direct) return FX_OK;
     const int n0 = d.n0, n1 = d.n1, n2 = d.n2;
     const size_t ntab = 2 * (size_t)n0 + 2 * (size_t)n1;
-    const size_t nck = (size_t)n0 * n1 * ((n2 + 15) / 16);
-    const size_t total = ntab + x->N + nck;
+    const size_t npiv = fxk::fft_pivot_count(n0, n1, n2);
+    const size_t total = ntab + x->N + npiv;
     std::vector<double> tab(ntab);
     double* tw0 = tab.data();
     double* tw1 = tw0 + n0;
@@ -720,7 +720,8 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
     const double pi = 3.14159265358979323846;
     auto fill_dir = [&](int n, double* tw, double* mu) {
         for (int q = 0; q < n / 2; ++q) { tw[2 * q] = std::cos(2.0 * pi * q / n); tw[2 * q + 1] = -std::sin(2.0 * pi * q / n); }
-        for (int m = 0; m < n; ++m) { const double sn = std::sin(pi * m / n); mu[m] = 4.0 * sn * sn; }   /* 2 - 2 cos(2 pi m / n) */
+        /* 2 - 2 cos(2 pi m / n), equal for m and n - m (the pivot table is shared by the two) */
+        for (int m = 0; m < n; ++m) { const double sn = std::sin(pi * std::min(m, n - m) / n); mu[m] = 4.0 * sn * sn; }
     };
     fill_dir(n0, tw0, mux);
     fill_dir(n1, tw1, muy);
@@ -743,10 +744,12 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
         for (int q = 0; q < 3; ++q) P.ih2[q] = d.ih2[q];
         P.dlo = dlo; P.dhi = dhi;
         P.S = base + ntab;
-        P.ck = base + ntab + x->N;
+        P.piv = base + ntab + x->N;
     };
     fill(x->fft64, x->fft_mem64, d.bxk, d.byk, d.bzk);
     if (fp32) fill(x->fft32, x->fft_mem32, x->lev32[0].bxk, x->lev32[0].byk, x->lev32[0].bzk);
+    fxk::fft_pivots(x->s, x->fft64, x->fft_mem64 + ntab + x->N, fp32 ? x->fft_mem32 + ntab + x->N : nullptr);
+    HIPCHK(x, hipStreamSynchronize(x->s));
     /* the inhomogeneous face term of the row operator is (2 w) v, w = beta_face / hz^2 */
     x->fft_w2lo = dlo ? 2.0 * (bz_lo * d.ih2[2]) : 0.0;
     x->fft_w2hi = dhi ? 2.0 * (bz_hi * d.ih2[2]) : 0.0;
@@ -1187,9 +1190,9 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
  * guess's residual, b plus the Dirichlet face terms) replaced by its lower
  * bound max(||b||, |face term| - ||b||), so the test is never looser; without
  * it the stats carry no norms (0). */
-int direct_solve(fx_ctx* x, const double bv[6], double tol, fx_solve_stats* st, bool check) {
+int direct_solve(fx_ctx* x, const double bv[6], double tol, fx_solve_stats* st, bool check, int z0, int z1) {
     const double blo = x->fft_w2lo * bv[2], bhi = x->fft_w2hi * bv[5];
-    fxk::fft_solve(x->s, x->fft64, x->rhs, x->phi, blo, bhi);
+    fxk::fft_solve(x->s, x->fft64, x->rhs, x->phi, blo, bhi, z0, z1);
     st->vcycles = 0; st->mg_resid0 = 0.0; st->mg_bnorm = 0.0; st->mg_resid = 0.0;
     if (!check) return FX_OK;
     fxk::mg_resid_norm(x->s, x->lev[0].d, dbc_of(x, bv), 1, nullptr, x->red, x->red_out);
@@ -1209,10 +1212,10 @@ int direct_solve(fx_ctx* x, const double bv[6], double tol, fx_solve_stats* st, 
  * the FP64 residual meets the stopping rule (with ||r0|| bounded below as in
  * direct_solve), r = b - A phi is formed in FP64, stored in FP32, and
  * phi += A^-1 r in FP32.  st->vcycles counts the FP32 solves. */
-int direct_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+int direct_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, int z0, int z1) {
     const DBC bc = dbc_of(x, bv);
     const double blo = x->fft_w2lo * bv[2], bhi = x->fft_w2hi * bv[5];
-    fxk::fft_solve(x->s, x->fft32, x->rhs, x->phi, blo, bhi);
+    fxk::fft_solve(x->s, x->fft32, x->rhs, x->phi, blo, bhi, z0, z1);
     st->vcycles = 1; st->mg_resid0 = 0.0; st->mg_bnorm = 0.0; st->mg_resid = 0.0;
     for (;;) {
         fxk::resid_to32(x->s, x->lev[0].d, bc, x->lev32[0].b, x->red, x->red_out);
@@ -1229,12 +1232,14 @@ int direct_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_
     }
 }
 
-/* phi_zero: the solve starts from phi = 0, whatever phi holds */
+/* phi_zero: the solve starts from phi = 0, whatever phi holds.  Planes of
+ * the right-hand side outside [z0, z1] hold zeros (z1 < z0: not known). */
 int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false,
-              bool check = false) {
+              bool check = false, int z0 = 0, int z1 = -1) {
     if (x->direct && x->lev[0].d.read_a == 0) {
-        if (x->in.precision == FX_PREC_FP32_REFINE && x->fft_mem32) return direct_solve_mixed(x, bv, tol, maxit, st);
-        return direct_solve(x, bv, tol, st, check || x->in.poisson_direct_check);
+        if (x->in.precision == FX_PREC_FP32_REFINE && x->fft_mem32)
+            return direct_solve_mixed(x, bv, tol, maxit, st, z0, z1);
+        return direct_solve(x, bv, tol, st, check || x->in.poisson_direct_check, z0, z1);
     }
     if (x->in.precision == FX_PREC_FP32_REFINE) {
         /* FP32 copy of the A-coefficient for the correction levels */
@@ -1283,11 +1288,13 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
     while (err > OUTER_TOL) {
         RhsArgs ra0 = rhs_args(x, P);
         if (!x->has_sc && x->alpha_clean) ra0.alpha = nullptr;   /* alpha is 0 outside SC cells: written once */
+        int z0 = 0, z1 = -1;   /* planes of the RHS outside [z0, z1] hold zeros (z1 < z0: not known) */
         /* the RHS of this P was formed by the last TDGL update (no SC cells) */
         if (x->rhs_next && !x->has_sc && x->alpha_clean && P == x->P) {
             std::swap(x->rhs, x->rhs_nx);
             std::swap(x->rhs_clean, x->rhs_nx_clean);
             x->lev[0].d.b = x->rhs;
+            if (x->fe_k0 >= 0) { z0 = x->fe_k0; z1 = x->fe_k1; }
         } else {
             ra0.rhs = x->rhs;
             /* without SC cells the RHS is 0 outside FE cells: once the buffer
@@ -1297,6 +1304,7 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
             if (narrow) {
                 ra0.kz0 = x->fe_k0 >= 0 ? x->fe_k0 : 1;
                 ra0.kz1 = x->fe_k0 >= 0 ? x->fe_k1 : 0;
+                if (x->fe_k0 >= 0) { z0 = x->fe_k0; z1 = x->fe_k1; }
             }
             if (narrow && x->pxy_zero && x->in.tdgl_stream && x->fe_k0 >= 0)
                 fxk::rhs_stream(x->s, x->s2, x->ev0, x->ev1, ra0, x->fe_k0, x->fe_k1, face_set(x), x->in.tdgl_stream);
@@ -1309,7 +1317,7 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
         set_read_a(x, x->has_sc ? 1 : 0);
         if (x->has_sc) restrict_acoef(x);
         fx_solve_stats one{};
-        int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one, !x->in.mg_warm_start);
+        int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one, !x->in.mg_warm_start, false, z0, z1);
         st->vcycles += one.vcycles;
         st->mg_resid0 = one.mg_resid0; st->mg_resid = one.mg_resid; st->mg_bnorm = one.mg_bnorm;
         if (rc) { st->outer_iters = iter + 1; return rc; }
@@ -1973,7 +1981,7 @@ int fx_poisson_solve_noalloc(fx_ctx* x, const double* rhs_dev, double* phi_dev) 
                                              "(needs x, y periodic, z-layered permittivity, a Dirichlet z face, "
                                              "lateral extents powers of two)");
     const FftPlan<double>& P0 = x->fft64;
-    const size_t need = (x->N + (size_t)P0.n0 * P0.n1 * ((P0.n2 + 15) / 16)) * sizeof(double);
+    const size_t need = x->N * sizeof(double);
     if (need > NOALLOC_SCRATCH_BYTES)
         return fail(x, FX_EINVAL, "fx_poisson_solve_noalloc: " + std::to_string(x->N) + " cells need " + std::to_string(need) +
                                       " bytes of workspace, more than the " + std::to_string(NOALLOC_SCRATCH_BYTES) +
@@ -1981,7 +1989,6 @@ int fx_poisson_solve_noalloc(fx_ctx* x, const double* rhs_dev, double* phi_dev) 
     if (!x->noalloc_scratch) return fail(x, FX_EHIP, "fx_poisson_solve_noalloc: workspace address unknown");
     FftPlan<double> P = P0;
     P.S = x->noalloc_scratch;
-    P.ck = x->noalloc_scratch + x->N;
     double bv[6];
     bc_values(x, bv);
     fxk::fft_solve(x->s, P, rhs_dev, phi_dev, x->fft_w2lo * bv[2], x->fft_w2hi * bv[5]);

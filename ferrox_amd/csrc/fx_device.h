@@ -346,7 +346,11 @@ struct FftPlan {
     T ih2[3];
     int dlo, dhi;
     T* S;
-    T* ck;
+    /* reciprocal pivots of the z systems, 1 / (d_k - a_k c'_k-1) of the Thomas
+     * recurrence, per plane k and mode class (min(mx, n0-mx), min(my, n1-my)):
+     * piv[k (n1/2+1) (n0/2+1) + myc (n0/2+1) + mxc] (mu_m = mu_n-m, so the
+     * four mirror modes share one system) */
+    const T* piv;
 };
 /* Plane kernels of the exact solve (one workgroup per plane and block of 16
  * columns): dynamic LDS values, the plane and the y basis with every row

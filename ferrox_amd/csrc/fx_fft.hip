@@ -26,6 +26,10 @@
  *   k_fft_tri              S -> S, Thomas solve in z per mode, scaled by 1/(n0 n1)
  *   k_fft_cols  (inverse)  S -> S
  *   k_fft_rows  (inverse)  S -> phi
+ * The forward transforms skip planes whose right-hand side is known to be 0
+ * (the non-FE planes of a stack without semiconductor cells).  The pivots
+ * of the z systems depend on the operator only; they are tabulated once
+ * (k_fft_pivots), one system per mirror class of modes.
  *
  * The Hartley transforms of two real rows come from one complex FFT of the
  * row pair packed as re + i im (radix-2, in LDS): with Z = FFT(re + i im),
@@ -42,8 +46,11 @@ using fxd::FftPlan;
 
 constexpr int FFT_THR = 128;  /* threads of a transform workgroup: R sequences x max(N1, N2) */
 constexpr int TRI_THR = 256;
-constexpr int TRI_BLK = 16;   /* planes per register block of the tridiagonal solve */
-constexpr int TRI_REG = 64;   /* planes of a column held in registers (the rest in LDS) */
+constexpr int TRI_BLK = 8;    /* planes per load block of the tridiagonal solve */
+#ifndef FX_TRI_REG
+#define FX_TRI_REG 64
+#endif
+constexpr int TRI_REG = FX_TRI_REG;   /* planes of a 128-plane column whose y stays in registers (the rest in LDS) */
 
 template <int N>
 struct Lg { static constexpr int v = N <= 1 ? 0 : 1 + Lg<N / 2>::v; };
@@ -166,10 +173,10 @@ __device__ __forceinline__ void load_tw(T* tw, const T* g, int n) {
  * S.  !FWD: input S, output phi (TOUT), stored or, with add, added. */
 template <class T, class TIN, class TOUT, bool FWD, int N1, int N2>
 __global__ __launch_bounds__(FFT_THR) void k_fft_rows(FftPlan<T> P, const TIN* __restrict__ in, TOUT* __restrict__ out,
-                                                      int R, double blo, double bhi, int add) {
+                                                      int R, double blo, double bhi, int add, int kz0) {
     extern __shared__ __align__(16) unsigned char fft_lds[];
     constexpr int n = N1 * N2;
-    const int k = blockIdx.y, j0 = blockIdx.x * 2 * R;
+    const int k = kz0 + blockIdx.y, j0 = blockIdx.x * 2 * R;
     T* const tw = reinterpret_cast<T*>(fft_lds);
     T* const Yr = tw + n;
     T* const Yi = Yr + R * N2 * (N1 + 1);
@@ -209,14 +216,14 @@ template <> struct Vec2<double> { using t = double2; };
 template <> struct Vec2<float> { using t = float2; };
 
 /* Hartley transform along y, in place, of the column pairs (i0 + 2s,
- * i0 + 2s + 1), s < R, of plane k = blockIdx.y: each row segment of 2R
- * values is read and written whole (one vector of two per thread). */
+ * i0 + 2s + 1), s < R, of plane k = kz0 + blockIdx.y: each row segment of
+ * 2R values is read and written whole (one vector of two per thread). */
 template <class T, int N1, int N2>
-__global__ __launch_bounds__(FFT_THR) void k_fft_cols(FftPlan<T> P, int R) {
+__global__ __launch_bounds__(FFT_THR) void k_fft_cols(FftPlan<T> P, int R, int kz0) {
     extern __shared__ __align__(16) unsigned char fft_lds[];
     using V2 = typename Vec2<T>::t;
     constexpr int n = N1 * N2;
-    const int n0 = P.n0, k = blockIdx.y, i0 = blockIdx.x * 2 * R;
+    const int n0 = P.n0, k = kz0 + blockIdx.y, i0 = blockIdx.x * 2 * R;
     T* const tw = reinterpret_cast<T*>(fft_lds);
     T* const Yr = tw + n;
     T* const Yi = Yr + R * N2 * (N1 + 1);
@@ -240,192 +247,141 @@ __global__ __launch_bounds__(FFT_THR) void k_fft_cols(FftPlan<T> P, int R) {
     }
 }
 
-/* One thread per lateral mode: the tridiagonal system in z (Thomas
- * algorithm), in place on S, the right-hand side scaled by `scale` (the
- * 1/(n0 n1) of the two inverse transforms, exact for powers of two).
- * NZ > 0 (n2 <= NZ): the whole column is loaded into registers at once,
- * eliminated there and stored once; the c' of the recurrence are kept every
- * TRI_BLK planes and recomputed block by block for the back substitution
- * with the same operations.  NZ = 0: any n2, block by block through S, with
- * the checkpoints in P.ck. */
-/* 1/t for the positive pivots of the tridiagonal recurrence: the hardware
- * reciprocal refined by Newton steps e = 1 - t r, r += r e (two for FP64,
- * one for FP32), shorter than the correctly rounded division sequence and
- * accurate to an ulp or two; the forward and the backward sweeps compute
- * the same c' from the same operations. */
-__device__ __forceinline__ double tri_recip(double t) {
-    double r = __builtin_amdgcn_rcp(t);
-    double e = __builtin_fma(-t, r, 1.0);
-    r = __builtin_fma(r, e, r);
-    e = __builtin_fma(-t, r, 1.0);
-    return __builtin_fma(r, e, r);
-}
-__device__ __forceinline__ float tri_recip(float t) {
-    const float r = __builtin_amdgcn_rcpf(t);
-    const float e = __builtin_fmaf(-t, r, 1.0f);
-    return __builtin_fmaf(r, e, r);
-}
-
-/* Row k of a mode's tridiagonal system, a x_k-1 + d x_k + cc x_k+1, from the
- * per-plane table staged in LDS: tab[5k..5k+4] = bx_k, by_k, the z part of
- * the diagonal, the sub- and the super-diagonal. */
+/* Reciprocal pivots of the z systems (FftPlan::piv), once per operator: one
+ * thread per mode class (mxc, myc), the Thomas recurrence's pivots
+ * t_k = d_k - a_k c'_k-1, c'_k = c_k / t_k, with
+ *   d_k = bx_k mu_x / hx^2 + by_k mu_y / hy^2 + (z couplings),
+ *   a_k = -bz_k / hz^2 (0 on the first plane), c_k = -bz_k+1 / hz^2 (0 on the last),
+ * the z couplings 2 bz / hz^2 across a Dirichlet face and 0 across a Neumann
+ * one; formed in FP64 (correctly rounded divisions) from the FP64 plan's
+ * coefficients and stored as T. */
 template <class T>
-struct TriCoef {
-    const T* tab;
-    T lx, ly;
-    __device__ __forceinline__ void operator()(int k, T& a, T& d, T& cc) const {
-        const T* r = tab + 5 * k;
-        T diag = r[0] * lx;
-        diag += r[1] * ly;
-        diag += r[2];
-        a = r[3];
-        cc = r[4];
-        d = diag;
+__global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __restrict__ piv) {
+    const int n0c = P.n0 / 2 + 1, n1c = P.n1 / 2 + 1, n2 = P.n2;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n0c * n1c) return;
+    const int mxc = q % n0c, myc = q / n0c;
+    const double lx = P.mux[mxc] * P.ih2[0], ly = P.muy[myc] * P.ih2[1], iz = P.ih2[2];
+    const size_t ps = (size_t)n0c * n1c;
+    double cp = 0.0;
+    for (int k = 0; k < n2; ++k) {
+        const double wzl = P.bzk[k] * iz, wzh = P.bzk[k + 1] * iz;
+        double d = P.bxk[k] * lx;
+        d += P.byk[k] * ly;
+        d += (k > 0 ? wzl : (P.dlo ? 2.0 * wzl : 0.0)) + (k < n2 - 1 ? wzh : (P.dhi ? 2.0 * wzh : 0.0));
+        const double a = k > 0 ? -wzl : 0.0, c = k < n2 - 1 ? -wzh : 0.0;
+        const double r = 1.0 / (d - a * cp);
+        cp = c * r;
+        piv[k * ps + q] = (T)r;
     }
-};
+}
 
-template <class T, int NZ>
-__global__ __launch_bounds__(TRI_THR, 1) void k_fft_tri(FftPlan<T> P, T scale) {
+/* One thread per lateral mode: the tridiagonal system in z by the Thomas
+ * algorithm with the pivots of the table, in place on S, the right-hand side
+ * scaled by `scale` (the 1/(n0 n1) of the two inverse transforms, exact for
+ * powers of two):
+ *   y_k = (b_k - a_k y_k-1) r_k,    x_k = y_k - c_k r_k x_k+1.
+ * Planes of S outside [z0, z1] were not transformed and count as 0; the
+ * Dirichlet face terms of a skipped first or last plane (flo, fhi, already
+ * scaled) enter mode (0, 0), the only mode of a constant plane.  y of
+ * planes [0, NZ - RB) waits in LDS for the backward sweep, the rest in
+ * registers; S and r stream in blocks of TRI_BLK planes, the next block's
+ * loads in flight while one is eliminated.  Every load is unconditional
+ * (clamped plane, the value selected afterwards).  NZ = 0: any n2, y
+ * written back to S between the sweeps. */
+template <class T, int NZ, int RB>
+__global__ __launch_bounds__(TRI_THR) void k_fft_tri(FftPlan<T> P, T scale, int z0, int z1, T flo, T fhi) {
     extern __shared__ __align__(16) unsigned char tri_lds[];
+    constexpr int LB = NZ > RB ? NZ - RB : 0;
+    constexpr int TB = TRI_BLK;
     const int s = P.n0 * P.n1, n2 = P.n2;
-    const int c = blockIdx.x * TRI_THR + threadIdx.x;
-    constexpr int LBV = NZ > TRI_REG ? NZ - TRI_REG : 0;
-    T* const tab = reinterpret_cast<T*>(tri_lds) + (size_t)LBV * TRI_THR;
+    T* const lv = reinterpret_cast<T*>(tri_lds) + threadIdx.x;                 /* y of plane k: lv[k TRI_THR] */
+    T* const tab = reinterpret_cast<T*>(tri_lds) + (size_t)LB * TRI_THR;         /* a_k, c_k */
     for (int k = threadIdx.x; k < n2; k += TRI_THR) {
         const T iz = P.ih2[2];
-        const T wzl = P.bzk[k] * iz, wzh = P.bzk[k + 1] * iz;
-        tab[5 * k + 0] = P.bxk[k];
-        tab[5 * k + 1] = P.byk[k];
-        tab[5 * k + 2] = (k > 0 ? wzl : (P.dlo ? T(2) * wzl : T(0))) + (k < n2 - 1 ? wzh : (P.dhi ? T(2) * wzh : T(0)));
-        tab[5 * k + 3] = k > 0 ? -wzl : T(0);
-        tab[5 * k + 4] = k < n2 - 1 ? -wzh : T(0);
+        tab[2 * k] = k > 0 ? -(P.bzk[k] * iz) : T(0);
+        tab[2 * k + 1] = k < n2 - 1 ? -(P.bzk[k + 1] * iz) : T(0);
     }
     __syncthreads();
+    const int c = blockIdx.x * TRI_THR + threadIdx.x;
     if (c >= s) return;
-    TriCoef<T> coef{tab, P.mux[c & (P.n0 - 1)] * P.ih2[0], P.muy[c >> P.l0] * P.ih2[1]};
+    const int h0 = P.n0 >> 1, h1 = P.n1 >> 1;
+    const int mx = c & (P.n0 - 1), my = c >> P.l0;
+    const int mxc = mx <= h0 ? mx : P.n0 - mx, myc = my <= h1 ? my : P.n1 - my;
+    const size_t ps = (size_t)(h0 + 1) * (h1 + 1);
+    const T* __restrict__ rp = P.piv + (size_t)myc * (h0 + 1) + mxc;
     T* const col = P.S + c;
+    const int zlo = z0 <= z1 ? z0 : 0, zhi = z0 <= z1 ? z1 : 0;
+    const bool any = z0 <= z1;
+    /* right-hand side of plane k from the loaded value v */
+    auto rhs = [&](int k, T v) -> T {
+        T b = (any && k >= z0 && k <= z1) ? v * scale : T(0);
+        if (c == 0) {
+            if (k == 0) b += flo;
+            if (k == n2 - 1) b += fhi;
+        }
+        return b;
+    };
     if constexpr (NZ > 0) {
-        /* planes [0, LB) go through LDS (one value per thread and plane),
-         * planes [LB, NZ) stay in registers */
-        constexpr int RB = NZ < TRI_REG ? NZ : TRI_REG, LB = NZ - RB;
-        constexpr int NB = (NZ + TRI_BLK - 1) / TRI_BLK;
-        T* const lv = reinterpret_cast<T*>(tri_lds) + threadIdx.x;
-        T v[RB], ck[NB];
+        constexpr int NB = NZ / TB;
+        T sb[2][TB], rb[2][TB], yr[RB > 0 ? RB : 1];
+        auto load = [&](int b, T (&sv)[TB], T (&rv)[TB]) {
 #pragma unroll
-        for (int q = 0; q < RB; ++q)
-            if (LB + q < n2) v[q] = col[(size_t)(LB + q) * s];
-        T cprev = T(0), dprev = T(0);
-        auto elim = [&](int k, T x) -> T {
-            T a, d, cc;
-            coef(k, a, d, cc);
-            const T r = tri_recip(d - a * cprev);
-            cprev = cc * r;
-            dprev = (x * scale - a * dprev) * r;
-            return dprev;
+            for (int q = 0; q < TB; ++q) {
+                const int k = b * TB + q;
+                const int kc = k < zlo ? zlo : k > zhi ? zhi : k;
+                sv[q] = col[(size_t)kc * s];
+                rv[q] = rp[(size_t)(k < n2 ? k : n2 - 1) * ps];
+            }
         };
+        T y = T(0);
+        load(0, sb[0], rb[0]);
 #pragma unroll
-        for (int b0 = 0; b0 < LB; b0 += TRI_BLK) {
-            T w[TRI_BLK];
+        for (int b = 0; b < NB; ++b) {
+            if (b + 1 < NB) load(b + 1, sb[(b + 1) & 1], rb[(b + 1) & 1]);
 #pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q)
-                if (b0 + q < n2) w[q] = col[(size_t)(b0 + q) * s];
-            ck[b0 / TRI_BLK] = cprev;
-#pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q)
-                if (b0 + q < n2) lv[(b0 + q) * TRI_THR] = elim(b0 + q, w[q]);
+            for (int q = 0; q < TB; ++q) {
+                const int k = b * TB + q;
+                if (k < n2) {
+                    y = (rhs(k, sb[b & 1][q]) - tab[2 * k] * y) * rb[b & 1][q];
+                    if (k < LB) lv[(size_t)k * TRI_THR] = y;
+                    else yr[k - LB] = y;
+                }
+            }
         }
+        auto loadr = [&](int b, T (&rv)[TB]) {
 #pragma unroll
-        for (int q = 0; q < RB; ++q) {
-            const int k = LB + q;
-            if (k % TRI_BLK == 0) ck[k / TRI_BLK] = cprev;
-            if (k < n2) v[q] = elim(k, v[q]);
-        }
+            for (int q = 0; q < TB; ++q) {
+                const int k = b * TB + q;
+                rv[q] = rp[(size_t)(k < n2 ? k : n2 - 1) * ps];
+            }
+        };
         T xn = T(0);
+        loadr(NB - 1, rb[(NB - 1) & 1]);
 #pragma unroll
         for (int b = NB - 1; b >= 0; --b) {
-            T cv[TRI_BLK];
-            T cp = ck[b];
+            if (b > 0) loadr(b - 1, rb[(b - 1) & 1]);
 #pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q) {
-                const int k = b * TRI_BLK + q;
+            for (int q = TB - 1; q >= 0; --q) {
+                const int k = b * TB + q;
                 if (k < n2) {
-                    T a, d, cc;
-                    coef(k, a, d, cc);
-                    const T r = tri_recip(d - a * cp);
-                    cp = cc * r;
-                    cv[q] = cp;
-                }
-            }
-#pragma unroll
-            for (int q = TRI_BLK - 1; q >= 0; --q) {
-                const int k = b * TRI_BLK + q;
-                if (k < n2) {
-                    if (k >= LB) {
-                        xn = v[k - LB] - cv[q] * xn;
-                        v[k - LB] = xn;
-                    } else {
-                        xn = lv[k * TRI_THR] - cv[q] * xn;
-                        col[(size_t)k * s] = xn;
-                    }
+                    const T yk = k < LB ? lv[(size_t)k * TRI_THR] : yr[k - LB];
+                    xn = yk - (tab[2 * k + 1] * rb[b & 1][q]) * xn;
+                    col[(size_t)k * s] = xn;
                 }
             }
         }
-#pragma unroll
-        for (int q = 0; q < RB; ++q)
-            if (LB + q < n2) col[(size_t)(LB + q) * s] = v[q];
     } else {
-        const int nb = (n2 + TRI_BLK - 1) / TRI_BLK;
-        T cprev = T(0), dprev = T(0);
-        for (int b = 0; b < nb; ++b) {
-            const int k0 = b * TRI_BLK;
-            P.ck[(size_t)b * s + c] = cprev;
-            T v[TRI_BLK];
-#pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q)
-                if (k0 + q < n2) v[q] = col[(size_t)(k0 + q) * s];
-#pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q) {
-                if (k0 + q < n2) {
-                    T a, d, cc;
-                    coef(k0 + q, a, d, cc);
-                    const T r = tri_recip(d - a * cprev);
-                    cprev = cc * r;
-                    dprev = (v[q] * scale - a * dprev) * r;
-                    v[q] = dprev;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q)
-                if (k0 + q < n2) col[(size_t)(k0 + q) * s] = v[q];
+        T y = T(0);
+        for (int k = 0; k < n2; ++k) {
+            const int kc = k < zlo ? zlo : k > zhi ? zhi : k;
+            y = (rhs(k, col[(size_t)kc * s]) - tab[2 * k] * y) * rp[(size_t)k * ps];
+            col[(size_t)k * s] = y;
         }
         T xn = T(0);
-        for (int b = nb - 1; b >= 0; --b) {
-            const int k0 = b * TRI_BLK;
-            T cp = P.ck[(size_t)b * s + c];
-            T cv[TRI_BLK], v[TRI_BLK];
-#pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q)
-                if (k0 + q < n2) v[q] = col[(size_t)(k0 + q) * s];
-#pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q) {
-                if (k0 + q < n2) {
-                    T a, d, cc;
-                    coef(k0 + q, a, d, cc);
-                    const T r = tri_recip(d - a * cp);
-                    cp = cc * r;
-                    cv[q] = cp;
-                }
-            }
-#pragma unroll
-            for (int q = TRI_BLK - 1; q >= 0; --q) {
-                if (k0 + q < n2) {
-                    xn = v[q] - cv[q] * xn;
-                    v[q] = xn;
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < TRI_BLK; ++q)
-                if (k0 + q < n2) col[(size_t)(k0 + q) * s] = v[q];
+        for (int k = n2 - 1; k >= 0; --k) {
+            xn = col[(size_t)k * s] - (tab[2 * k + 1] * rp[(size_t)k * ps]) * xn;
+            col[(size_t)k * s] = xn;
         }
     }
 }
@@ -444,86 +400,98 @@ size_t fft_lds_bytes(int N1, int N2, int R) {
 }
 
 template <class T, class TIN, class TOUT, int N1, int N2>
-void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add) {
+void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add,
+                 int kz0, int nk) {
     const int R = seqs_per_block(N1, N2, P.n1);
     const size_t lds = fft_lds_bytes<T>(N1, N2, R);
-    const dim3 g(P.n1 / (2 * R), P.n2);
+    const dim3 g(P.n1 / (2 * R), nk);
     if (fwd)
         hipLaunchKernelGGL((k_fft_rows<T, TIN, T, true, N1, N2>), g, dim3(FFT_THR), lds, s, P,
-                           static_cast<const TIN*>(in), static_cast<T*>(out), R, blo, bhi, 0);
+                           static_cast<const TIN*>(in), static_cast<T*>(out), R, blo, bhi, 0, kz0);
     else
         hipLaunchKernelGGL((k_fft_rows<T, T, TOUT, false, N1, N2>), g, dim3(FFT_THR), lds, s, P,
-                           static_cast<const T*>(in), static_cast<TOUT*>(out), R, 0.0, 0.0, add);
+                           static_cast<const T*>(in), static_cast<TOUT*>(out), R, 0.0, 0.0, add, kz0);
 }
 
 template <class T, int N1, int N2>
-void cols_launch(hipStream_t s, const FftPlan<T>& P) {
+void cols_launch(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
     const int R = seqs_per_block(N1, N2, P.n0);
     const size_t lds = fft_lds_bytes<T>(N1, N2, R);
-    hipLaunchKernelGGL((k_fft_cols<T, N1, N2>), dim3(P.n0 / (2 * R), P.n2), dim3(FFT_THR), lds, s, P, R);
+    hipLaunchKernelGGL((k_fft_cols<T, N1, N2>), dim3(P.n0 / (2 * R), nk), dim3(FFT_THR), lds, s, P, R, kz0);
 }
 
-/* n = N1 N2 with N1 >= N2, both powers of two up to 32 */
+/* n = N1 N2 with N1 >= N2, both powers of two up to 32; planes kz0 .. kz0 + nk - 1 */
 template <class T, class TIN, class TOUT>
-void rows(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add) {
+void rows(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add,
+          int kz0, int nk) {
+    if (nk <= 0) return;
     switch (P.l0) {
-    case 2: rows_launch<T, TIN, TOUT, 2, 2>(s, P, fwd, in, out, blo, bhi, add); break;
-    case 3: rows_launch<T, TIN, TOUT, 4, 2>(s, P, fwd, in, out, blo, bhi, add); break;
-    case 4: rows_launch<T, TIN, TOUT, 4, 4>(s, P, fwd, in, out, blo, bhi, add); break;
-    case 5: rows_launch<T, TIN, TOUT, 8, 4>(s, P, fwd, in, out, blo, bhi, add); break;
-    case 6: rows_launch<T, TIN, TOUT, 8, 8>(s, P, fwd, in, out, blo, bhi, add); break;
-    case 7: rows_launch<T, TIN, TOUT, 16, 8>(s, P, fwd, in, out, blo, bhi, add); break;
-    case 8: rows_launch<T, TIN, TOUT, 16, 16>(s, P, fwd, in, out, blo, bhi, add); break;
-    case 9: rows_launch<T, TIN, TOUT, 32, 16>(s, P, fwd, in, out, blo, bhi, add); break;
-    default: rows_launch<T, TIN, TOUT, 32, 32>(s, P, fwd, in, out, blo, bhi, add); break;
+    case 2: rows_launch<T, TIN, TOUT, 2, 2>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 3: rows_launch<T, TIN, TOUT, 4, 2>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 4: rows_launch<T, TIN, TOUT, 4, 4>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 5: rows_launch<T, TIN, TOUT, 8, 4>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 6: rows_launch<T, TIN, TOUT, 8, 8>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 7: rows_launch<T, TIN, TOUT, 16, 8>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 8: rows_launch<T, TIN, TOUT, 16, 16>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 9: rows_launch<T, TIN, TOUT, 32, 16>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    default: rows_launch<T, TIN, TOUT, 32, 32>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
     }
 }
 
 template <class T>
-void cols(hipStream_t s, const FftPlan<T>& P) {
+void cols(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
+    if (nk <= 0) return;
     switch (P.l1) {
-    case 2: cols_launch<T, 2, 2>(s, P); break;
-    case 3: cols_launch<T, 4, 2>(s, P); break;
-    case 4: cols_launch<T, 4, 4>(s, P); break;
-    case 5: cols_launch<T, 8, 4>(s, P); break;
-    case 6: cols_launch<T, 8, 8>(s, P); break;
-    case 7: cols_launch<T, 16, 8>(s, P); break;
-    case 8: cols_launch<T, 16, 16>(s, P); break;
-    case 9: cols_launch<T, 32, 16>(s, P); break;
-    default: cols_launch<T, 32, 32>(s, P); break;
+    case 2: cols_launch<T, 2, 2>(s, P, kz0, nk); break;
+    case 3: cols_launch<T, 4, 2>(s, P, kz0, nk); break;
+    case 4: cols_launch<T, 4, 4>(s, P, kz0, nk); break;
+    case 5: cols_launch<T, 8, 4>(s, P, kz0, nk); break;
+    case 6: cols_launch<T, 8, 8>(s, P, kz0, nk); break;
+    case 7: cols_launch<T, 16, 8>(s, P, kz0, nk); break;
+    case 8: cols_launch<T, 16, 16>(s, P, kz0, nk); break;
+    case 9: cols_launch<T, 32, 16>(s, P, kz0, nk); break;
+    default: cols_launch<T, 32, 32>(s, P, kz0, nk); break;
     }
 }
 
-template <class T, int NZ>
-void tri_launch(hipStream_t s, const FftPlan<T>& P, T scale) {
-    constexpr int LB = NZ > TRI_REG ? NZ - TRI_REG : 0;
-    const size_t lds = ((size_t)LB * TRI_THR + 5 * (size_t)P.n2) * sizeof(T);
+template <class T, int NZ, int RB>
+void tri_launch(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi) {
+    constexpr int LB = NZ > RB ? NZ - RB : 0;
+    const size_t lds = ((size_t)LB * TRI_THR + 2 * (size_t)P.n2) * sizeof(T);
     static bool attr = false;   /* the LDS limit is raised once per kernel */
     if (lds > 64 * 1024 && !attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_tri<T, NZ>), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_tri<T, NZ, RB>),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((k_fft_tri<T, NZ>), dim3((P.n0 * P.n1 + TRI_THR - 1) / TRI_THR), dim3(TRI_THR), lds, s, P, scale);
+    hipLaunchKernelGGL((k_fft_tri<T, NZ, RB>), dim3((P.n0 * P.n1 + TRI_THR - 1) / TRI_THR), dim3(TRI_THR), lds, s, P,
+                       scale, z0, z1, flo, fhi);
 }
 
 template <class T>
-void tri(hipStream_t s, const FftPlan<T>& P, T scale) {
-    if (P.n2 <= 16) tri_launch<T, 16>(s, P, scale);
-    else if (P.n2 <= 32) tri_launch<T, 32>(s, P, scale);
-    else if (P.n2 <= 64) tri_launch<T, 64>(s, P, scale);
-    else if (P.n2 <= 128) tri_launch<T, 128>(s, P, scale);
-    else tri_launch<T, 0>(s, P, scale);
+void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi) {
+    if (P.n2 <= 16) tri_launch<T, 16, 16>(s, P, scale, z0, z1, flo, fhi);
+    else if (P.n2 <= 32) tri_launch<T, 32, 32>(s, P, scale, z0, z1, flo, fhi);
+    else if (P.n2 <= 64) tri_launch<T, 64, 64>(s, P, scale, z0, z1, flo, fhi);
+    else if (P.n2 <= 128) tri_launch<T, 128, TRI_REG>(s, P, scale, z0, z1, flo, fhi);
+    else tri_launch<T, 0, 0>(s, P, scale, z0, z1, flo, fhi);
 }
 
+/* planes of b outside [z0, z1] hold zeros: their forward transforms are
+ * skipped (z1 < z0: none is, every plane is transformed) */
 template <class T, class TIN, class TOUT>
-void fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi, double blo, double bhi, int add) {
+void fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi, double blo, double bhi, int add,
+                    int z0, int z1) {
     const T scale = T(1) / T(P.n0 * P.n1);
-    rows<T, TIN, TOUT>(s, P, true, b, P.S, blo, bhi, 0);
-    cols<T>(s, P);
-    tri<T>(s, P, scale);
-    cols<T>(s, P);
-    rows<T, TIN, TOUT>(s, P, false, P.S, phi, 0.0, 0.0, add);
+    if (z1 < z0) { z0 = 0; z1 = P.n2 - 1; }
+    const int nk = z1 - z0 + 1;
+    /* the Dirichlet face terms of a skipped face plane go to the z solve */
+    const T flo = z0 > 0 ? T(blo) : T(0), fhi = z1 < P.n2 - 1 ? T(bhi) : T(0);
+    rows<T, TIN, TOUT>(s, P, true, b, P.S, blo, bhi, 0, z0, nk);
+    cols<T>(s, P, z0, nk);
+    tri<T>(s, P, scale, z0, z1, flo, fhi);
+    cols<T>(s, P, 0, P.n2);
+    rows<T, TIN, TOUT>(s, P, false, P.S, phi, 0.0, 0.0, add, 0, P.n2);
 }
 
 }  // namespace
@@ -532,23 +500,33 @@ namespace fxk {
 
 bool fft_size_ok(int n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; }
 
-void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi) {
-    fft_solve_impl<double, double, double>(s, P, b, phi, blo, bhi, 0);
+size_t fft_pivot_count(int n0, int n1, int n2) { return (size_t)(n0 / 2 + 1) * (n1 / 2 + 1) * n2; }
+
+void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32) {
+    const int n = (P.n0 / 2 + 1) * (P.n1 / 2 + 1);
+    hipLaunchKernelGGL(k_fft_pivots<double>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv64);
+    if (piv32) hipLaunchKernelGGL(k_fft_pivots<float>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv32);
 }
-void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi) {
-    fft_solve_impl<float, double, double>(s, P, b, phi, blo, bhi, 0);
+
+void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi, int z0,
+               int z1) {
+    fft_solve_impl<double, double, double>(s, P, b, phi, blo, bhi, 0, z0, z1);
+}
+void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi, int z0,
+               int z1) {
+    fft_solve_impl<float, double, double>(s, P, b, phi, blo, bhi, 0, z0, z1);
 }
 void fft_solve_add(hipStream_t s, const fxd::FftPlan<float>& P, const float* r, double* phi) {
-    fft_solve_impl<float, float, double>(s, P, r, phi, 0.0, 0.0, 1);
+    fft_solve_impl<float, float, double>(s, P, r, phi, 0.0, 0.0, 1, 0, -1);
 }
 
 void fft_pass(hipStream_t s, const fxd::FftPlan<double>& P, int pass, const double* b, double* phi, double blo,
               double bhi) {
     switch (pass) {
-    case 0: rows<double, double, double>(s, P, true, b, P.S, blo, bhi, 0); break;
-    case 1: cols<double>(s, P); break;
-    case 2: tri<double>(s, P, 1.0 / (double)(P.n0 * P.n1)); break;
-    default: rows<double, double, double>(s, P, false, P.S, phi, 0.0, 0.0, 0); break;
+    case 0: rows<double, double, double>(s, P, true, b, P.S, blo, bhi, 0, 0, P.n2); break;
+    case 1: cols<double>(s, P, 0, P.n2); break;
+    case 2: tri<double>(s, P, 1.0 / (double)(P.n0 * P.n1), 0, P.n2 - 1, 0.0, 0.0); break;
+    default: rows<double, double, double>(s, P, false, P.S, phi, 0.0, 0.0, 0, 0, P.n2); break;
     }
 }
 

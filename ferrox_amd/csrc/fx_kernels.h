@@ -98,8 +98,15 @@ void fill(hipStream_t s, double* p, long long n, double v);
  * plane); the FP32 plan transforms in FP32 and stores phi in FP64;
  * fft_solve_add: phi += A^-1 r (homogeneous faces) */
 bool fft_size_ok(int n);
-void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi);
-void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi);
+/* values of the pivot table (FftPlan::piv) and its set-up from the FP64
+ * plan (piv32 may be null) */
+size_t fft_pivot_count(int n0, int n1, int n2);
+void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32);
+/* planes of b outside [z0, z1] hold zeros (z1 < z0: nothing is known) */
+void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi,
+               int z0 = 0, int z1 = -1);
+void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi,
+               int z0 = 0, int z1 = -1);
 void fft_solve_add(hipStream_t s, const fxd::FftPlan<float>& P, const float* r, double* phi);
 /* one pass of the FP64 direct solve: 0 forward x transform (b -> S), 1 y
  * transform of S in place (forward and inverse are the same), 2 tridiagonal

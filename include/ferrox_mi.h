@@ -297,10 +297,10 @@ int fx_poisson_solve(fx_ctx* ctx, const double* rhs, const double* acoef,
  * the context's stream (fx_stream) with no host wait and no allocation.
  * Only where fx_poisson_solve takes the direct path (x and y periodic,
  * permittivity depending on z only, a Dirichlet z face, lateral extents
- * powers of two, no Helmholtz term).  The transform workspace,
- * N + n0 n1 ceil(n2/16) doubles, lives in a static device array of 64 MiB
- * per GPU (about 7.9M cells at most), shared by every call on that GPU and
- * left holding the last call's values: calls that overlap on one GPU race.
+ * powers of two, no Helmholtz term).  The transform workspace, N doubles,
+ * lives in a static device array of 64 MiB per GPU (8388608 cells at most,
+ * e.g. 256 x 256 x 128), shared by every call on that GPU and left holding
+ * the last call's values: calls that overlap on one GPU race.
  * FX_EINVAL off the direct path or beyond the limit. */
 int fx_poisson_solve_noalloc(fx_ctx* ctx, const double* rhs_dev, double* phi_dev);
 

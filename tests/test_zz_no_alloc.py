@@ -8,6 +8,11 @@ import ctypes as C
 
 import numpy as np
 import pytest
+# torch first: its wheel carries its own HIP runtime under the system
+# runtime's soname (libamdhip64.so.7), and whichever is loaded first serves
+# the whole process; torch finds no GPU on the system runtime, the library
+# runs on either
+import torch  # noqa: F401
 
 import ferrox_amd as fx
 from ferrox_amd import configs
@@ -18,7 +23,6 @@ FX_EINVAL = 1
 
 
 def solve_both(text, seed):
-    import torch
     inp = fx.from_text(text, mg_tol_rel=1e-13)
     g = fx.FerroX(inp)
     lib = g._lib
@@ -47,7 +51,6 @@ def test_noalloc_matches_poisson_solve_two_calls():
 
 
 def test_noalloc_rejects_what_it_cannot_do():
-    import torch
     buf = torch.zeros(8, dtype=torch.float64, device="cuda")
     p = C.c_void_p(buf.data_ptr())
     # off the direct path: a lateral extent that is not a power of two
@@ -55,8 +58,10 @@ def test_noalloc_rejects_what_it_cannot_do():
     assert g._lib.fx_poisson_solve_noalloc(g._h, p, p) == FX_EINVAL
     print(g._lib.fx_last_error(g._h), flush=True)
     g.close()
-    # beyond the 64 MiB workspace: 256 x 256 x 128 cells need about 71 MB
-    g = fx.FerroX(fx.from_text(configs.mfim_256(nsteps=1)))
+    # beyond the 64 MiB workspace (N doubles, at most 256 x 256 x 128 cells):
+    # 256 x 256 x 144 cells need 75.5 MB
+    g = fx.FerroX(fx.from_text(configs.mfim(nx=256, ny=256, nz=144, lx=128e-9, ly=128e-9, lz=14.4e-9, de_top=4e-9,
+                                            fe_top=12e-9, nsteps=1)))
     assert g._lib.fx_poisson_solve_noalloc(g._h, p, p) == FX_EINVAL
     msg = g._lib.fx_last_error(g._h)
     print(msg, flush=True)
