@@ -266,7 +266,10 @@ def main():
     steps_per_s = a.steps / secs
     n_v = st["total_vcycles"] / max(1, st["steps_done"])
     n_outer = st["total_outer_iters"] / max(1, st["steps_done"])
-    N_FE = int((g.get("mask") == 0).sum())
+    mask = g.get("mask")
+    N_FE = int((mask == 0).sum())
+    fe_planes = np.flatnonzero((mask == 0).any(axis=(1, 2)))   # mask is (nz, ny, nx)
+    k0, k1 = int(fe_planes[0]), int(fe_planes[-1])
     print(f"# {a.steps} steps in {secs:.4f} s, {n_v:.2f} V-cycles/step, {n_outer:.2f} outer/step, {N_FE} FE cells",
           file=sys.stderr, flush=True)
 
@@ -274,11 +277,20 @@ def main():
     # grain-table index): the RHS pass reads P, the material id and the grain
     # index and writes the RHS of each FE cell (21 B); the TDGL update reads
     # P, phi, the material id and the grain index and writes P (29 B per FE
-    # cell); the direct solve makes five passes that each read and write the
-    # field once (x, y, tridiagonal z, y, x: 80 B per cell); V-cycles (154 B
-    # per cell) where the multigrid runs instead
+    # cell).  The direct solve: the forward x and y transforms read and write
+    # the planes with a nonzero right-hand side (the FE planes: 16 B per cell
+    # each), the z solve reads those, writes every plane and reads the pivot
+    # table once ((n0/2+1)(n1/2+1) n2 doubles), the inverse y and x
+    # transforms read and write the planes the update reads (the FE planes
+    # and one either side: 16 B per cell each).  V-cycles (154 B per cell)
+    # where the multigrid runs instead.
     direct = n_v == 0
-    B_step = N_FE * (21 + 29) + N * ((80.0 if direct else 0.0) + n_v * 154 + (n_outer - 1) * 58)
+    plane = nx * ny
+    n_in = (k1 - k0 + 1) * plane
+    n_out = (min(k1 + 1, nz - 1) - max(k0 - 1, 0) + 1) * plane
+    n_piv = (nx // 2 + 1) * (ny // 2 + 1) * nz
+    B_direct = 32.0 * n_in + 8.0 * n_in + 8.0 * N + 8.0 * n_piv + 32.0 * n_out
+    B_step = N_FE * (21 + 29) + (B_direct if direct else 0.0) + N * (n_v * 154 + (n_outer - 1) * 58)
 
     roof = None
     kernels = None
@@ -287,13 +299,15 @@ def main():
         kernels = {}
         # (name, algorithmic bytes per launch, description)
         table = [("smooth0", 24.0 * N, "k_mg_gsrb_stream: one red+black sweep of the finest multigrid level")] if not direct else []
-        table += [("tdgl", 29.0 * N_FE, "TDGL update: k_tdgl_stream (interior FE cells) + k_tdgl_face_cells (second stream)"),
-                 ("rhs", 21.0 * N_FE, "Poisson RHS: k_rhs_stream + k_rhs_face_list (second stream)"),
+        table += [("tdgl", 29.0 * N_FE, "TDGL update: k_tdgl_stream over the interior FE cells, then over the planes "
+                                        "holding FE cells next to a face"),
+                 ("rhs", 21.0 * N_FE, "Poisson RHS: k_rhs_stream over the interior FE cells, then over the planes "
+                                      "holding FE cells next to a face"),
                  ("fft_x", 16.0 * N, "k_fft_rows forward (Hartley along x)"),
                  ("fft_y", 16.0 * N, "k_fft_cols (Hartley along y, in place)"),
-                 ("fft_tri", 16.0 * N, "k_fft_tri (Thomas solve in z per mode, column in registers/LDS)"),
+                 ("fft_tri", 16.0 * N + 8.0 * n_piv, "k_fft_tri (Thomas solve in z per mode with tabulated pivots)"),
                  ("fft_xinv", 16.0 * N, "k_fft_rows inverse"),
-                 ("fft_solve", 80.0 * N, "the whole direct solve (five launches)"),
+                 ("fft_solve", 80.0 * N + 8.0 * n_piv, "the whole direct solve on every plane (five launches)"),
                  ("copy", 16.0 * N, "plain copy (device bandwidth reference)")]
         if not direct:
             table = [t for t in table if not t[0].startswith("fft")]
@@ -378,6 +392,8 @@ def main():
                    "vcycles_per_step": round(n_v, 3), "outer_iters_per_step": round(n_outer, 3)},
         "cell_updates_per_s": round(N * steps_per_s, 1),
         "step_bytes_algorithmic": B_step,
+        "direct_solve_planes": {"rhs_nonzero": [k0, k1], "phi_formed": [max(k0 - 1, 0), min(k1 + 1, nz - 1)],
+                                "of": nz} if direct else None,
         "step_hbm_GBs_algorithmic": round(B_step * steps_per_s / 1e9, 1),
         "step_roofline_frac": round(B_step * steps_per_s / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roof,

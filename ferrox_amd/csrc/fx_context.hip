@@ -101,6 +101,11 @@ struct fx_ctx {
     int nface = 0;
     int nface_planes = 0;
     double* noalloc_scratch = nullptr;   /* device address of the static workspace on this context's GPU */
+    /* phi_z0 <= phi_z1: the last direct solve formed phi on those planes
+     * only (what the TDGL update reads); the other planes' solution waits,
+     * transformed, in fft64.S until phi_sync() finishes it */
+    int phi_z0 = 0, phi_z1 = -1;
+    bool phi_exposed = false;      /* a device pointer to phi was handed out: always form all of it */
     /* the fused update writes the next RHS into rhs_nx, which the next solve
      * swaps in (rhs keeps the RHS of the last solve, the PoissonRHS the
      * reference writes at plot time); the fused kernel writes only planes
@@ -903,6 +908,14 @@ RotArgs rot_of(const fx_ctx* x) {
     std::memcpy(r.Ru, x->Ru, sizeof(r.Ru));
     return r;
 }
+/* Finish the planes of phi a partial direct solve left in its workspace. */
+void phi_sync(fx_ctx* x) {
+    if (x->phi_z0 > x->phi_z1) return;
+    fxk::fft_finish(x->s, x->fft64, x->phi, x->phi_z0, x->phi_z1);
+    x->phi_z0 = 0;
+    x->phi_z1 = -1;
+}
+
 PhiView phiview_of(const fx_ctx* x) {
     PhiView p;
     p.p = x->phi;
@@ -1190,9 +1203,13 @@ int mg_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solv
  * guess's residual, b plus the Dirichlet face terms) replaced by its lower
  * bound max(||b||, |face term| - ||b||), so the test is never looser; without
  * it the stats carry no norms (0). */
-int direct_solve(fx_ctx* x, const double bv[6], double tol, fx_solve_stats* st, bool check, int z0, int z1) {
+int direct_solve(fx_ctx* x, const double bv[6], double tol, fx_solve_stats* st, bool check, int z0, int z1, int o0,
+                 int o1) {
     const double blo = x->fft_w2lo * bv[2], bhi = x->fft_w2hi * bv[5];
-    fxk::fft_solve(x->s, x->fft64, x->rhs, x->phi, blo, bhi, z0, z1);
+    if (check || x->phi_exposed) o1 = o0 - 1;   /* the residual (or a caller) reads all of phi */
+    fxk::fft_solve(x->s, x->fft64, x->rhs, x->phi, blo, bhi, z0, z1, o0, o1);
+    x->phi_z0 = o0 <= o1 ? o0 : 0;
+    x->phi_z1 = o0 <= o1 ? o1 : -1;
     st->vcycles = 0; st->mg_resid0 = 0.0; st->mg_bnorm = 0.0; st->mg_resid = 0.0;
     if (!check) return FX_OK;
     fxk::mg_resid_norm(x->s, x->lev[0].d, dbc_of(x, bv), 1, nullptr, x->red, x->red_out);
@@ -1233,14 +1250,19 @@ int direct_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_
 }
 
 /* phi_zero: the solve starts from phi = 0, whatever phi holds.  Planes of
- * the right-hand side outside [z0, z1] hold zeros (z1 < z0: not known). */
+ * the right-hand side outside [z0, z1] hold zeros (z1 < z0: not known).
+ * The FP64 direct solve may form phi on planes [o0, o1] only (o1 < o0:
+ * all); every other solve forms all of it. */
 int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st, bool phi_zero = false,
-              bool check = false, int z0 = 0, int z1 = -1) {
+              bool check = false, int z0 = 0, int z1 = -1, int o0 = 0, int o1 = -1) {
     if (x->direct && x->lev[0].d.read_a == 0) {
-        if (x->in.precision == FX_PREC_FP32_REFINE && x->fft_mem32)
+        if (x->in.precision == FX_PREC_FP32_REFINE && x->fft_mem32) {
+            phi_sync(x);
             return direct_solve_mixed(x, bv, tol, maxit, st, z0, z1);
-        return direct_solve(x, bv, tol, st, check || x->in.poisson_direct_check, z0, z1);
+        }
+        return direct_solve(x, bv, tol, st, check || x->in.poisson_direct_check, z0, z1, o0, o1);
     }
+    phi_sync(x);
     if (x->in.precision == FX_PREC_FP32_REFINE) {
         /* FP32 copy of the A-coefficient for the correction levels */
         if (x->lev32[0].read_a) {
@@ -1277,8 +1299,11 @@ fxk::FaceSet face_set(const fx_ctx* x) {
     return f;
 }
 
-/* ComputePhi_Rho (ElectrostaticSolver.cpp:678-747). */
-int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
+/* ComputePhi_Rho (ElectrostaticSolver.cpp:678-747).  partial_ok: only the
+ * forward-Euler update reads phi before the next solve, so a direct solve
+ * may form it on the FE planes and their neighbours only (phi_sync
+ * finishes it for every other reader). */
+int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st, bool partial_ok = false) {
     double err = 1.0;
     int iter = 0;
     double bv[6];
@@ -1317,7 +1342,12 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st) {
         set_read_a(x, x->has_sc ? 1 : 0);
         if (x->has_sc) restrict_acoef(x);
         fx_solve_stats one{};
-        int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one, !x->in.mg_warm_start, false, z0, z1);
+        int o0 = 0, o1 = -1;
+        if (partial_ok && !x->has_sc && x->fe_k0 >= 0) {
+            o0 = std::max(x->fe_k0 - 1, 0);
+            o1 = std::min(x->fe_k1 + 1, x->n[2] - 1);
+        }
+        int rc = solve_any(x, bv, x->in.mg_tol_rel, x->in.mg_max_iter, &one, !x->in.mg_warm_start, false, z0, z1, o0, o1);
         st->vcycles += one.vcycles;
         st->mg_resid0 = one.mg_resid0; st->mg_resid = one.mg_resid; st->mg_bnorm = one.mg_bnorm;
         if (rc) { st->outer_iters = iter + 1; return rc; }
@@ -1440,6 +1470,7 @@ int steady_read(fx_ctx* x, double* metric) {
     return FX_OK;
 }
 int steady_check(fx_ctx* x, int step, double* metric) {
+    phi_sync(x);
     fxk::maxabs2(x->s, x->phi, x->phi_old, (long long)x->N, x->red, x->steady_dev);
     fxk::phidiff(x->s, x->phi, x->phi_old, x->phidiff, (long long)x->N, x->steady_dev);
     x->steady_pending = true;
@@ -1725,6 +1756,7 @@ int fx_get_field(fx_ctx* x, int f, double* host) {
         int rc = fx_compute_E(x);
         if (rc) return rc;
     }
+    if (f == FX_FIELD_PHI) phi_sync(x);
     double* d = field_dev(x, f);
     if (!d) return fail(x, FX_EINVAL, "unknown field id");
     HIPCHK(x, hipMemcpyAsync(host, d, N * sizeof(double), hipMemcpyDeviceToHost, x->s));
@@ -1758,6 +1790,7 @@ int fx_set_field(fx_ctx* x, int f, const double* host) {
         return FX_OK;
     }
     case FX_FIELD_PHI:
+        x->phi_z0 = 0; x->phi_z1 = -1;   /* all of phi rewritten */
         HIPCHK(x, hipMemcpyAsync(x->phi, host, B, hipMemcpyHostToDevice, x->s));
         HIPCHK(x, hipStreamSynchronize(x->s));
         return FX_OK;
@@ -1835,6 +1868,7 @@ int fx_compute_phi_rho(fx_ctx* x, fx_solve_stats* st) {
 
 int fx_compute_E(fx_ctx* x) {
     if (!x) return FX_EINVAL;
+    phi_sync(x);
     EArgs a;
     a.g = geom_of(x); a.phv = phiview_of(x); a.rot = rot_of(x);
     for (int d = 0; d < 3; ++d) a.E[d] = x->E[d];
@@ -1896,12 +1930,13 @@ int fx_step(fx_ctx* x, int nsteps, fx_step_stats* out) {
         const int step = ++x->step;
         const bool check_now = !lazy || it == nsteps - 1 || (in.plot_int > 0 && step % in.plot_int == 0);
         if (check_now && x->phi_old_stale) {
+            phi_sync(x);
             fxk::copy(x->s, x->phi_old, x->phi, (long long)x->N);
             x->phi_old_stale = false;
         }
         if (!x->waveform.empty()) x->Phi_Bc_hi = x->waveform[std::min((size_t)step, x->waveform.size()) - 1];
         fx_solve_stats ss{};
-        rc = compute_phi_rho(x, x->P, &ss);
+        rc = compute_phi_rho(x, x->P, &ss, in.TimeIntegratorOrder == 1);
         st.total_vcycles += ss.vcycles; st.total_outer_iters += ss.outer_iters; st.total_solves += ss.outer_iters;
         if (rc) break;
         fx_solve_stats ss2{};
@@ -2004,6 +2039,7 @@ int fx_field_device_ptr(fx_ctx* x, int f, void** dptr) {
     if (d == x->rho) x->rho_exposed = true, x->rho_clean = false;
     if (d == x->rhs) x->rhs_exposed = true;   /* the buffers are not swapped from then on */
     if (d == x->alpha_cc) x->alpha_exposed = true, x->alpha_clean = false;
+    if (d == x->phi) phi_sync(x), x->phi_exposed = true;   /* every later solve forms all of phi */
     *dptr = d;
     return FX_OK;
 }
@@ -2082,7 +2118,7 @@ static void diff_set_P(fx_ctx* x, const double* src) {
 static int diff_step(fx_ctx* x, double V) {
     x->Phi_Bc_hi = V;
     fx_solve_stats ss{}, ss2{};
-    int rc = compute_phi_rho(x, x->P, &ss);
+    int rc = compute_phi_rho(x, x->P, &ss, true);
     if (rc) return rc;
     return update(x, 1, &ss2);
 }
@@ -2222,6 +2258,7 @@ int fx_diff_backward(fx_ctx* x, const double* dL_dm, double* g_theta, double* g_
  * the measured run. */
 int fx_bench_kernel(fx_ctx* x, const char* which, int reps, double* avg_ms) {
     if (!x || !which || reps < 1 || !avg_ms) return FX_EINVAL;
+    phi_sync(x);   /* the transform passes reuse the direct solve's workspace */
     const std::string w(which);
     double bv[6];
     bc_values(x, bv);

@@ -478,20 +478,23 @@ void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T f
 }
 
 /* planes of b outside [z0, z1] hold zeros: their forward transforms are
- * skipped (z1 < z0: none is, every plane is transformed) */
+ * skipped (z1 < z0: none is, every plane is transformed); phi is formed on
+ * planes [o0, o1] only (o1 < o0: all), the other planes' solution left
+ * transformed in S for fft_finish */
 template <class T, class TIN, class TOUT>
 void fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi, double blo, double bhi, int add,
-                    int z0, int z1) {
+                    int z0, int z1, int o0 = 0, int o1 = -1) {
     const T scale = T(1) / T(P.n0 * P.n1);
     if (z1 < z0) { z0 = 0; z1 = P.n2 - 1; }
+    if (o1 < o0) { o0 = 0; o1 = P.n2 - 1; }
     const int nk = z1 - z0 + 1;
     /* the Dirichlet face terms of a skipped face plane go to the z solve */
     const T flo = z0 > 0 ? T(blo) : T(0), fhi = z1 < P.n2 - 1 ? T(bhi) : T(0);
     rows<T, TIN, TOUT>(s, P, true, b, P.S, blo, bhi, 0, z0, nk);
     cols<T>(s, P, z0, nk);
     tri<T>(s, P, scale, z0, z1, flo, fhi);
-    cols<T>(s, P, 0, P.n2);
-    rows<T, TIN, TOUT>(s, P, false, P.S, phi, 0.0, 0.0, add, 0, P.n2);
+    cols<T>(s, P, o0, o1 - o0 + 1);
+    rows<T, TIN, TOUT>(s, P, false, P.S, phi, 0.0, 0.0, add, o0, o1 - o0 + 1);
 }
 
 }  // namespace
@@ -509,8 +512,18 @@ void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, flo
 }
 
 void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi, int z0,
-               int z1) {
-    fft_solve_impl<double, double, double>(s, P, b, phi, blo, bhi, 0, z0, z1);
+               int z1, int o0, int o1) {
+    fft_solve_impl<double, double, double>(s, P, b, phi, blo, bhi, 0, z0, z1, o0, o1);
+}
+void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1) {
+    if (o0 > 0) {
+        cols<double>(s, P, 0, o0);
+        rows<double, double, double>(s, P, false, P.S, phi, 0.0, 0.0, 0, 0, o0);
+    }
+    if (o1 < P.n2 - 1) {
+        cols<double>(s, P, o1 + 1, P.n2 - 1 - o1);
+        rows<double, double, double>(s, P, false, P.S, phi, 0.0, 0.0, 0, o1 + 1, P.n2 - 1 - o1);
+    }
 }
 void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi, int z0,
                int z1) {

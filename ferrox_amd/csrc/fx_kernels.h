@@ -102,9 +102,12 @@ bool fft_size_ok(int n);
  * plan (piv32 may be null) */
 size_t fft_pivot_count(int n0, int n1, int n2);
 void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32);
-/* planes of b outside [z0, z1] hold zeros (z1 < z0: nothing is known) */
+/* planes of b outside [z0, z1] hold zeros (z1 < z0: nothing is known);
+ * phi is formed on planes [o0, o1] (o1 < o0: all), the others by a later
+ * fft_finish(o0, o1) while P.S is untouched */
 void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi,
-               int z0 = 0, int z1 = -1);
+               int z0 = 0, int z1 = -1, int o0 = 0, int o1 = -1);
+void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1);
 void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi,
                int z0 = 0, int z1 = -1);
 void fft_solve_add(hipStream_t s, const fxd::FftPlan<float>& P, const float* r, double* phi);
