@@ -56,6 +56,7 @@ class FxInputs(C.Structure):
         ("mg_block_levels", C.c_int), ("mg_stream_min_cells", C.c_int), ("mg_coarsen_ratio", C.c_double), ("mg_exact_tail", C.c_int),
         ("mg_refine_nu1", C.c_int), ("mg_refine_nu2", C.c_int),
         ("poisson_direct", C.c_int), ("poisson_direct_check", C.c_int), ("tdgl_fused_rhs", C.c_int), ("tdgl_stream", C.c_int),
+        ("poisson_pcg", C.c_int),
         ("plot_Phi", C.c_int), ("plot_PoissonRHS", C.c_int), ("plot_E", C.c_int), ("plot_holes", C.c_int), ("plot_electrons", C.c_int), ("plot_charge", C.c_int), ("plot_epsilon", C.c_int), ("plot_mask", C.c_int), ("plot_tphase", C.c_int), ("plot_alpha", C.c_int), ("plot_beta", C.c_int), ("plot_theta", C.c_int), ("plot_PhiDiff", C.c_int),
         ("tphase_geom_function", C.c_char * 4096), ("angle_alpha_function", C.c_char * 2048),
         ("angle_beta_function", C.c_char * 2048), ("angle_theta_function", C.c_char * 2048),

@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-LIB_SOURCES = ["fx_context.hip", "fx_kernels.hip", "fx_mg.hip", "fx_adjoint.hip", "fx_batch.hip", "fx_fft.hip", "fx_inputs.cpp", "fx_plotfile.cpp"]
+LIB_SOURCES = ["fx_context.hip", "fx_kernels.hip", "fx_mg.hip", "fx_adjoint.hip", "fx_batch.hip", "fx_fft.hip", "fx_pcg.hip", "fx_inputs.cpp", "fx_plotfile.cpp"]
 
 
 def _run(cmd):

@@ -164,7 +164,11 @@ struct fx_ctx {
     bool direct = false;
     fxd::FftPlan<double> fft64{};
     fxd::FftPlan<float> fft32{};
-    double* fft_mem64 = nullptr;      /* twiddles, eigenvalues, S, checkpoints */
+    double* fft_mem64 = nullptr;      /* twiddles, eigenvalues, S, the z-pivot table */
+    /* preconditioned CG (semiconductor stacks on the direct path's geometry):
+     * r, z, p, q (N each), the plane means of the A-coefficient (n2), the
+     * preconditioner's z-pivots, three sets of partials, r.z of two iterations */
+    double* pcg_mem = nullptr;
     float* fft_mem32 = nullptr;
     double fft_w2lo = 0.0, fft_w2hi = 0.0;   /* 2 w of the first / last plane's Dirichlet face (0: Neumann) */
     int tail_ts64 = 0, tail_ts32 = 0;          /* first tail level staged in LDS */
@@ -1249,6 +1253,78 @@ int direct_solve_mixed(fx_ctx* x, const double bv[6], double tol, int maxit, fx_
     }
 }
 
+/* Preconditioned conjugate gradients (fx_pcg.hip) from phi = 0 on the
+ * direct path's geometry with a Helmholtz term.  The preconditioner is the
+ * direct solve of the operator with each plane's A-coefficient replaced by
+ * its plane mean (z-pivots tabulated for this operator).  The stopping rule
+ * is the multigrid's, ||r|| <= tol max(||b||, ||r0||) in the max-norm, r0
+ * the residual of phi = 0 with the Dirichlet face terms; it is tested on the
+ * updated residual every iteration (the host reads max|r| while the next
+ * preconditioner solve runs) and, once that meets it, on the residual
+ * b - A phi formed anew.  st->vcycles counts the iterations. */
+int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_stats* st) {
+    const size_t N = x->N;
+    const int n2 = x->n[2];
+    const size_t npiv = fxk::fft_pivot_count(x->n[0], x->n[1], n2);
+    const int PB = fxk::PCG_PARTS;
+    int rc;
+    if (!x->pcg_mem &&
+        (rc = dalloc(x, (void**)&x->pcg_mem, (4 * N + (size_t)n2 + npiv + 3 * (size_t)PB + 2) * sizeof(double))))
+        return rc;
+    double* r = x->pcg_mem;
+    double* z = r + N;
+    double* p = z + N;
+    double* q = p + N;
+    double* abar = q + N;
+    double* piv = abar + n2;
+    double* part_pq = piv + npiv;
+    double* part_r = part_pq + PB;
+    double* part_rz = part_r + PB;
+    double* sc = part_rz + PB;
+    const DBC bc = dbc_of(x, bv);
+    const DLev& L = x->lev[0].d;
+    /* the preconditioner: the plane-averaged operator's pivots */
+    fxk::pcg_abar(x->s, L, abar);
+    fxk::fft_pivots(x->s, x->fft64, piv, nullptr, abar);
+    FftPlan<double> M = x->fft64;
+    M.piv = piv;
+    /* r0 = b - A 0 with the face terms */
+    fxk::fill(x->s, x->phi, (long long)N, 0.0);
+    fxk::mg_resid_norm(x->s, L, bc, 1, r, x->red, x->red_out);
+    fxk::fft_solve(x->s, M, r, z, 0.0, 0.0);
+    fxk::pcg_dot(x->s, r, z, (long long)N, part_rz);
+    fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, -1);
+    double nr[2];
+    if ((rc = read_red(x, 2, nr))) return rc;
+    const double r0 = nr[0], bn = nr[1];
+    const double target = tol * std::max(bn, r0);
+    st->mg_resid0 = r0; st->mg_bnorm = bn; st->mg_resid = r0; st->vcycles = 0;
+    if (!(r0 == r0) || !(bn == bn)) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
+    if (r0 <= target) return FX_OK;
+    for (int it = 0; it < maxit; ++it) {
+        fxk::pcg_matvec(x->s, L, bc, p, q, part_pq);
+        fxk::pcg_update(x->s, x->phi, r, p, q, (long long)N, part_pq, sc, it, part_r);
+        fxk::final1024(x->s, part_r, PB, 1, 1, x->red_out);
+        const unsigned v = post_red(x, 1);
+        /* the next direction is queued before the host looks at max|r| */
+        fxk::fft_solve(x->s, M, r, z, 0.0, 0.0);
+        fxk::pcg_dot(x->s, r, z, (long long)N, part_rz);
+        fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, it);
+        double rm;
+        if ((rc = wait_red(x, v, 1, &rm))) return rc;
+        st->vcycles = it + 1;
+        st->mg_resid = rm;
+        if (!(rm == rm)) return fail(x, FX_ENAN, "conjugate gradients produced a non-finite residual");
+        if (rm <= target) {
+            fxk::mg_resid_norm(x->s, L, bc, 1, nullptr, x->red, x->red_out);
+            if ((rc = read_red(x, 2, nr))) return rc;
+            st->mg_resid = nr[0];
+            if (nr[0] <= target) return FX_OK;
+        }
+    }
+    return fail(x, FX_ENOCONV_MG, "preconditioned conjugate gradients did not converge in mg_max_iter iterations");
+}
+
 /* phi_zero: the solve starts from phi = 0, whatever phi holds.  Planes of
  * the right-hand side outside [z0, z1] hold zeros (z1 < z0: not known).
  * The FP64 direct solve may form phi on planes [o0, o1] only (o1 < o0:
@@ -1263,6 +1339,7 @@ int solve_any(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
         return direct_solve(x, bv, tol, st, check || x->in.poisson_direct_check, z0, z1, o0, o1);
     }
     phi_sync(x);
+    if (x->direct && x->in.poisson_pcg && x->in.precision == FX_PREC_FP64) return pcg_solve(x, bv, tol, maxit, st);
     if (x->in.precision == FX_PREC_FP32_REFINE) {
         /* FP32 copy of the A-coefficient for the correction levels */
         if (x->lev32[0].read_a) {
@@ -1502,7 +1579,7 @@ void free_all(fx_ctx* x) {
         F(f.phi); F(f.b); F(f.a); F(f.bx); F(f.by); F(f.bz); F((void*)f.bxk); F((void*)f.byk); F((void*)f.bzk); F(f.phi2);
     }
     for (float* p : x->up32) F(p);
-    F(x->spec_basis64); F(x->spec_basis32); F(x->fft_mem64); F(x->fft_mem32);
+    F(x->spec_basis64); F(x->spec_basis32); F(x->fft_mem64); F(x->fft_mem32); F(x->pcg_mem);
     F(x->cg_work32); F(x->phi_alt); F(x->d_tail64); F(x->d_tail32);
     F(x->lu64); F(x->lu32); F(x->perm64); F(x->perm32);
     if (x->h_red) hipHostFree(x->h_red);

@@ -253,10 +253,10 @@ __global__ __launch_bounds__(FFT_THR) void k_fft_cols(FftPlan<T> P, int R, int k
  *   d_k = bx_k mu_x / hx^2 + by_k mu_y / hy^2 + (z couplings),
  *   a_k = -bz_k / hz^2 (0 on the first plane), c_k = -bz_k+1 / hz^2 (0 on the last),
  * the z couplings 2 bz / hz^2 across a Dirichlet face and 0 across a Neumann
- * one; formed in FP64 (correctly rounded divisions) from the FP64 plan's
+ * one, and -abar_k where a Helmholtz term per plane is given; formed in FP64 (correctly rounded divisions) from the FP64 plan's
  * coefficients and stored as T. */
 template <class T>
-__global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __restrict__ piv) {
+__global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __restrict__ piv, const double* __restrict__ abar) {
     const int n0c = P.n0 / 2 + 1, n1c = P.n1 / 2 + 1, n2 = P.n2;
     const int q = blockIdx.x * 256 + threadIdx.x;
     if (q >= n0c * n1c) return;
@@ -269,6 +269,7 @@ __global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __rest
         double d = P.bxk[k] * lx;
         d += P.byk[k] * ly;
         d += (k > 0 ? wzl : (P.dlo ? 2.0 * wzl : 0.0)) + (k < n2 - 1 ? wzh : (P.dhi ? 2.0 * wzh : 0.0));
+        if (abar) d -= abar[k];
         const double a = k > 0 ? -wzl : 0.0, c = k < n2 - 1 ? -wzh : 0.0;
         const double r = 1.0 / (d - a * cp);
         cp = c * r;
@@ -505,10 +506,10 @@ bool fft_size_ok(int n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; }
 
 size_t fft_pivot_count(int n0, int n1, int n2) { return (size_t)(n0 / 2 + 1) * (n1 / 2 + 1) * n2; }
 
-void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32) {
+void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar) {
     const int n = (P.n0 / 2 + 1) * (P.n1 / 2 + 1);
-    hipLaunchKernelGGL(k_fft_pivots<double>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv64);
-    if (piv32) hipLaunchKernelGGL(k_fft_pivots<float>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv32);
+    hipLaunchKernelGGL(k_fft_pivots<double>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv64, abar);
+    if (piv32) hipLaunchKernelGGL(k_fft_pivots<float>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv32, abar);
 }
 
 void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi, int z0,

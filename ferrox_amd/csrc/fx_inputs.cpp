@@ -493,7 +493,7 @@ struct Applier {
         D1(mg_tol_rel) I1(mg_max_iter) I1(mg_nu1) I1(mg_nu2) I1(mg_warm_start) I1(outer_max_iter)
         I1(phi_ghost_mode) I1(precision) I1(bias_ramp_steps) D1(bias_ramp_start) D1(bias_ramp_end) I1(verbose)
         I1(mg_fused_sweep) I1(mg_block_levels) I1(mg_stream_min_cells) D1(mg_coarsen_ratio) I1(mg_exact_tail)
-        I1(mg_refine_nu1) I1(mg_refine_nu2) I1(poisson_direct) I1(poisson_direct_check) I1(tdgl_fused_rhs) I1(tdgl_stream)
+        I1(mg_refine_nu1) I1(mg_refine_nu2) I1(poisson_direct) I1(poisson_direct_check) I1(tdgl_fused_rhs) I1(tdgl_stream) I1(poisson_pcg)
 #undef D1
 #undef D3
 #undef I1
@@ -586,6 +586,7 @@ int fx_inputs_default(fx_inputs* in) {
     in->mg_coarsen_ratio = 1.9;
     in->mg_exact_tail = 1;
     in->poisson_direct = 1; in->poisson_direct_check = 0; in->tdgl_fused_rhs = 0; in->tdgl_stream = 1;
+    in->poisson_pcg = 1;
     in->mg_refine_nu1 = 1; in->mg_refine_nu2 = 2;   /* measured on the 256x256x128 workload: 281.6 against 266.5 steps/s with (2, 2) */
     return FX_OK;
 }

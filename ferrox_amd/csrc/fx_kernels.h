@@ -99,9 +99,20 @@ void fill(hipStream_t s, double* p, long long n, double v);
  * fft_solve_add: phi += A^-1 r (homogeneous faces) */
 bool fft_size_ok(int n);
 /* values of the pivot table (FftPlan::piv) and its set-up from the FP64
- * plan (piv32 may be null) */
+ * plan (piv32 may be null); abar (n2 values, may be null): a Helmholtz term
+ * per plane, the row operator's -a on the diagonal */
 size_t fft_pivot_count(int n0, int n1, int n2);
-void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32);
+void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar = nullptr);
+/* preconditioned conjugate gradients (fx_pcg.hip): reductions leave
+ * PCG_PARTS partials, summed by the kernel that uses them; sc holds r.z of
+ * the last two iterations */
+constexpr int PCG_PARTS = 1024;
+void pcg_abar(hipStream_t s, const fxd::DLev& l, double* abar);
+void pcg_matvec(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* p, double* q, double* part);
+void pcg_update(hipStream_t s, double* phi, double* r, const double* p, const double* q, long long N,
+                const double* part_pq, const double* sc, int it, double* part_r);
+void pcg_dot(hipStream_t s, const double* a, const double* b, long long N, double* part);
+void pcg_dir(hipStream_t s, double* p, const double* z, long long N, const double* part_rz, double* sc, int it);
 /* planes of b outside [z0, z1] hold zeros (z1 < z0: nothing is known);
  * phi is formed on planes [o0, o1] (o1 < o0: all), the others by a later
  * fft_finish(o0, o1) while P.S is untouched */

@@ -156,6 +156,10 @@ typedef struct fx_inputs {
                                      them when those are at most a quarter of the FE planes, else by list;
                                      2: always by list; 3: always over their planes; 0: one thread per cell
                                      with gathered operands (k_tdgl, k_rhs_newton) */
+    int    poisson_pcg;           /* 1 (default): where the direct solve applies but for a Helmholtz term
+                                     (semiconductor stacks), FP64 solves are preconditioned conjugate gradients
+                                     with the direct solve of the plane-averaged operator as preconditioner,
+                                     under the multigrid stopping rule; 0: multigrid */
     /* plotfile variable switches, default 1 (FerroX.cpp:300-325) */
     int    plot_Phi, plot_PoissonRHS, plot_E, plot_holes, plot_electrons, plot_charge, plot_epsilon;
     int    plot_mask, plot_tphase, plot_alpha, plot_beta, plot_theta, plot_PhiDiff;

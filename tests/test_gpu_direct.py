@@ -132,6 +132,8 @@ def test_other_problems_stay_on_the_multigrid(why):
         kw["poisson_direct"] = 0
     elif why == "tphase":
         kw["epsilonX_fe_tphase"] = 30.0       # the t-phase cells break the layering
+    elif why == "helmholtz":
+        kw["poisson_pcg"] = 0                 # else the direct solve preconditions CG (tests/test_gpu_pcg.py)
     inp = fx.from_text(base, mg_tol_rel=1e-12, **kw)
     g, o = fx.FerroX(inp), Oracle(inp)
     if why == "tphase":
