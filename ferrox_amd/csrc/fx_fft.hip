@@ -44,7 +44,26 @@ namespace {
 
 using fxd::FftPlan;
 
-constexpr int FFT_THR = 128;  /* threads of a transform workgroup: R sequences x max(N1, N2) */
+/* Threads of a transform workgroup (R sequences x max(N1, N2)) and the y
+ * transform's LDS layout: rows of the four-step buffer padded by one value
+ * (PAD), twiddles staged in LDS (TWL).  Measured on 256 x 256 planes: the x
+ * transform is fastest with 256 threads (23.7 against 25.4 us per pass); the
+ * y transform with 128 threads, padded rows and twiddles in LDS (31.0 us;
+ * without padding and with twiddles read from the cache five workgroups fit
+ * a CU, but 36.9 us; 64 threads, 44.9 us). */
+#ifndef FX_FFT_ROWS_THR
+#define FX_FFT_ROWS_THR 256
+#endif
+#ifndef FX_FFT_COLS_THR
+#define FX_FFT_COLS_THR 128
+#endif
+#ifndef FX_FFT_COLS_PAD
+#define FX_FFT_COLS_PAD 1
+#endif
+#ifndef FX_FFT_COLS_TWL
+#define FX_FFT_COLS_TWL 1
+#endif
+constexpr int ROWS_THR = FX_FFT_ROWS_THR, COLS_THR = FX_FFT_COLS_THR;
 constexpr int TRI_THR = 256;
 constexpr int TRI_BLK = 8;    /* planes per load block of the tridiagonal solve */
 #ifndef FX_TRI_REG
@@ -110,9 +129,9 @@ __device__ __forceinline__ void reg_dft(T (&vr)[N], T (&vi)[N], const T* tw, int
  *           stored in LDS as X[s][k] over the Y buffer.
  * load(s, a, b, re, im) reads element N2 a + b of sequence s; after the
  * call, X is in Xr/Xi (Xr[s n + k]) for the Hartley step. */
-template <class T, int N1, int N2, class Load>
+template <class T, int N1, int N2, int PAD, class Load>
 __device__ __forceinline__ void four_step(int R, const T* tw, T* Yr, T* Yi, Load load, int step1_s_fast) {
-    constexpr int n = N1 * N2, YS = N1 + 1;
+    constexpr int n = N1 * N2, YS = N1 + PAD;
     const int t = threadIdx.x;
     if (t < R * N2) {
         const int s = step1_s_fast ? t % R : t / N2;
@@ -172,8 +191,8 @@ __device__ __forceinline__ void load_tw(T* tw, const T* g, int n) {
  * (TIN) plus the Dirichlet face terms on the first and last plane, output
  * S.  !FWD: input S, output phi (TOUT), stored or, with add, added. */
 template <class T, class TIN, class TOUT, bool FWD, int N1, int N2>
-__global__ __launch_bounds__(FFT_THR) void k_fft_rows(FftPlan<T> P, const TIN* __restrict__ in, TOUT* __restrict__ out,
-                                                      int R, double blo, double bhi, int add, int kz0) {
+__global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* __restrict__ in, TOUT* __restrict__ out,
+                                                       int R, double blo, double bhi, int add, int kz0) {
     extern __shared__ __align__(16) unsigned char fft_lds[];
     constexpr int n = N1 * N2;
     const int k = kz0 + blockIdx.y, j0 = blockIdx.x * 2 * R;
@@ -195,7 +214,7 @@ __global__ __launch_bounds__(FFT_THR) void k_fft_rows(FftPlan<T> P, const TIN* _
             im = (T)in[o + n];
         }
     };
-    four_step<T, N1, N2>(R, tw, Yr, Yi, load, 0);
+    four_step<T, N1, N2, 1>(R, tw, Yr, Yi, load, 0);
     for (int t = threadIdx.x; t < R * n; t += blockDim.x) {
         const int s = t / n, kk = t % n;
         T h0, h1;
@@ -219,23 +238,27 @@ template <> struct Vec2<float> { using t = float2; };
  * i0 + 2s + 1), s < R, of plane k = kz0 + blockIdx.y: each row segment of
  * 2R values is read and written whole (one vector of two per thread). */
 template <class T, int N1, int N2>
-__global__ __launch_bounds__(FFT_THR) void k_fft_cols(FftPlan<T> P, int R, int kz0) {
+__global__ __launch_bounds__(COLS_THR) void k_fft_cols(FftPlan<T> P, int R, int kz0) {
     extern __shared__ __align__(16) unsigned char fft_lds[];
     using V2 = typename Vec2<T>::t;
-    constexpr int n = N1 * N2;
+    constexpr int n = N1 * N2, PAD = FX_FFT_COLS_PAD;
     const int n0 = P.n0, k = kz0 + blockIdx.y, i0 = blockIdx.x * 2 * R;
-    T* const tw = reinterpret_cast<T*>(fft_lds);
-    T* const Yr = tw + n;
-    T* const Yi = Yr + R * N2 * (N1 + 1);
-    load_tw<T>(tw, P.tw1, n);
-    __syncthreads();
+    T* const twl = reinterpret_cast<T*>(fft_lds);
+    T* const Yr = twl + (FX_FFT_COLS_TWL ? n : 0);
+    T* const Yi = Yr + R * N2 * (N1 + PAD);
+    const T* tw = P.tw1;
+    if (FX_FFT_COLS_TWL) {
+        load_tw<T>(twl, P.tw1, n);
+        __syncthreads();
+        tw = twl;
+    }
     T* const S = P.S + (size_t)k * n * n0 + i0;
     auto load = [&](int s, int a, int b, T& re, T& im) {
         const V2 v = *reinterpret_cast<const V2*>(S + (size_t)(N2 * a + b) * n0 + 2 * s);
         re = v.x;
         im = v.y;
     };
-    four_step<T, N1, N2>(R, tw, Yr, Yi, load, 1);
+    four_step<T, N1, N2, PAD>(R, tw, Yr, Yi, load, 1);
     for (int t = threadIdx.x; t < R * n; t += blockDim.x) {
         const int s = t % R, j = t / R;
         T h0, h1;
@@ -387,38 +410,53 @@ __global__ __launch_bounds__(TRI_THR) void k_fft_tri(FftPlan<T> P, T scale, int 
     }
 }
 
-/* sequences per workgroup: FFT_THR / max(N1, N2), and 2R no more than the
+/* sequences per workgroup: thr / max(N1, N2), and 2R no more than the
  * other lateral extent */
-inline int seqs_per_block(int n1f, int n2f, int other) {
-    int R = FFT_THR / (n1f > n2f ? n1f : n2f);
+inline int seqs_per_block(int thr, int n1f, int n2f, int other) {
+    int R = thr / (n1f > n2f ? n1f : n2f);
     if (2 * R > other) R = other / 2;
     return R;
 }
 
 template <class T>
-size_t fft_lds_bytes(int N1, int N2, int R) {
-    return ((size_t)N1 * N2 + 2 * (size_t)R * N2 * (N1 + 1)) * sizeof(T);
+size_t fft_lds_bytes(int N1, int N2, int R, int pad, int twl) {
+    return ((twl ? (size_t)N1 * N2 : 0) + 2 * (size_t)R * N2 * (N1 + pad)) * sizeof(T);
+}
+
+/* raise the dynamic LDS limit of a kernel once, where a launch needs more than 64 KB */
+template <class K>
+void lds_limit(K* kernel, size_t lds, bool& done) {
+    if (lds > 64 * 1024 && !done) {
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        done = true;
+    }
 }
 
 template <class T, class TIN, class TOUT, int N1, int N2>
 void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add,
                  int kz0, int nk) {
-    const int R = seqs_per_block(N1, N2, P.n1);
-    const size_t lds = fft_lds_bytes<T>(N1, N2, R);
+    const int R = seqs_per_block(ROWS_THR, N1, N2, P.n1);
+    const size_t lds = fft_lds_bytes<T>(N1, N2, R, 1, 1);
     const dim3 g(P.n1 / (2 * R), nk);
+    static bool lf = false, li = false;
+    if (fwd) lds_limit(&k_fft_rows<T, TIN, T, true, N1, N2>, lds, lf);
+    else lds_limit(&k_fft_rows<T, T, TOUT, false, N1, N2>, lds, li);
     if (fwd)
-        hipLaunchKernelGGL((k_fft_rows<T, TIN, T, true, N1, N2>), g, dim3(FFT_THR), lds, s, P,
+        hipLaunchKernelGGL((k_fft_rows<T, TIN, T, true, N1, N2>), g, dim3(ROWS_THR), lds, s, P,
                            static_cast<const TIN*>(in), static_cast<T*>(out), R, blo, bhi, 0, kz0);
     else
-        hipLaunchKernelGGL((k_fft_rows<T, T, TOUT, false, N1, N2>), g, dim3(FFT_THR), lds, s, P,
+        hipLaunchKernelGGL((k_fft_rows<T, T, TOUT, false, N1, N2>), g, dim3(ROWS_THR), lds, s, P,
                            static_cast<const T*>(in), static_cast<TOUT*>(out), R, 0.0, 0.0, add, kz0);
 }
 
 template <class T, int N1, int N2>
 void cols_launch(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
-    const int R = seqs_per_block(N1, N2, P.n0);
-    const size_t lds = fft_lds_bytes<T>(N1, N2, R);
-    hipLaunchKernelGGL((k_fft_cols<T, N1, N2>), dim3(P.n0 / (2 * R), nk), dim3(FFT_THR), lds, s, P, R, kz0);
+    const int R = seqs_per_block(COLS_THR, N1, N2, P.n0);
+    const size_t lds = fft_lds_bytes<T>(N1, N2, R, FX_FFT_COLS_PAD, FX_FFT_COLS_TWL);
+    static bool lc = false;
+    lds_limit(&k_fft_cols<T, N1, N2>, lds, lc);
+    hipLaunchKernelGGL((k_fft_cols<T, N1, N2>), dim3(P.n0 / (2 * R), nk), dim3(COLS_THR), lds, s, P, R, kz0);
 }
 
 /* n = N1 N2 with N1 >= N2, both powers of two up to 32; planes kz0 .. kz0 + nk - 1 */

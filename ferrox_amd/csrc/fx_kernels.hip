@@ -888,10 +888,17 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int z
     auto fetch = [&](PlaneRegs<true>& R, int q) { plane_fetch<true, ROT>(R, q, qmax, P, ph, hs, plane, a.M, cic, cjc, ridx, cc0); };
     auto stash = [&](const PlaneRegs<true>& R, int q) { plane_stash<true>(R, q, nz, hs, Pw, Fw, Rr); };
     PlaneRegs<true> RA, RB;
-    fetch(RA, ks - 1); stash(RA, ks - 1);
-    fetch(RA, ks); stash(RA, ks);
-    fetch(RA, ks + 1); stash(RA, ks + 1);
-    fetch(RA, ks + 2);
+    if (FACES) {
+        /* one plane: its three planes' loads in flight together */
+        PlaneRegs<true> RC;
+        fetch(RA, ks - 1); fetch(RB, ks); fetch(RC, ks + 1);
+        stash(RA, ks - 1); stash(RB, ks); stash(RC, ks + 1);
+    } else {
+        fetch(RA, ks - 1); stash(RA, ks - 1);
+        fetch(RA, ks); stash(RA, ks);
+        fetch(RA, ks + 1); stash(RA, ks + 1);
+        fetch(RA, ks + 2);
+    }
     __syncthreads();
     const double Gam = a.BigGamma;
 
@@ -1001,6 +1008,10 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int z
             a.Pout[2][c] = r + a.dt * g;
         }
     };
+    if (FACES) {
+        update(ks);
+        return;
+    }
     for (int p = ks; p <= ke; p += 2) {
         fetch(RB, p + 3);
         update(p);
@@ -1087,10 +1098,16 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, in
     auto fetch = [&](PlaneRegs<false>& R, int q) { plane_fetch<false, ROT>(R, q, qmax, P, nullptr, hs, plane, a.M, cic, cjc, ridx, cc0); };
     auto stash = [&](const PlaneRegs<false>& R, int q) { plane_stash<false>(R, q, nz, hs, Pw, nullptr, Rr); };
     PlaneRegs<false> RA, RB;
-    fetch(RA, ks - 1); stash(RA, ks - 1);
-    fetch(RA, ks); stash(RA, ks);
-    fetch(RA, ks + 1); stash(RA, ks + 1);
-    fetch(RA, ks + 2);
+    if (FACES) {
+        PlaneRegs<false> RC;
+        fetch(RA, ks - 1); fetch(RB, ks); fetch(RC, ks + 1);
+        stash(RA, ks - 1); stash(RB, ks); stash(RC, ks + 1);
+    } else {
+        fetch(RA, ks - 1); stash(RA, ks - 1);
+        fetch(RA, ks); stash(RA, ks);
+        fetch(RA, ks + 1); stash(RA, ks + 1);
+        fetch(RA, ks + 2);
+    }
     __syncthreads();
     const double* dx = a.g.dx;
     auto rhs_of = [&](int p) {
@@ -1128,6 +1145,10 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, in
             a.rhs[c] = -x;
         }
     };
+    if (FACES) {
+        rhs_of(ks);
+        return;
+    }
     for (int p = ks; p <= ke; p += 2) {
         fetch(RB, p + 3);
         rhs_of(p);
