@@ -743,6 +743,25 @@ struct WinPhi {
     }
 };
 
+/* Tile of a streamed pass's workgroup.  The dispatcher hands workgroup L
+ * (in x, y, z order) to XCD L mod 8; each XCD takes one contiguous share of
+ * the tiles instead (x fastest), so that neighbouring tiles, whose halos
+ * overlap, share an L2.  A bijection of the grid onto itself. */
+#ifndef FX_STREAM_XCD
+#define FX_STREAM_XCD 1
+#endif
+__device__ __forceinline__ void stream_tile(int& bx, int& by, int& bz) {
+    if (!FX_STREAM_XCD) { bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z; return; }
+    const int T = gridDim.x * gridDim.y * gridDim.z;
+    const int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    const int x = L & 7, q = T >> 3, r = T & 7;
+    const int t = x * q + min(x, r) + (L >> 3);
+    bx = t % gridDim.x;
+    const int rest = t / gridDim.x;
+    by = rest % gridDim.y;
+    bz = rest / gridDim.y;
+}
+
 /* In-plane source of window column (or row) v of a domain of n cells:
  * cells 0 .. n-1 themselves, -1 and n their periodic images (or nothing
  * across a non-periodic side); positions further out, which arise when the
@@ -870,8 +889,10 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int z
     double wu[9];   /* ROT 0: the uniform rotation's weights */
     if (ROT == 0 && !FACES) interior_weights(a.rot.Ru, a.g11, g44m, ih, wu);
     const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    const int i0 = blockIdx.x * TDS_TX, j0 = blockIdx.y * TDS_TY;
-    const int ks = FACES ? zlist[blockIdx.z] : A.k0 + blockIdx.z * zc;
+    int bx, by, bz;
+    stream_tile(bx, by, bz);
+    const int i0 = bx * TDS_TX, j0 = by * TDS_TY;
+    const int ks = FACES ? zlist[bz] : A.k0 + bz * zc;
     const int ke = FACES ? ks : min(ks + zc - 1, A.k1);
     const int qmax = min(ke + 1, nz - 1);
     const double* __restrict__ P = a.P[2];
@@ -1081,8 +1102,10 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, in
         for (int t = threadIdx.x; t < 3 * nr; t += TDS_THR) Rt[t] = a.rot.tab[9 * (t / 3) + 6 + t % 3];
     }
     const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
-    const int i0 = blockIdx.x * TDS_TX, j0 = blockIdx.y * TDS_TY;
-    const int ks = FACES ? zlist[blockIdx.z] : k0 + blockIdx.z * zc;
+    int bx, by, bz;
+    stream_tile(bx, by, bz);
+    const int i0 = bx * TDS_TX, j0 = by * TDS_TY;
+    const int ks = FACES ? zlist[bz] : k0 + bz * zc;
     const int ke = FACES ? ks : min(ks + zc - 1, k1);
     const int qmax = min(ke + 1, nz - 1);
     const double* __restrict__ P = a.P[2];
