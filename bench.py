@@ -83,9 +83,9 @@ def diff_leg(large: bool) -> dict:
     from ferrox_amd import configs
     from ferrox_amd.diff import central_fd, triangular_sweep
 
-    def case(nx, ny, nz, lx, ly, lz, de_top, fe_top):
+    def case(nx, ny, nz, lx, ly, lz, de_top, fe_top, precision=0):
         text = configs.mfim(nx=nx, ny=ny, nz=nz, lx=lx, ly=ly, lz=lz, de_top=de_top, fe_top=fe_top, nsteps=400)
-        sim = fx.FerroX(fx.from_text(text, mg_tol_rel=1e-13))
+        sim = fx.FerroX(fx.from_text(text, mg_tol_rel=1e-13, precision=precision))
         return sim, sim.get("Pz")
 
     nsteps, K = 400, 20
@@ -113,8 +113,19 @@ def diff_leg(large: bool) -> dict:
     for q, name in ((0, "alpha"), (5, "epsilonX_fe")):
         g_fd = central_fd(lambda t: loss(curve(t)), theta, q)
         fd[name] = abs(gt[q] - g_fd) / abs(g_fd)
+    # the precision form's gradient: the same curve and adjoint with FP32
+    # solves inside FP64 refinement (same stopping rule)
+    s32, P32 = case(64, 64, 36, 32e-9, 32e-9, 9e-9, 4e-9, 9e-9, precision=1)
+    s32.diff_set_params(theta)
+    s32.set("Pz", P0)
+    m32 = s32.diff_forward(V, K)
+    g32, _ = s32.diff_backward(2 * (m32 - target) / nsteps)
+    s32.close()
+    sel = np.abs(gt) > 0
+    grad32_err = float(np.max(np.abs(g32[sel] - gt[sel]) / np.abs(gt[sel])))
     out = {"workload": "64x64x36 MFIM, triangular +-3 V sweep, 400 Euler steps, checkpoints every 20 steps, "
                        "solves to 1e-13",
+           "fp32_refine_grad_max_rel_err": grad32_err,
            "loss": loss(m), "forward_s": round(t1 - t0, 3), "backward_s": round(t2 - t1, 3),
            "grad_over_forward": round((t2 - t1) / (t1 - t0), 2),
            "checkpoint_MB": round((nsteps // K + 1 + 2 * K) * P0.size * 8 / 2**20, 1),
