@@ -129,7 +129,8 @@ struct fx_ctx {
     double* red = nullptr;
     int nred = 0;                  /* capacity of red, in partials per quantity */
     double* red_out = nullptr;
-    double* h_red = nullptr;       /* pinned: 8 reduction values, then the publish sequence number */
+    double* h_red = nullptr;       /* pinned: 8 reduction values, then the publish sequence number; from
+                                      h_red + 16, PCG_SLOTS (value, sequence number) pairs of the CG solve */
     double* d_red_host = nullptr;  /* device view of h_red */
     unsigned pub_seq = 0;
     int* d_iters = nullptr;
@@ -167,7 +168,8 @@ struct fx_ctx {
     double* fft_mem64 = nullptr;      /* twiddles, eigenvalues, S, the z-pivot table */
     /* preconditioned CG (semiconductor stacks on the direct path's geometry):
      * r, z, p, q (N each), the plane means of the A-coefficient (n2), the
-     * preconditioner's z-pivots, three sets of partials, r.z of two iterations */
+     * preconditioner's z-pivots, three sets of partials, four scalars (r.z of
+     * two iterations, the converged flag, the target), the plane-mean partials */
     double* pcg_mem = nullptr;
     float* fft_mem32 = nullptr;
     double fft_w2lo = 0.0, fft_w2hi = 0.0;   /* 2 w of the first / last plane's Dirichlet face (0: Neumann) */
@@ -975,6 +977,31 @@ int wait_red(fx_ctx* x, unsigned v, int nq, double* out) {
 }
 int read_red(fx_ctx* x, int nq, double* out, const double* src = nullptr) { return wait_red(x, post_red(x, nq, src), nq, out); }
 
+/* CG iterations in flight: each publishes max|r| into its own slot */
+constexpr int PCG_SLOTS = 4;
+double* pcg_slot_host(fx_ctx* x, int w) { return x->h_red + 16 + 2 * w; }
+double* pcg_slot_dev(fx_ctx* x, int w) { return x->d_red_host + 16 + 2 * w; }
+/* wait_red for a CG slot */
+int wait_slot(fx_ctx* x, int w, unsigned v, double* out) {
+    HIPCHK(x, hipGetLastError());
+    volatile unsigned* seq = reinterpret_cast<volatile unsigned*>(pcg_slot_host(x, w) + 1);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned spins = 1; *seq != v; ++spins) {
+        if ((spins & 1023) == 0) {
+            const hipError_t q = hipStreamQuery(x->s);
+            if (q != hipSuccess && q != hipErrorNotReady) return fail(x, FX_EHIP, hipGetErrorString(q));
+            if (q == hipSuccess || std::chrono::steady_clock::now() - t0 > std::chrono::seconds(5)) break;
+        }
+    }
+    if (*seq != v) {
+        HIPCHK(x, hipStreamSynchronize(x->s));
+        if (*seq != v) return fail(x, FX_EHIP, "reduction results did not reach the host");
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    *out = pcg_slot_host(x, w)[0];
+    return FX_OK;
+}
+
 /* --------------------------------- multigrid ------------------------------ */
 /* pre_done: the first pre-smoothing sweep of level 0 has already run
  * (phi -> phi2, queued speculatively with the previous convergence check) */
@@ -1268,8 +1295,11 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     const size_t npiv = fxk::fft_pivot_count(x->n[0], x->n[1], n2);
     const int PB = fxk::PCG_PARTS;
     int rc;
+    if (N >= (size_t)1 << 31) return fail(x, FX_EINVAL, "conjugate gradients index cells with 32-bit integers");
+    const size_t nab = (size_t)n2 * fxk::PCG_ABAR_PARTS;
     if (!x->pcg_mem &&
-        (rc = dalloc(x, (void**)&x->pcg_mem, (4 * N + (size_t)n2 + npiv + 3 * (size_t)PB + 2) * sizeof(double))))
+        (rc = dalloc(x, (void**)&x->pcg_mem,
+                     (4 * N + (size_t)n2 + npiv + 3 * (size_t)PB + 4 + nab) * sizeof(double))))
         return rc;
     double* r = x->pcg_mem;
     double* z = r + N;
@@ -1280,19 +1310,23 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     double* part_pq = piv + npiv;
     double* part_r = part_pq + PB;
     double* part_rz = part_r + PB;
-    double* sc = part_rz + PB;
+    double* sc = part_rz + PB;   /* r.z of two iterations, the converged flag, the target */
+    double* part_ab = sc + 4;
     const DBC bc = dbc_of(x, bv);
     const DLev& L = x->lev[0].d;
     /* the preconditioner: the plane-averaged operator's pivots */
-    fxk::pcg_abar(x->s, L, abar);
+    fxk::pcg_abar(x->s, L, part_ab, abar);
     fxk::fft_pivots(x->s, x->fft64, piv, nullptr, abar);
     FftPlan<double> M = x->fft64;
     M.piv = piv;
-    /* r0 = b - A 0 with the face terms */
+    FftPlan<double> Mg = M;   /* the preconditioner as the iterations use it: idle once converged */
+    Mg.skip = sc + 2;
+    /* r0 = b - A 0 with the face terms; z0, p0 */
+    fxk::fill(x->s, sc + 2, 1, 0.0);
     fxk::fill(x->s, x->phi, (long long)N, 0.0);
     fxk::mg_resid_norm(x->s, L, bc, 1, r, x->red, x->red_out);
     fxk::fft_solve(x->s, M, r, z, 0.0, 0.0);
-    fxk::pcg_dot(x->s, r, z, (long long)N, part_rz);
+    fxk::pcg_dot(x->s, r, z, (long long)N, part_rz, sc);
     fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, -1);
     double nr[2];
     if ((rc = read_red(x, 2, nr))) return rc;
@@ -1301,26 +1335,49 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     st->mg_resid0 = r0; st->mg_bnorm = bn; st->mg_resid = r0; st->vcycles = 0;
     if (!(r0 == r0) || !(bn == bn)) return fail(x, FX_ENAN, "non-finite Poisson residual or right-hand side");
     if (r0 <= target) return FX_OK;
-    for (int it = 0; it < maxit; ++it) {
-        fxk::pcg_matvec(x->s, L, bc, p, q, part_pq);
+    fxk::fill(x->s, sc + 3, 1, target);
+    /* PCG_SLOTS - 1 iterations in flight: the host reads max|r| of iteration
+     * it while later ones run; the one that meets the rule sets the device
+     * flag, after which every queued pass does nothing, so phi is that
+     * iteration's, as if the host had stopped there */
+    std::vector<unsigned> vseq(maxit > 0 ? maxit : 1);
+    int queued = 0;
+    auto queue_iter = [&](int it) {
+        fxk::pcg_matvec(x->s, L, bc, p, q, part_pq, sc);
         fxk::pcg_update(x->s, x->phi, r, p, q, (long long)N, part_pq, sc, it, part_r);
-        fxk::final1024(x->s, part_r, PB, 1, 1, x->red_out);
-        const unsigned v = post_red(x, 1);
-        /* the next direction is queued before the host looks at max|r| */
-        fxk::fft_solve(x->s, M, r, z, 0.0, 0.0);
-        fxk::pcg_dot(x->s, r, z, (long long)N, part_rz);
+        vseq[it] = ++x->pub_seq;
+        double* slot = pcg_slot_dev(x, it % PCG_SLOTS);
+        fxk::pcg_publish_max(x->s, part_r, slot, reinterpret_cast<unsigned*>(slot + 1), vseq[it], sc);
+        fxk::fft_solve(x->s, Mg, r, z, 0.0, 0.0);
+        fxk::pcg_dot(x->s, r, z, (long long)N, part_rz, sc);
         fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, it);
+    };
+    auto fill_window = [&](int from) {
+        queued = std::max(queued, from);
+        while (queued < maxit && queued < from + PCG_SLOTS - 1) queue_iter(queued++);
+    };
+    fill_window(0);
+    for (int it = 0; it < maxit; ++it) {
         double rm;
-        if ((rc = wait_red(x, v, 1, &rm))) return rc;
+        if ((rc = wait_slot(x, it % PCG_SLOTS, vseq[it], &rm))) return rc;
         st->vcycles = it + 1;
         st->mg_resid = rm;
         if (!(rm == rm)) return fail(x, FX_ENAN, "conjugate gradients produced a non-finite residual");
         if (rm <= target) {
+            /* the rule on b - A phi formed anew; if it fails (rounding drift of
+             * the updated residual), clear the flag and resume from here */
             fxk::mg_resid_norm(x->s, L, bc, 1, nullptr, x->red, x->red_out);
             if ((rc = read_red(x, 2, nr))) return rc;
             st->mg_resid = nr[0];
             if (nr[0] <= target) return FX_OK;
+            fxk::fill(x->s, sc + 2, 1, 0.0);
+            fxk::fill(x->s, sc + 3, 1, 0.0);    /* no further early stop: the host decides */
+            fxk::fft_solve(x->s, Mg, r, z, 0.0, 0.0);
+            fxk::pcg_dot(x->s, r, z, (long long)N, part_rz, sc);
+            fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, it);
+            queued = it + 1;
         }
+        fill_window(it + 1);
     }
     return fail(x, FX_ENOCONV_MG, "preconditioned conjugate gradients did not converge in mg_max_iter iterations");
 }
@@ -1417,7 +1474,9 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st, bool partia
         x->rhs_next = false;
         x->alpha_clean = !x->has_sc && !x->alpha_exposed;
         set_read_a(x, x->has_sc ? 1 : 0);
-        if (x->has_sc) restrict_acoef(x);
+        /* the coarse levels' A-coefficient, for the multigrid (the CG path
+         * forms its preconditioner from the finest level's plane means) */
+        if (x->has_sc && !(x->direct && x->in.poisson_pcg && x->in.precision == FX_PREC_FP64)) restrict_acoef(x);
         fx_solve_stats one{};
         int o0 = 0, o1 = -1;
         if (partial_ok && !x->has_sc && x->fe_k0 >= 0) {
@@ -1701,11 +1760,11 @@ int fx_create(const fx_inputs* in_, fx_ctx** out) {
     ALLOC(x->steady_dev, sizeof(double) * 2);
     ALLOC(x->d_iters, sizeof(int) * 4);
 #undef ALLOC
-    if (hipHostMalloc((void**)&x->h_red, 8 * sizeof(double) + 64, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
+    if (hipHostMalloc((void**)&x->h_red, 64 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess ||
         hipHostGetDevicePointer((void**)&x->d_red_host, x->h_red, 0) != hipSuccess) {
         free_all(x); delete x; return FX_EHIP;
     }
-    *reinterpret_cast<volatile unsigned*>(x->h_red + 8) = 0;
+    for (int q = 8; q < 64; ++q) x->h_red[q] = 0.0;   /* every sequence number 0 */
     if ((rc = build_levels(x))) { std::fprintf(stderr, "fx_create: %s\n", x->err.c_str()); free_all(x); delete x; return rc; }
     x->h_tphase.assign(x->N, 0.0);
     for (int d = 0; d < 3; ++d) x->h_ang[d].assign(x->N, 0.0);

@@ -351,6 +351,9 @@ struct FftPlan {
      * piv[k (n1/2+1) (n0/2+1) + myc (n0/2+1) + mxc] (mu_m = mu_n-m, so the
      * four mirror modes share one system) */
     const T* piv;
+    /* non-null: every pass of the solve does nothing while *skip != 0 (a
+     * device flag set when an iterative solve around it has converged) */
+    const double* skip;
 };
 /* Plane kernels of the exact solve (one workgroup per plane and block of 16
  * columns): dynamic LDS values, the plane and the y basis with every row

@@ -194,6 +194,7 @@ template <class T, class TIN, class TOUT, bool FWD, int N1, int N2>
 __global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* __restrict__ in, TOUT* __restrict__ out,
                                                        int R, double blo, double bhi, int add, int kz0) {
     extern __shared__ __align__(16) unsigned char fft_lds[];
+    if (P.skip && *P.skip != 0.0) return;
     constexpr int n = N1 * N2;
     const int k = kz0 + blockIdx.y, j0 = blockIdx.x * 2 * R;
     T* const tw = reinterpret_cast<T*>(fft_lds);
@@ -240,6 +241,7 @@ template <> struct Vec2<float> { using t = float2; };
 template <class T, int N1, int N2>
 __global__ __launch_bounds__(COLS_THR) void k_fft_cols(FftPlan<T> P, int R, int kz0) {
     extern __shared__ __align__(16) unsigned char fft_lds[];
+    if (P.skip && *P.skip != 0.0) return;
     using V2 = typename Vec2<T>::t;
     constexpr int n = N1 * N2, PAD = FX_FFT_COLS_PAD;
     const int n0 = P.n0, k = kz0 + blockIdx.y, i0 = blockIdx.x * 2 * R;
@@ -313,21 +315,22 @@ __global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __rest
  * loads in flight while one is eliminated.  Every load is unconditional
  * (clamped plane, the value selected afterwards).  NZ = 0: any n2, y
  * written back to S between the sweeps. */
-template <class T, int NZ, int RB>
-__global__ __launch_bounds__(TRI_THR) void k_fft_tri(FftPlan<T> P, T scale, int z0, int z1, T flo, T fhi) {
+template <class T, int NZ, int RB, int THR>
+__global__ __launch_bounds__(THR) void k_fft_tri(FftPlan<T> P, T scale, int z0, int z1, T flo, T fhi) {
     extern __shared__ __align__(16) unsigned char tri_lds[];
+    if (P.skip && *P.skip != 0.0) return;
     constexpr int LB = NZ > RB ? NZ - RB : 0;
     constexpr int TB = TRI_BLK;
     const int s = P.n0 * P.n1, n2 = P.n2;
-    T* const lv = reinterpret_cast<T*>(tri_lds) + threadIdx.x;                 /* y of plane k: lv[k TRI_THR] */
-    T* const tab = reinterpret_cast<T*>(tri_lds) + (size_t)LB * TRI_THR;         /* a_k, c_k */
-    for (int k = threadIdx.x; k < n2; k += TRI_THR) {
+    T* const lv = reinterpret_cast<T*>(tri_lds) + threadIdx.x;                 /* y of plane k: lv[k THR] */
+    T* const tab = reinterpret_cast<T*>(tri_lds) + (size_t)LB * THR;             /* a_k, c_k */
+    for (int k = threadIdx.x; k < n2; k += THR) {
         const T iz = P.ih2[2];
         tab[2 * k] = k > 0 ? -(P.bzk[k] * iz) : T(0);
         tab[2 * k + 1] = k < n2 - 1 ? -(P.bzk[k + 1] * iz) : T(0);
     }
     __syncthreads();
-    const int c = blockIdx.x * TRI_THR + threadIdx.x;
+    const int c = blockIdx.x * THR + threadIdx.x;
     if (c >= s) return;
     const int h0 = P.n0 >> 1, h1 = P.n1 >> 1;
     const int mx = c & (P.n0 - 1), my = c >> P.l0;
@@ -368,7 +371,7 @@ __global__ __launch_bounds__(TRI_THR) void k_fft_tri(FftPlan<T> P, T scale, int 
                 const int k = b * TB + q;
                 if (k < n2) {
                     y = (rhs(k, sb[b & 1][q]) - tab[2 * k] * y) * rb[b & 1][q];
-                    if (k < LB) lv[(size_t)k * TRI_THR] = y;
+                    if (k < LB) lv[(size_t)k * THR] = y;
                     else yr[k - LB] = y;
                 }
             }
@@ -389,7 +392,7 @@ __global__ __launch_bounds__(TRI_THR) void k_fft_tri(FftPlan<T> P, T scale, int 
             for (int q = TB - 1; q >= 0; --q) {
                 const int k = b * TB + q;
                 if (k < n2) {
-                    const T yk = k < LB ? lv[(size_t)k * TRI_THR] : yr[k - LB];
+                    const T yk = k < LB ? lv[(size_t)k * THR] : yr[k - LB];
                     xn = yk - (tab[2 * k + 1] * rb[b & 1][q]) * xn;
                     col[(size_t)k * s] = xn;
                 }
@@ -493,18 +496,26 @@ void cols(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
     }
 }
 
-template <class T, int NZ, int RB>
-void tri_launch(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi) {
+template <class T, int NZ, int RB, int THR>
+void tri_launch_thr(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi) {
     constexpr int LB = NZ > RB ? NZ - RB : 0;
-    const size_t lds = ((size_t)LB * TRI_THR + 2 * (size_t)P.n2) * sizeof(T);
+    const size_t lds = ((size_t)LB * THR + 2 * (size_t)P.n2) * sizeof(T);
     static bool attr = false;   /* the LDS limit is raised once per kernel */
     if (lds > 64 * 1024 && !attr) {
-        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_tri<T, NZ, RB>),
+        (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_tri<T, NZ, RB, THR>),
                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         attr = true;
     }
-    hipLaunchKernelGGL((k_fft_tri<T, NZ, RB>), dim3((P.n0 * P.n1 + TRI_THR - 1) / TRI_THR), dim3(TRI_THR), lds, s, P,
-                       scale, z0, z1, flo, fhi);
+    hipLaunchKernelGGL((k_fft_tri<T, NZ, RB, THR>), dim3((P.n0 * P.n1 + THR - 1) / THR), dim3(THR), lds, s, P, scale,
+                       z0, z1, flo, fhi);
+}
+
+/* one thread per mode: workgroups of TRI_THR threads, or of 64 where that
+ * would leave CUs idle (fewer modes than 256 workgroups of TRI_THR) */
+template <class T, int NZ, int RB>
+void tri_launch(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi) {
+    if (P.n0 * P.n1 < 256 * TRI_THR) tri_launch_thr<T, NZ, RB, 64>(s, P, scale, z0, z1, flo, fhi);
+    else tri_launch_thr<T, NZ, RB, TRI_THR>(s, P, scale, z0, z1, flo, fhi);
 }
 
 template <class T>

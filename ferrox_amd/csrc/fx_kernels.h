@@ -104,14 +104,21 @@ bool fft_size_ok(int n);
 size_t fft_pivot_count(int n0, int n1, int n2);
 void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar = nullptr);
 /* preconditioned conjugate gradients (fx_pcg.hip): reductions leave
- * PCG_PARTS partials, summed by the kernel that uses them; sc holds r.z of
- * the last two iterations */
+ * PCG_PARTS partials, summed by the kernel that uses them; sc[0..1] hold r.z
+ * of the last two iterations, sc[2] the converged flag (set by
+ * pcg_publish_max when max|r| <= sc[3]; matvec, update, dot and dir do
+ * nothing while it is set) */
 constexpr int PCG_PARTS = 1024;
-void pcg_abar(hipStream_t s, const fxd::DLev& l, double* abar);
-void pcg_matvec(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* p, double* q, double* part);
+constexpr int PCG_ABAR_PARTS = 16;   /* partials per plane of pcg_abar (part holds n2 of them) */
+/* abar[k] = mean of the A-coefficient over plane k (lateral extents powers of two) */
+void pcg_abar(hipStream_t s, const fxd::DLev& l, double* part, double* abar);
+/* the max of the PCG_PARTS partials to pinned host memory, then seq = v */
+void pcg_publish_max(hipStream_t s, const double* part, double* host, unsigned* seq, unsigned v, double* sc);
+void pcg_matvec(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* p, double* q, double* part,
+                const double* sc);
 void pcg_update(hipStream_t s, double* phi, double* r, const double* p, const double* q, long long N,
                 const double* part_pq, const double* sc, int it, double* part_r);
-void pcg_dot(hipStream_t s, const double* a, const double* b, long long N, double* part);
+void pcg_dot(hipStream_t s, const double* a, const double* b, long long N, double* part, const double* sc);
 void pcg_dir(hipStream_t s, double* p, const double* z, long long N, const double* part_rz, double* sc, int it);
 /* planes of b outside [z0, z1] hold zeros (z1 < z0: nothing is known);
  * phi is formed on planes [o0, o1] (o1 < o0: all), the others by a later

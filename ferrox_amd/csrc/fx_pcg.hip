@@ -28,6 +28,7 @@ using fxd::DLev;
 
 constexpr int PB = fxk::PCG_PARTS;   /* workgroups of the reducing kernels = partials per reduction */
 constexpr int PT = 256;
+constexpr int AB = 16;               /* workgroups per plane of the plane-mean kernel */
 
 __device__ __forceinline__ double partial_sum(const double* __restrict__ part, double* sh) {
     double v = 0.0;
@@ -36,40 +37,65 @@ __device__ __forceinline__ double partial_sum(const double* __restrict__ part, d
 }
 
 /* the A-coefficient the row operator applies at cell c (mg_coef) */
-__device__ __forceinline__ double acoef(const DLev& l, int i, int j, int k, long long c) {
+__device__ __forceinline__ double acoef(const DLev& l, int i, int j, int k, int c) {
     if (l.read_a == 0) return 0.0;
     if (l.read_a == 2) return l.a[c];
     const int g = (i + 1) + l.msx * (j + 1) + l.msy * (k + 1);
     return ((l.mat[g] & 3) >= 2) ? l.a[c] : 0.0;
 }
 
-/* abar[k] = mean over plane k of the A-coefficient; one workgroup per plane */
-__global__ __launch_bounds__(PT) void k_pcg_abar(DLev l, double* __restrict__ abar) {
+/* partial sums of the A-coefficient over plane k = blockIdx.x, AB workgroups
+ * per plane: part[k AB + blockIdx.y]; lateral extents 2^l0 x 2^l1 */
+__global__ __launch_bounds__(PT) void k_pcg_abar(DLev l, int l0, double* __restrict__ part) {
     __shared__ double sh[PT / 64 + 1];
     const int k = blockIdx.x, s = l.n0 * l.n1;
     double v = 0.0;
-    for (int q = threadIdx.x; q < s; q += PT) v += acoef(l, q % l.n0, q / l.n0, k, (long long)s * k + q);
+    for (int q = blockIdx.y * PT + threadIdx.x; q < s; q += AB * PT)
+        v += acoef(l, q & (l.n0 - 1), q >> l0, k, s * k + q);
     v = fxd::block_sum<PT>(v, sh);
-    if (threadIdx.x == 0) abar[k] = v / (double)s;
+    if (threadIdx.x == 0) part[k * AB + blockIdx.y] = v;
 }
 
-/* q = A p (homogeneous faces) and the partial sums of p.q */
+/* abar[k] = the plane mean from the partials */
+__global__ __launch_bounds__(64) void k_pcg_abar_fin(const double* __restrict__ part, int n2, double inv_s,
+                                                    double* __restrict__ abar) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k >= n2) return;
+    double v = 0.0;
+    for (int b = 0; b < AB; ++b) v += part[k * AB + b];
+    abar[k] = v * inv_s;
+}
+
+/* q = A p (homogeneous faces) and the partial sums of p.q: bricks of 64 x 4
+ * columns x MZ planes (the planes unrolled, so their loads are in flight
+ * together), strided over the PB workgroups */
+constexpr int MZ = 4;
 __global__ __launch_bounds__(PT) void k_pcg_matvec(DLev l, DBC bc, const double* __restrict__ p, double* __restrict__ q,
-                                                  double* __restrict__ part) {
+                                                  double* __restrict__ part, const double* __restrict__ done) {
     __shared__ double sh[PT / 64 + 1];
-    const int n0 = l.n0, n1 = l.n1;
-    const long long N = (long long)n0 * n1 * l.n2;
+    if (*done != 0.0) return;
+    const int n0 = l.n0, n1 = l.n1, n2 = l.n2;
+    const int bx = (n0 + 63) / 64, by = (n1 + 3) / 4, bz = (n2 + MZ - 1) / MZ;
+    const int nbrick = bx * by * bz;
     double acc = 0.0;
-    for (long long c = blockIdx.x * (long long)PT + threadIdx.x; c < N; c += (long long)PB * PT) {
-        const int i = (int)(c % n0);
-        const long long rest = c / n0;
-        const int j = (int)(rest % n1), k = (int)(rest / n1);
-        double D, S;
-        fxd::mg_row(l, p, i, j, k, bc, false, D, S);
-        const double pc = p[c];
-        const double v = D * pc - S;
-        q[c] = v;
-        acc += pc * v;
+    for (int t = blockIdx.x; t < nbrick; t += PB) {
+        const int tx = t % bx, rest = t / bx, ty = rest % by, tz = rest / by;
+        const int i = tx * 64 + (threadIdx.x & 63), j = ty * 4 + (threadIdx.x >> 6), k0 = tz * MZ;
+        if (i < n0 && j < n1) {
+#pragma unroll
+            for (int z = 0; z < MZ; ++z) {
+                const int k = k0 + z;
+                if (k < n2) {
+                    const int c = i + n0 * (j + n1 * k);
+                    double D, S;
+                    fxd::mg_row(l, p, i, j, k, bc, false, D, S);
+                    const double pc = p[c];
+                    const double v = D * pc - S;
+                    q[c] = v;
+                    acc += pc * v;
+                }
+            }
+        }
     }
     acc = fxd::block_sum<PT>(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
@@ -77,13 +103,14 @@ __global__ __launch_bounds__(PT) void k_pcg_matvec(DLev l, DBC bc, const double*
 
 /* alpha = rz / pq (rz = sc[it & 1]); phi += alpha p; r -= alpha q; partial max|r| */
 __global__ __launch_bounds__(PT) void k_pcg_update(double* __restrict__ phi, double* __restrict__ r,
-                                                  const double* __restrict__ p, const double* __restrict__ q, long long N,
+                                                  const double* __restrict__ p, const double* __restrict__ q, int N,
                                                   const double* __restrict__ part_pq, const double* __restrict__ sc,
                                                   int it, double* __restrict__ part_r) {
     __shared__ double sh[PT / 64 + 1];
+    if (sc[2] != 0.0) return;
     const double alpha = sc[it & 1] / partial_sum(part_pq, sh);
     double m = 0.0;
-    for (long long c = blockIdx.x * (long long)PT + threadIdx.x; c < N; c += (long long)PB * PT) {
+    for (int c = blockIdx.x * PT + threadIdx.x; c < N; c += PB * PT) {
         phi[c] += alpha * p[c];
         const double rv = r[c] - alpha * q[c];
         r[c] = rv;
@@ -93,12 +120,30 @@ __global__ __launch_bounds__(PT) void k_pcg_update(double* __restrict__ phi, dou
     if (threadIdx.x == 0) part_r[blockIdx.x] = m;
 }
 
-/* partial sums of a.b */
-__global__ __launch_bounds__(PT) void k_pcg_dot(const double* __restrict__ a, const double* __restrict__ b, long long N,
-                                               double* __restrict__ part) {
+/* the max of the PB partials to pinned host memory, then the sequence number
+ * v (system-scope release), as k_publish; sc[2] = 1 once the max meets the
+ * target sc[3] (every later pass of the solve then does nothing) */
+__global__ __launch_bounds__(PT) void k_pcg_publish_max(const double* __restrict__ part, double* host, unsigned* seq,
+                                                       unsigned v, double* __restrict__ sc) {
     __shared__ double sh[PT / 64 + 1];
+    double m = 0.0;
+    for (int q = threadIdx.x; q < PB; q += PT) m = fxd::nmax(m, part[q]);
+    m = fxd::block_max<PT>(m, sh);
+    if (threadIdx.x == 0) {
+        if (m <= sc[3]) sc[2] = 1.0;
+        host[0] = m;
+        __threadfence_system();
+        __hip_atomic_store(seq, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+/* partial sums of a.b */
+__global__ __launch_bounds__(PT) void k_pcg_dot(const double* __restrict__ a, const double* __restrict__ b, int N,
+                                               double* __restrict__ part, const double* __restrict__ done) {
+    __shared__ double sh[PT / 64 + 1];
+    if (done && *done != 0.0) return;
     double acc = 0.0;
-    for (long long c = blockIdx.x * (long long)PT + threadIdx.x; c < N; c += (long long)PB * PT) acc += a[c] * b[c];
+    for (int c = blockIdx.x * PT + threadIdx.x; c < N; c += PB * PT) acc += a[c] * b[c];
     acc = fxd::block_sum<PT>(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
@@ -106,39 +151,50 @@ __global__ __launch_bounds__(PT) void k_pcg_dot(const double* __restrict__ a, co
 /* rz_new = r.z from the partials; p = z + (rz_new / sc[it & 1]) p, or p = z
  * for it < 0; sc[(it + 1) & 1] = rz_new (the slot the next update reads;
  * no workgroup of this launch reads it) */
-__global__ __launch_bounds__(PT) void k_pcg_dir(double* __restrict__ p, const double* __restrict__ z, long long N,
+__global__ __launch_bounds__(PT) void k_pcg_dir(double* __restrict__ p, const double* __restrict__ z, int N,
                                                const double* __restrict__ part_rz, double* __restrict__ sc, int it) {
     __shared__ double sh[PT / 64 + 1];
+    if (sc[2] != 0.0) return;
     const double rz = partial_sum(part_rz, sh);
     if (blockIdx.x == 0 && threadIdx.x == 0) sc[(it + 1) & 1] = rz;
     if (it < 0) {
-        for (long long c = blockIdx.x * (long long)PT + threadIdx.x; c < N; c += (long long)PB * PT) p[c] = z[c];
+        for (int c = blockIdx.x * PT + threadIdx.x; c < N; c += PB * PT) p[c] = z[c];
     } else {
         const double beta = rz / sc[it & 1];
-        for (long long c = blockIdx.x * (long long)PT + threadIdx.x; c < N; c += (long long)PB * PT)
-            p[c] = z[c] + beta * p[c];
+        for (int c = blockIdx.x * PT + threadIdx.x; c < N; c += PB * PT) p[c] = z[c] + beta * p[c];
     }
+}
+
+inline int log2i(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
 }
 
 }  // namespace
 
 namespace fxk {
 
-void pcg_abar(hipStream_t s, const DLev& l, double* abar) {
-    hipLaunchKernelGGL(k_pcg_abar, dim3(l.n2), dim3(PT), 0, s, l, abar);
+void pcg_abar(hipStream_t s, const DLev& l, double* part, double* abar) {
+    hipLaunchKernelGGL(k_pcg_abar, dim3(l.n2, AB), dim3(PT), 0, s, l, log2i(l.n0), part);
+    hipLaunchKernelGGL(k_pcg_abar_fin, dim3((l.n2 + 63) / 64), dim3(64), 0, s, part, l.n2, 1.0 / ((double)l.n0 * l.n1),
+                       abar);
 }
-void pcg_matvec(hipStream_t s, const DLev& l, const DBC& bc, const double* p, double* q, double* part) {
-    hipLaunchKernelGGL(k_pcg_matvec, dim3(PB), dim3(PT), 0, s, l, bc, p, q, part);
+void pcg_matvec(hipStream_t s, const DLev& l, const DBC& bc, const double* p, double* q, double* part, const double* sc) {
+    hipLaunchKernelGGL(k_pcg_matvec, dim3(PB), dim3(PT), 0, s, l, bc, p, q, part, sc + 2);
 }
 void pcg_update(hipStream_t s, double* phi, double* r, const double* p, const double* q, long long N,
                 const double* part_pq, const double* sc, int it, double* part_r) {
-    hipLaunchKernelGGL(k_pcg_update, dim3(PB), dim3(PT), 0, s, phi, r, p, q, N, part_pq, sc, it, part_r);
+    hipLaunchKernelGGL(k_pcg_update, dim3(PB), dim3(PT), 0, s, phi, r, p, q, (int)N, part_pq, sc, it, part_r);
 }
-void pcg_dot(hipStream_t s, const double* a, const double* b, long long N, double* part) {
-    hipLaunchKernelGGL(k_pcg_dot, dim3(PB), dim3(PT), 0, s, a, b, N, part);
+void pcg_publish_max(hipStream_t s, const double* part, double* host, unsigned* seq, unsigned v, double* sc) {
+    hipLaunchKernelGGL(k_pcg_publish_max, dim3(1), dim3(PT), 0, s, part, host, seq, v, sc);
+}
+void pcg_dot(hipStream_t s, const double* a, const double* b, long long N, double* part, const double* sc) {
+    hipLaunchKernelGGL(k_pcg_dot, dim3(PB), dim3(PT), 0, s, a, b, (int)N, part, sc ? sc + 2 : nullptr);
 }
 void pcg_dir(hipStream_t s, double* p, const double* z, long long N, const double* part_rz, double* sc, int it) {
-    hipLaunchKernelGGL(k_pcg_dir, dim3(PB), dim3(PT), 0, s, p, z, N, part_rz, sc, it);
+    hipLaunchKernelGGL(k_pcg_dir, dim3(PB), dim3(PT), 0, s, p, z, (int)N, part_rz, sc, it);
 }
 
 }  // namespace fxk
