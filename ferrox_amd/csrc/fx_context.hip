@@ -1296,11 +1296,19 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     const int PB = fxk::PCG_PARTS;
     int rc;
     if (N >= (size_t)1 << 31) return fail(x, FX_EINVAL, "conjugate gradients index cells with 32-bit integers");
+    const DBC bc = dbc_of(x, bv);
+    const DLev& L = x->lev[0].d;
+    if (!L.layered || bc.lo[0] != fxd::BC_PER || bc.lo[1] != fxd::BC_PER)
+        return fail(x, FX_EINVAL, "conjugate gradients need a layered level periodic in x and y");
     const size_t nab = (size_t)n2 * fxk::PCG_ABAR_PARTS;
-    if (!x->pcg_mem &&
-        (rc = dalloc(x, (void**)&x->pcg_mem,
-                     (4 * N + (size_t)n2 + npiv + 3 * (size_t)PB + 4 + nab) * sizeof(double))))
-        return rc;
+    const int nrz = fxk::fft_dot_parts(x->fft64);
+    const size_t nrzb = (size_t)std::max(nrz, PB);
+    if (!x->pcg_mem) {
+        if ((rc = dalloc(x, (void**)&x->pcg_mem,
+                         (4 * N + (size_t)n2 + npiv + 2 * (size_t)PB + nrzb + 4 + nab) * sizeof(double))))
+            return rc;
+        fxk::fill(x->s, x->pcg_mem, (long long)(4 * N), 0.0);   /* p and q are read (times 0) by the first pass */
+    }
     double* r = x->pcg_mem;
     double* z = r + N;
     double* p = z + N;
@@ -1310,10 +1318,8 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     double* part_pq = piv + npiv;
     double* part_r = part_pq + PB;
     double* part_rz = part_r + PB;
-    double* sc = part_rz + PB;   /* r.z of two iterations, the converged flag, the target */
+    double* sc = part_rz + nrzb;   /* r.z of two iterations, the converged flag, the target */
     double* part_ab = sc + 4;
-    const DBC bc = dbc_of(x, bv);
-    const DLev& L = x->lev[0].d;
     /* the preconditioner: the plane-averaged operator's pivots */
     fxk::pcg_abar(x->s, L, part_ab, abar);
     fxk::fft_pivots(x->s, x->fft64, piv, nullptr, abar);
@@ -1325,9 +1331,8 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     fxk::fill(x->s, sc + 2, 1, 0.0);
     fxk::fill(x->s, x->phi, (long long)N, 0.0);
     fxk::mg_resid_norm(x->s, L, bc, 1, r, x->red, x->red_out);
-    fxk::fft_solve(x->s, M, r, z, 0.0, 0.0);
-    fxk::pcg_dot(x->s, r, z, (long long)N, part_rz, sc);
-    fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, -1);
+    fxk::fft_solve_dot(x->s, M, r, z, part_rz);
+    fxk::pcg_dirmat(x->s, L, bc, z, p, q, part_rz, nrz, sc, -1, part_pq);
     double nr[2];
     if ((rc = read_red(x, 2, nr))) return rc;
     const double r0 = nr[0], bn = nr[1];
@@ -1343,14 +1348,12 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     std::vector<unsigned> vseq(maxit > 0 ? maxit : 1);
     int queued = 0;
     auto queue_iter = [&](int it) {
-        fxk::pcg_matvec(x->s, L, bc, p, q, part_pq, sc);
         fxk::pcg_update(x->s, x->phi, r, p, q, (long long)N, part_pq, sc, it, part_r);
         vseq[it] = ++x->pub_seq;
         double* slot = pcg_slot_dev(x, it % PCG_SLOTS);
         fxk::pcg_publish_max(x->s, part_r, slot, reinterpret_cast<unsigned*>(slot + 1), vseq[it], sc);
-        fxk::fft_solve(x->s, Mg, r, z, 0.0, 0.0);
-        fxk::pcg_dot(x->s, r, z, (long long)N, part_rz, sc);
-        fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, it);
+        fxk::fft_solve_dot(x->s, Mg, r, z, part_rz);
+        fxk::pcg_dirmat(x->s, L, bc, z, p, q, part_rz, nrz, sc, it, part_pq);
     };
     auto fill_window = [&](int from) {
         queued = std::max(queued, from);
@@ -1372,9 +1375,8 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
             if (nr[0] <= target) return FX_OK;
             fxk::fill(x->s, sc + 2, 1, 0.0);
             fxk::fill(x->s, sc + 3, 1, 0.0);    /* no further early stop: the host decides */
-            fxk::fft_solve(x->s, Mg, r, z, 0.0, 0.0);
-            fxk::pcg_dot(x->s, r, z, (long long)N, part_rz, sc);
-            fxk::pcg_dir(x->s, p, z, (long long)N, part_rz, sc, it);
+            fxk::fft_solve_dot(x->s, Mg, r, z, part_rz);
+            fxk::pcg_dirmat(x->s, L, bc, z, p, q, part_rz, nrz, sc, it, part_pq);
             queued = it + 1;
         }
         fill_window(it + 1);

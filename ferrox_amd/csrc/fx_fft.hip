@@ -189,11 +189,15 @@ __device__ __forceinline__ void load_tw(T* tw, const T* g, int n) {
 /* Hartley transform along x of rows j0 .. j0 + 2R - 1 of plane k =
  * blockIdx.y: rows 2s and 2s + 1 of the block are sequence s.  FWD: input b
  * (TIN) plus the Dirichlet face terms on the first and last plane, output
- * S.  !FWD: input S, output phi (TOUT), stored or, with add, added. */
+ * S.  !FWD: input S, output phi (TOUT), stored or, with add, added; with
+ * dotp, also the workgroup's partial sum of phi . dotv (the stored values)
+ * to dotp[blockIdx.y gridDim.x + blockIdx.x]. */
 template <class T, class TIN, class TOUT, bool FWD, int N1, int N2>
 __global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* __restrict__ in, TOUT* __restrict__ out,
-                                                       int R, double blo, double bhi, int add, int kz0) {
+                                                       int R, double blo, double bhi, int add, int kz0,
+                                                       const double* __restrict__ dotv, double* __restrict__ dotp) {
     extern __shared__ __align__(16) unsigned char fft_lds[];
+    __shared__ double dsh[ROWS_THR / 64 + 1];
     if (P.skip && *P.skip != 0.0) return;
     constexpr int n = N1 * N2;
     const int k = kz0 + blockIdx.y, j0 = blockIdx.x * 2 * R;
@@ -216,6 +220,7 @@ __global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* 
         }
     };
     four_step<T, N1, N2, 1>(R, tw, Yr, Yi, load, 0);
+    double dacc = 0.0;
     for (int t = threadIdx.x; t < R * n; t += blockDim.x) {
         const int s = t / n, kk = t % n;
         T h0, h1;
@@ -224,10 +229,15 @@ __global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* 
         if (FWD || !add) {
             out[o] = (TOUT)h0;
             out[o + n] = (TOUT)h1;
+            if (!FWD && dotp) dacc += (double)(TOUT)h0 * dotv[o] + (double)(TOUT)h1 * dotv[o + n];
         } else {
             out[o] = out[o] + (TOUT)h0;
             out[o + n] = out[o + n] + (TOUT)h1;
         }
+    }
+    if (!FWD && dotp) {
+        dacc = fxd::block_sum<ROWS_THR>(dacc, dsh);
+        if (threadIdx.x == 0) dotp[blockIdx.y * gridDim.x + blockIdx.x] = dacc;
     }
 }
 
@@ -438,7 +448,7 @@ void lds_limit(K* kernel, size_t lds, bool& done) {
 
 template <class T, class TIN, class TOUT, int N1, int N2>
 void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add,
-                 int kz0, int nk) {
+                 int kz0, int nk, const double* dotv, double* dotp) {
     const int R = seqs_per_block(ROWS_THR, N1, N2, P.n1);
     const size_t lds = fft_lds_bytes<T>(N1, N2, R, 1, 1);
     const dim3 g(P.n1 / (2 * R), nk);
@@ -447,10 +457,17 @@ void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, v
     else lds_limit(&k_fft_rows<T, T, TOUT, false, N1, N2>, lds, li);
     if (fwd)
         hipLaunchKernelGGL((k_fft_rows<T, TIN, T, true, N1, N2>), g, dim3(ROWS_THR), lds, s, P,
-                           static_cast<const TIN*>(in), static_cast<T*>(out), R, blo, bhi, 0, kz0);
+                           static_cast<const TIN*>(in), static_cast<T*>(out), R, blo, bhi, 0, kz0, nullptr, nullptr);
     else
         hipLaunchKernelGGL((k_fft_rows<T, T, TOUT, false, N1, N2>), g, dim3(ROWS_THR), lds, s, P,
-                           static_cast<const T*>(in), static_cast<TOUT*>(out), R, 0.0, 0.0, add, kz0);
+                           static_cast<const T*>(in), static_cast<TOUT*>(out), R, 0.0, 0.0, add, kz0, dotv, dotp);
+}
+
+/* workgroups of a rows pass over nk planes (the partials of its dot product) */
+inline int rows_blocks(int l0, int n1, int nk) {
+    const int n1f = l0 <= 2 ? 2 : l0 <= 4 ? 4 : l0 <= 6 ? 8 : l0 <= 8 ? 16 : 32;
+    const int n2f = (l0 <= 3 ? 2 : l0 <= 5 ? 4 : l0 <= 7 ? 8 : l0 <= 9 ? 16 : 32);
+    return n1 / (2 * seqs_per_block(ROWS_THR, n1f, n2f, n1)) * nk;
 }
 
 template <class T, int N1, int N2>
@@ -465,18 +482,18 @@ void cols_launch(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
 /* n = N1 N2 with N1 >= N2, both powers of two up to 32; planes kz0 .. kz0 + nk - 1 */
 template <class T, class TIN, class TOUT>
 void rows(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add,
-          int kz0, int nk) {
+          int kz0, int nk, const double* dotv = nullptr, double* dotp = nullptr) {
     if (nk <= 0) return;
     switch (P.l0) {
-    case 2: rows_launch<T, TIN, TOUT, 2, 2>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    case 3: rows_launch<T, TIN, TOUT, 4, 2>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    case 4: rows_launch<T, TIN, TOUT, 4, 4>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    case 5: rows_launch<T, TIN, TOUT, 8, 4>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    case 6: rows_launch<T, TIN, TOUT, 8, 8>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    case 7: rows_launch<T, TIN, TOUT, 16, 8>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    case 8: rows_launch<T, TIN, TOUT, 16, 16>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    case 9: rows_launch<T, TIN, TOUT, 32, 16>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
-    default: rows_launch<T, TIN, TOUT, 32, 32>(s, P, fwd, in, out, blo, bhi, add, kz0, nk); break;
+    case 2: rows_launch<T, TIN, TOUT, 2, 2>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    case 3: rows_launch<T, TIN, TOUT, 4, 2>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    case 4: rows_launch<T, TIN, TOUT, 4, 4>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    case 5: rows_launch<T, TIN, TOUT, 8, 4>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    case 6: rows_launch<T, TIN, TOUT, 8, 8>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    case 7: rows_launch<T, TIN, TOUT, 16, 8>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    case 8: rows_launch<T, TIN, TOUT, 16, 16>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    case 9: rows_launch<T, TIN, TOUT, 32, 16>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
+    default: rows_launch<T, TIN, TOUT, 32, 32>(s, P, fwd, in, out, blo, bhi, add, kz0, nk, dotv, dotp); break;
     }
 }
 
@@ -533,7 +550,7 @@ void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T f
  * transformed in S for fft_finish */
 template <class T, class TIN, class TOUT>
 void fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi, double blo, double bhi, int add,
-                    int z0, int z1, int o0 = 0, int o1 = -1) {
+                    int z0, int z1, int o0 = 0, int o1 = -1, const double* dotv = nullptr, double* dotp = nullptr) {
     const T scale = T(1) / T(P.n0 * P.n1);
     if (z1 < z0) { z0 = 0; z1 = P.n2 - 1; }
     if (o1 < o0) { o0 = 0; o1 = P.n2 - 1; }
@@ -544,7 +561,7 @@ void fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi,
     cols<T>(s, P, z0, nk);
     tri<T>(s, P, scale, z0, z1, flo, fhi);
     cols<T>(s, P, o0, o1 - o0 + 1);
-    rows<T, TIN, TOUT>(s, P, false, P.S, phi, 0.0, 0.0, add, o0, o1 - o0 + 1);
+    rows<T, TIN, TOUT>(s, P, false, P.S, phi, 0.0, 0.0, add, o0, o1 - o0 + 1, dotv, dotp);
 }
 
 }  // namespace
@@ -565,6 +582,10 @@ void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, do
                int z1, int o0, int o1) {
     fft_solve_impl<double, double, double>(s, P, b, phi, blo, bhi, 0, z0, z1, o0, o1);
 }
+void fft_solve_dot(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double* dotp) {
+    fft_solve_impl<double, double, double>(s, P, b, phi, 0.0, 0.0, 0, 0, -1, 0, -1, b, dotp);
+}
+int fft_dot_parts(const fxd::FftPlan<double>& P) { return rows_blocks(P.l0, P.n1, P.n2); }
 void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1) {
     if (o0 > 0) {
         cols<double>(s, P, 0, o0);

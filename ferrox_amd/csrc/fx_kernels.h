@@ -104,28 +104,35 @@ bool fft_size_ok(int n);
 size_t fft_pivot_count(int n0, int n1, int n2);
 void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar = nullptr);
 /* preconditioned conjugate gradients (fx_pcg.hip): reductions leave
- * PCG_PARTS partials, summed by the kernel that uses them; sc[0..1] hold r.z
- * of the last two iterations, sc[2] the converged flag (set by
- * pcg_publish_max when max|r| <= sc[3]; matvec, update, dot and dir do
- * nothing while it is set) */
+ * partials (PCG_PARTS, or the preconditioner's fft_dot_parts for r.z),
+ * summed by the kernel that uses them; sc[0..1] hold r.z of the last two
+ * iterations, sc[2] the converged flag (set by pcg_publish_max when
+ * max|r| <= sc[3]; every other pass does nothing while it is set) */
 constexpr int PCG_PARTS = 1024;
 constexpr int PCG_ABAR_PARTS = 16;   /* partials per plane of pcg_abar (part holds n2 of them) */
 /* abar[k] = mean of the A-coefficient over plane k (lateral extents powers of two) */
 void pcg_abar(hipStream_t s, const fxd::DLev& l, double* part, double* abar);
 /* the max of the PCG_PARTS partials to pinned host memory, then seq = v */
 void pcg_publish_max(hipStream_t s, const double* part, double* host, unsigned* seq, unsigned v, double* sc);
-void pcg_matvec(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* p, double* q, double* part,
-                const double* sc);
+/* the new search direction and its image, from z and A z: rz = the sum of
+ * the nrz partials part_rz (written to sc[(it + 1) & 1]); it < 0: p = z,
+ * q = A z; else p = z + beta p, q = A z + beta q, beta = rz / sc[it & 1];
+ * the partial sums of p.q.  A: the level's row operator with homogeneous
+ * faces, for a layered level periodic in x and y */
+void pcg_dirmat(hipStream_t s, const fxd::DLev& l, const fxd::DBC& bc, const double* z, double* p, double* q,
+                const double* part_rz, int nrz, double* sc, int it, double* part_pq);
 void pcg_update(hipStream_t s, double* phi, double* r, const double* p, const double* q, long long N,
                 const double* part_pq, const double* sc, int it, double* part_r);
-void pcg_dot(hipStream_t s, const double* a, const double* b, long long N, double* part, const double* sc);
-void pcg_dir(hipStream_t s, double* p, const double* z, long long N, const double* part_rz, double* sc, int it);
 /* planes of b outside [z0, z1] hold zeros (z1 < z0: nothing is known);
  * phi is formed on planes [o0, o1] (o1 < o0: all), the others by a later
  * fft_finish(o0, o1) while P.S is untouched */
 void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi,
                int z0 = 0, int z1 = -1, int o0 = 0, int o1 = -1);
 void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1);
+/* homogeneous solve phi = A^-1 b on every plane, with the partial sums of
+ * phi . b (fft_dot_parts(P) of them) to dotp */
+void fft_solve_dot(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double* dotp);
+int fft_dot_parts(const fxd::FftPlan<double>& P);
 void fft_solve(hipStream_t s, const fxd::FftPlan<float>& P, const double* b, double* phi, double blo, double bhi,
                int z0 = 0, int z1 = -1);
 void fft_solve_add(hipStream_t s, const fxd::FftPlan<float>& P, const float* r, double* phi);

@@ -1293,6 +1293,9 @@ void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a) {
  * so running beside it would halve the interior pass's occupancy); else one
  * thread per listed cell beside the interior pass on the second stream s2
  * (events e0, e1 order it after the work queued on s, and s after it). */
+#ifndef FX_FACE_ROT_GLOBAL
+#define FX_FACE_ROT_GLOBAL 0
+#endif
 #ifndef FX_FACE_PLANES_BESIDE
 #define FX_FACE_PLANES_BESIDE 0
 #endif
@@ -1310,7 +1313,7 @@ void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, co
         if (planes) {
             const dim3 gf(gx, gy, fs.nplanes);
             if (rot == 0) hipLaunchKernelGGL((k_tdgl_stream<0, true>), gf, dim3(TDS_THR), 0, q, a, 1, fs.planes);
-            else if (rot == 1) hipLaunchKernelGGL((k_tdgl_stream<1, true>), gf, dim3(TDS_THR), tab, q, a, 1, fs.planes);
+            else if (rot == 1 && !FX_FACE_ROT_GLOBAL) hipLaunchKernelGGL((k_tdgl_stream<1, true>), gf, dim3(TDS_THR), tab, q, a, 1, fs.planes);
             else hipLaunchKernelGGL((k_tdgl_stream<2, true>), gf, dim3(TDS_THR), 0, q, a, 1, fs.planes);
         } else {
             hipLaunchKernelGGL(k_tdgl_face_cells, dim3((fs.ncells + 255) / 256), dim3(256), 0, q, a);
@@ -1342,7 +1345,7 @@ void rhs_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, con
         if (planes) {
             const dim3 gf(gx, gy, fs.nplanes);
             if (rot == 0) hipLaunchKernelGGL((k_rhs_stream<0, true>), gf, dim3(TDS_THR), 0, q, a, k0, k1, 1, fs.planes);
-            else if (rot == 1) hipLaunchKernelGGL((k_rhs_stream<1, true>), gf, dim3(TDS_THR), tab, q, a, k0, k1, 1, fs.planes);
+            else if (rot == 1 && !FX_FACE_ROT_GLOBAL) hipLaunchKernelGGL((k_rhs_stream<1, true>), gf, dim3(TDS_THR), tab, q, a, k0, k1, 1, fs.planes);
             else hipLaunchKernelGGL((k_rhs_stream<2, true>), gf, dim3(TDS_THR), 0, q, a, k0, k1, 1, fs.planes);
         } else {
             hipLaunchKernelGGL(k_rhs_face_list, dim3((fs.ncells + 255) / 256), dim3(256), 0, q, a, fs.cells, fs.ncells);

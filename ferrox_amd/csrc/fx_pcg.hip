@@ -12,12 +12,13 @@
  * the carriers.  That layered operator, solved directly (its z-pivots
  * tabulated per solve), is the preconditioner.
  *
- * The kernels below are one pass each over the level: A p with the
- * multigrid's row operator (homogeneous faces) and the partial sums of p.q;
- * the updates of phi and r with the max-norm of r; r.z; the new search
- * direction.  Every reduction leaves PB partials, which the next kernel sums
- * itself (PB loads per workgroup from L2), so the scalars alpha and beta
- * never travel to the host; only max|r| does, for the stopping rule.
+ * An iteration is three passes over the level besides the preconditioner:
+ * the updates of phi and r with the max-norm of r; its publication; the new
+ * search direction p and q = A p formed together from z and A z.  r.z comes
+ * from the preconditioner's last transform (fft_solve_dot).  Every
+ * reduction leaves partials, which the next kernel sums itself (a few
+ * hundred loads per workgroup from L2), so the scalars alpha and beta never
+ * travel to the host; only max|r| does, for the stopping rule.
  */
 #include "fx_kernels.h"
 
@@ -66,39 +67,66 @@ __global__ __launch_bounds__(64) void k_pcg_abar_fin(const double* __restrict__ 
     abar[k] = v * inv_s;
 }
 
-/* q = A p (homogeneous faces) and the partial sums of p.q: bricks of 64 x 4
- * columns x MZ planes (the planes unrolled, so their loads are in flight
- * together), strided over the PB workgroups */
+/* The search direction and its image in one pass (fx_kernels.h,
+ * pcg_dirmat): A z by the row operator of a layered level periodic in x
+ * and y (mg_coef's layered branch and mg_acc, homogeneous faces, the same
+ * accumulation order), then p = z + beta p and q = A z + beta q, which is
+ * A p without a second stencil pass.  Bricks of 64 x 4 columns x MZ planes
+ * (the planes unrolled, so their loads are in flight together), strided
+ * over the PB workgroups. */
 constexpr int MZ = 4;
-__global__ __launch_bounds__(PT) void k_pcg_matvec(DLev l, DBC bc, const double* __restrict__ p, double* __restrict__ q,
-                                                  double* __restrict__ part, const double* __restrict__ done) {
+__global__ __launch_bounds__(PT) void k_pcg_dirmat(DLev l, DBC bc, const double* __restrict__ z, double* __restrict__ p,
+                                                  double* __restrict__ q, const double* __restrict__ part_rz, int nrz,
+                                                  double* __restrict__ sc, int it, double* __restrict__ part_pq) {
     __shared__ double sh[PT / 64 + 1];
-    if (*done != 0.0) return;
-    const int n0 = l.n0, n1 = l.n1, n2 = l.n2;
+    if (sc[2] != 0.0) return;
+    double v = 0.0;
+    for (int t = threadIdx.x; t < nrz; t += PT) v += part_rz[t];
+    const double rz = fxd::block_sum<PT>(v, sh);
+    const bool first = it < 0;
+    const double beta = first ? 0.0 : rz / sc[it & 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[(it + 1) & 1] = rz;   /* the slot the next update reads */
+    const int n0 = l.n0, n1 = l.n1, n2 = l.n2, s = n0 * n1;
     const int bx = (n0 + 63) / 64, by = (n1 + 3) / 4, bz = (n2 + MZ - 1) / MZ;
     const int nbrick = bx * by * bz;
+    const bool zlo_per = bc.lo[2] == fxd::BC_PER, zhi_per = bc.hi[2] == fxd::BC_PER;
+    const double dlo = bc.lo[2] == fxd::BC_DIR ? 2.0 : 0.0, dhi = bc.hi[2] == fxd::BC_DIR ? 2.0 : 0.0;
     double acc = 0.0;
     for (int t = blockIdx.x; t < nbrick; t += PB) {
         const int tx = t % bx, rest = t / bx, ty = rest % by, tz = rest / by;
         const int i = tx * 64 + (threadIdx.x & 63), j = ty * 4 + (threadIdx.x >> 6), k0 = tz * MZ;
         if (i < n0 && j < n1) {
+            const int im = i > 0 ? -1 : n0 - 1, ip = i < n0 - 1 ? 1 : -(n0 - 1);
+            const int jm = j > 0 ? -n0 : n0 * (n1 - 1), jp = j < n1 - 1 ? n0 : -n0 * (n1 - 1);
 #pragma unroll
-            for (int z = 0; z < MZ; ++z) {
-                const int k = k0 + z;
+            for (int dz = 0; dz < MZ; ++dz) {
+                const int k = k0 + dz;
                 if (k < n2) {
                     const int c = i + n0 * (j + n1 * k);
-                    double D, S;
-                    fxd::mg_row(l, p, i, j, k, bc, false, D, S);
-                    const double pc = p[c];
-                    const double v = D * pc - S;
-                    q[c] = v;
-                    acc += pc * v;
+                    const double wx = l.bxk[k] * l.ih2[0], wy = l.byk[k] * l.ih2[1];
+                    const double wzl = l.bzk[k] * l.ih2[2], wzh = l.bzk[k + 1] * l.ih2[2];
+                    const double zc = z[c];
+                    double D = -acoef(l, i, j, k, c), S = 0.0;
+                    D += wx; S += wx * z[c + im];
+                    D += wx; S += wx * z[c + ip];
+                    D += wy; S += wy * z[c + jm];
+                    D += wy; S += wy * z[c + jp];
+                    if (k > 0 || zlo_per) { D += wzl; S += wzl * z[k > 0 ? c - s : c + s * (n2 - 1)]; }
+                    else D += dlo * wzl;
+                    if (k < n2 - 1 || zhi_per) { D += wzh; S += wzh * z[k < n2 - 1 ? c + s : c - s * (n2 - 1)]; }
+                    else D += dhi * wzh;
+                    const double az = D * zc - S;
+                    const double pn = first ? zc : zc + beta * p[c];
+                    const double qn = first ? az : az + beta * q[c];
+                    p[c] = pn;
+                    q[c] = qn;
+                    acc += pn * qn;
                 }
             }
         }
     }
     acc = fxd::block_sum<PT>(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part_pq[blockIdx.x] = acc;
 }
 
 /* alpha = rz / pq (rz = sc[it & 1]); phi += alpha p; r -= alpha q; partial max|r| */
@@ -137,34 +165,6 @@ __global__ __launch_bounds__(PT) void k_pcg_publish_max(const double* __restrict
     }
 }
 
-/* partial sums of a.b */
-__global__ __launch_bounds__(PT) void k_pcg_dot(const double* __restrict__ a, const double* __restrict__ b, int N,
-                                               double* __restrict__ part, const double* __restrict__ done) {
-    __shared__ double sh[PT / 64 + 1];
-    if (done && *done != 0.0) return;
-    double acc = 0.0;
-    for (int c = blockIdx.x * PT + threadIdx.x; c < N; c += PB * PT) acc += a[c] * b[c];
-    acc = fxd::block_sum<PT>(acc, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
-
-/* rz_new = r.z from the partials; p = z + (rz_new / sc[it & 1]) p, or p = z
- * for it < 0; sc[(it + 1) & 1] = rz_new (the slot the next update reads;
- * no workgroup of this launch reads it) */
-__global__ __launch_bounds__(PT) void k_pcg_dir(double* __restrict__ p, const double* __restrict__ z, int N,
-                                               const double* __restrict__ part_rz, double* __restrict__ sc, int it) {
-    __shared__ double sh[PT / 64 + 1];
-    if (sc[2] != 0.0) return;
-    const double rz = partial_sum(part_rz, sh);
-    if (blockIdx.x == 0 && threadIdx.x == 0) sc[(it + 1) & 1] = rz;
-    if (it < 0) {
-        for (int c = blockIdx.x * PT + threadIdx.x; c < N; c += PB * PT) p[c] = z[c];
-    } else {
-        const double beta = rz / sc[it & 1];
-        for (int c = blockIdx.x * PT + threadIdx.x; c < N; c += PB * PT) p[c] = z[c] + beta * p[c];
-    }
-}
-
 inline int log2i(int n) {
     int l = 0;
     while ((1 << l) < n) ++l;
@@ -180,8 +180,9 @@ void pcg_abar(hipStream_t s, const DLev& l, double* part, double* abar) {
     hipLaunchKernelGGL(k_pcg_abar_fin, dim3((l.n2 + 63) / 64), dim3(64), 0, s, part, l.n2, 1.0 / ((double)l.n0 * l.n1),
                        abar);
 }
-void pcg_matvec(hipStream_t s, const DLev& l, const DBC& bc, const double* p, double* q, double* part, const double* sc) {
-    hipLaunchKernelGGL(k_pcg_matvec, dim3(PB), dim3(PT), 0, s, l, bc, p, q, part, sc + 2);
+void pcg_dirmat(hipStream_t s, const DLev& l, const DBC& bc, const double* z, double* p, double* q,
+                const double* part_rz, int nrz, double* sc, int it, double* part_pq) {
+    hipLaunchKernelGGL(k_pcg_dirmat, dim3(PB), dim3(PT), 0, s, l, bc, z, p, q, part_rz, nrz, sc, it, part_pq);
 }
 void pcg_update(hipStream_t s, double* phi, double* r, const double* p, const double* q, long long N,
                 const double* part_pq, const double* sc, int it, double* part_r) {
@@ -189,12 +190,6 @@ void pcg_update(hipStream_t s, double* phi, double* r, const double* p, const do
 }
 void pcg_publish_max(hipStream_t s, const double* part, double* host, unsigned* seq, unsigned v, double* sc) {
     hipLaunchKernelGGL(k_pcg_publish_max, dim3(1), dim3(PT), 0, s, part, host, seq, v, sc);
-}
-void pcg_dot(hipStream_t s, const double* a, const double* b, long long N, double* part, const double* sc) {
-    hipLaunchKernelGGL(k_pcg_dot, dim3(PB), dim3(PT), 0, s, a, b, (int)N, part, sc ? sc + 2 : nullptr);
-}
-void pcg_dir(hipStream_t s, double* p, const double* z, long long N, const double* part_rz, double* sc, int it) {
-    hipLaunchKernelGGL(k_pcg_dir, dim3(PB), dim3(PT), 0, s, p, z, (int)N, part_rz, sc, it);
 }
 
 }  // namespace fxk
