@@ -436,13 +436,16 @@ size_t fft_lds_bytes(int N1, int N2, int R, int pad, int twl) {
     return ((twl ? (size_t)N1 * N2 : 0) + 2 * (size_t)R * N2 * (N1 + pad)) * sizeof(T);
 }
 
-/* raise the dynamic LDS limit of a kernel once, where a launch needs more than 64 KB */
+/* raise the dynamic LDS limit of a kernel to what a launch needs, where that
+ * is more than 64 KB (the limit plus the kernel's static LDS must stay
+ * within the 160 KB of a CU, so it is raised to the need, not to 160 KB);
+ * set: the largest limit set so far */
 template <class K>
-void lds_limit(K* kernel, size_t lds, bool& done) {
-    if (lds > 64 * 1024 && !done) {
+void lds_limit(K* kernel, size_t lds, size_t& set) {
+    if (lds > 64 * 1024 && lds > set) {
         (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
-        done = true;
+                                  (int)lds);
+        set = lds;
     }
 }
 
@@ -452,7 +455,7 @@ void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, v
     const int R = seqs_per_block(ROWS_THR, N1, N2, P.n1);
     const size_t lds = fft_lds_bytes<T>(N1, N2, R, 1, 1);
     const dim3 g(P.n1 / (2 * R), nk);
-    static bool lf = false, li = false;
+    static size_t lf = 0, li = 0;
     if (fwd) lds_limit(&k_fft_rows<T, TIN, T, true, N1, N2>, lds, lf);
     else lds_limit(&k_fft_rows<T, T, TOUT, false, N1, N2>, lds, li);
     if (fwd)
@@ -474,7 +477,7 @@ template <class T, int N1, int N2>
 void cols_launch(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
     const int R = seqs_per_block(COLS_THR, N1, N2, P.n0);
     const size_t lds = fft_lds_bytes<T>(N1, N2, R, FX_FFT_COLS_PAD, FX_FFT_COLS_TWL);
-    static bool lc = false;
+    static size_t lc = 0;
     lds_limit(&k_fft_cols<T, N1, N2>, lds, lc);
     hipLaunchKernelGGL((k_fft_cols<T, N1, N2>), dim3(P.n0 / (2 * R), nk), dim3(COLS_THR), lds, s, P, R, kz0);
 }
