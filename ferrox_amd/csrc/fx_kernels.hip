@@ -1292,9 +1292,13 @@ void tdgl_rhs_stream(hipStream_t s, const TdglRhsArgs& a) {
  * the interior pass on s (it needs twice the interior kernel's registers,
  * so running beside it would halve the interior pass's occupancy); else one
  * thread per listed cell beside the interior pass on the second stream s2
- * (events e0, e1 order it after the work queued on s, and s after it). */
+ * (events e0, e1 order it after the work queued on s, and s after it).
+ * FX_FACE_ROT_GLOBAL 1: the face-plane kernels read the rotation rows from
+ * the cache instead of staging the whole table in LDS (one plane per
+ * workgroup leaves nothing to amortise the staging over; 212.6 against
+ * 214.3 us per step on the 256x256x128 polycrystal, same box). */
 #ifndef FX_FACE_ROT_GLOBAL
-#define FX_FACE_ROT_GLOBAL 0
+#define FX_FACE_ROT_GLOBAL 1
 #endif
 #ifndef FX_FACE_PLANES_BESIDE
 #define FX_FACE_PLANES_BESIDE 0
