@@ -100,6 +100,11 @@ struct fx_ctx {
     int* face_list = nullptr;      /* FE cells without centred stencils (global indices), then the planes holding them */
     int nface = 0;
     int nface_planes = 0;
+    /* the same without the cells next to a z face only (MAT_ZFACE_LO/HI),
+     * which the streamed interior passes update */
+    int* face2_list = nullptr;
+    int nface2 = 0;
+    int nface2_planes = 0;
     double* noalloc_scratch = nullptr;   /* device address of the static workspace on this context's GPU */
     /* phi_z0 <= phi_z1: the last direct solve formed phi on those planes
      * only (what the TDGL update reads); the other planes' solution waits,
@@ -772,7 +777,9 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
  * cells whose polarization stencils and grad(phi) are all the centred ones:
  * the 18 face and edge neighbours are FE cells inside the domain or across a
  * periodic face, and k is neither the first nor the last plane (where the
- * z-derivative of phi is one-sided).  The host shadow keeps bits 0-2. */
+ * z-derivative of phi is one-sided); bit 4 or 5 (MAT_ZFACE_LO / _HI) on the
+ * FE cells of such planes and lateral neighbourhoods whose only masked
+ * stencils are the z ones of one side.  The host shadow keeps bits 0-2. */
 void mark_interior_fe(const fx_ctx* x, std::vector<uint8_t>& dm) {
     const int nx = x->n[0], ny = x->n[1], nz = x->n[2];
     const bool px = x->in.is_periodic[0], py = x->in.is_periodic[1];
@@ -791,7 +798,25 @@ void mark_interior_fe(const fx_ctx* x, std::vector<uint8_t>& dm) {
                             if (nn == 0 || nn == 3) continue;
                             ok = (x->h_mat[gidx(x, i + di, j + dj, k + dk)] & 3) == 0;
                         }
-                if (ok) dm[g] |= fxd::MAT_INTERIOR_FE;
+                if (ok) { dm[g] |= fxd::MAT_INTERIOR_FE; continue; }
+                /* next to a z face only: the cell's in-plane neighbours FE, and
+                 * of the five cells below it (its own and its four lateral
+                 * neighbours' z neighbours) and the five above, one group all FE
+                 * and the other none, on a side whose flag is 0..3 */
+                bool plane_ok = true;
+                int below = 0, above = 0;
+                for (int dj = -1; dj <= 1; ++dj)
+                    for (int di = -1; di <= 1; ++di) {
+                        if ((di || dj) && (x->h_mat[gidx(x, i + di, j + dj, k)] & 3)) plane_ok = false;
+                        if (di && dj) continue;
+                        below += (x->h_mat[gidx(x, i + di, j + dj, k - 1)] & 3) == 0;
+                        above += (x->h_mat[gidx(x, i + di, j + dj, k + 1)] & 3) == 0;
+                    }
+                if (!plane_ok) continue;
+                if (below == 0 && above == 5 && x->in.P_BC_flag_lo[2] >= 0 && x->in.P_BC_flag_lo[2] <= 3)
+                    dm[g] |= fxd::MAT_ZFACE_LO;
+                else if (above == 0 && below == 5 && x->in.P_BC_flag_hi[2] >= 0 && x->in.P_BC_flag_hi[2] <= 3)
+                    dm[g] |= fxd::MAT_ZFACE_HI;
             }
         }
 }
@@ -801,23 +826,32 @@ int upload_mat(fx_ctx* x) {
     HIPCHK(x, hipMemcpyAsync(x->mat, dm.data(), dm.size(), hipMemcpyHostToDevice, x->s));
     /* FE cells whose stencils are not all centred: the fused TDGL + RHS
      * update forms them on their own (k_tdgl_face_cells, k_rhs_face_cells) */
-    std::vector<int> face;
+    std::vector<int> face, face2;
     for (int k = 0; k < x->n[2]; ++k)
         for (int j = 0; j < x->n[1]; ++j)
             for (int i = 0; i < x->n[0]; ++i) {
                 const uint8_t r = dm[gidx(x, i, j, k)];
                 if ((r & 3) == 0 && !(r & fxd::MAT_INTERIOR_FE)) face.push_back(i + x->n[0] * (j + x->n[1] * k));
+                if ((r & 3) == 0 && !(r & (fxd::MAT_INTERIOR_FE | fxd::MAT_ZFACE_LO | fxd::MAT_ZFACE_HI)))
+                    face2.push_back(i + x->n[0] * (j + x->n[1] * k));
             }
-    if (x->face_list) { (void)hipFree(x->face_list); x->face_list = nullptr; }
-    x->nface = (int)face.size();
     const int plane = x->n[0] * x->n[1];
-    x->nface_planes = 0;
-    for (size_t q = 0, n = face.size(); q < n; ++q)   /* face is in plane order */
-        if (q == 0 || face[q] / plane != face[q - 1] / plane) { face.push_back(face[q] / plane); ++x->nface_planes; }
-    if (!face.empty()) {
-        HIPCHK(x, hipMalloc((void**)&x->face_list, face.size() * sizeof(int)));
-        HIPCHK(x, hipMemcpy(x->face_list, face.data(), face.size() * sizeof(int), hipMemcpyHostToDevice));
-    }
+    /* a list of cells (in plane order) followed by the planes that hold them */
+    auto upload = [&](std::vector<int>& f, int*& dev, int& ncells, int& nplanes) -> int {
+        if (dev) { (void)hipFree(dev); dev = nullptr; }
+        ncells = (int)f.size();
+        nplanes = 0;
+        for (int q = 0; q < ncells; ++q)
+            if (q == 0 || f[q] / plane != f[q - 1] / plane) { f.push_back(f[q] / plane); ++nplanes; }
+        if (!f.empty()) {
+            HIPCHK(x, hipMalloc((void**)&dev, f.size() * sizeof(int)));
+            HIPCHK(x, hipMemcpy(dev, f.data(), f.size() * sizeof(int), hipMemcpyHostToDevice));
+        }
+        return FX_OK;
+    };
+    int rc = upload(face, x->face_list, x->nface, x->nface_planes);
+    if (!rc) rc = upload(face2, x->face2_list, x->nface2, x->nface2_planes);
+    if (rc) return rc;
     x->has_sc = false;
     for (int k = 0; k < x->n[2]; ++k)
         for (int j = 0; j < x->n[1]; ++j)
@@ -1430,8 +1464,8 @@ RhsArgs rhs_args(fx_ctx* x, double* const* P) {
 
 fxk::FaceSet face_set(const fx_ctx* x) {
     fxk::FaceSet f;
-    f.cells = x->face_list; f.ncells = x->nface;
-    f.planes = x->face_list ? x->face_list + x->nface : nullptr; f.nplanes = x->nface_planes;
+    f.cells = x->face2_list; f.ncells = x->nface2;
+    f.planes = x->face2_list ? x->face2_list + x->nface2 : nullptr; f.nplanes = x->nface2_planes;
     return f;
 }
 
@@ -1625,7 +1659,7 @@ int steady_check(fx_ctx* x, int step, double* metric) {
 void free_all(fx_ctx* x) {
     auto F = [](void* p) { if (p) hipFree(p); };
     for (int d = 0; d < 3; ++d) { F(x->P[d]); F(x->Pn[d]); F(x->E[d]); F(x->GL[d]); F(x->GL1[d]); }
-    F(x->phi); F(x->phi_old); F(x->phi_prev); F(x->rhs); F(x->rhs_nx); F(x->face_list); F(x->rho); F(x->e_den); F(x->h_den);
+    F(x->phi); F(x->phi_old); F(x->phi_prev); F(x->rhs); F(x->rhs_nx); F(x->face_list); F(x->face2_list); F(x->rho); F(x->e_den); F(x->h_den);
     F(x->alpha_cc); F(x->phidiff); F(x->eps); F(x->mat); F(x->rot_idx); F(x->rot_tab); F(x->rot_atab);
     F(x->diff.ck); F(x->diff.segP); F(x->diff.segPhi); F(x->diff.lam[0]); F(x->diff.lam[1]); F(x->diff.s);
     F(x->diff.fin); F(x->diff.acc); F(x->diff.m); F(x->diff.part);

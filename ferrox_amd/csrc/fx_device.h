@@ -48,6 +48,44 @@ struct PView {
  * grad(phi) are all centred (see upload_mat). */
 constexpr int MAT_INTERIOR_FE = 8;
 
+/* Material id bits 4 and 5 (device copy only): an FE cell whose stencils are
+ * the centred ones except in z, where the cells below it (MAT_ZFACE_LO) or
+ * above it (MAT_ZFACE_HI) and below or above its four lateral neighbours
+ * are not FE, and the P_BC_flag of that z side is 0..3 (see upload_mat).
+ * Its masked z stencils of DerivativeAlgorithm.H are then linear in P at
+ * planes k - 1, k, k + 1 with coefficients of the flag alone (zface_coef). */
+constexpr int MAT_ZFACE_LO = 16;
+constexpr int MAT_ZFACE_HI = 32;
+
+/* The masked z stencils of a MAT_ZFACE_LO (hi = false) or MAT_ZFACE_HI cell
+ * as linear forms (h = dz, F the P values at planes k - 1, k, k + 1):
+ *   h^2 DoubleDPDz = z[0] F_k + z[1] F_k+1 + z[2] F_k-1   (DerivativeAlgorithm.H:257-467),
+ *   2 h DPDz(q)    = z[3] F_k + z[4] F_k+1 + z[5] F_k-1   (:45-252),
+ * the latter at the cell and at its lateral neighbours q, which take the
+ * same branch; flag 3 gives the centred forms.  The products reassociate
+ * the reference's expressions (rounding-level differences). */
+__host__ __device__ inline void zface_coef(int flag, bool hi, double h, double lambda, double z[6]) {
+    const double s = hi ? h / (1 - h / 2 / lambda) / lambda : h / (1 + h / 2 / lambda) / lambda;
+    switch (flag) {
+    case 0:
+        z[0] = -4.0; z[1] = hi ? 0.0 : 4.0 / 3.0; z[2] = hi ? 4.0 / 3.0 : 0.0;
+        z[3] = hi ? -2.0 : 2.0; z[4] = hi ? 0.0 : 2.0 / 3.0; z[5] = hi ? -2.0 / 3.0 : 0.0;
+        break;
+    case 1:
+        z[0] = hi ? s - 1.0 : -s - 1.0; z[1] = hi ? 0.0 : 1.0; z[2] = hi ? 1.0 : 0.0;
+        z[3] = hi ? s + 1.0 : s - 1.0; z[4] = hi ? 0.0 : 1.0; z[5] = hi ? -1.0 : 0.0;
+        break;
+    case 2:
+        z[0] = -1.0; z[1] = hi ? 0.0 : 1.0; z[2] = hi ? 1.0 : 0.0;
+        z[3] = hi ? 1.0 : -1.0; z[4] = hi ? 0.0 : 1.0; z[5] = hi ? -1.0 : 0.0;
+        break;
+    default:
+        z[0] = -2.0; z[1] = 1.0; z[2] = 1.0;
+        z[3] = 0.0; z[4] = 1.0; z[5] = -1.0;
+        break;
+    }
+}
+
 /* Material id on the grown box. */
 struct MView {
     const uint8_t* __restrict__ m;

@@ -888,6 +888,9 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int z
     }
     double wu[9];   /* ROT 0: the uniform rotation's weights */
     if (ROT == 0 && !FACES) interior_weights(a.rot.Ru, a.g11, g44m, ih, wu);
+    double zl[6], zh[6];   /* the masked z stencils of the cells next to a z face (MAT_ZFACE_LO/HI) */
+    zface_coef(a.pbc.lo[2], false, dx[2], a.pbc.lambda, zl);
+    zface_coef(a.pbc.hi[2], true, dx[2], a.pbc.lambda, zh);
     const int nx = a.g.n[0], ny = a.g.n[1], nz = a.g.n[2];
     int bx, by, bz;
     stream_tile(bx, by, bz);
@@ -927,7 +930,7 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int z
         const int rr = Rr[((p + 3) % 3) * TDS_THR + threadIdx.x];
         const int raw_c = rr & 255, row_c = rr >> 8;
         const int s0 = ((p - 1) & 3) * TDS_HP, s1 = (p & 3) * TDS_HP, s2 = ((p + 1) & 3) * TDS_HP;
-        if (!FACES && in_dom && (raw_c & 3) == 0 && (raw_c & MAT_INTERIOR_FE)) {
+        if (!FACES && in_dom && (raw_c & 3) == 0 && (raw_c & (MAT_INTERIOR_FE | MAT_ZFACE_LO | MAT_ZFACE_HI))) {
             const int c = ci + nx * (cj + ny * p);
             const double* f0 = Fw + s0;
             const double* f1 = Fw + s1;
@@ -961,10 +964,23 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int z
             if (a.incG == 1) {
                 const double Dxx = (F(1, 0, 0) - 2. * r) + F(-1, 0, 0);
                 const double Dyy = (F(0, 1, 0) - 2. * r) + F(0, -1, 0);
-                const double Dzz = (F(0, 0, 1) - 2. * r) + F(0, 0, -1);
+                double Dzz = (F(0, 0, 1) - 2. * r) + F(0, 0, -1);
                 const double Dxy = (F(1, 1, 0) - F(1, -1, 0)) - (F(-1, 1, 0) - F(-1, -1, 0));
-                const double Dyz = (F(0, 1, 1) - F(0, 1, -1)) - (F(0, -1, 1) - F(0, -1, -1));
-                const double Dxz = (F(1, 0, 1) - F(1, 0, -1)) - (F(-1, 0, 1) - F(-1, 0, -1));
+                double Dyz = (F(0, 1, 1) - F(0, 1, -1)) - (F(0, -1, 1) - F(0, -1, -1));
+                double Dxz = (F(1, 0, 1) - F(1, 0, -1)) - (F(-1, 0, 1) - F(-1, 0, -1));
+                if (!(raw_c & MAT_INTERIOR_FE)) {
+                    /* next to a z face: the masked z stencils as linear forms
+                     * (h^2 DoubleDPDz, and 2h DPDz at the lateral neighbours of
+                     * the mixed entries) */
+                    const bool lo = (raw_c & MAT_ZFACE_LO) != 0;
+                    double z[6];
+#pragma unroll
+                    for (int q = 0; q < 6; ++q) z[q] = lo ? zl[q] : zh[q];
+                    Dzz = z[0] * r + z[1] * F(0, 0, 1) + z[2] * F(0, 0, -1);
+                    auto D2 = [&](int di, int dj) { return z[3] * F(di, dj, 0) + z[4] * F(di, dj, 1) + z[5] * F(di, dj, -1); };
+                    Dyz = D2(0, 1) - D2(0, -1);
+                    Dxz = D2(1, 0) - D2(-1, 0);
+                }
                 const double gr = w[0] * Dxx + w[1] * Dyy + w[2] * Dzz + w[3] * Dxy + w[4] * Dyz + w[5] * Dxz;
                 G = Gam * gr;                          /* -Gam * grad_r_of, grad_r_of = -gr */
             }
@@ -979,7 +995,7 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_tdgl_stream(TdglRhsArgs A, int z
             if (a.incG == 1) g += G;
             if (a.incE == 1) g += El;
             a.Pout[2][c] = r + a.dt * g;
-        } else if (FACES && in_dom && (raw_c & 3) == 0 && !(raw_c & MAT_INTERIOR_FE)) {
+        } else if (FACES && in_dom && (raw_c & 3) == 0 && !(raw_c & (MAT_INTERIOR_FE | MAT_ZFACE_LO | MAT_ZFACE_HI))) {
             /* an FE cell next to a face: the masked stencils (tdgl_scalar_cell),
              * operands from the window */
             const int c = ci + nx * (cj + ny * p);
@@ -1133,13 +1149,16 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, in
     }
     __syncthreads();
     const double* dx = a.g.dx;
+    double zl[6], zh[6];   /* the masked z stencils of the cells next to a z face (MAT_ZFACE_LO/HI) */
+    zface_coef(a.pbc.lo[2], false, dx[2], a.pbc.lambda, zl);
+    zface_coef(a.pbc.hi[2], true, dx[2], a.pbc.lambda, zh);
     auto rhs_of = [&](int p) {
         const int rr = Rr[((p + 3) % 3) * TDS_THR + threadIdx.x];
         const int raw_c = rr & 255, row_c = rr >> 8;
         const double* p0 = Pw + ((p - 1) & 3) * TDS_HP;
         const double* p1 = Pw + (p & 3) * TDS_HP;
         const double* p2 = Pw + ((p + 1) & 3) * TDS_HP;
-        if (!FACES && in_dom && (raw_c & 3) == 0 && (raw_c & MAT_INTERIOR_FE)) {
+        if (!FACES && in_dom && (raw_c & 3) == 0 && (raw_c & (MAT_INTERIOR_FE | MAT_ZFACE_LO | MAT_ZFACE_HI))) {
             const int c = ci + nx * (cj + ny * p);
             double R6, R7, R8;
             if (ROT == 1) { R6 = Rt[3 * row_c]; R7 = Rt[3 * row_c + 1]; R8 = Rt[3 * row_c + 2]; }
@@ -1147,13 +1166,18 @@ __global__ __launch_bounds__(TDS_THR, 2) void k_rhs_stream(RhsArgs a, int k0, in
             else { R6 = a.rot.Ru[6]; R7 = a.rot.Ru[7]; R8 = a.rot.Ru[8]; }
             const double gx = (p1[wo + 1] - p1[wo - 1]) * (0.5 / dx[0]);
             const double gy = (p1[wo + TDS_HX] - p1[wo - TDS_HX]) * (0.5 / dx[1]);
-            const double gz = (p2[wo] - p0[wo]) * (0.5 / dx[2]);
+            double gz = (p2[wo] - p0[wo]) * (0.5 / dx[2]);
+            if (!(raw_c & MAT_INTERIOR_FE)) {   /* next to a z face: 2h DPDz as a linear form */
+                const bool lo = (raw_c & MAT_ZFACE_LO) != 0;
+                gz = ((lo ? zl[3] : zh[3]) * p1[wo] + (lo ? zl[4] : zh[4]) * p2[wo] + (lo ? zl[5] : zh[5]) * p0[wo]) *
+                     (0.5 / dx[2]);
+            }
             double x = 0.0;
             if (R6 != 0.0) x += R6 * gx;
             if (R7 != 0.0) x += R7 * gy;
             x += R8 * gz;
             a.rhs[c] = -x;
-        } else if (FACES && in_dom && (raw_c & 3) == 0 && !(raw_c & MAT_INTERIOR_FE)) {
+        } else if (FACES && in_dom && (raw_c & 3) == 0 && !(raw_c & (MAT_INTERIOR_FE | MAT_ZFACE_LO | MAT_ZFACE_HI))) {
             /* an FE cell next to a face: the masked branch (k_rhs_face_list) on the window */
             const int c = ci + nx * (cj + ny * p);
             double R6, R7, R8;
@@ -1320,7 +1344,10 @@ void tdgl_stream(hipStream_t s, hipStream_t s2, hipEvent_t e0, hipEvent_t e1, co
             else if (rot == 1 && !FX_FACE_ROT_GLOBAL) hipLaunchKernelGGL((k_tdgl_stream<1, true>), gf, dim3(TDS_THR), tab, q, a, 1, fs.planes);
             else hipLaunchKernelGGL((k_tdgl_stream<2, true>), gf, dim3(TDS_THR), 0, q, a, 1, fs.planes);
         } else {
-            hipLaunchKernelGGL(k_tdgl_face_cells, dim3((fs.ncells + 255) / 256), dim3(256), 0, q, a);
+            TdglRhsArgs af = a;   /* the cells of fs (the update's own list may hold more) */
+            af.face = fs.cells;
+            af.nface = fs.ncells;
+            hipLaunchKernelGGL(k_tdgl_face_cells, dim3((fs.ncells + 255) / 256), dim3(256), 0, q, af);
         }
     };
     if (beside) {
