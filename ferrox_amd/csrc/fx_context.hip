@@ -728,6 +728,7 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
     const size_t ntab = 2 * (size_t)n0 + 2 * (size_t)n1;
     const size_t npiv = fxk::fft_pivot_count(n0, n1, n2);
     const size_t total = ntab + x->N + npiv;
+    const bool twoway = n2 == 128 && ((size_t)n0 * n1) % 128 == 0;   /* upward pivots too (FP64) */
     std::vector<double> tab(ntab);
     double* tw0 = tab.data();
     double* tw1 = tw0 + n0;
@@ -742,7 +743,7 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
     fill_dir(n0, tw0, mux);
     fill_dir(n1, tw1, muy);
     int rc;
-    if (!x->fft_mem64 && (rc = dalloc(x, (void**)&x->fft_mem64, total * sizeof(double)))) return rc;
+    if (!x->fft_mem64 && (rc = dalloc(x, (void**)&x->fft_mem64, (total + (twoway ? npiv : 0)) * sizeof(double)))) return rc;
     HIPCHK(x, hipMemcpy(x->fft_mem64, tab.data(), ntab * sizeof(double), hipMemcpyHostToDevice));
     const bool fp32 = in.precision == FX_PREC_FP32_REFINE && !x->lev32.empty() && x->lev32[0].bxk;
     if (fp32) {
@@ -761,10 +762,15 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
         P.dlo = dlo; P.dhi = dhi;
         P.S = base + ntab;
         P.piv = base + ntab + x->N;
+        P.pivu = nullptr;
     };
     fill(x->fft64, x->fft_mem64, d.bxk, d.byk, d.bzk);
     if (fp32) fill(x->fft32, x->fft_mem32, x->lev32[0].bxk, x->lev32[0].byk, x->lev32[0].bzk);
     fxk::fft_pivots(x->s, x->fft64, x->fft_mem64 + ntab + x->N, fp32 ? x->fft_mem32 + ntab + x->N : nullptr);
+    if (twoway) {
+        fxk::fft_pivots_up(x->s, x->fft64, x->fft_mem64 + total);
+        x->fft64.pivu = x->fft_mem64 + total;
+    }
     HIPCHK(x, hipStreamSynchronize(x->s));
     /* the inhomogeneous face term of the row operator is (2 w) v, w = beta_face / hz^2 */
     x->fft_w2lo = dlo ? 2.0 * (bz_lo * d.ih2[2]) : 0.0;
@@ -1359,6 +1365,7 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     fxk::fft_pivots(x->s, x->fft64, piv, nullptr, abar);
     FftPlan<double> M = x->fft64;
     M.piv = piv;
+    M.pivu = nullptr;   /* the upward pivots are of the operator without the plane means */
     FftPlan<double> Mg = M;   /* the preconditioner as the iterations use it: idle once converged */
     Mg.skip = sc + 2;
     /* r0 = b - A 0 with the face terms; z0, p0 */

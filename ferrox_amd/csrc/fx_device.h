@@ -389,6 +389,10 @@ struct FftPlan {
      * piv[k (n1/2+1) (n0/2+1) + myc (n0/2+1) + mxc] (mu_m = mu_n-m, so the
      * four mirror modes share one system) */
     const T* piv;
+    /* non-null: the pivots of the same systems eliminated upward from the last
+     * plane, 1 / (d_k - c_k a'_k+1) with a'_k = a_k pivu_k, laid out as piv;
+     * the z solve then runs from both ends and meets in the middle */
+    const T* pivu;
     /* non-null: every pass of the solve does nothing while *skip != 0 (a
      * device flag set when an iterative solve around it has converged) */
     const double* skip;

@@ -38,6 +38,8 @@
  * n0 and n1 are powers of two (4 .. 1024).  The result is exact to rounding:
  * no iteration and no stopping test.
  */
+#include <type_traits>
+
 #include "fx_kernels.h"
 
 namespace {
@@ -312,6 +314,176 @@ __global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __rest
     }
 }
 
+/* Upward pivots of the same systems (FftPlan::pivu): eliminating from the
+ * last plane, g_k = d_k - c_k a'_k+1 with a'_k = a_k / g_k, stored as 1 / g_k
+ * (FP64, correctly rounded divisions; no Helmholtz term). */
+__global__ __launch_bounds__(256) void k_fft_pivots_up(FftPlan<double> P, double* __restrict__ pivu) {
+    const int n0c = P.n0 / 2 + 1, n1c = P.n1 / 2 + 1, n2 = P.n2;
+    const int q = blockIdx.x * 256 + threadIdx.x;
+    if (q >= n0c * n1c) return;
+    const int mxc = q % n0c, myc = q / n0c;
+    const double lx = P.mux[mxc] * P.ih2[0], ly = P.muy[myc] * P.ih2[1], iz = P.ih2[2];
+    const size_t ps = (size_t)n0c * n1c;
+    double ap = 0.0;   /* a'_k+1 */
+    for (int k = n2 - 1; k >= 0; --k) {
+        const double wzl = P.bzk[k] * iz, wzh = P.bzk[k + 1] * iz;
+        double d = P.bxk[k] * lx;
+        d += P.byk[k] * ly;
+        d += (k > 0 ? wzl : (P.dlo ? 2.0 * wzl : 0.0)) + (k < n2 - 1 ? wzh : (P.dhi ? 2.0 * wzh : 0.0));
+        const double a = k > 0 ? -wzl : 0.0, c = k < n2 - 1 ? -wzh : 0.0;
+        const double r = 1.0 / (d - c * ap);
+        ap = a * r;
+        pivu[k * ps + q] = r;
+    }
+}
+
+/* The z systems of 2H planes solved from both ends (FP64, FftPlan::pivu
+ * set): in a workgroup of 256 threads, threads 0..127 eliminate planes
+ * 0..H-1 of 128 modes downward (y_k = (b_k - a_k y_k-1) r_k, the Thomas
+ * recurrence of k_fft_tri) and threads 128..255 planes 2H-1..H of the same
+ * modes upward (u_k = (b_k - c_k u_k+1) s_k); they meet at planes H-1 and H,
+ *   x_H-1 = (y_H-1 - c'_H-1 u_H) / (1 - c'_H-1 a'_H),  x_H = u_H - a'_H x_H-1,
+ * (c'_k = c_k r_k, a'_k = a_k s_k) through LDS, and substitute back outward.
+ * Each thread keeps H values (half in registers, half in LDS: 64 KB per
+ * workgroup for H = 64) instead of 2H, so two workgroups fit a CU where
+ * k_fft_tri fits one.  Right-hand side, skipped planes and face terms as
+ * k_fft_tri. */
+#ifndef FX_TRI2_BLK
+#define FX_TRI2_BLK 8
+#endif
+#ifndef FX_TRI2_LDS_EIGHTHS
+#define FX_TRI2_LDS_EIGHTHS 4   /* eighths of a half's H values that wait in LDS */
+#endif
+/* One half of the two-ended z solve (UP false: planes 0..H-1 downward, true:
+ * planes 2H-1..H upward), step j at plane k = UP ? 2H-1-j : j; the first LB
+ * steps' values wait in LDS (lv[j 256]), the rest in registers (yr). */
+template <int H, int LB, bool UP>
+__device__ __forceinline__ void tri2_sweep(const double* __restrict__ col, const double* __restrict__ rp, size_t s,
+                                           size_t ps, const double* tab, double* lv, double (&yr)[H - LB],
+                                           int zlo, int zhi, bool any, int z0, int z1, bool c0, double scale,
+                                           double flo, double fhi, double& y, double& rlast) {
+    constexpr int TB = FX_TRI2_BLK, NB = H / TB, N2 = 2 * H;
+    double sb[2][TB], rb[2][TB];
+    auto load = [&](int b, double (&sv)[TB], double (&rv)[TB]) {
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
+            const int kc = k < zlo ? zlo : k > zhi ? zhi : k;
+            sv[q] = col[(size_t)kc * s];
+            rv[q] = rp[(size_t)k * ps];
+        }
+    };
+    y = 0.0;
+    load(0, sb[0], rb[0]);
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        if (b + 1 < NB) load(b + 1, sb[(b + 1) & 1], rb[(b + 1) & 1]);
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
+            double bk = (any && k >= z0 && k <= z1) ? sb[b & 1][q] * scale : 0.0;
+            if (c0) {
+                if (k == 0) bk += flo;
+                if (k == N2 - 1) bk += fhi;
+            }
+            /* downward: the coupling to the plane above (a_k); upward: below (c_k) */
+            y = (bk - tab[2 * k + (UP ? 1 : 0)] * y) * rb[b & 1][q];
+            if (j < LB) lv[(size_t)j * 256] = y;
+            else yr[j - LB] = y;
+        }
+    }
+    rlast = rb[(NB - 1) & 1][TB - 1];
+}
+
+/* the back substitution of that half from x of its step H-1's neighbour
+ * across the middle (xn): downward x_k = y_k - c'_k x_k+1, upward x_k = u_k -
+ * a'_k x_k-1; the downward half's step H-1 is the meeting's x_H-1 itself */
+template <int H, int LB, bool UP>
+__device__ __forceinline__ void tri2_back(double* col, const double* __restrict__ rp, size_t s, size_t ps,
+                                          const double* tab, const double* lv, const double (&yr)[H - LB], double xn) {
+    constexpr int TB = FX_TRI2_BLK, NB = H / TB, N2 = 2 * H;
+    double rb[2][TB];
+    auto loadr = [&](int b, double (&rv)[TB]) {
+#pragma unroll
+        for (int q = 0; q < TB; ++q) {
+            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
+            rv[q] = rp[(size_t)k * ps];
+        }
+    };
+    loadr(NB - 1, rb[(NB - 1) & 1]);
+#pragma unroll
+    for (int b = NB - 1; b >= 0; --b) {
+        if (b > 0) loadr(b - 1, rb[(b - 1) & 1]);
+#pragma unroll
+        for (int q = TB - 1; q >= 0; --q) {
+            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
+            if (UP || j != H - 1) {
+                const double yj = j < LB ? lv[(size_t)j * 256] : yr[j - LB];
+                xn = yj - (tab[2 * k + (UP ? 0 : 1)] * rb[b & 1][q]) * xn;
+            }
+            col[(size_t)k * s] = xn;
+        }
+    }
+}
+
+/* The z systems of 2H planes solved from both ends (FP64, FftPlan::pivu
+ * set): in a workgroup of 256 threads, threads 0..127 eliminate planes
+ * 0..H-1 of 128 modes downward (y_k = (b_k - a_k y_k-1) r_k, the Thomas
+ * recurrence of k_fft_tri) and threads 128..255 planes 2H-1..H of the same
+ * modes upward (u_k = (b_k - c_k u_k+1) s_k); they meet at planes H-1 and H,
+ *   x_H-1 = (y_H-1 - c'_H-1 u_H) / (1 - c'_H-1 a'_H),  x_H = u_H - a'_H x_H-1,
+ * (c'_k = c_k r_k, a'_k = a_k s_k) through LDS, and substitute back outward.
+ * Each thread keeps H values (half in registers, half in LDS: 64 KB per
+ * workgroup for H = 64) instead of 2H, so two workgroups fit a CU where
+ * k_fft_tri fits one.  The halves are whole waves, so each runs code with
+ * its direction fixed at compile time.  Right-hand side, skipped planes and
+ * face terms as k_fft_tri. */
+template <int H>
+__global__ __launch_bounds__(256, 2) void k_fft_tri2(FftPlan<double> P, double scale, int z0, int z1, double flo, double fhi) {
+    __shared__ double tab[4 * H];   /* a_k, c_k */
+    __shared__ double xch[2][128];
+    constexpr int LB = H * FX_TRI2_LDS_EIGHTHS / 8, N2 = 2 * H;
+    extern __shared__ __align__(16) unsigned char tri2_lds[];
+    double* const lv = reinterpret_cast<double*>(tri2_lds) + threadIdx.x;
+    if (P.skip && *P.skip != 0.0) return;
+    for (int k = threadIdx.x; k < N2; k += 256) {
+        const double iz = P.ih2[2];
+        tab[2 * k] = k > 0 ? -(P.bzk[k] * iz) : 0.0;
+        tab[2 * k + 1] = k < N2 - 1 ? -(P.bzk[k + 1] * iz) : 0.0;
+    }
+    __syncthreads();
+    const bool up = threadIdx.x >= 128;
+    const int lane = threadIdx.x & 127;
+    const int c = blockIdx.x * 128 + lane;   /* n0 n1 is a multiple of 128 (the launch checks) */
+    const size_t s = (size_t)P.n0 * P.n1;
+    const int h0 = P.n0 >> 1, h1 = P.n1 >> 1;
+    const int mx = c & (P.n0 - 1), my = c >> P.l0;
+    const int mxc = mx <= h0 ? mx : P.n0 - mx, myc = my <= h1 ? my : P.n1 - my;
+    const size_t ps = (size_t)(h0 + 1) * (h1 + 1);
+    const double* __restrict__ rp = (up ? P.pivu : P.piv) + (size_t)myc * (h0 + 1) + mxc;
+    double* const col = P.S + c;
+    const int zlo = z0 <= z1 ? z0 : 0, zhi = z0 <= z1 ? z1 : 0;
+    const bool any = z0 <= z1;
+    double yr[H - LB];
+    double y, rlast;
+    if (up) tri2_sweep<H, LB, true>(col, rp, s, ps, tab, lv, yr, zlo, zhi, any, z0, z1, c == 0, scale, flo, fhi, y, rlast);
+    else tri2_sweep<H, LB, false>(col, rp, s, ps, tab, lv, yr, zlo, zhi, any, z0, z1, c == 0, scale, flo, fhi, y, rlast);
+    /* the meeting: the upward half hands over u_H and a'_H */
+    if (up) {
+        xch[0][lane] = y;
+        xch[1][lane] = tab[2 * H] * rlast;
+    }
+    __syncthreads();
+    if (!up) {
+        const double cp = tab[2 * (H - 1) + 1] * rlast;
+        xch[0][lane] = (y - cp * xch[0][lane]) / (1.0 - cp * xch[1][lane]);   /* x_H-1 */
+    }
+    __syncthreads();
+    const double xm = xch[0][lane];   /* x_H-1 */
+    if (up) tri2_back<H, LB, true>(col, rp, s, ps, tab, lv, yr, xm);
+    else tri2_back<H, LB, false>(col, rp, s, ps, tab, lv, yr, xm);
+}
+
 /* One thread per lateral mode: the tridiagonal system in z by the Thomas
  * algorithm with the pivots of the table, in place on S, the right-hand side
  * scaled by `scale` (the 1/(n0 n1) of the two inverse transforms, exact for
@@ -540,6 +712,19 @@ void tri_launch(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T f
 
 template <class T>
 void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (P.pivu && P.n2 == 128 && (P.n0 * P.n1) % 128 == 0) {
+            const size_t lds = (size_t)(64 * FX_TRI2_LDS_EIGHTHS / 8) * 256 * sizeof(double);
+            static bool attr = false;   /* 64 KB of dynamic LDS: the limit raised once */
+            if (!attr) {
+                (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fft_tri2<64>),
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                attr = true;
+            }
+            hipLaunchKernelGGL(k_fft_tri2<64>, dim3(P.n0 * P.n1 / 128), dim3(256), lds, s, P, scale, z0, z1, flo, fhi);
+            return;
+        }
+    }
     if (P.n2 <= 16) tri_launch<T, 16, 16>(s, P, scale, z0, z1, flo, fhi);
     else if (P.n2 <= 32) tri_launch<T, 32, 32>(s, P, scale, z0, z1, flo, fhi);
     else if (P.n2 <= 64) tri_launch<T, 64, 64>(s, P, scale, z0, z1, flo, fhi);
@@ -575,6 +760,10 @@ bool fft_size_ok(int n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; }
 
 size_t fft_pivot_count(int n0, int n1, int n2) { return (size_t)(n0 / 2 + 1) * (n1 / 2 + 1) * n2; }
 
+void fft_pivots_up(hipStream_t s, const fxd::FftPlan<double>& P, double* pivu) {
+    const int n = (P.n0 / 2 + 1) * (P.n1 / 2 + 1);
+    hipLaunchKernelGGL(k_fft_pivots_up, dim3((n + 255) / 256), dim3(256), 0, s, P, pivu);
+}
 void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar) {
     const int n = (P.n0 / 2 + 1) * (P.n1 / 2 + 1);
     hipLaunchKernelGGL(k_fft_pivots<double>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv64, abar);

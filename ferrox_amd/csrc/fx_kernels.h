@@ -103,6 +103,10 @@ bool fft_size_ok(int n);
  * per plane, the row operator's -a on the diagonal */
 size_t fft_pivot_count(int n0, int n1, int n2);
 void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar = nullptr);
+/* the upward pivots (FftPlan::pivu, fft_pivot_count values) of the same
+ * operator without a Helmholtz term: with them an FP64 solve of 128 planes
+ * eliminates from both ends */
+void fft_pivots_up(hipStream_t s, const fxd::FftPlan<double>& P, double* pivu);
 /* preconditioned conjugate gradients (fx_pcg.hip): reductions leave
  * partials (PCG_PARTS, or the preconditioner's fft_dot_parts for r.z),
  * summed by the kernel that uses them; sc[0..1] hold r.z of the last two

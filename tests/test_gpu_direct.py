@@ -39,6 +39,9 @@ CASES = {
                                             fe_top=6e-9, nsteps=1)),
     "slab_256": configs.mfim(nx=256, ny=256, nz=32, lx=128e-9, ly=128e-9, lz=3.2e-9, de_top=1e-9, fe_top=3e-9,
                              bias=0.5, nsteps=1),
+    # 128 planes: the z systems solved from both ends (k_fft_tri2)
+    "tall_z_128": configs.mfim(nx=32, ny=32, nz=128, lx=16e-9, ly=16e-9, lz=12.8e-9, de_top=4e-9, fe_top=12e-9,
+                               bias=0.5, nsteps=1),
 }
 
 
@@ -83,6 +86,25 @@ def test_direct_time_loop_matches_oracle(precision):
     for name in ("Pz", "Phi", "Ez", "PhiDiff"):
         e = rel(g.get(name), o.get(name))
         print(precision, name, e, flush=True)
+        assert e < 1e-10, (name, e)
+    g.close(); o.close()
+
+
+@pytest.mark.parametrize("fe_top", [12e-9, 12.8e-9])
+def test_direct_128_planes_time_loop(fe_top):
+    """128 planes (the z systems solved from both ends, meeting between
+    planes 63 and 64), the FE layer ending below the top face (DE spacer
+    above) or at it; 20 steps against the oracle, phi read at the end (the
+    planes the time loop left transformed finished then)."""
+    text = configs.mfim(nx=32, ny=32, nz=128, lx=16e-9, ly=16e-9, lz=12.8e-9, de_top=4e-9, fe_top=fe_top,
+                        nsteps=20, bias=0.5)
+    inp = fx.from_text(text, mg_tol_rel=1e-13)
+    g, o = fx.FerroX(inp), Oracle(inp)
+    sg, so = g.step(20), o.step(20)
+    assert sg["total_vcycles"] == 0 and sg["steps_done"] == so["steps_done"] == 20
+    for name in ("Pz", "Phi", "Ez"):
+        e = rel(g.get(name), o.get(name))
+        print(fe_top, name, e, flush=True)
         assert e < 1e-10, (name, e)
     g.close(); o.close()
 
