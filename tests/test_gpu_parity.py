@@ -348,7 +348,9 @@ def test_throughput_workload_one_step(relerr, tol):
     launch shapes: streamed sweeps on the three largest levels, block legs of
     8x8 and 4x4 columns on the next two, the LDS-resident tail and the
     explicit-inverse bottom.  At 1e-12 both sides converge far below the bar;
-    at the reference's 1e-10 they stop on the same V-cycle."""
+    at the reference's 1e-10 they stop on the same V-cycle, and their
+    iterates then differ only by the rounding of that cycle (measured 2.2e-14
+    in Pz and 6.3e-15 in phi), so the bar is 1e-10 in both cases."""
     inp = fx.from_text(configs.mfim_poly_256(nsteps=1), mg_tol_rel=tol, poisson_direct=0)
     f = configs.polycrystal_fields(inp, n_grains=400, fe_fraction=0.7, alpha_pe=1.0e9, seed=7)
     g, o = fx.FerroX(inp), Oracle(inp)
@@ -361,6 +363,6 @@ def test_throughput_workload_one_step(relerr, tol):
     for name in ("Pz", "Phi"):
         e = relerr(g.get(name), o.get(name))
         print(name, e)
-        assert e < (TOL if tol < 1e-11 else 1e-9), name
+        assert e < TOL, name
     g.close()
     o.close()
