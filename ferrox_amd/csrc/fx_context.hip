@@ -1541,10 +1541,14 @@ int compute_phi_rho(fx_ctx* x, double* const* P, fx_solve_stats* st, bool partia
             err = 0.;
             st->outer_iters = 1;
         } else {
-            fxk::final1024(x->s, x->red, nbr, 2, 0, x->red_out);
-            double nd[2];
-            if ((rc = read_red(x, 2, nd))) return rc;
-            if (iter > 0) err = nd[0] / nd[1];
+            /* the error of the first iteration is never used (it is formed
+             * from the second on), so the host does not wait for it */
+            if (iter > 0) {
+                fxk::final1024(x->s, x->red, nbr, 2, 0, x->red_out);
+                double nd[2];
+                if ((rc = read_red(x, 2, nd))) return rc;
+                err = nd[0] / nd[1];
+            }
             iter = iter + 1;
             st->outer_iters = iter;
             st->outer_err = err;

@@ -292,21 +292,44 @@ __global__ __launch_bounds__(COLS_THR) void k_fft_cols(FftPlan<T> P, int R, int 
  * the z couplings 2 bz / hz^2 across a Dirichlet face and 0 across a Neumann
  * one, and -abar_k where a Helmholtz term per plane is given; formed in FP64 (correctly rounded divisions) from the FP64 plan's
  * coefficients and stored as T. */
+/* the per-plane coefficients of the pivot recurrences (bx, by, bz with its
+ * last face, and the Helmholtz term) staged in LDS: each step of a
+ * recurrence then waits for LDS, not for the cache */
+__device__ __forceinline__ void stage_plane_coefs(const FftPlan<double>& P, const double* abar, double* sx, double* sy,
+                                                  double* sz, double* sa) {
+    const int n2 = P.n2;
+    for (int k = threadIdx.x; k <= n2; k += blockDim.x) {
+        sz[k] = P.bzk[k];
+        if (k < n2) {
+            sx[k] = P.bxk[k];
+            sy[k] = P.byk[k];
+            sa[k] = abar ? abar[k] : 0.0;
+        }
+    }
+    __syncthreads();
+}
+
 template <class T>
 __global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __restrict__ piv, const double* __restrict__ abar) {
+    extern __shared__ __align__(16) unsigned char piv_lds[];
     const int n0c = P.n0 / 2 + 1, n1c = P.n1 / 2 + 1, n2 = P.n2;
-    const int q = blockIdx.x * 256 + threadIdx.x;
+    double* const sx = reinterpret_cast<double*>(piv_lds);
+    double* const sy = sx + n2;
+    double* const sa = sy + n2;
+    double* const sz = sa + n2;
+    stage_plane_coefs(P, abar, sx, sy, sz, sa);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n0c * n1c) return;
     const int mxc = q % n0c, myc = q / n0c;
     const double lx = P.mux[mxc] * P.ih2[0], ly = P.muy[myc] * P.ih2[1], iz = P.ih2[2];
     const size_t ps = (size_t)n0c * n1c;
     double cp = 0.0;
     for (int k = 0; k < n2; ++k) {
-        const double wzl = P.bzk[k] * iz, wzh = P.bzk[k + 1] * iz;
-        double d = P.bxk[k] * lx;
-        d += P.byk[k] * ly;
+        const double wzl = sz[k] * iz, wzh = sz[k + 1] * iz;
+        double d = sx[k] * lx;
+        d += sy[k] * ly;
         d += (k > 0 ? wzl : (P.dlo ? 2.0 * wzl : 0.0)) + (k < n2 - 1 ? wzh : (P.dhi ? 2.0 * wzh : 0.0));
-        if (abar) d -= abar[k];
+        if (abar) d -= sa[k];
         const double a = k > 0 ? -wzl : 0.0, c = k < n2 - 1 ? -wzh : 0.0;
         const double r = 1.0 / (d - a * cp);
         cp = c * r;
@@ -318,17 +341,23 @@ __global__ __launch_bounds__(256) void k_fft_pivots(FftPlan<double> P, T* __rest
  * last plane, g_k = d_k - c_k a'_k+1 with a'_k = a_k / g_k, stored as 1 / g_k
  * (FP64, correctly rounded divisions; no Helmholtz term). */
 __global__ __launch_bounds__(256) void k_fft_pivots_up(FftPlan<double> P, double* __restrict__ pivu) {
+    extern __shared__ __align__(16) unsigned char piv_lds[];
     const int n0c = P.n0 / 2 + 1, n1c = P.n1 / 2 + 1, n2 = P.n2;
-    const int q = blockIdx.x * 256 + threadIdx.x;
+    double* const sx = reinterpret_cast<double*>(piv_lds);
+    double* const sy = sx + n2;
+    double* const sa = sy + n2;
+    double* const sz = sa + n2;
+    stage_plane_coefs(P, nullptr, sx, sy, sz, sa);
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= n0c * n1c) return;
     const int mxc = q % n0c, myc = q / n0c;
     const double lx = P.mux[mxc] * P.ih2[0], ly = P.muy[myc] * P.ih2[1], iz = P.ih2[2];
     const size_t ps = (size_t)n0c * n1c;
     double ap = 0.0;   /* a'_k+1 */
     for (int k = n2 - 1; k >= 0; --k) {
-        const double wzl = P.bzk[k] * iz, wzh = P.bzk[k + 1] * iz;
-        double d = P.bxk[k] * lx;
-        d += P.byk[k] * ly;
+        const double wzl = sz[k] * iz, wzh = sz[k + 1] * iz;
+        double d = sx[k] * lx;
+        d += sy[k] * ly;
         d += (k > 0 ? wzl : (P.dlo ? 2.0 * wzl : 0.0)) + (k < n2 - 1 ? wzh : (P.dhi ? 2.0 * wzh : 0.0));
         const double a = k > 0 ? -wzl : 0.0, c = k < n2 - 1 ? -wzh : 0.0;
         const double r = 1.0 / (d - c * ap);
@@ -760,14 +789,18 @@ bool fft_size_ok(int n) { return n >= 4 && n <= 1024 && (n & (n - 1)) == 0; }
 
 size_t fft_pivot_count(int n0, int n1, int n2) { return (size_t)(n0 / 2 + 1) * (n1 / 2 + 1) * n2; }
 
+/* the recurrences are sequential in the plane: one thread per mode class in
+ * workgroups of 64, so that the classes spread over the CUs */
 void fft_pivots_up(hipStream_t s, const fxd::FftPlan<double>& P, double* pivu) {
     const int n = (P.n0 / 2 + 1) * (P.n1 / 2 + 1);
-    hipLaunchKernelGGL(k_fft_pivots_up, dim3((n + 255) / 256), dim3(256), 0, s, P, pivu);
+    const size_t lds = (4 * (size_t)P.n2 + 1) * sizeof(double);
+    hipLaunchKernelGGL(k_fft_pivots_up, dim3((n + 63) / 64), dim3(64), lds, s, P, pivu);
 }
 void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar) {
     const int n = (P.n0 / 2 + 1) * (P.n1 / 2 + 1);
-    hipLaunchKernelGGL(k_fft_pivots<double>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv64, abar);
-    if (piv32) hipLaunchKernelGGL(k_fft_pivots<float>, dim3((n + 255) / 256), dim3(256), 0, s, P, piv32, abar);
+    const size_t lds = (4 * (size_t)P.n2 + 1) * sizeof(double);
+    hipLaunchKernelGGL(k_fft_pivots<double>, dim3((n + 63) / 64), dim3(64), lds, s, P, piv64, abar);
+    if (piv32) hipLaunchKernelGGL(k_fft_pivots<float>, dim3((n + 63) / 64), dim3(64), lds, s, P, piv32, abar);
 }
 
 void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi, int z0,
