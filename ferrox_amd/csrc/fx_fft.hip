@@ -65,6 +65,12 @@ using fxd::FftPlan;
 #ifndef FX_FFT_COLS_TWL
 #define FX_FFT_COLS_TWL 1
 #endif
+#ifndef FX_FFT_REG_HARTLEY
+#define FX_FFT_REG_HARTLEY 1
+#endif
+#ifndef FX_DPP_ROR
+#define FX_DPP_ROR 1
+#endif
 constexpr int ROWS_THR = FX_FFT_ROWS_THR, COLS_THR = FX_FFT_COLS_THR;
 constexpr int TRI_THR = 256;
 constexpr int TRI_BLK = 8;    /* planes per load block of the tridiagonal solve */
@@ -132,7 +138,7 @@ __device__ __forceinline__ void reg_dft(T (&vr)[N], T (&vi)[N], const T* tw, int
  * load(s, a, b, re, im) reads element N2 a + b of sequence s; after the
  * call, X is in Xr/Xi (Xr[s n + k]) for the Hartley step. */
 template <class T, int N1, int N2, int PAD, class Load>
-__device__ __forceinline__ void four_step(int R, const T* tw, T* Yr, T* Yi, Load load, int step1_s_fast) {
+__device__ __forceinline__ void four_step_1(int R, const T* tw, T* Yr, T* Yi, Load load, int step1_s_fast) {
     constexpr int n = N1 * N2, YS = N1 + PAD;
     const int t = threadIdx.x;
     if (t < R * N2) {
@@ -152,6 +158,13 @@ __device__ __forceinline__ void four_step(int R, const T* tw, T* Yr, T* Yi, Load
         }
     }
     __syncthreads();
+}
+
+template <class T, int N1, int N2, int PAD, class Load>
+__device__ __forceinline__ void four_step(int R, const T* tw, T* Yr, T* Yi, Load load, int step1_s_fast) {
+    constexpr int n = N1 * N2, YS = N1 + PAD;
+    const int t = threadIdx.x;
+    four_step_1<T, N1, N2, PAD>(R, tw, Yr, Yi, load, step1_s_fast);
     T ur[N2], ui[N2];
     const int s3 = t / N1, c3 = t % N1;
     const bool act3 = t < R * N1;
@@ -181,6 +194,62 @@ __device__ __forceinline__ void hartley2(const T* Xr, const T* Xi, int s, int n,
     const T A = Xr[s * n + k], B = Xi[s * n + k], C = Xr[s * n + mk], D = Xi[s * n + mk];
     h0 = T(0.5) * ((A - B) + (C + D));
     h1 = T(0.5) * ((A + B) + (D - C));
+}
+
+/* Lane c of each row of 16 lanes receives v of lane (16 - c) mod 16:
+ * row_mirror (lane c <- lane 15 - c), then row_ror:1 (lane c <- lane c - 1),
+ * one 32-bit word at a time. */
+template <class T>
+__device__ __forceinline__ T lane_partner16(T v) {
+    constexpr int W = sizeof(T) / 4;
+    union U { T v; int w[W]; };
+    U a, b;
+    a.v = v;
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+        const int m = __builtin_amdgcn_update_dpp(0, a.w[i], 0x140, 0xF, 0xF, false);
+        b.w[i] = __builtin_amdgcn_update_dpp(0, m, 0x120 + FX_DPP_ROR, 0xF, 0xF, false);
+    }
+    return b.v;
+}
+
+/* The four-step FFT with the Hartley step in registers, for N1 = 16: after
+ * step 3, thread (s, c) holds X[c + 16 d], d < N2, of sequence s, and
+ * X[n - c - 16 d] is X[(16 - c) + 16 (N2 - 1 - d)] of the thread 16 - c of the
+ * same sequence (the lane (16 - c) mod 16 of the same 16-lane row; for c = 0,
+ * X[16 ((N2 - d) mod N2)] of the thread itself).  The pair is exchanged by
+ * DPP, so X is never written back to LDS: one barrier fewer.  The Hartley
+ * values are formed as hartley2 forms them (bitwise the same);
+ * store(s, k, h0, h1) receives index k of sequence s. */
+template <class T, int N1, int N2, int PAD, class Load, class Store>
+__device__ __forceinline__ void four_step_hartley16(int R, const T* tw, T* Yr, T* Yi, Load load, int step1_s_fast,
+                                                    Store store) {
+    static_assert(N1 == 16, "the lane exchange works on rows of 16 lanes");
+    constexpr int YS = N1 + PAD;
+    const int t = threadIdx.x;
+    four_step_1<T, N1, N2, PAD>(R, tw, Yr, Yi, load, step1_s_fast);
+    const int s3 = t / N1, c3 = t % N1;
+    if (t < R * N1) {
+        T ur[N2], ui[N2], pr[N2], pi[N2];
+#pragma unroll
+        for (int b = 0; b < N2; ++b) {
+            ur[b] = Yr[(s3 * N2 + b) * YS + c3];
+            ui[b] = Yi[(s3 * N2 + b) * YS + c3];
+        }
+        reg_dft<T, N2>(ur, ui, tw, N1);
+#pragma unroll
+        for (int r = 0; r < N2; ++r) {
+            pr[r] = lane_partner16<T>(ur[r]);
+            pi[r] = lane_partner16<T>(ui[r]);
+        }
+#pragma unroll
+        for (int d = 0; d < N2; ++d) {
+            const T A = ur[brev_c<N2>(d)], B = ui[brev_c<N2>(d)];
+            const T C = c3 ? pr[brev_c<N2>(N2 - 1 - d)] : ur[brev_c<N2>((N2 - d) & (N2 - 1))];
+            const T D = c3 ? pi[brev_c<N2>(N2 - 1 - d)] : ui[brev_c<N2>((N2 - d) & (N2 - 1))];
+            store(s3, c3 + N1 * d, T(0.5) * ((A - B) + (C + D)), T(0.5) * ((A + B) + (D - C)));
+        }
+    }
 }
 
 template <class T>
@@ -221,6 +290,21 @@ __global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* 
             im = (T)in[o + n];
         }
     };
+    if constexpr (N1 == 16 && FX_FFT_REG_HARTLEY) {
+        if (FWD || !dotp) {
+            four_step_hartley16<T, N1, N2, 1>(R, tw, Yr, Yi, load, 0, [&](int s, int kk, T h0, T h1) {
+                const size_t o = row0 + (size_t)(2 * s) * n + kk;
+                if (FWD || !add) {
+                    out[o] = (TOUT)h0;
+                    out[o + n] = (TOUT)h1;
+                } else {
+                    out[o] = out[o] + (TOUT)h0;
+                    out[o + n] = out[o + n] + (TOUT)h1;
+                }
+            });
+            return;
+        }
+    }
     four_step<T, N1, N2, 1>(R, tw, Yr, Yi, load, 0);
     double dacc = 0.0;
     for (int t = threadIdx.x; t < R * n; t += blockDim.x) {
@@ -272,6 +356,15 @@ __global__ __launch_bounds__(COLS_THR) void k_fft_cols(FftPlan<T> P, int R, int 
         re = v.x;
         im = v.y;
     };
+    if constexpr (N1 == 16 && FX_FFT_REG_HARTLEY) {
+        four_step_hartley16<T, N1, N2, PAD>(R, tw, Yr, Yi, load, 1, [&](int s, int j, T h0, T h1) {
+            V2 v;
+            v.x = h0;
+            v.y = h1;
+            *reinterpret_cast<V2*>(S + (size_t)j * n0 + 2 * s) = v;
+        });
+        return;
+    }
     four_step<T, N1, N2, PAD>(R, tw, Yr, Yi, load, 1);
     for (int t = threadIdx.x; t < R * n; t += blockDim.x) {
         const int s = t % R, j = t / R;
