@@ -110,6 +110,10 @@ struct fx_ctx {
      * only (what the TDGL update reads); the other planes' solution waits,
      * transformed, in fft64.S until phi_sync() finishes it */
     int phi_z0 = 0, phi_z1 = -1;
+    /* the z solve, too, stored only planes phi_z0 .. phi_z1 of fft64.S:
+     * phi_sync continues it from there with the solve's face terms */
+    bool phi_cont = false;
+    double phi_face[2] = {0.0, 0.0};
     bool phi_exposed = false;      /* a device pointer to phi was handed out: always form all of it */
     /* the fused update writes the next RHS into rhs_nx, which the next solve
      * swaps in (rhs keeps the RHS of the last solve, the PoissonRHS the
@@ -743,7 +747,8 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
     fill_dir(n0, tw0, mux);
     fill_dir(n1, tw1, muy);
     int rc;
-    if (!x->fft_mem64 && (rc = dalloc(x, (void**)&x->fft_mem64, (total + (twoway ? npiv : 0)) * sizeof(double)))) return rc;
+    if (!x->fft_mem64 && (rc = dalloc(x, (void**)&x->fft_mem64, (total + (twoway ? npiv + 2 * (size_t)n2 : 0)) * sizeof(double))))
+        return rc;
     HIPCHK(x, hipMemcpy(x->fft_mem64, tab.data(), ntab * sizeof(double), hipMemcpyHostToDevice));
     const bool fp32 = in.precision == FX_PREC_FP32_REFINE && !x->lev32.empty() && x->lev32[0].bxk;
     if (fp32) {
@@ -763,6 +768,7 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
         P.S = base + ntab;
         P.piv = base + ntab + x->N;
         P.pivu = nullptr;
+        P.green = nullptr;
     };
     fill(x->fft64, x->fft_mem64, d.bxk, d.byk, d.bzk);
     if (fp32) fill(x->fft32, x->fft_mem32, x->lev32[0].bxk, x->lev32[0].byk, x->lev32[0].bzk);
@@ -770,6 +776,8 @@ int build_direct(fx_ctx* x, double bz_lo, double bz_hi) {
     if (twoway) {
         fxk::fft_pivots_up(x->s, x->fft64, x->fft_mem64 + total);
         x->fft64.pivu = x->fft_mem64 + total;
+        fxk::fft_face_green(x->s, x->fft64, x->fft_mem64 + total + npiv);
+        x->fft64.green = x->fft_mem64 + total + npiv;
     }
     HIPCHK(x, hipStreamSynchronize(x->s));
     /* the inhomogeneous face term of the row operator is (2 w) v, w = beta_face / hz^2 */
@@ -957,9 +965,10 @@ RotArgs rot_of(const fx_ctx* x) {
 /* Finish the planes of phi a partial direct solve left in its workspace. */
 void phi_sync(fx_ctx* x) {
     if (x->phi_z0 > x->phi_z1) return;
-    fxk::fft_finish(x->s, x->fft64, x->phi, x->phi_z0, x->phi_z1);
+    fxk::fft_finish(x->s, x->fft64, x->phi, x->phi_z0, x->phi_z1, x->phi_cont ? x->phi_face : nullptr);
     x->phi_z0 = 0;
     x->phi_z1 = -1;
+    x->phi_cont = false;
 }
 
 PhiView phiview_of(const fx_ctx* x) {
@@ -1278,7 +1287,7 @@ int direct_solve(fx_ctx* x, const double bv[6], double tol, fx_solve_stats* st, 
                  int o1) {
     const double blo = x->fft_w2lo * bv[2], bhi = x->fft_w2hi * bv[5];
     if (check || x->phi_exposed) o1 = o0 - 1;   /* the residual (or a caller) reads all of phi */
-    fxk::fft_solve(x->s, x->fft64, x->rhs, x->phi, blo, bhi, z0, z1, o0, o1);
+    x->phi_cont = fxk::fft_solve(x->s, x->fft64, x->rhs, x->phi, blo, bhi, z0, z1, o0, o1, x->phi_face);
     x->phi_z0 = o0 <= o1 ? o0 : 0;
     x->phi_z1 = o0 <= o1 ? o1 : -1;
     st->vcycles = 0; st->mg_resid0 = 0.0; st->mg_bnorm = 0.0; st->mg_resid = 0.0;
@@ -1366,6 +1375,7 @@ int pcg_solve(fx_ctx* x, const double bv[6], double tol, int maxit, fx_solve_sta
     FftPlan<double> M = x->fft64;
     M.piv = piv;
     M.pivu = nullptr;   /* the upward pivots are of the operator without the plane means */
+    M.green = nullptr;
     FftPlan<double> Mg = M;   /* the preconditioner as the iterations use it: idle once converged */
     Mg.skip = sc + 2;
     /* r0 = b - A 0 with the face terms; z0, p0 */
@@ -1973,7 +1983,7 @@ int fx_set_field(fx_ctx* x, int f, const double* host) {
         return FX_OK;
     }
     case FX_FIELD_PHI:
-        x->phi_z0 = 0; x->phi_z1 = -1;   /* all of phi rewritten */
+        x->phi_z0 = 0; x->phi_z1 = -1; x->phi_cont = false;   /* all of phi rewritten */
         HIPCHK(x, hipMemcpyAsync(x->phi, host, B, hipMemcpyHostToDevice, x->s));
         HIPCHK(x, hipStreamSynchronize(x->s));
         return FX_OK;

@@ -393,6 +393,10 @@ struct FftPlan {
      * plane, 1 / (d_k - c_k a'_k+1) with a'_k = a_k pivu_k, laid out as piv;
      * the z solve then runs from both ends and meets in the middle */
     const T* pivu;
+    /* with pivu: the Thomas values of mode (0, 0) for a unit right-hand side
+     * on the first plane (downward, green[k]) and on the last (upward,
+     * green[n2 + k]); null: the z solve always runs over every plane */
+    const T* green;
     /* non-null: every pass of the solve does nothing while *skip != 0 (a
      * device flag set when an iterative solve around it has converged) */
     const double* skip;

@@ -459,109 +459,128 @@ __global__ __launch_bounds__(256) void k_fft_pivots_up(FftPlan<double> P, double
     }
 }
 
-/* The z systems of 2H planes solved from both ends (FP64, FftPlan::pivu
- * set): in a workgroup of 256 threads, threads 0..127 eliminate planes
- * 0..H-1 of 128 modes downward (y_k = (b_k - a_k y_k-1) r_k, the Thomas
- * recurrence of k_fft_tri) and threads 128..255 planes 2H-1..H of the same
- * modes upward (u_k = (b_k - c_k u_k+1) s_k); they meet at planes H-1 and H,
- *   x_H-1 = (y_H-1 - c'_H-1 u_H) / (1 - c'_H-1 a'_H),  x_H = u_H - a'_H x_H-1,
- * (c'_k = c_k r_k, a'_k = a_k s_k) through LDS, and substitute back outward.
- * Each thread keeps H values (half in registers, half in LDS: 64 KB per
- * workgroup for H = 64) instead of 2H, so two workgroups fit a CU where
- * k_fft_tri fits one.  Right-hand side, skipped planes and face terms as
- * k_fft_tri. */
+/* the two-ended z solve (k_fft_tri2): planes per load block, and the share
+ * of a half's values kept in LDS */
 #ifndef FX_TRI2_BLK
 #define FX_TRI2_BLK 8
 #endif
 #ifndef FX_TRI2_LDS_EIGHTHS
 #define FX_TRI2_LDS_EIGHTHS 4   /* eighths of a half's H values that wait in LDS */
 #endif
-/* One half of the two-ended z solve (UP false: planes 0..H-1 downward, true:
- * planes 2H-1..H upward), step j at plane k = UP ? 2H-1-j : j; the first LB
- * steps' values wait in LDS (lv[j 256]), the rest in registers (yr). */
+/* One half of the two-ended z solve (UP false: downward from plane kb, true:
+ * upward from plane kb), nf steps, step j at plane k = UP ? kb - j : kb + j,
+ * from y, the value of the plane before the first; the first LB steps'
+ * values wait in LDS (lv[j 256]), the rest in registers (yr).  Loads of
+ * steps past nf read a plane clamped into [0, 2H) and are not used. */
 template <int H, int LB, bool UP>
 __device__ __forceinline__ void tri2_sweep(const double* __restrict__ col, const double* __restrict__ rp, size_t s,
-                                           size_t ps, const double* tab, double* lv, double (&yr)[H - LB],
-                                           int zlo, int zhi, bool any, int z0, int z1, bool c0, double scale,
+                                           size_t ps, const double* tab, double* lv, double (&yr)[H - LB], int kb,
+                                           int nf, int zlo, int zhi, bool any, int z0, int z1, bool c0, double scale,
                                            double flo, double fhi, double& y, double& rlast) {
     constexpr int TB = FX_TRI2_BLK, NB = H / TB, N2 = 2 * H;
     double sb[2][TB], rb[2][TB];
     auto load = [&](int b, double (&sv)[TB], double (&rv)[TB]) {
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
-            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
+            const int j = b * TB + q, k0 = UP ? kb - j : kb + j;
+            const int k = k0 < 0 ? 0 : k0 > N2 - 1 ? N2 - 1 : k0;
             const int kc = k < zlo ? zlo : k > zhi ? zhi : k;
             sv[q] = col[(size_t)kc * s];
             rv[q] = rp[(size_t)k * ps];
         }
     };
-    y = 0.0;
     load(0, sb[0], rb[0]);
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-        if (b + 1 < NB) load(b + 1, sb[(b + 1) & 1], rb[(b + 1) & 1]);
+        if (b * TB >= nf) break;
+        if ((b + 1) * TB < nf) load(b + 1, sb[(b + 1) & 1], rb[(b + 1) & 1]);
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
-            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
-            double bk = (any && k >= z0 && k <= z1) ? sb[b & 1][q] * scale : 0.0;
-            if (c0) {
-                if (k == 0) bk += flo;
-                if (k == N2 - 1) bk += fhi;
+            const int j = b * TB + q, k = UP ? kb - j : kb + j;
+            if (j < nf) {
+                double bk = (any && k >= z0 && k <= z1) ? sb[b & 1][q] * scale : 0.0;
+                if (c0) {
+                    if (k == 0) bk += flo;
+                    if (k == N2 - 1) bk += fhi;
+                }
+                /* downward: the coupling to the plane above (a_k); upward: below (c_k) */
+                y = (bk - tab[2 * k + (UP ? 1 : 0)] * y) * rb[b & 1][q];
+                if (j < LB) lv[(size_t)j * 256] = y;
+                else yr[j - LB] = y;
+                rlast = rb[b & 1][q];
             }
-            /* downward: the coupling to the plane above (a_k); upward: below (c_k) */
-            y = (bk - tab[2 * k + (UP ? 1 : 0)] * y) * rb[b & 1][q];
-            if (j < LB) lv[(size_t)j * 256] = y;
-            else yr[j - LB] = y;
         }
     }
-    rlast = rb[(NB - 1) & 1][TB - 1];
 }
 
-/* the back substitution of that half from x of its step H-1's neighbour
- * across the middle (xn): downward x_k = y_k - c'_k x_k+1, upward x_k = u_k -
- * a'_k x_k-1; the downward half's step H-1 is the meeting's x_H-1 itself */
+/* the back substitution of that half, from its last step (the plane next to
+ * the meeting) back to step 0 and on outward to plane oe: downward x_k = y_k -
+ * c'_k x_k+1, upward x_k = u_k - a'_k x_k-1, from x of the neighbour across
+ * the middle (xn; the downward half's last step is the meeting's x_M-1
+ * itself).  Past step 0 the right-hand side is 0: y_k = 0, or for mode (0, 0)
+ * fv gr[k], the face term's share (FftPlan::green).  x is stored on the planes
+ * from the meeting out to oe. */
 template <int H, int LB, bool UP>
 __device__ __forceinline__ void tri2_back(double* col, const double* __restrict__ rp, size_t s, size_t ps,
-                                          const double* tab, const double* lv, const double (&yr)[H - LB], double xn) {
+                                          const double* tab, const double* lv, const double (&yr)[H - LB], int kb,
+                                          int nf, int oe, bool c0, double fv, const double* __restrict__ gr,
+                                          double xn) {
     constexpr int TB = FX_TRI2_BLK, NB = H / TB, N2 = 2 * H;
     double rb[2][TB];
     auto loadr = [&](int b, double (&rv)[TB]) {
 #pragma unroll
         for (int q = 0; q < TB; ++q) {
-            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
+            const int j = b * TB + q, k0 = UP ? kb - j : kb + j;
+            const int k = k0 < 0 ? 0 : k0 > N2 - 1 ? N2 - 1 : k0;
             rv[q] = rp[(size_t)k * ps];
         }
     };
-    loadr(NB - 1, rb[(NB - 1) & 1]);
+    const int bl = (nf - 1) / TB;   /* the block of the last step */
 #pragma unroll
     for (int b = NB - 1; b >= 0; --b) {
+        if (b > bl) continue;
+        if (b == bl) loadr(b, rb[b & 1]);
         if (b > 0) loadr(b - 1, rb[(b - 1) & 1]);
 #pragma unroll
         for (int q = TB - 1; q >= 0; --q) {
-            const int j = b * TB + q, k = UP ? N2 - 1 - j : j;
-            if (UP || j != H - 1) {
-                const double yj = j < LB ? lv[(size_t)j * 256] : yr[j - LB];
-                xn = yj - (tab[2 * k + (UP ? 0 : 1)] * rb[b & 1][q]) * xn;
+            const int j = b * TB + q, k = UP ? kb - j : kb + j;
+            if (j < nf) {
+                if (UP || j != nf - 1) {
+                    const double yj = j < LB ? lv[(size_t)j * 256] : yr[j - LB];
+                    xn = yj - (tab[2 * k + (UP ? 0 : 1)] * rb[b & 1][q]) * xn;
+                }
+                if (UP ? k <= oe : k >= oe) col[(size_t)k * s] = xn;
             }
-            col[(size_t)k * s] = xn;
         }
+    }
+    for (int k = UP ? kb + 1 : kb - 1; UP ? k <= oe : k >= oe; k += UP ? 1 : -1) {
+        const double yk = c0 ? fv * gr[k] : 0.0;
+        xn = yk - (tab[2 * k + (UP ? 0 : 1)] * rp[(size_t)k * ps]) * xn;
+        col[(size_t)k * s] = xn;
     }
 }
 
 /* The z systems of 2H planes solved from both ends (FP64, FftPlan::pivu
  * set): in a workgroup of 256 threads, threads 0..127 eliminate planes
- * 0..H-1 of 128 modes downward (y_k = (b_k - a_k y_k-1) r_k, the Thomas
- * recurrence of k_fft_tri) and threads 128..255 planes 2H-1..H of the same
- * modes upward (u_k = (b_k - c_k u_k+1) s_k); they meet at planes H-1 and H,
- *   x_H-1 = (y_H-1 - c'_H-1 u_H) / (1 - c'_H-1 a'_H),  x_H = u_H - a'_H x_H-1,
- * (c'_k = c_k r_k, a'_k = a_k s_k) through LDS, and substitute back outward.
- * Each thread keeps H values (half in registers, half in LDS: 64 KB per
- * workgroup for H = 64) instead of 2H, so two workgroups fit a CU where
- * k_fft_tri fits one.  The halves are whole waves, so each runs code with
- * its direction fixed at compile time.  Right-hand side, skipped planes and
- * face terms as k_fft_tri. */
+ * kd0..M-1 of 128 modes downward (y_k = (b_k - a_k y_k-1) r_k, the Thomas
+ * recurrence of k_fft_tri) and threads 128..255 planes ku1..M of the same
+ * modes upward (u_k = (b_k - c_k u_k+1) s_k); they meet at planes M-1 and M,
+ *   x_M-1 = (y_M-1 - c'_M-1 u_M) / (1 - c'_M-1 a'_M),  x_M = u_M - a'_M x_M-1,
+ * (c'_k = c_k r_k, a'_k = a_k s_k) through LDS, and substitute back outward,
+ * storing x on planes o0..o1.  The whole system is kd0 = 0, ku1 = 2H-1,
+ * M = H, o0 = 0, o1 = 2H-1.  With a right-hand side on planes z0..z1 only,
+ * kd0 = z0, ku1 = z1 and M their middle: below z0 and above z1 y and u are 0
+ * (mode (0, 0): the face terms' Green's values times flo, fhi), so the sweeps
+ * start there, and the back substitution stops at o0 and o1 (the planes the
+ * inverse transforms read; k_fft_tri2_cont continues it when the rest is
+ * needed).  Each thread keeps at most H values (half in registers, half in
+ * LDS: 64 KB per workgroup for H = 64), so two workgroups fit a CU.  The
+ * halves are whole waves, so each runs code with its direction fixed at
+ * compile time.  Right-hand side, skipped planes and face terms as
+ * k_fft_tri. */
 template <int H>
-__global__ __launch_bounds__(256, 2) void k_fft_tri2(FftPlan<double> P, double scale, int z0, int z1, double flo, double fhi) {
+__global__ __launch_bounds__(256, 2) void k_fft_tri2(FftPlan<double> P, double scale, int z0, int z1, double flo,
+                                                     double fhi, int kd0, int ku1, int M, int o0, int o1) {
     __shared__ double tab[4 * H];   /* a_k, c_k */
     __shared__ double xch[2][128];
     constexpr int LB = H * FX_TRI2_LDS_EIGHTHS / 8, N2 = 2 * H;
@@ -585,25 +604,91 @@ __global__ __launch_bounds__(256, 2) void k_fft_tri2(FftPlan<double> P, double s
     const double* __restrict__ rp = (up ? P.pivu : P.piv) + (size_t)myc * (h0 + 1) + mxc;
     double* const col = P.S + c;
     const int zlo = z0 <= z1 ? z0 : 0, zhi = z0 <= z1 ? z1 : 0;
-    const bool any = z0 <= z1;
+    const bool any = z0 <= z1, c0 = c == 0;
     double yr[H - LB];
-    double y, rlast;
-    if (up) tri2_sweep<H, LB, true>(col, rp, s, ps, tab, lv, yr, zlo, zhi, any, z0, z1, c == 0, scale, flo, fhi, y, rlast);
-    else tri2_sweep<H, LB, false>(col, rp, s, ps, tab, lv, yr, zlo, zhi, any, z0, z1, c == 0, scale, flo, fhi, y, rlast);
-    /* the meeting: the upward half hands over u_H and a'_H */
+    double y, rlast = 0.0;
+    if (up) {
+        y = (c0 && ku1 < N2 - 1) ? fhi * P.green[N2 + ku1 + 1] : 0.0;
+        tri2_sweep<H, LB, true>(col, rp, s, ps, tab, lv, yr, ku1, ku1 - M + 1, zlo, zhi, any, z0, z1, c0, scale, flo,
+                                fhi, y, rlast);
+    } else {
+        y = (c0 && kd0 > 0) ? flo * P.green[kd0 - 1] : 0.0;
+        tri2_sweep<H, LB, false>(col, rp, s, ps, tab, lv, yr, kd0, M - kd0, zlo, zhi, any, z0, z1, c0, scale, flo,
+                                 fhi, y, rlast);
+    }
+    /* the meeting: the upward half hands over u_M and a'_M */
     if (up) {
         xch[0][lane] = y;
-        xch[1][lane] = tab[2 * H] * rlast;
+        xch[1][lane] = tab[2 * M] * rlast;
     }
     __syncthreads();
     if (!up) {
-        const double cp = tab[2 * (H - 1) + 1] * rlast;
-        xch[0][lane] = (y - cp * xch[0][lane]) / (1.0 - cp * xch[1][lane]);   /* x_H-1 */
+        const double cp = tab[2 * (M - 1) + 1] * rlast;
+        xch[0][lane] = (y - cp * xch[0][lane]) / (1.0 - cp * xch[1][lane]);   /* x_M-1 */
     }
     __syncthreads();
-    const double xm = xch[0][lane];   /* x_H-1 */
-    if (up) tri2_back<H, LB, true>(col, rp, s, ps, tab, lv, yr, xm);
-    else tri2_back<H, LB, false>(col, rp, s, ps, tab, lv, yr, xm);
+    const double xm = xch[0][lane];   /* x_M-1 */
+    if (up) tri2_back<H, LB, true>(col, rp, s, ps, tab, lv, yr, ku1, ku1 - M + 1, o1, c0, fhi, P.green + N2, xm);
+    else tri2_back<H, LB, false>(col, rp, s, ps, tab, lv, yr, kd0, M - kd0, o0, c0, flo, P.green, xm);
+}
+
+/* The rest of a z solution that k_fft_tri2 stored on planes o0..o1 only
+ * (still untransformed there), where the right-hand side was 0 outside
+ * them: its back substitution continued from plane o0 down to 0 and from o1
+ * up to the last plane, one thread per mode (y_k, u_k as in k_fft_tri2 past
+ * its sweeps). */
+__global__ __launch_bounds__(256) void k_fft_tri2_cont(FftPlan<double> P, int o0, int o1, double flo, double fhi) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= P.n0 * P.n1) return;
+    const int n2 = P.n2;
+    const size_t s = (size_t)P.n0 * P.n1;
+    const int h0 = P.n0 >> 1, h1 = P.n1 >> 1;
+    const int mx = c & (P.n0 - 1), my = c >> P.l0;
+    const int mxc = mx <= h0 ? mx : P.n0 - mx, myc = my <= h1 ? my : P.n1 - my;
+    const size_t ps = (size_t)(h0 + 1) * (h1 + 1), cls = (size_t)myc * (h0 + 1) + mxc;
+    const double iz = P.ih2[2];
+    const bool c0 = c == 0;
+    double* const col = P.S + c;
+    if (o0 > 0) {
+        double x = col[(size_t)o0 * s];
+        for (int k = o0 - 1; k >= 0; --k) {
+            const double ck = -(P.bzk[k + 1] * iz);
+            const double yk = c0 ? flo * P.green[k] : 0.0;
+            x = yk - (ck * P.piv[k * ps + cls]) * x;
+            col[(size_t)k * s] = x;
+        }
+    }
+    if (o1 < n2 - 1) {
+        double x = col[(size_t)o1 * s];
+        for (int k = o1 + 1; k < n2; ++k) {
+            const double ak = -(P.bzk[k] * iz);
+            const double uk = c0 ? fhi * P.green[n2 + k] : 0.0;
+            x = uk - (ak * P.pivu[k * ps + cls]) * x;
+            col[(size_t)k * s] = x;
+        }
+    }
+}
+
+/* FftPlan::green: the Thomas values of mode (0, 0) for a unit right-hand
+ * side on the first plane (downward, green[k]) and on the last (upward,
+ * green[n2 + k]), with the pivots of the tables; one thread */
+__global__ void k_fft_face_green(FftPlan<double> P, double* g) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int n2 = P.n2;
+    const size_t ps = (size_t)(P.n0 / 2 + 1) * (P.n1 / 2 + 1);
+    const double iz = P.ih2[2];
+    double y = 0.0;
+    for (int k = 0; k < n2; ++k) {
+        const double a = k > 0 ? -(P.bzk[k] * iz) : 0.0;
+        y = ((k == 0 ? 1.0 : 0.0) - a * y) * P.piv[k * ps];
+        g[k] = y;
+    }
+    double u = 0.0;
+    for (int k = n2 - 1; k >= 0; --k) {
+        const double cc = k < n2 - 1 ? -(P.bzk[k + 1] * iz) : 0.0;
+        u = ((k == n2 - 1 ? 1.0 : 0.0) - cc * u) * P.pivu[k * ps];
+        g[n2 + k] = u;
+    }
 }
 
 /* One thread per lateral mode: the tridiagonal system in z by the Thomas
@@ -832,10 +917,29 @@ void tri_launch(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T f
     else tri_launch_thr<T, NZ, RB, TRI_THR>(s, P, scale, z0, z1, flo, fhi);
 }
 
+/* returns true where the two-ended solve stored x on planes o0..o1 only (the
+ * rest by fft_finish's continuation): an FP64 plan with upward pivots, a
+ * right-hand side on z0..z1 only and o0 <= z0, z1 <= o1, not every plane */
 template <class T>
-void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi) {
+bool tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T fhi, int o0 = 0, int o1 = -1) {
     if constexpr (std::is_same<T, double>::value) {
         if (P.pivu && P.n2 == 128 && (P.n0 * P.n1) % 128 == 0) {
+            const int N2 = P.n2;
+            const bool part = z0 <= z1 && P.green && o0 <= o1 && o0 <= z0 && z1 <= o1 && (o0 > 0 || o1 < N2 - 1);
+            int kd0 = 0, ku1 = N2 - 1;
+            if (part) {
+                kd0 = z0;
+                ku1 = z1;
+                if (ku1 == kd0) { if (ku1 < N2 - 1) ++ku1; else --kd0; }   /* one plane a side at least */
+                /* one plane more each side: the continuation starts there, since
+                 * the inverse transforms overwrite planes o0..o1 in place */
+                o0 = o0 > 0 ? o0 - 1 : 0;
+                o1 = o1 < N2 - 1 ? o1 + 1 : N2 - 1;
+            } else {
+                o0 = 0;
+                o1 = N2 - 1;
+            }
+            const int M = (kd0 + ku1 + 1) / 2;   /* kd0 < M <= ku1, at most N2/2 planes a side */
             const size_t lds = (size_t)(64 * FX_TRI2_LDS_EIGHTHS / 8) * 256 * sizeof(double);
             static bool attr = false;   /* 64 KB of dynamic LDS: the limit raised once */
             if (!attr) {
@@ -843,8 +947,9 @@ void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T f
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 attr = true;
             }
-            hipLaunchKernelGGL(k_fft_tri2<64>, dim3(P.n0 * P.n1 / 128), dim3(256), lds, s, P, scale, z0, z1, flo, fhi);
-            return;
+            hipLaunchKernelGGL(k_fft_tri2<64>, dim3(P.n0 * P.n1 / 128), dim3(256), lds, s, P, scale, z0, z1, flo, fhi,
+                               kd0, ku1, M, o0, o1);
+            return part;
         }
     }
     if (P.n2 <= 16) tri_launch<T, 16, 16>(s, P, scale, z0, z1, flo, fhi);
@@ -852,6 +957,7 @@ void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T f
     else if (P.n2 <= 64) tri_launch<T, 64, 64>(s, P, scale, z0, z1, flo, fhi);
     else if (P.n2 <= 128) tri_launch<T, 128, TRI_REG>(s, P, scale, z0, z1, flo, fhi);
     else tri_launch<T, 0, 0>(s, P, scale, z0, z1, flo, fhi);
+    return false;
 }
 
 /* planes of b outside [z0, z1] hold zeros: their forward transforms are
@@ -859,8 +965,9 @@ void tri(hipStream_t s, const FftPlan<T>& P, T scale, int z0, int z1, T flo, T f
  * planes [o0, o1] only (o1 < o0: all), the other planes' solution left
  * transformed in S for fft_finish */
 template <class T, class TIN, class TOUT>
-void fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi, double blo, double bhi, int add,
-                    int z0, int z1, int o0 = 0, int o1 = -1, const double* dotv = nullptr, double* dotp = nullptr) {
+bool fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi, double blo, double bhi, int add,
+                    int z0, int z1, int o0 = 0, int o1 = -1, const double* dotv = nullptr, double* dotp = nullptr,
+                    double* face = nullptr) {
     const T scale = T(1) / T(P.n0 * P.n1);
     if (z1 < z0) { z0 = 0; z1 = P.n2 - 1; }
     if (o1 < o0) { o0 = 0; o1 = P.n2 - 1; }
@@ -869,9 +976,11 @@ void fft_solve_impl(hipStream_t s, const FftPlan<T>& P, const TIN* b, TOUT* phi,
     const T flo = z0 > 0 ? T(blo) : T(0), fhi = z1 < P.n2 - 1 ? T(bhi) : T(0);
     rows<T, TIN, TOUT>(s, P, true, b, P.S, blo, bhi, 0, z0, nk);
     cols<T>(s, P, z0, nk);
-    tri<T>(s, P, scale, z0, z1, flo, fhi);
+    const bool part = tri<T>(s, P, scale, z0, z1, flo, fhi, o0, o1);
+    if (face) { face[0] = (double)flo; face[1] = (double)fhi; }
     cols<T>(s, P, o0, o1 - o0 + 1);
     rows<T, TIN, TOUT>(s, P, false, P.S, phi, 0.0, 0.0, add, o0, o1 - o0 + 1, dotv, dotp);
+    return part;
 }
 
 }  // namespace
@@ -889,6 +998,9 @@ void fft_pivots_up(hipStream_t s, const fxd::FftPlan<double>& P, double* pivu) {
     const size_t lds = (4 * (size_t)P.n2 + 1) * sizeof(double);
     hipLaunchKernelGGL(k_fft_pivots_up, dim3((n + 63) / 64), dim3(64), lds, s, P, pivu);
 }
+void fft_face_green(hipStream_t s, const fxd::FftPlan<double>& P, double* g) {
+    hipLaunchKernelGGL(k_fft_face_green, dim3(1), dim3(64), 0, s, P, g);
+}
 void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, float* piv32, const double* abar) {
     const int n = (P.n0 / 2 + 1) * (P.n1 / 2 + 1);
     const size_t lds = (4 * (size_t)P.n2 + 1) * sizeof(double);
@@ -896,15 +1008,20 @@ void fft_pivots(hipStream_t s, const fxd::FftPlan<double>& P, double* piv64, flo
     if (piv32) hipLaunchKernelGGL(k_fft_pivots<float>, dim3((n + 63) / 64), dim3(64), lds, s, P, piv32, abar);
 }
 
-void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi, int z0,
-               int z1, int o0, int o1) {
-    fft_solve_impl<double, double, double>(s, P, b, phi, blo, bhi, 0, z0, z1, o0, o1);
+bool fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi, int z0,
+               int z1, int o0, int o1, double* face) {
+    return fft_solve_impl<double, double, double>(s, P, b, phi, blo, bhi, 0, z0, z1, o0, o1, nullptr, nullptr, face);
 }
 void fft_solve_dot(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double* dotp) {
     fft_solve_impl<double, double, double>(s, P, b, phi, 0.0, 0.0, 0, 0, -1, 0, -1, b, dotp);
 }
 int fft_dot_parts(const fxd::FftPlan<double>& P) { return rows_blocks(P.l0, P.n1, P.n2); }
-void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1) {
+void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1, const double* face) {
+    if (face && (o0 > 1 || o1 < P.n2 - 2)) {   /* the solve stored planes o0 - 1 and o1 + 1 too */
+        const int n = P.n0 * P.n1;
+        hipLaunchKernelGGL(k_fft_tri2_cont, dim3((n + 255) / 256), dim3(256), 0, s, P, o0 > 0 ? o0 - 1 : 0,
+                           o1 < P.n2 - 1 ? o1 + 1 : P.n2 - 1, face[0], face[1]);
+    }
     if (o0 > 0) {
         cols<double>(s, P, 0, o0);
         rows<double, double, double>(s, P, false, P.S, phi, 0.0, 0.0, 0, 0, o0);

@@ -129,10 +129,15 @@ void pcg_update(hipStream_t s, double* phi, double* r, const double* p, const do
                 const double* part_pq, const double* sc, int it, double* part_r);
 /* planes of b outside [z0, z1] hold zeros (z1 < z0: nothing is known);
  * phi is formed on planes [o0, o1] (o1 < o0: all), the others by a later
- * fft_finish(o0, o1) while P.S is untouched */
-void fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi,
-               int z0 = 0, int z1 = -1, int o0 = 0, int o1 = -1);
-void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1);
+ * fft_finish(o0, o1) while P.S is untouched.  Returns true where the z
+ * solve, too, stored only planes [o0, o1] of P.S: fft_finish then takes the
+ * solve's face terms (face[0], face[1], written by the solve when face is
+ * given) to continue it. */
+bool fft_solve(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double blo, double bhi,
+               int z0 = 0, int z1 = -1, int o0 = 0, int o1 = -1, double* face = nullptr);
+void fft_finish(hipStream_t s, const fxd::FftPlan<double>& P, double* phi, int o0, int o1, const double* face);
+/* FftPlan::green of an FP64 plan with upward pivots (2 n2 values) */
+void fft_face_green(hipStream_t s, const fxd::FftPlan<double>& P, double* g);
 /* homogeneous solve phi = A^-1 b on every plane, with the partial sums of
  * phi . b (fft_dot_parts(P) of them) to dotp */
 void fft_solve_dot(hipStream_t s, const fxd::FftPlan<double>& P, const double* b, double* phi, double* dotp);

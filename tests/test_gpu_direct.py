@@ -91,14 +91,16 @@ def test_direct_time_loop_matches_oracle(precision):
 
 
 @pytest.mark.parametrize("fe_top,v_lo,top", [(12e-9, 0.0, "dir"), (12.8e-9, 0.0, "dir"), (12e-9, -0.3, "dir"),
-                                              (12e-9, -0.3, "neu")])
+                                              (12e-9, -0.3, "neu"), (4.1e-9, -0.3, "dir")])
 def test_direct_128_planes_time_loop(fe_top, v_lo, top):
-    """128 planes (the z systems solved from both ends, meeting between
-    planes 63 and 64), the FE layer ending below the top face (DE spacer
-    above) or at it, a nonzero bottom potential (the face term of mode (0, 0)
-    on the downward half, planes below the FE layer skipped), a Neumann top
-    face; 20 steps against the oracle, phi read at the end (the planes the
-    time loop left transformed finished then)."""
+    """128 planes (the z systems solved from both ends, meeting in the middle
+    of the planes with a right-hand side, the sweeps starting at its first and
+    last plane), the FE layer ending below the top face (DE spacer above) or
+    at it, or one plane thick (one plane a side), a nonzero bottom potential
+    (the face term of mode (0, 0) on the downward half, planes below the FE
+    layer skipped), a Neumann top face; 20 steps against the oracle, phi read
+    at the end (the z solution continued to the planes the time loop did not
+    store, and those planes transformed then)."""
     text = configs.mfim(nx=32, ny=32, nz=128, lx=16e-9, ly=16e-9, lz=12.8e-9, de_top=4e-9, fe_top=fe_top,
                         nsteps=20, bias=0.5)
     text += f"\nboundary.lo = per per dir({v_lo!r})\nPhi_Bc_lo = {v_lo!r}\n"
