@@ -71,6 +71,17 @@ using fxd::FftPlan;
 #ifndef FX_DPP_ROR
 #define FX_DPP_ROR 1
 #endif
+/* the four-step buffer carries the real parts across, then the imaginary
+ * ones (two barriers more, half the LDS): on 80 planes of 256 x 256 the y
+ * transform's 1280 workgroups then fit the CUs at once (eight a CU instead of
+ * four) and the x transform's 640 (three instead of two); 170.8 -> 160.5 us
+ * per time step with both */
+#ifndef FX_FFT_COLS_SPLIT
+#define FX_FFT_COLS_SPLIT 1
+#endif
+#ifndef FX_FFT_ROWS_SPLIT
+#define FX_FFT_ROWS_SPLIT 1
+#endif
 constexpr int ROWS_THR = FX_FFT_ROWS_THR, COLS_THR = FX_FFT_COLS_THR;
 constexpr int TRI_THR = 256;
 constexpr int TRI_BLK = 8;    /* planes per load block of the tridiagonal solve */
@@ -221,21 +232,63 @@ __device__ __forceinline__ T lane_partner16(T v) {
  * DPP, so X is never written back to LDS: one barrier fewer.  The Hartley
  * values are formed as hartley2 forms them (bitwise the same);
  * store(s, k, h0, h1) receives index k of sequence s. */
-template <class T, int N1, int N2, int PAD, class Load, class Store>
+template <class T, int N1, int N2, int PAD, bool SPLIT, class Load, class Store>
 __device__ __forceinline__ void four_step_hartley16(int R, const T* tw, T* Yr, T* Yi, Load load, int step1_s_fast,
                                                     Store store) {
     static_assert(N1 == 16, "the lane exchange works on rows of 16 lanes");
-    constexpr int YS = N1 + PAD;
+    constexpr int n = N1 * N2, YS = N1 + PAD;
     const int t = threadIdx.x;
-    four_step_1<T, N1, N2, PAD>(R, tw, Yr, Yi, load, step1_s_fast);
     const int s3 = t / N1, c3 = t % N1;
-    if (t < R * N1) {
-        T ur[N2], ui[N2], pr[N2], pi[N2];
+    T ur[N2], ui[N2];
+    if (!SPLIT) {
+        four_step_1<T, N1, N2, PAD>(R, tw, Yr, Yi, load, step1_s_fast);
+        if (t < R * N1) {
 #pragma unroll
-        for (int b = 0; b < N2; ++b) {
-            ur[b] = Yr[(s3 * N2 + b) * YS + c3];
-            ui[b] = Yi[(s3 * N2 + b) * YS + c3];
+            for (int b = 0; b < N2; ++b) {
+                ur[b] = Yr[(s3 * N2 + b) * YS + c3];
+                ui[b] = Yi[(s3 * N2 + b) * YS + c3];
+            }
         }
+    } else {
+        /* one buffer of real values (Yr): the real parts of step 1 go across,
+         * then the imaginary parts, so the workgroup needs half the LDS */
+        T xr[N1], xi[N1];
+        const bool act1 = t < R * N2;
+        const int s1 = step1_s_fast ? t % R : t / N2, b1 = step1_s_fast ? t / R : t % N2;
+        if (act1) {
+            T vr[N1], vi[N1];
+#pragma unroll
+            for (int a = 0; a < N1; ++a) load(s1, a, b1, vr[a], vi[a]);
+            reg_dft<T, N1>(vr, vi, tw, N2);
+#pragma unroll
+            for (int c = 0; c < N1; ++c) {
+                T wc, ws;
+                omega<T>(tw, b1 * c, n / 2, wc, ws);
+                const T ar = vr[brev_c<N1>(c)], ai = vi[brev_c<N1>(c)];
+                xr[c] = ar * wc - ai * ws;
+                xi[c] = ar * ws + ai * wc;
+            }
+#pragma unroll
+            for (int c = 0; c < N1; ++c) Yr[(s1 * N2 + b1) * YS + c] = xr[c];
+        }
+        __syncthreads();
+        if (t < R * N1) {
+#pragma unroll
+            for (int b = 0; b < N2; ++b) ur[b] = Yr[(s3 * N2 + b) * YS + c3];
+        }
+        __syncthreads();
+        if (act1) {
+#pragma unroll
+            for (int c = 0; c < N1; ++c) Yr[(s1 * N2 + b1) * YS + c] = xi[c];
+        }
+        __syncthreads();
+        if (t < R * N1) {
+#pragma unroll
+            for (int b = 0; b < N2; ++b) ui[b] = Yr[(s3 * N2 + b) * YS + c3];
+        }
+    }
+    if (t < R * N1) {
+        T pr[N2], pi[N2];
         reg_dft<T, N2>(ur, ui, tw, N1);
 #pragma unroll
         for (int r = 0; r < N2; ++r) {
@@ -292,7 +345,7 @@ __global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* 
     };
     if constexpr (N1 == 16 && FX_FFT_REG_HARTLEY) {
         if (FWD || !dotp) {
-            four_step_hartley16<T, N1, N2, 1>(R, tw, Yr, Yi, load, 0, [&](int s, int kk, T h0, T h1) {
+            four_step_hartley16<T, N1, N2, 1, FX_FFT_ROWS_SPLIT != 0>(R, tw, Yr, Yi, load, 0, [&](int s, int kk, T h0, T h1) {
                 const size_t o = row0 + (size_t)(2 * s) * n + kk;
                 if (FWD || !add) {
                     out[o] = (TOUT)h0;
@@ -357,7 +410,7 @@ __global__ __launch_bounds__(COLS_THR) void k_fft_cols(FftPlan<T> P, int R, int 
         im = v.y;
     };
     if constexpr (N1 == 16 && FX_FFT_REG_HARTLEY) {
-        four_step_hartley16<T, N1, N2, PAD>(R, tw, Yr, Yi, load, 1, [&](int s, int j, T h0, T h1) {
+        four_step_hartley16<T, N1, N2, PAD, FX_FFT_COLS_SPLIT != 0>(R, tw, Yr, Yi, load, 1, [&](int s, int j, T h0, T h1) {
             V2 v;
             v.x = h0;
             v.y = h1;
@@ -811,8 +864,8 @@ inline int seqs_per_block(int thr, int n1f, int n2f, int other) {
 }
 
 template <class T>
-size_t fft_lds_bytes(int N1, int N2, int R, int pad, int twl) {
-    return ((twl ? (size_t)N1 * N2 : 0) + 2 * (size_t)R * N2 * (N1 + pad)) * sizeof(T);
+size_t fft_lds_bytes(int N1, int N2, int R, int pad, int twl, bool half = false) {
+    return ((twl ? (size_t)N1 * N2 : 0) + (half ? 1 : 2) * (size_t)R * N2 * (N1 + pad)) * sizeof(T);
 }
 
 /* raise the dynamic LDS limit of a kernel to what a launch needs, where that
@@ -832,7 +885,9 @@ template <class T, class TIN, class TOUT, int N1, int N2>
 void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, void* out, double blo, double bhi, int add,
                  int kz0, int nk, const double* dotv, double* dotp) {
     const int R = seqs_per_block(ROWS_THR, N1, N2, P.n1);
-    const size_t lds = fft_lds_bytes<T>(N1, N2, R, 1, 1);
+    /* the split four-step path (real, then imaginary parts across) needs half the buffer */
+    const bool half = FX_FFT_ROWS_SPLIT && FX_FFT_REG_HARTLEY && N1 == 16 && (fwd || !dotp);
+    const size_t lds = fft_lds_bytes<T>(N1, N2, R, 1, 1, half);
     const dim3 g(P.n1 / (2 * R), nk);
     static size_t lf = 0, li = 0;
     if (fwd) lds_limit(&k_fft_rows<T, TIN, T, true, N1, N2>, lds, lf);
@@ -855,7 +910,8 @@ inline int rows_blocks(int l0, int n1, int nk) {
 template <class T, int N1, int N2>
 void cols_launch(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
     const int R = seqs_per_block(COLS_THR, N1, N2, P.n0);
-    const size_t lds = fft_lds_bytes<T>(N1, N2, R, FX_FFT_COLS_PAD, FX_FFT_COLS_TWL);
+    const bool half = FX_FFT_COLS_SPLIT && FX_FFT_REG_HARTLEY && N1 == 16;
+    const size_t lds = fft_lds_bytes<T>(N1, N2, R, FX_FFT_COLS_PAD, FX_FFT_COLS_TWL, half);
     static size_t lc = 0;
     lds_limit(&k_fft_cols<T, N1, N2>, lds, lc);
     hipLaunchKernelGGL((k_fft_cols<T, N1, N2>), dim3(P.n0 / (2 * R), nk), dim3(COLS_THR), lds, s, P, R, kz0);
