@@ -290,17 +290,21 @@ def main():
     # P, phi, the material id and the grain index and writes P (29 B per FE
     # cell).  The direct solve: the forward x and y transforms read and write
     # the planes with a nonzero right-hand side (the FE planes: 16 B per cell
-    # each), the z solve reads those, writes every plane and reads the pivot
-    # table once ((n0/2+1)(n1/2+1) n2 doubles), the inverse y and x
-    # transforms read and write the planes the update reads (the FE planes
-    # and one either side: 16 B per cell each).  V-cycles (154 B per cell)
-    # where the multigrid runs instead.
+    # each), the z solve reads those, writes the planes the inverse
+    # transforms read and one more either side (where it continues when phi
+    # is read elsewhere) and reads the pivot table of those planes once
+    # ((n0/2+1)(n1/2+1) doubles a plane), the inverse y and x transforms read
+    # and write the planes the update reads (the FE planes and one either
+    # side: 16 B per cell each).  V-cycles (154 B per cell) where the
+    # multigrid runs instead.
     direct = n_v == 0
     plane = nx * ny
     n_in = (k1 - k0 + 1) * plane
     n_out = (min(k1 + 1, nz - 1) - max(k0 - 1, 0) + 1) * plane
+    z_planes = min(nz, n_out // plane + 2)
     n_piv = (nx // 2 + 1) * (ny // 2 + 1) * nz
-    B_direct = 32.0 * n_in + 8.0 * n_in + 8.0 * N + 8.0 * n_piv + 32.0 * n_out
+    B_direct = 32.0 * n_in + 8.0 * n_in + 8.0 * z_planes * plane + 8.0 * (nx // 2 + 1) * (ny // 2 + 1) * z_planes \
+        + 32.0 * n_out
     B_step = N_FE * (21 + 29) + (B_direct if direct else 0.0) + N * (n_v * 154 + (n_outer - 1) * 58)
 
     roof = None
@@ -316,7 +320,8 @@ def main():
                                       "only), then over any plane holding other FE cells next to a face"),
                  ("fft_x", 16.0 * N, "k_fft_rows forward (Hartley along x)"),
                  ("fft_y", 16.0 * N, "k_fft_cols (Hartley along y, in place)"),
-                 ("fft_tri", 16.0 * N + 8.0 * n_piv, "k_fft_tri (Thomas solve in z per mode with tabulated pivots)"),
+                 ("fft_tri", 16.0 * N + 8.0 * n_piv, "k_fft_tri2 (the z systems of every plane solved from both ends "
+                                                      "with tabulated pivots)"),
                  ("fft_xinv", 16.0 * N, "k_fft_rows inverse"),
                  ("fft_solve", 80.0 * N + 8.0 * n_piv, "the whole direct solve on every plane (five launches)"),
                  ("copy", 16.0 * N, "plain copy (device bandwidth reference)")]
