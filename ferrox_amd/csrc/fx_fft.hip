@@ -82,6 +82,16 @@ using fxd::FftPlan;
 #ifndef FX_FFT_ROWS_SPLIT
 #define FX_FFT_ROWS_SPLIT 1
 #endif
+/* one value of padding after each sequence of the split buffer: the y
+ * transform's first-step stores (thread = sequence s, row b; its 16 values
+ * along c) otherwise put the eight sequences of a row on the same banks
+ * (159.9 -> 157.1 us per time step) */
+#ifndef FX_FFT_COLS_SPAD
+#define FX_FFT_COLS_SPAD 1
+#endif
+#ifndef FX_FFT_ROWS_SPAD
+#define FX_FFT_ROWS_SPAD 0
+#endif
 constexpr int ROWS_THR = FX_FFT_ROWS_THR, COLS_THR = FX_FFT_COLS_THR;
 constexpr int TRI_THR = 256;
 constexpr int TRI_BLK = 8;    /* planes per load block of the tridiagonal solve */
@@ -232,7 +242,7 @@ __device__ __forceinline__ T lane_partner16(T v) {
  * DPP, so X is never written back to LDS: one barrier fewer.  The Hartley
  * values are formed as hartley2 forms them (bitwise the same);
  * store(s, k, h0, h1) receives index k of sequence s. */
-template <class T, int N1, int N2, int PAD, bool SPLIT, class Load, class Store>
+template <class T, int N1, int N2, int PAD, bool SPLIT, int SP, class Load, class Store>
 __device__ __forceinline__ void four_step_hartley16(int R, const T* tw, T* Yr, T* Yi, Load load, int step1_s_fast,
                                                     Store store) {
     static_assert(N1 == 16, "the lane exchange works on rows of 16 lanes");
@@ -254,6 +264,9 @@ __device__ __forceinline__ void four_step_hartley16(int R, const T* tw, T* Yr, T
          * then the imaginary parts, so the workgroup needs half the LDS */
         T xr[N1], xi[N1];
         const bool act1 = t < R * N2;
+        /* sequence s at s (N2 YS + SP): SP = 1 moves consecutive sequences to
+         * other LDS banks */
+        constexpr int SS = N2 * YS + SP;
         const int s1 = step1_s_fast ? t % R : t / N2, b1 = step1_s_fast ? t / R : t % N2;
         if (act1) {
             T vr[N1], vi[N1];
@@ -269,22 +282,22 @@ __device__ __forceinline__ void four_step_hartley16(int R, const T* tw, T* Yr, T
                 xi[c] = ar * ws + ai * wc;
             }
 #pragma unroll
-            for (int c = 0; c < N1; ++c) Yr[(s1 * N2 + b1) * YS + c] = xr[c];
+            for (int c = 0; c < N1; ++c) Yr[s1 * SS + b1 * YS + c] = xr[c];
         }
         __syncthreads();
         if (t < R * N1) {
 #pragma unroll
-            for (int b = 0; b < N2; ++b) ur[b] = Yr[(s3 * N2 + b) * YS + c3];
+            for (int b = 0; b < N2; ++b) ur[b] = Yr[s3 * SS + b * YS + c3];
         }
         __syncthreads();
         if (act1) {
 #pragma unroll
-            for (int c = 0; c < N1; ++c) Yr[(s1 * N2 + b1) * YS + c] = xi[c];
+            for (int c = 0; c < N1; ++c) Yr[s1 * SS + b1 * YS + c] = xi[c];
         }
         __syncthreads();
         if (t < R * N1) {
 #pragma unroll
-            for (int b = 0; b < N2; ++b) ui[b] = Yr[(s3 * N2 + b) * YS + c3];
+            for (int b = 0; b < N2; ++b) ui[b] = Yr[s3 * SS + b * YS + c3];
         }
     }
     if (t < R * N1) {
@@ -345,7 +358,7 @@ __global__ __launch_bounds__(ROWS_THR) void k_fft_rows(FftPlan<T> P, const TIN* 
     };
     if constexpr (N1 == 16 && FX_FFT_REG_HARTLEY) {
         if (FWD || !dotp) {
-            four_step_hartley16<T, N1, N2, 1, FX_FFT_ROWS_SPLIT != 0>(R, tw, Yr, Yi, load, 0, [&](int s, int kk, T h0, T h1) {
+            four_step_hartley16<T, N1, N2, 1, FX_FFT_ROWS_SPLIT != 0, FX_FFT_ROWS_SPAD>(R, tw, Yr, Yi, load, 0, [&](int s, int kk, T h0, T h1) {
                 const size_t o = row0 + (size_t)(2 * s) * n + kk;
                 if (FWD || !add) {
                     out[o] = (TOUT)h0;
@@ -410,7 +423,7 @@ __global__ __launch_bounds__(COLS_THR) void k_fft_cols(FftPlan<T> P, int R, int 
         im = v.y;
     };
     if constexpr (N1 == 16 && FX_FFT_REG_HARTLEY) {
-        four_step_hartley16<T, N1, N2, PAD, FX_FFT_COLS_SPLIT != 0>(R, tw, Yr, Yi, load, 1, [&](int s, int j, T h0, T h1) {
+        four_step_hartley16<T, N1, N2, PAD, FX_FFT_COLS_SPLIT != 0, FX_FFT_COLS_SPAD>(R, tw, Yr, Yi, load, 1, [&](int s, int j, T h0, T h1) {
             V2 v;
             v.x = h0;
             v.y = h1;
@@ -887,7 +900,7 @@ void rows_launch(hipStream_t s, const FftPlan<T>& P, bool fwd, const void* in, v
     const int R = seqs_per_block(ROWS_THR, N1, N2, P.n1);
     /* the split four-step path (real, then imaginary parts across) needs half the buffer */
     const bool half = FX_FFT_ROWS_SPLIT && FX_FFT_REG_HARTLEY && N1 == 16 && (fwd || !dotp);
-    const size_t lds = fft_lds_bytes<T>(N1, N2, R, 1, 1, half);
+    const size_t lds = fft_lds_bytes<T>(N1, N2, R, 1, 1, half) + (half ? (size_t)R * FX_FFT_ROWS_SPAD * sizeof(T) : 0);
     const dim3 g(P.n1 / (2 * R), nk);
     static size_t lf = 0, li = 0;
     if (fwd) lds_limit(&k_fft_rows<T, TIN, T, true, N1, N2>, lds, lf);
@@ -911,7 +924,8 @@ template <class T, int N1, int N2>
 void cols_launch(hipStream_t s, const FftPlan<T>& P, int kz0, int nk) {
     const int R = seqs_per_block(COLS_THR, N1, N2, P.n0);
     const bool half = FX_FFT_COLS_SPLIT && FX_FFT_REG_HARTLEY && N1 == 16;
-    const size_t lds = fft_lds_bytes<T>(N1, N2, R, FX_FFT_COLS_PAD, FX_FFT_COLS_TWL, half);
+    const size_t lds = fft_lds_bytes<T>(N1, N2, R, FX_FFT_COLS_PAD, FX_FFT_COLS_TWL, half) +
+                       (half ? (size_t)R * FX_FFT_COLS_SPAD * sizeof(T) : 0);
     static size_t lc = 0;
     lds_limit(&k_fft_cols<T, N1, N2>, lds, lc);
     hipLaunchKernelGGL((k_fft_cols<T, N1, N2>), dim3(P.n0 / (2 * R), nk), dim3(COLS_THR), lds, s, P, R, kz0);
